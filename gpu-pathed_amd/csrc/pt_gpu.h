@@ -1,0 +1,68 @@
+// pt_gpu.h -- device-resident builders shared by the renderer and the unit entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "pt_types.h"
+
+namespace pt {
+
+#define PT_HIP_TRY(expr)                                                                   \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess) {                                                            \
+            throw HipFailure(std::string(#expr) + ": " + hipGetErrorString(_e));           \
+        }                                                                                  \
+    } while (0)
+
+struct HipFailure {
+    std::string msg;
+    explicit HipFailure(std::string m) : msg(std::move(m)) {}
+};
+
+// Owning device allocation (hipMalloc / hipFree).
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t n) {
+        if (n == bytes && p) return;
+        release();
+        if (n == 0) return;
+        PT_HIP_TRY(hipMalloc(&p, n));
+        bytes = n;
+    }
+    template <class T> T *as() const { return (T *)p; }
+};
+
+struct BvhInfo {
+    int numNodes = 0;
+    int depth = 0;
+};
+
+// Builds the LBVH over `n` primitive records (3 float4 each, input order) on `stream`:
+// writes 4 float4 per internal node to `nodes` (capacity max(n - 1, 1) nodes) and the
+// records in BVH order to `primsSorted`.  lo/hi bound all primitive centroids.
+void buildBvh(const float4 *primsIn, int n, const float lo[3], const float hi[3], float4 *nodes,
+              float4 *primsSorted, hipStream_t stream, BvhInfo *info);
+
+// Environment tables in FP64 on the GPU: per-row and marginal alias tables packed as
+// (prob as float, alias as int bits), texel pdf.  Optionally returns the FP64 tables.
+void buildEnvTables(const float4 *rgba, int w, int h, int sinWeight, float2 *rows, float2 *marginal,
+                    float *texelPdf, double *rowProbOut, int *rowAliasOut, double *margProbOut,
+                    int *margAliasOut, hipStream_t stream);
+
+int deviceCUs();
+
+}  // namespace pt

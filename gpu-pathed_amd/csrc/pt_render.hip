@@ -1,0 +1,1480 @@
+// pt_render.hip -- queue-driven wavefront path tracer for one MI355X (gfx950).
+//
+// One progressive pass of `spp` samples per pixel is cut into chunks of at most `pool`
+// paths (whole sample rows of the image, or pixel ranges when the image alone exceeds the
+// pool).  A chunk runs:
+//
+//   raygen            one thread per path: PCG32 stream, camera ray, path record
+//   for each bounce:
+//     extend          persistent waves fetch 64 rays at a time; closest hit through the BVH
+//                     with a traversal stack in LDS
+//     shade           emission with MIS weight, BSDF sample, light sample; appends the
+//                     continuation ray and the shadow ray to compacted queues (one atomic
+//                     per wave, slots from the wave's ballot prefix)
+//     shadow          persistent any-hit traversal; adds the unoccluded light contribution
+//   accumulate        per pixel: FP64 sum of the chunk's sample radiances
+//
+// Path state is structure-of-arrays in HBM, double-buffered between bounces so every queue
+// kernel reads and writes contiguously:
+//   rayO  float4 (origin, tmin)        rayD  float4 (direction, tmax)
+//   state uint4  (rng lo, rng hi, path id, depth)
+//   beta  float4 (throughput, pdf of the BSDF sample that led here; < 0: camera or delta)
+//   hit   float4 (t, u, v, primitive slot bits)
+//   shadow: sO float4 (origin, tmax), sD float4 (direction, path id), sL float4 (contribution)
+//   Lpath float4 per path id (radiance of the path so far)
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <string>
+#include <vector>
+
+#include "pt_device.h"
+#include "pt_gpu.h"
+#include "pt_internal.h"
+
+namespace pt {
+
+static constexpr int kMaxBounces = 64;
+// counters per bounce: ray count, shadow count, extend fetch, shadow fetch, radiance adds
+// from shade, radiance adds from shadow
+enum { C_RAYS = 0, C_SHADOW = 1, C_FETCH_EXT = 2, C_FETCH_SHADOW = 3, C_LADD_SHADE = 4, C_LADD_SHADOW = 5, C_PER_BOUNCE = 6 };
+static constexpr int kNumCounters = C_PER_BOUNCE * kMaxBounces + 8;
+static constexpr int kOverflowCounter = C_PER_BOUNCE * kMaxBounces;
+// device statistics (uint64): per bounce rays, shadows, shade adds, shadow adds; totals
+enum { S_RAYS = 0, S_SHADOW = kMaxBounces, S_LADD_SHADE = 2 * kMaxBounces, S_LADD_SHADOW = 3 * kMaxBounces, S_PATHS = 4 * kMaxBounces, S_COUNT };
+
+static constexpr int kTraceThreads = 128;
+static constexpr float kShadowEps = 1e-4f;   // path_tracer.cu:104-110
+static constexpr float kBounceEps = 1e-3f;   // path_tracer.cu:177
+
+struct ChunkParams {
+    int pixBegin;     // first pixel of the chunk
+    int npix;         // pixels per sample row of the chunk
+    int samples;      // sample rows in the chunk
+    int sampleBase;   // global index of the first sample row
+    int nPaths;       // npix * samples
+    int maxDepth;
+    int nee;
+    int W, H;
+    uint64_t seed;
+};
+
+// ------------------------------------------------------------------------------------------
+// wave helpers
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ int laneId() { return (int)(threadIdx.x & 63); }
+__device__ __forceinline__ uint32_t lanesBelow(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+// Appends for every lane with `want`: one atomic per wave; returns this lane's slot.
+__device__ __forceinline__ int waveAppend(bool want, int *counter) {
+    const uint64_t m = __ballot(want);
+    if (m == 0) return -1;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    int base = 0;
+    if (laneId() == leader) base = atomicAdd(counter, __popcll(m));
+    base = __shfl(base, leader);
+    return want ? base + (int)lanesBelow(m) : -1;
+}
+__device__ __forceinline__ void waveCount(bool c, int *counter) {
+    const uint64_t m = __ballot(c);
+    if (m != 0 && laneId() == __ffsll((unsigned long long)m) - 1) atomicAdd(counter, __popcll(m));
+}
+
+// ------------------------------------------------------------------------------------------
+// ray generation (camera.h:40-61, PCG32 stream per (pixel, sample))
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ void cameraRay(const PtCamera &c, int row, int col, float sx, float sy, f3 *o, f3 *d) {
+    const float xc = (col + sx) / c.W;
+    const float yc = (row + sy) / c.H;
+    const float y = yc * c.height - c.top;
+    const float x = xc * c.width - c.right;
+    const f3 dl = normalize(mk(x, y, -1.f));
+    *o = mk(c.m[0][3], c.m[1][3], c.m[2][3]);
+    *d = mk(c.m[0][0] * dl.x + c.m[0][1] * dl.y + c.m[0][2] * dl.z, c.m[1][0] * dl.x + c.m[1][1] * dl.y + c.m[1][2] * dl.z,
+            c.m[2][0] * dl.x + c.m[2][1] * dl.y + c.m[2][2] * dl.z);
+}
+
+__global__ __launch_bounds__(256) void k_raygen(ChunkParams cp, PtCamera cam, float4 *rayO, float4 *rayD, uint4 *state, float4 *beta,
+                                                float4 *Lpath, int *counters) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0) {
+        for (int k = threadIdx.x; k < kNumCounters; k += blockDim.x) counters[k] = (k == C_RAYS) ? cp.nPaths : 0;
+    }
+    if (i >= cp.nPaths) return;
+    const int s = i / cp.npix;
+    const int pixel = cp.pixBegin + (i - s * cp.npix);
+    const int sample = cp.sampleBase + s;
+    const int row = pixel / cp.W, col = pixel - (pixel / cp.W) * cp.W;
+    Rng rng = seedRng(cp.seed, (uint32_t)pixel, (uint32_t)sample);
+    const float sx = rng.uniform(), sy = rng.uniform();
+    f3 o, d;
+    cameraRay(cam, row, col, sx, sy, &o, &d);
+    rayO[i] = f4(o, 0.f);
+    rayD[i] = f4(d, FLT_MAX);
+    state[i] = make_uint4((uint32_t)rng.state, (uint32_t)(rng.state >> 32), (uint32_t)i, 0u);
+    beta[i] = make_float4(1.f, 1.f, 1.f, -1.f);
+    Lpath[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// ------------------------------------------------------------------------------------------
+// BVH traversal (closest hit / any hit).  Ordered traversal of the binary BVH: both child
+// boxes per 64-byte node, nearer child first, farther one pushed on a per-lane stack in LDS
+// (stack[level * blockDim + lane]).
+// ------------------------------------------------------------------------------------------
+struct TraceHit {
+    float t, u, v;
+    int prim;
+};
+
+template <bool ANY, int STACK>
+__device__ __forceinline__ bool traverse(const PtSceneView &sc, f3 o, f3 d, float tmin, float tmax, int *stack, TraceHit *out, int *overflow) {
+    if (sc.numNodes == 0) return false;
+    const WtRay wr = wtSetup(o, d);
+    const float ix = 1.f / (fabsf(d.x) > 1e-20f ? d.x : copysignf(1e-20f, d.x));
+    const float iy = 1.f / (fabsf(d.y) > 1e-20f ? d.y : copysignf(1e-20f, d.y));
+    const float iz = 1.f / (fabsf(d.z) > 1e-20f ? d.z : copysignf(1e-20f, d.z));
+    const float ox = o.x * ix, oy = o.y * iy, oz = o.z * iz;
+    const int stride = blockDim.x;
+    int sp = 0;
+    int node = 0;
+    bool hit = false;
+    while (true) {
+        if (node >= 0) {
+            const float4 *n = sc.nodes + 4 * (size_t)node;
+            const float4 n0 = n[0], n1 = n[1], n2 = n[2], n3 = n[3];
+            const float a0 = n0.x * ix - ox, a1 = n0.y * ix - ox, a2 = n0.z * iy - oy, a3 = n0.w * iy - oy;
+            const float a4 = n2.x * iz - oz, a5 = n2.y * iz - oz;
+            const float b0 = n1.x * ix - ox, b1 = n1.y * ix - ox, b2 = n1.z * iy - oy, b3 = n1.w * iy - oy;
+            const float b4 = n2.z * iz - oz, b5 = n2.w * iz - oz;
+            const float c0lo = fmaxf(fmaxf(fminf(a0, a1), fminf(a2, a3)), fmaxf(fminf(a4, a5), tmin));
+            const float c0hi = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fminf(fmaxf(a4, a5), tmax));
+            const float c1lo = fmaxf(fmaxf(fminf(b0, b1), fminf(b2, b3)), fmaxf(fminf(b4, b5), tmin));
+            const float c1hi = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fminf(fmaxf(b4, b5), tmax));
+            const bool h0 = c0lo <= c0hi, h1 = c1lo <= c1hi;
+            const int ch0 = asInt(n3.x), ch1 = asInt(n3.y);
+            if (h0 && h1) {
+                int nearN = ch0, farN = ch1;
+                if (c1lo < c0lo) { nearN = ch1; farN = ch0; }
+                if (sp < STACK) {
+                    stack[sp * stride] = farN;
+                    sp++;
+                } else {
+                    atomicAdd(overflow, 1);
+                }
+                node = nearN;
+                continue;
+            }
+            if (h0) { node = ch0; continue; }
+            if (h1) { node = ch1; continue; }
+        } else {
+            const int leaf = ~node;
+            const int first = leaf & ((1 << PT_LEAF_COUNT_SHIFT) - 1);
+            const int cnt = (leaf >> PT_LEAF_COUNT_SHIFT) + 1;
+            for (int k = first; k < first + cnt; k++) {
+                const float4 p0 = sc.prims[3 * (size_t)k], p1 = sc.prims[3 * (size_t)k + 1];
+                float t, u = 0.f, v = 0.f;
+                bool h;
+                if (asInt(p1.w) == PT_TAG_SPHERE) {
+                    h = sphereHit(xyz(p0), p1.x, o, d, tmin, tmax, &t);
+                } else {
+                    const float4 p2 = sc.prims[3 * (size_t)k + 2];
+                    h = wtHit(wr, xyz(p0), xyz(p1), xyz(p2), tmin, tmax, &t, &u, &v);
+                }
+                if (h) {
+                    hit = true;
+                    tmax = t;
+                    out->t = t;
+                    out->u = u;
+                    out->v = v;
+                    out->prim = k;
+                    if (ANY) return true;
+                }
+            }
+        }
+        if (sp == 0) break;
+        sp--;
+        node = stack[sp * stride];
+    }
+    return hit;
+}
+
+// Persistent closest-hit kernel: each wave takes 64 rays per atomic until the queue is empty.
+template <int STACK>
+__global__ __launch_bounds__(kTraceThreads) void k_extend(PtSceneView sc, const int *count, int *fetch, const float4 *rayO, const float4 *rayD,
+                                                         float4 *hits, int *overflow) {
+    __shared__ int sstack[STACK * kTraceThreads];
+    int *stack = sstack + threadIdx.x;
+    const int n = *count;
+    while (true) {
+        int base = 0;
+        if (laneId() == 0) base = atomicAdd(fetch, 64);
+        base = __shfl(base, 0);
+        if (base >= n) break;
+        const int i = base + laneId();
+        if (i < n) {
+            const float4 ro = rayO[i], rd = rayD[i];
+            TraceHit th;
+            th.t = FLT_MAX; th.u = 0.f; th.v = 0.f; th.prim = INT32_MIN;
+            traverse<false, STACK>(sc, xyz(ro), xyz(rd), ro.w, rd.w, stack, &th, overflow);
+            hits[i] = make_float4(th.t, th.u, th.v, asFloat(th.prim));
+        }
+    }
+}
+
+// Persistent any-hit kernel for shadow rays (interval [1e-4, tmax]).
+template <int STACK>
+__global__ __launch_bounds__(kTraceThreads) void k_shadow(PtSceneView sc, const int *count, int *fetch, const float4 *sO, const float4 *sD,
+                                                         const float4 *sL, float4 *Lpath, int *laddCounter, int *overflow) {
+    __shared__ int sstack[STACK * kTraceThreads];
+    int *stack = sstack + threadIdx.x;
+    const int n = *count;
+    while (true) {
+        int base = 0;
+        if (laneId() == 0) base = atomicAdd(fetch, 64);
+        base = __shfl(base, 0);
+        if (base >= n) break;
+        const int i = base + laneId();
+        bool add = false;
+        if (i < n) {
+            const float4 o = sO[i], d = sD[i];
+            TraceHit th;
+            const bool occluded = traverse<true, STACK>(sc, xyz(o), xyz(d), kShadowEps, o.w, stack, &th, overflow);
+            if (!occluded) {
+                const int pid = asInt(d.w);
+                const float4 l = sL[i];
+                float4 acc = Lpath[pid];
+                acc.x += l.x; acc.y += l.y; acc.z += l.z;
+                Lpath[pid] = acc;
+                add = true;
+            }
+        }
+        waveCount(add, laddCounter);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// shading (path_tracer.cu:38-184 NEE + MIS, :186-233 naive), one path vertex per lane
+// ------------------------------------------------------------------------------------------
+struct SurfacePoint {
+    f3 p, n;
+    int mat;
+    float area;  // area of the primitive (for the light pdf of BSDF-sampled emitter hits)
+};
+
+__device__ __forceinline__ SurfacePoint surfaceAt(const PtSceneView &sc, int slot, f3 o, f3 d, float t, float u, float v) {
+    SurfacePoint s;
+    const float4 p0 = sc.prims[3 * (size_t)slot], p1 = sc.prims[3 * (size_t)slot + 1];
+    s.mat = asInt(p0.w);
+    const int tag = asInt(p1.w);
+    if (tag == PT_TAG_SPHERE) {
+        const f3 c = xyz(p0);
+        const float r = p1.x;
+        s.p = o + d * t;
+        s.n = normalize((s.p - c) / r);
+        s.area = sphereArea(r);
+    } else {
+        const float4 p2 = sc.prims[3 * (size_t)slot + 2];
+        const f3 a = xyz(p0), b = xyz(p1), c = xyz(p2);
+        s.p = a * (1.f - u - v) + b * u + c * v;  // triangle.cu:36
+        if (tag >= 0) {
+            const f3 na = xyz(sc.normals[3 * (size_t)tag]), nb = xyz(sc.normals[3 * (size_t)tag + 1]), nc = xyz(sc.normals[3 * (size_t)tag + 2]);
+            s.n = normalize(na * (1.f - u - v) + nb * u + nc * v);
+        } else {
+            s.n = normalize(cross(b - a, c - a));
+        }
+        s.area = triArea(a, b, c);
+    }
+    return s;
+}
+
+struct LightSample {
+    f3 wi, n, emitted;
+    float dist, pdf;
+};
+
+// sampler.h:142-183 uniform choice over emitters (+ environment), triangle.h:66-90,
+// sphere.h:30-44, surface_sample.h:13-24
+__device__ __forceinline__ LightSample sampleLights(const PtSceneView &sc, f3 p, float l1, float l2, float l3) {
+    LightSample ls;
+    const int nl = sc.numEmitters;
+    const int choiceSize = nl + (sc.env.present ? 1 : 0);
+    const int choice = (int)fminf((float)(choiceSize - 1), floorf(l1 * (float)choiceSize));
+    const float choicePdf = 1.f / (float)choiceSize;
+    if (choice < nl) {
+        const float4 e0 = sc.emitters[3 * choice], e1 = sc.emitters[3 * choice + 1];
+        f3 pt, n;
+        float pdfA;
+        if (asInt(e1.w) == PT_TAG_SPHERE) {
+            const f3 pl = uniformSphere(l2, l3);
+            pt = pl * e1.x + xyz(e0);
+            n = pl;
+            pdfA = 1.f / sphereArea(e1.x);
+        } else {
+            const float4 e2 = sc.emitters[3 * choice + 2];
+            const f3 a = xyz(e0), b = xyz(e1), c = xyz(e2);
+            const float sr = sqrtf(l2);
+            const float wa = 1.f - sr, wb = sr * (1.f - l3);
+            const float wc = 1.f - wa - wb;
+            pt = a * wa + b * wb + c * wc;
+            n = normalize(cross(b - a, c - a));
+            pdfA = 1.f / triArea(a, b, c);
+        }
+        const f3 dir = pt - p;
+        const float dist = length(dir);
+        const f3 wdir = normalize(dir);
+        const float proj = fabsf(dot(n, -wdir));
+        const float sa = proj == 0.f ? 0.f : pdfA * (dist * dist) / proj;
+        const PtMaterial m = sc.materials[asInt(e0.w)];
+        ls.wi = wdir;
+        ls.n = n;
+        ls.dist = dist;
+        ls.pdf = sa * choicePdf;
+        ls.emitted = mk(m.b.x, m.b.y, m.b.z);
+    } else {
+        float pdf;
+        const f3 w = envSample(sc.env, l2, l3, &pdf);
+        ls.wi = normalize(w);
+        ls.n = -ls.wi;
+        ls.dist = 1e16f;
+        ls.pdf = pdf * choicePdf;
+        ls.emitted = envEmit(sc.env, w);
+    }
+    return ls;
+}
+
+// sampler.h:89-125 light pdf (solid angle) of a BSDF-sampled emitter hit, max(0, cos)
+__device__ __forceinline__ float pdfSceneLight(const PtSceneView &sc, f3 ref, const SurfacePoint &s) {
+    const f3 dir = ref - s.p;
+    const f3 wo = normalize(dir);
+    const float dist = length(dir);
+    const float proj = fmaxf(0.f, dot(s.n, wo));
+    const float pdfSA = (1.f / s.area) * (dist * dist) / proj;
+    return pdfSA / (float)(sc.numEmitters + (sc.env.present ? 1 : 0));
+}
+// sampler.h:127-140
+__device__ __forceinline__ float pdfEnvLight(const PtSceneView &sc, f3 w) {
+    if (!sc.env.present) return 0.f;
+    return envPdf(sc.env, w) * (1.f / (float)(sc.numEmitters + 1));
+}
+
+__global__ __launch_bounds__(256) void k_shade(PtSceneView sc, ChunkParams cp, int bounce, int *counters, const float4 *rayO, const float4 *rayD,
+                                               const uint4 *state, const float4 *beta, const float4 *hits, float4 *nRayO, float4 *nRayD,
+                                               uint4 *nState, float4 *nBeta, float4 *sO, float4 *sD, float4 *sL, float4 *Lpath) {
+    int *c = counters + C_PER_BOUNCE * bounce;
+    int *cNext = counters + C_PER_BOUNCE * (bounce + 1);
+    const int n = c[C_RAYS];
+    const int waveStride = gridDim.x * blockDim.x;
+    for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n; base += waveStride) {
+        const int i = base + laneId();
+        const bool valid = i < n;
+        bool pushRay = false, pushShadow = false, addL = false;
+        float4 outRO, outRD, outB, shO, shD, shL;
+        uint4 outS;
+        f3 L = mk(0.f);
+        int pathId = 0;
+        if (valid) {
+            const float4 ro = rayO[i], rd = rayD[i], h = hits[i], bp = beta[i];
+            const uint4 st = state[i];
+            pathId = (int)st.z;
+            const int depth = (int)st.w;
+            const f3 o = xyz(ro), d = xyz(rd), thr = xyz(bp);
+            const float prevPdf = bp.w;
+            const int slot = asInt(h.w);
+            if (slot == INT32_MIN) {
+                // miss: environment (camera miss: environment_light.h:68-83 as is)
+                const f3 e = envEmit(sc.env, d);
+                if (depth == 0 || !cp.nee || prevPdf < 0.f) L = thr * e;
+                else L = thr * e * balance(prevPdf, pdfEnvLight(sc, d));
+            } else {
+                const SurfacePoint sp = surfaceAt(sc, slot, o, d, h.x, h.y, h.z);
+                const Frame fr = makeFrame(sp.n);
+                const f3 wo = normalize(toLocal(fr, -d));
+                const PtMaterial m = sc.materials[sp.mat];
+                const int type = matType(m);
+                // lambertian.cu:5-11: emission seen from the front only
+                if (type == PT_MAT_LAMBERTIAN && wo.z >= 0.f && (m.b.x != 0.f || m.b.y != 0.f || m.b.z != 0.f)) {
+                    const f3 e = mk(m.b.x, m.b.y, m.b.z);
+                    if (depth == 0 || !cp.nee || prevPdf < 0.f) L = thr * e;
+                    else L = thr * e * balance(prevPdf, pdfSceneLight(sc, o, sp));
+                }
+                if (depth + 1 < cp.maxDepth) {
+                    Rng rng;
+                    const int s = pathId / cp.npix;
+                    const int pixel = cp.pixBegin + (pathId - s * cp.npix);
+                    rng.inc = streamInc(cp.seed, (uint32_t)pixel, (uint32_t)(cp.sampleBase + s));
+                    rng.state = ((uint64_t)st.y << 32) | st.x;
+                    const int dims = bsdfDims(type);
+                    const float x1 = dims >= 1 ? rng.uniform() : 0.f;
+                    const float x2 = dims >= 2 ? rng.uniform() : 0.f;
+                    const BsdfSample bs = bsdfSample(m, wo, x1, x2);
+                    if (cp.nee && !bs.delta) {
+                        // directSampleLights (path_tracer.cu:83-126); a scene without lights
+                        // draws the three uniforms and adds nothing
+                        const float l1 = rng.uniform(), l2 = rng.uniform(), l3 = rng.uniform();
+                        const bool anyLight = sc.numEmitters > 0 || sc.env.present;
+                        LightSample ls;
+                        if (anyLight) ls = sampleLights(sc, sp.p, l1, l2, l3);
+                        if (anyLight && dot(ls.n, ls.wi) < 0.f) {
+                            const f3 wiL = toLocal(fr, ls.wi);
+                            const float pB = bsdfPdf(m, wo, wiL);
+                            const float w = balance(ls.pdf, pB);
+                            const f3 contrib = thr * (ls.emitted * w * bsdfF(m, wo, wiL) * fabsf(dot(sp.n, ls.wi)) / ls.pdf);
+                            if (!isZero(contrib) && !(contrib.x != contrib.x)) {
+                                pushShadow = true;
+                                shO = f4(sp.p, ls.dist - 2e-4f);
+                                shD = f4(ls.wi, asFloat(pathId));
+                                shL = f4(contrib, 0.f);
+                            }
+                        }
+                    }
+                    if (!isZero(bs.f) && bs.pdf > 0.f) {
+                        const f3 nb = thr * (bs.f * fabsf(bs.wi.z) / bs.pdf);
+                        if (!isZero(nb)) {
+                            pushRay = true;
+                            outRO = f4(sp.p, kBounceEps);
+                            outRD = f4(toWorld(fr, bs.wi), FLT_MAX);
+                            outS = make_uint4((uint32_t)rng.state, (uint32_t)(rng.state >> 32), (uint32_t)pathId, (uint32_t)(depth + 1));
+                            outB = f4(nb, bs.delta ? -1.f : bs.pdf);
+                        }
+                    }
+                }
+            }
+            addL = !isZero(L);
+            if (addL) {
+                float4 acc = Lpath[pathId];
+                acc.x += L.x; acc.y += L.y; acc.z += L.z;
+                Lpath[pathId] = acc;
+            }
+        }
+        const int rs = waveAppend(pushRay, &cNext[C_RAYS]);
+        if (pushRay) {
+            nRayO[rs] = outRO;
+            nRayD[rs] = outRD;
+            nState[rs] = outS;
+            nBeta[rs] = outB;
+        }
+        const int ss = waveAppend(pushShadow, &c[C_SHADOW]);
+        if (pushShadow) {
+            sO[ss] = shO;
+            sD[ss] = shD;
+            sL[ss] = shL;
+        }
+        waveCount(addL, &c[C_LADD_SHADE]);
+    }
+}
+
+// Per pixel of the chunk: FP64 sum over the chunk's sample rows (framebuffer.cu:7-41 keeps a
+// running FP32 mean instead).  Block 0 also folds the chunk's queue counters into the stats.
+__global__ __launch_bounds__(256) void k_accumulate(ChunkParams cp, const float4 *Lpath, double *accum, const int *counters, unsigned long long *stats) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0) {
+        for (int b = threadIdx.x; b < kMaxBounces; b += blockDim.x) {
+            const int *c = counters + C_PER_BOUNCE * b;
+            atomicAdd(&stats[S_RAYS + b], (unsigned long long)c[C_RAYS]);
+            atomicAdd(&stats[S_SHADOW + b], (unsigned long long)c[C_SHADOW]);
+            atomicAdd(&stats[S_LADD_SHADE + b], (unsigned long long)c[C_LADD_SHADE]);
+            atomicAdd(&stats[S_LADD_SHADOW + b], (unsigned long long)c[C_LADD_SHADOW]);
+        }
+        if (threadIdx.x == 0) atomicAdd(&stats[S_PATHS], (unsigned long long)cp.nPaths);
+    }
+    if (p >= cp.npix) return;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int s = 0; s < cp.samples; s++) {
+        const float4 l = Lpath[(size_t)s * cp.npix + p];
+        sx += (double)l.x;
+        sy += (double)l.y;
+        sz += (double)l.z;
+    }
+    double *a = accum + 3 * (size_t)(cp.pixBegin + p);
+    a[0] += sx;
+    a[1] += sy;
+    a[2] += sz;
+}
+
+// mean radiance + the uchar4 framebuffer of framebuffer.cu:35-40 (color.h:7-28 sRGB)
+__global__ __launch_bounds__(256) void k_tonemap(const double *accum, int npix, double invSpp, float *radiance, uchar4 *rgba) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    float v[3];
+    unsigned char o[3];
+    for (int k = 0; k < 3; k++) {
+        v[k] = (float)(accum[3 * (size_t)p + k] * invSpp);
+        radiance[3 * (size_t)p + k] = v[k];
+        const float x = v[k];
+        const float s = x <= 0.0031308f ? x * 12.92f : 1.055f * powf(x, 1.f / 2.2f) - 0.055f;
+        o[k] = (unsigned char)(fmaxf(0.f, fminf(1.f, s)) * 255.f);
+    }
+    rgba[p] = make_uchar4(o[0], o[1], o[2], 255);
+}
+
+// Normals view (g_buffer.cu:27-57): colour (n + 1) / 2 of the primary hit at the pixel centre
+__global__ __launch_bounds__(256) void k_normals_shade(PtSceneView sc, int npix, const float4 *rayO, const float4 *rayD, const float4 *hits,
+                                                       double *accum) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    const float4 h = hits[p];
+    const int slot = asInt(h.w);
+    f3 c = mk(0.f);
+    if (slot != INT32_MIN) {
+        const SurfacePoint s = surfaceAt(sc, slot, xyz(rayO[p]), xyz(rayD[p]), h.x, h.y, h.z);
+        c = (s.n + mk(1.f)) / 2.f;
+    }
+    double *a = accum + 3 * (size_t)p;
+    a[0] += c.x;
+    a[1] += c.y;
+    a[2] += c.z;
+}
+
+__global__ void k_centre_rays(PtCamera cam, int npix, int W, float4 *rayO, float4 *rayD, int *counters) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0)
+        for (int k = threadIdx.x; k < kNumCounters; k += blockDim.x) counters[k] = (k == C_RAYS) ? npix : 0;
+    if (p >= npix) return;
+    f3 o, d;
+    cameraRay(cam, p / W, p % W, 0.5f, 0.5f, &o, &d);
+    rayO[p] = f4(o, 0.f);
+    rayD[p] = f4(d, FLT_MAX);
+}
+
+__global__ void k_set_counter(int *counters, int idx, int value) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        for (int k = 0; k < kNumCounters; k++) counters[k] = 0;
+        counters[idx] = value;
+    }
+}
+
+__global__ void k_map_prims(const PtSceneView sc, const float4 *hits, int n, float4 *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 h = hits[i];
+    const int slot = asInt(h.w);
+    if (slot != INT32_MIN) h.w = sc.prims[3 * (size_t)slot + 2].w;  // original index
+    else h.x = -1.f;
+    out[i] = h;
+}
+
+__global__ void k_zero_d(double *p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+}  // namespace pt
+
+// ============================================================================================
+// Renderer object
+// ============================================================================================
+using namespace pt;
+
+namespace {
+enum { K_RAYGEN = 0, K_EXTEND, K_SHADE, K_SHADOW, K_ACCUM, K_TONEMAP, K_NUM };
+const char *kKernelNames[K_NUM] = {"raygen", "extend", "shade", "shadow", "accumulate", "tonemap"};
+}  // namespace
+
+struct pt_renderer {
+    int W = 0, H = 0;
+    hipStream_t stream = nullptr;
+    int cus = 256;
+    // scene on the device
+    DevBuf nodes, prims, primsIn, normals, materials, emitters, envRgba, envRows, envMarg, envPdf;
+    PtSceneView view{};
+    BvhInfo bvh;
+    int stackSize = 32;
+    bool hasScene = false;
+    // host copies needed for edits
+    std::vector<HostMaterial> mats;
+    std::vector<int> triMat, sphMat;
+    std::vector<float> triPos, sph;
+    HostCamera camHost;
+    PtCamera cam{};
+    int maxDepth = 3;  // scene.h:18 defaultMaxDepth
+    int nee = 1;
+    int type = PT_RENDERER_OPTIX;
+    uint64_t seed = 0;
+    // path pool
+    int64_t pool = 0, poolWanted = 1 << 22;
+    DevBuf rayO[2], rayD[2], st[2], beta[2], hits, sO, sD, sL, Lpath, counters, overflow;
+    int traceBlocksPerCU = 8, shadeBlocksPerCU = 8;
+    // accumulation
+    DevBuf accum, radiance, rgba, stats;
+    int64_t spp = 0;
+    bool resetPending = false;
+    // pass tracking
+    hipEvent_t evBegin = nullptr, evEnd = nullptr;
+    bool inFlight = false;
+    int passSpp = 0;
+    int64_t lastPassPaths = 0;
+    bool lastPassSingleChunk = false;
+    double gpuMs = 0.0;
+    int64_t passes = 0;
+    // kernel profiling
+    bool profile = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> kevents;
+    std::vector<hipEvent_t> eventPool;
+    size_t eventNext = 0;
+    pt_kernel_stat kstat[K_NUM] = {};
+    double samplesAccumulated = 0.0, pixelsAccumulated = 0.0;
+    std::vector<ChunkParams> lastChunks;
+};
+
+namespace {
+
+hipEvent_t takeEvent(pt_renderer *r) {
+    if (r->eventNext >= r->eventPool.size()) {
+        hipEvent_t e;
+        PT_HIP_TRY(hipEventCreate(&e));
+        r->eventPool.push_back(e);
+    }
+    return r->eventPool[r->eventNext++];
+}
+
+struct KTimer {
+    pt_renderer *r;
+    int k;
+    hipEvent_t a = nullptr, b = nullptr;
+    KTimer(pt_renderer *rr, int kk) : r(rr), k(kk) {
+        if (r->profile) {
+            a = takeEvent(r);
+            b = takeEvent(r);
+            PT_HIP_TRY(hipEventRecord(a, r->stream));
+        }
+    }
+    ~KTimer() {
+        if (r->profile) {
+            (void)hipEventRecord(b, r->stream);
+            r->kevents.push_back({k, {a, b}});
+        }
+    }
+};
+
+void computeCamera(pt_renderer *r) {
+    PtCamera &c = r->cam;
+    lookAt(r->camHost.origin, r->camHost.target, r->camHost.up, r->camHost.flip, c.m);
+    // camera.h:41-46
+    c.top = tanf(r->camHost.vfov / 2.f);
+    c.height = c.top * 2.f;
+    const float aspect = 1.f * r->W / r->H;
+    c.width = c.height * aspect;
+    c.right = c.width / 2.f;
+    c.W = r->W;
+    c.H = r->H;
+}
+
+void uploadMaterials(pt_renderer *r) {
+    std::vector<PtMaterial> m(r->mats.size());
+    for (size_t i = 0; i < m.size(); i++) {
+        const HostMaterial &h = r->mats[i];
+        const bool lamb = h.type == PT_LAMBERTIAN;
+        m[i].a = make_float4(h.albedo[0], h.albedo[1], h.albedo[2], (float)h.type);
+        const float param = h.type == PT_GLASS ? h.ior : (h.type == PT_MICROFACET ? h.alpha : 0.f);
+        m[i].b = make_float4(lamb ? h.emit[0] : 0.f, lamb ? h.emit[1] : 0.f, lamb ? h.emit[2] : 0.f, param);
+    }
+    r->materials.alloc(std::max<size_t>(1, m.size()) * sizeof(PtMaterial));
+    if (!m.empty()) PT_HIP_TRY(hipMemcpyAsync(r->materials.p, m.data(), m.size() * sizeof(PtMaterial), hipMemcpyHostToDevice, r->stream));
+    r->view.materials = r->materials.as<PtMaterial>();
+    r->view.numMaterials = (int)m.size();
+}
+
+// Emitter records in the reference light order (scene_data.cu:92-115); rebuilt after edits
+// so that emission changes also change the light list.
+void uploadEmitters(pt_renderer *r) {
+    std::vector<float4> e;
+    const int64_t nt = (int64_t)r->triMat.size();
+    for (int64_t i = 0; i < nt; i++) {
+        if (!isEmissive(r->mats[r->triMat[i]])) continue;
+        const float *p = &r->triPos[i * 9];
+        e.push_back(make_float4(p[0], p[1], p[2], asFloat(r->triMat[i])));
+        e.push_back(make_float4(p[3], p[4], p[5], asFloat(PT_TAG_FLAT)));
+        e.push_back(make_float4(p[6], p[7], p[8], asFloat((int)i)));
+    }
+    for (size_t i = 0; i < r->sphMat.size(); i++) {
+        if (!isEmissive(r->mats[r->sphMat[i]])) continue;
+        const float *s = &r->sph[i * 4];
+        e.push_back(make_float4(s[0], s[1], s[2], asFloat(r->sphMat[i])));
+        e.push_back(make_float4(s[3], 0.f, 0.f, asFloat(PT_TAG_SPHERE)));
+        e.push_back(make_float4(0.f, 0.f, 0.f, asFloat(~(int)i)));
+    }
+    r->emitters.alloc(std::max<size_t>(1, e.size()) * sizeof(float4));
+    if (!e.empty()) PT_HIP_TRY(hipMemcpyAsync(r->emitters.p, e.data(), e.size() * sizeof(float4), hipMemcpyHostToDevice, r->stream));
+    r->view.emitters = r->emitters.as<float4>();
+    r->view.numEmitters = (int)(e.size() / 3);
+}
+
+void ensurePool(pt_renderer *r) {
+    const int64_t want = std::max<int64_t>(4096, r->poolWanted);
+    if (r->pool == want) return;
+    const size_t n = (size_t)want;
+    for (int k = 0; k < 2; k++) {
+        r->rayO[k].alloc(n * 16);
+        r->rayD[k].alloc(n * 16);
+        r->st[k].alloc(n * 16);
+        r->beta[k].alloc(n * 16);
+    }
+    r->hits.alloc(n * 16);
+    r->sO.alloc(n * 16);
+    r->sD.alloc(n * 16);
+    r->sL.alloc(n * 16);
+    r->Lpath.alloc(n * 16);
+    r->pool = want;
+}
+
+void zeroAccum(pt_renderer *r) {
+    const size_t n = 3 * (size_t)r->W * r->H;
+    k_zero_d<<<(unsigned)((n + 255) / 256), 256, 0, r->stream>>>(r->accum.as<double>(), n);
+    PT_HIP_TRY(hipGetLastError());
+    r->spp = 0;
+    r->resetPending = false;
+}
+
+void launchExtend(pt_renderer *r, const int *count, int *fetch, const float4 *ro, const float4 *rd, float4 *hits) {
+    const int grid = r->cus * r->traceBlocksPerCU;
+    int *ov = r->overflow.as<int>();
+    switch (r->stackSize) {
+    case 32: k_extend<32><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, ro, rd, hits, ov); break;
+    case 64: k_extend<64><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, ro, rd, hits, ov); break;
+    default: k_extend<96><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, ro, rd, hits, ov); break;
+    }
+    PT_HIP_TRY(hipGetLastError());
+}
+void launchShadow(pt_renderer *r, const int *count, int *fetch, int *ladd) {
+    const int grid = r->cus * r->traceBlocksPerCU;
+    int *ov = r->overflow.as<int>();
+    const float4 *o = r->sO.as<float4>(), *d = r->sD.as<float4>(), *l = r->sL.as<float4>();
+    float4 *L = r->Lpath.as<float4>();
+    switch (r->stackSize) {
+    case 32: k_shadow<32><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, o, d, l, L, ladd, ov); break;
+    case 64: k_shadow<64><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, o, d, l, L, ladd, ov); break;
+    default: k_shadow<96><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, o, d, l, L, ladd, ov); break;
+    }
+    PT_HIP_TRY(hipGetLastError());
+}
+
+void runChunk(pt_renderer *r, const ChunkParams &cp) {
+    int *ctr = r->counters.as<int>();
+    {
+        KTimer t(r, K_RAYGEN);
+        k_raygen<<<(cp.nPaths + 255) / 256, 256, 0, r->stream>>>(cp, r->cam, r->rayO[0].as<float4>(), r->rayD[0].as<float4>(), r->st[0].as<uint4>(),
+                                                                r->beta[0].as<float4>(), r->Lpath.as<float4>(), ctr);
+        PT_HIP_TRY(hipGetLastError());
+    }
+    const int shadeGrid = (int)std::min<int64_t>((cp.nPaths + 255) / 256, (int64_t)r->cus * r->shadeBlocksPerCU);
+    int cur = 0;
+    for (int b = 0; b < cp.maxDepth; b++) {
+        int *c = ctr + C_PER_BOUNCE * b;
+        {
+            KTimer t(r, K_EXTEND);
+            launchExtend(r, &c[C_RAYS], &c[C_FETCH_EXT], r->rayO[cur].as<float4>(), r->rayD[cur].as<float4>(), r->hits.as<float4>());
+        }
+        {
+            KTimer t(r, K_SHADE);
+            k_shade<<<shadeGrid, 256, 0, r->stream>>>(r->view, cp, b, ctr, r->rayO[cur].as<float4>(), r->rayD[cur].as<float4>(), r->st[cur].as<uint4>(),
+                                                      r->beta[cur].as<float4>(), r->hits.as<float4>(), r->rayO[1 - cur].as<float4>(),
+                                                      r->rayD[1 - cur].as<float4>(), r->st[1 - cur].as<uint4>(), r->beta[1 - cur].as<float4>(),
+                                                      r->sO.as<float4>(), r->sD.as<float4>(), r->sL.as<float4>(), r->Lpath.as<float4>());
+            PT_HIP_TRY(hipGetLastError());
+        }
+        if (cp.nee) {
+            KTimer t(r, K_SHADOW);
+            launchShadow(r, &c[C_SHADOW], &c[C_FETCH_SHADOW], &c[C_LADD_SHADOW]);
+        }
+        cur = 1 - cur;
+    }
+    {
+        KTimer t(r, K_ACCUM);
+        k_accumulate<<<(cp.npix + 255) / 256, 256, 0, r->stream>>>(cp, r->Lpath.as<float4>(), r->accum.as<double>(), ctr,
+                                                                  r->stats.as<unsigned long long>());
+        PT_HIP_TRY(hipGetLastError());
+    }
+}
+
+void tonemap(pt_renderer *r) {
+    KTimer t(r, K_TONEMAP);
+    const int npix = r->W * r->H;
+    const double inv = r->spp > 0 ? 1.0 / (double)r->spp : 0.0;
+    k_tonemap<<<(npix + 255) / 256, 256, 0, r->stream>>>(r->accum.as<double>(), npix, inv, r->radiance.as<float>(), r->rgba.as<uchar4>());
+    PT_HIP_TRY(hipGetLastError());
+}
+
+int guard(const char *fn, pt_renderer *r) {
+    if (!r) return fail(PT_ERR_INVALID, std::string(fn) + ": renderer is NULL");
+    return PT_OK;
+}
+
+#define PT_API_BEGIN try {
+#define PT_API_END(fn)                                                        \
+    }                                                                         \
+    catch (const HipFailure &e) {                                             \
+        return fail(PT_ERR_DEVICE, std::string(fn) + ": " + e.msg);           \
+    }                                                                         \
+    catch (const std::bad_alloc &) {                                          \
+        return fail(PT_ERR_MEMORY, std::string(fn) + ": out of host memory"); \
+    }
+
+void finishPass(pt_renderer *r) {
+    // called once the end event completed
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, r->evBegin, r->evEnd);
+    r->gpuMs += ms;
+    r->passes++;
+    if (r->profile) {
+        // launch counts and times accumulate until pt_renderer_reset_stats
+        for (auto &ke : r->kevents) {
+            float t = 0.f;
+            (void)hipEventElapsedTime(&t, ke.second.first, ke.second.second);
+            r->kstat[ke.first].launches++;
+            r->kstat[ke.first].ms += t;
+        }
+    }
+    r->kevents.clear();
+    r->eventNext = 0;
+    r->inFlight = false;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pt_device_count(int *count) {
+    if (!count) return fail(PT_ERR_INVALID, "pt_device_count: count is NULL");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return PT_OK;
+}
+
+int pt_renderer_create(int width, int height, pt_renderer **out) {
+    if (!out || width <= 0 || height <= 0) return fail(PT_ERR_INVALID, "pt_renderer_create: bad arguments");
+    if ((int64_t)width * height > (1ll << 28)) return fail(PT_ERR_INVALID, "pt_renderer_create: image too large");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(PT_ERR_DEVICE, "pt_renderer_create: no HIP device");
+    pt_renderer *r = new pt_renderer();
+    PT_API_BEGIN
+    r->W = width;
+    r->H = height;
+    PT_HIP_TRY(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
+    PT_HIP_TRY(hipEventCreate(&r->evBegin));
+    PT_HIP_TRY(hipEventCreate(&r->evEnd));
+    r->cus = deviceCUs();
+    const size_t npix = (size_t)width * height;
+    r->accum.alloc(npix * 3 * sizeof(double));
+    r->radiance.alloc(npix * 3 * sizeof(float));
+    r->rgba.alloc(npix * 4);
+    r->stats.alloc(S_COUNT * sizeof(unsigned long long));
+    r->counters.alloc(kNumCounters * sizeof(int));
+    r->overflow.alloc(sizeof(int));
+    PT_HIP_TRY(hipMemsetAsync(r->overflow.p, 0, sizeof(int), r->stream));
+    k_zero_u64<<<1, 512, 0, r->stream>>>(r->stats.as<unsigned long long>(), S_COUNT);
+    zeroAccum(r);
+    computeCamera(r);
+    PT_HIP_TRY(hipStreamSynchronize(r->stream));
+    *out = r;
+    return PT_OK;
+    }
+    catch (const HipFailure &e) {
+        delete r;
+        return fail(PT_ERR_DEVICE, "pt_renderer_create: " + e.msg);
+    }
+}
+
+int pt_renderer_destroy(pt_renderer *r) {
+    if (!r) return PT_OK;
+    if (r->stream) (void)hipStreamSynchronize(r->stream);
+    for (hipEvent_t e : r->eventPool) (void)hipEventDestroy(e);
+    if (r->evBegin) (void)hipEventDestroy(r->evBegin);
+    if (r->evEnd) (void)hipEventDestroy(r->evEnd);
+    hipStream_t s = r->stream;
+    delete r;
+    if (s) (void)hipStreamDestroy(s);
+    return PT_OK;
+}
+
+int pt_renderer_set_scene(pt_renderer *r, const pt_scene *s) {
+    if (guard("pt_renderer_set_scene", r)) return PT_ERR_INVALID;
+    if (!s) return fail(PT_ERR_INVALID, "pt_renderer_set_scene: scene is NULL");
+    if (r->inFlight) return fail(PT_ERR_STATE, "pt_renderer_set_scene: a pass is still running");
+    const int64_t nt = (int64_t)s->triMat.size(), ns = (int64_t)s->sphMat.size();
+    if (nt + ns >= (1ll << PT_LEAF_COUNT_SHIFT)) return fail(PT_ERR_INVALID, "pt_renderer_set_scene: too many primitives");
+    if (s->mats.empty()) return fail(PT_ERR_INVALID, "pt_renderer_set_scene: scene has no materials");
+    PT_API_BEGIN
+    PT_HIP_TRY(hipStreamSynchronize(r->stream));
+    r->mats = s->mats;
+    r->triMat = s->triMat;
+    r->sphMat = s->sphMat;
+    r->triPos = s->pos;
+    r->sph = s->sph;
+    r->camHost = s->cam;
+    computeCamera(r);
+    // primitive records in input order (triangles, then spheres)
+    const int64_t n = nt + ns;
+    std::vector<float4> recs((size_t)std::max<int64_t>(n, 1) * 3);
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    auto grow = [&](float x, float y, float z) {
+        lo[0] = std::min(lo[0], x); lo[1] = std::min(lo[1], y); lo[2] = std::min(lo[2], z);
+        hi[0] = std::max(hi[0], x); hi[1] = std::max(hi[1], y); hi[2] = std::max(hi[2], z);
+    };
+    std::vector<float4> nrm;
+    for (int64_t i = 0; i < nt; i++) {
+        const float *p = &s->pos[i * 9];
+        int tag = PT_TAG_FLAT;
+        if (s->hasNormals) {
+            // keep per-vertex normals only where they differ from the flat normal
+            const float e1[3] = {p[3] - p[0], p[4] - p[1], p[5] - p[2]}, e2[3] = {p[6] - p[0], p[7] - p[1], p[8] - p[2]};
+            float g[3] = {e1[1] * e2[2] - e1[2] * e2[1], -(e1[0] * e2[2] - e1[2] * e2[0]), e1[0] * e2[1] - e1[1] * e2[0]};
+            const float gl = sqrtf(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+            bool flat = gl > 0.f;
+            const float *nv = &s->nrm[i * 9];
+            for (int v = 0; v < 3 && flat; v++)
+                for (int k = 0; k < 3; k++)
+                    if (fabsf(nv[v * 3 + k] - g[k] / gl) > 1e-6f) flat = false;
+            if (!flat) {
+                tag = (int)(nrm.size() / 3);
+                for (int v = 0; v < 3; v++) nrm.push_back(make_float4(nv[v * 3], nv[v * 3 + 1], nv[v * 3 + 2], 0.f));
+            }
+        }
+        recs[i * 3] = make_float4(p[0], p[1], p[2], asFloat(s->triMat[i]));
+        recs[i * 3 + 1] = make_float4(p[3], p[4], p[5], asFloat(tag));
+        recs[i * 3 + 2] = make_float4(p[6], p[7], p[8], asFloat((int)i));
+        for (int v = 0; v < 3; v++) grow(p[v * 3], p[v * 3 + 1], p[v * 3 + 2]);
+    }
+    for (int64_t i = 0; i < ns; i++) {
+        const float *q = &s->sph[i * 4];
+        const int64_t k = nt + i;
+        recs[k * 3] = make_float4(q[0], q[1], q[2], asFloat(s->sphMat[i]));
+        recs[k * 3 + 1] = make_float4(q[3], 0.f, 0.f, asFloat(PT_TAG_SPHERE));
+        recs[k * 3 + 2] = make_float4(0.f, 0.f, 0.f, asFloat(~(int)i));
+        grow(q[0], q[1], q[2]);
+    }
+    r->primsIn.alloc(recs.size() * sizeof(float4));
+    PT_HIP_TRY(hipMemcpyAsync(r->primsIn.p, recs.data(), recs.size() * sizeof(float4), hipMemcpyHostToDevice, r->stream));
+    r->prims.alloc(recs.size() * sizeof(float4));
+    r->nodes.alloc((size_t)std::max<int64_t>(n - 1, 1) * 4 * sizeof(float4));
+    r->normals.alloc(std::max<size_t>(nrm.size(), 1) * sizeof(float4));
+    if (!nrm.empty()) PT_HIP_TRY(hipMemcpyAsync(r->normals.p, nrm.data(), nrm.size() * sizeof(float4), hipMemcpyHostToDevice, r->stream));
+    buildBvh(r->primsIn.as<float4>(), (int)n, lo, hi, r->nodes.as<float4>(), r->prims.as<float4>(), r->stream, &r->bvh);
+    r->primsIn.release();
+    const int need = r->bvh.depth + 1;
+    if (need > 96) return fail(PT_ERR_NUMERIC, "pt_renderer_set_scene: BVH deeper than 96 levels");
+    r->stackSize = need <= 32 ? 32 : (need <= 64 ? 64 : 96);
+    r->view.nodes = r->nodes.as<float4>();
+    r->view.prims = r->prims.as<float4>();
+    r->view.normals = r->normals.as<float4>();
+    r->view.numPrims = (int)n;
+    r->view.numNodes = r->bvh.numNodes;
+    uploadMaterials(r);
+    uploadEmitters(r);
+    // environment
+    r->view.env = PtEnv{};
+    if (s->envW > 0 && s->envH > 0) {
+        const size_t ne = (size_t)s->envW * s->envH;
+        r->envRgba.alloc(ne * sizeof(float4));
+        r->envRows.alloc(ne * sizeof(float2));
+        r->envMarg.alloc((size_t)s->envH * sizeof(float2));
+        r->envPdf.alloc(ne * sizeof(float));
+        PT_HIP_TRY(hipMemcpyAsync(r->envRgba.p, s->env.data(), ne * sizeof(float4), hipMemcpyHostToDevice, r->stream));
+        buildEnvTables(r->envRgba.as<float4>(), s->envW, s->envH, s->envSinWeight, r->envRows.as<float2>(), r->envMarg.as<float2>(),
+                       r->envPdf.as<float>(), nullptr, nullptr, nullptr, nullptr, r->stream);
+        r->view.env.rgba = r->envRgba.as<float4>();
+        r->view.env.rows = r->envRows.as<float2>();
+        r->view.env.marginal = r->envMarg.as<float2>();
+        r->view.env.texelPdf = r->envPdf.as<float>();
+        r->view.env.width = s->envW;
+        r->view.env.height = s->envH;
+        r->view.env.present = 1;
+    } else {
+        r->envRgba.release();
+        r->envRows.release();
+        r->envMarg.release();
+        r->envPdf.release();
+    }
+    ensurePool(r);
+    zeroAccum(r);
+    PT_HIP_TRY(hipStreamSynchronize(r->stream));
+    r->hasScene = true;
+    return PT_OK;
+    PT_API_END("pt_renderer_set_scene")
+}
+
+int pt_renderer_set_camera(pt_renderer *r, const float origin[3], const float target[3], const float up[3], float vfov, int flip) {
+    if (guard("pt_renderer_set_camera", r)) return PT_ERR_INVALID;
+    if (!origin || !target || !up) return fail(PT_ERR_INVALID, "pt_renderer_set_camera: NULL vector");
+    float m[3][4];
+    if (!lookAt(origin, target, up, flip, m)) return fail(PT_ERR_NUMERIC, "pt_renderer_set_camera: view direction equals the up vector");
+    if (!(vfov > 0.f && vfov < 3.14159f)) return fail(PT_ERR_INVALID, "pt_renderer_set_camera: vfov out of (0, pi)");
+    memcpy(r->camHost.origin, origin, 12);
+    memcpy(r->camHost.target, target, 12);
+    memcpy(r->camHost.up, up, 12);
+    r->camHost.vfov = vfov;
+    r->camHost.flip = flip ? 1 : 0;
+    computeCamera(r);
+    r->resetPending = true;
+    return PT_OK;
+}
+
+int pt_renderer_set_material(pt_renderer *r, int id, int type, const float albedo[3], const float emit[3], float ior, float alpha) {
+    if (guard("pt_renderer_set_material", r)) return PT_ERR_INVALID;
+    if (!r->hasScene) return fail(PT_ERR_STATE, "pt_renderer_set_material: no scene");
+    if (id < 0 || id >= (int)r->mats.size()) return fail(PT_ERR_INVALID, "pt_renderer_set_material: material id out of range");
+    if (type < PT_LAMBERTIAN || type > PT_MICROFACET) return fail(PT_ERR_INVALID, "pt_renderer_set_material: bad material type");
+    if (r->inFlight) return fail(PT_ERR_STATE, "pt_renderer_set_material: a pass is still running");
+    PT_API_BEGIN
+    HostMaterial &m = r->mats[id];
+    const bool wasEmissive = isEmissive(m);
+    m.type = type;
+    if (albedo) memcpy(m.albedo, albedo, 12);
+    if (emit) memcpy(m.emit, emit, 12);
+    m.ior = ior;
+    m.alpha = alpha;
+    PT_HIP_TRY(hipStreamSynchronize(r->stream));
+    uploadMaterials(r);
+    if (wasEmissive || isEmissive(m)) uploadEmitters(r);
+    PT_HIP_TRY(hipStreamSynchronize(r->stream));
+    r->resetPending = true;
+    return PT_OK;
+    PT_API_END("pt_renderer_set_material")
+}
+
+int pt_renderer_set_max_depth(pt_renderer *r, int d) {
+    if (guard("pt_renderer_set_max_depth", r)) return PT_ERR_INVALID;
+    if (d < 0 || d >= kMaxBounces) return fail(PT_ERR_INVALID, "pt_renderer_set_max_depth: depth must be in [0, 63]");
+    r->maxDepth = d;
+    r->resetPending = true;
+    return PT_OK;
+}
+
+int pt_renderer_set_nee(pt_renderer *r, int on) {
+    if (guard("pt_renderer_set_nee", r)) return PT_ERR_INVALID;
+    r->nee = on ? 1 : 0;
+    r->resetPending = true;
+    return PT_OK;
+}
+
+int pt_renderer_set_type(pt_renderer *r, int t) {
+    if (guard("pt_renderer_set_type", r)) return PT_ERR_INVALID;
+    if (t < PT_RENDERER_CUDA || t > PT_RENDERER_GUIDED) return fail(PT_ERR_INVALID, "pt_renderer_set_type: unknown renderer type");
+    if (t == PT_RENDERER_GUIDED) return fail(PT_ERR_STATE, "pt_renderer_set_type: the guided integrator is not built into this library");
+    r->type = t;
+    r->resetPending = true;
+    return PT_OK;
+}
+
+int pt_renderer_set_seed(pt_renderer *r, uint64_t seed) {
+    if (guard("pt_renderer_set_seed", r)) return PT_ERR_INVALID;
+    r->seed = seed;
+    r->resetPending = true;
+    return PT_OK;
+}
+
+int pt_renderer_set_option(pt_renderer *r, int opt, int64_t v) {
+    if (guard("pt_renderer_set_option", r)) return PT_ERR_INVALID;
+    if (r->inFlight) return fail(PT_ERR_STATE, "pt_renderer_set_option: a pass is still running");
+    PT_API_BEGIN
+    switch (opt) {
+    case PT_OPT_POOL_PATHS:
+        if (v > (1ll << 27)) return fail(PT_ERR_INVALID, "pt_renderer_set_option: pool larger than 2^27 paths");
+        r->poolWanted = v > 0 ? v : (1 << 22);
+        if (r->hasScene) {
+            PT_HIP_TRY(hipStreamSynchronize(r->stream));
+            ensurePool(r);
+        }
+        break;
+    case PT_OPT_TRACE_BLOCKS_PER_CU: r->traceBlocksPerCU = v > 0 ? (int)std::min<int64_t>(v, 64) : 8; break;
+    case PT_OPT_SHADE_BLOCKS_PER_CU: r->shadeBlocksPerCU = v > 0 ? (int)std::min<int64_t>(v, 64) : 8; break;
+    case PT_OPT_PROFILE_KERNELS: r->profile = v != 0; break;
+    default: return fail(PT_ERR_INVALID, "pt_renderer_set_option: unknown option");
+    }
+    return PT_OK;
+    PT_API_END("pt_renderer_set_option")
+}
+
+int pt_renderer_get_option(pt_renderer *r, int opt, int64_t *v) {
+    if (guard("pt_renderer_get_option", r) || !v) return fail(PT_ERR_INVALID, "pt_renderer_get_option: bad arguments");
+    switch (opt) {
+    case PT_OPT_POOL_PATHS: *v = r->poolWanted; break;
+    case PT_OPT_TRACE_BLOCKS_PER_CU: *v = r->traceBlocksPerCU; break;
+    case PT_OPT_SHADE_BLOCKS_PER_CU: *v = r->shadeBlocksPerCU; break;
+    case PT_OPT_PROFILE_KERNELS: *v = r->profile ? 1 : 0; break;
+    default: return fail(PT_ERR_INVALID, "pt_renderer_get_option: unknown option");
+    }
+    return PT_OK;
+}
+
+int pt_renderer_render_async(pt_renderer *r, int spp) {
+    if (guard("pt_renderer_render_async", r)) return PT_ERR_INVALID;
+    if (!r->hasScene) return fail(PT_ERR_STATE, "pt_renderer_render_async: no scene");
+    if (r->inFlight) return fail(PT_ERR_STATE, "pt_renderer_render_async: previous pass not polled");
+    if (spp <= 0) return fail(PT_ERR_INVALID, "pt_renderer_render_async: spp must be positive");
+    PT_API_BEGIN
+    if (r->resetPending) zeroAccum(r);
+    PT_HIP_TRY(hipEventRecord(r->evBegin, r->stream));
+    const int npix = r->W * r->H;
+    r->lastChunks.clear();
+    if (r->type == PT_RENDERER_NORMALS) {
+        // one centre ray per pixel, added once per pass (the reference's framebuffer sees
+        // spp = 1 each pass, g_buffer.cu:108-117)
+        if (npix > r->poolWanted) r->poolWanted = npix;
+        ensurePool(r);
+        int *c = r->counters.as<int>();
+        k_centre_rays<<<(npix + 255) / 256, 256, 0, r->stream>>>(r->cam, npix, r->W, r->rayO[0].as<float4>(), r->rayD[0].as<float4>(), c);
+        launchExtend(r, &c[C_RAYS], &c[C_FETCH_EXT], r->rayO[0].as<float4>(), r->rayD[0].as<float4>(), r->hits.as<float4>());
+        k_normals_shade<<<(npix + 255) / 256, 256, 0, r->stream>>>(r->view, npix, r->rayO[0].as<float4>(), r->rayD[0].as<float4>(),
+                                                                  r->hits.as<float4>(), r->accum.as<double>());
+        PT_HIP_TRY(hipGetLastError());
+        r->spp += 1;
+        r->passSpp = spp;
+    } else {
+        ensurePool(r);
+        const int64_t pool = r->pool;
+        if (npix <= pool) {
+            const int rowsPerChunk = (int)std::min<int64_t>(spp, pool / npix);
+            for (int s0 = 0; s0 < spp; s0 += rowsPerChunk) {
+                ChunkParams cp;
+                cp.pixBegin = 0;
+                cp.npix = npix;
+                cp.samples = std::min(rowsPerChunk, spp - s0);
+                cp.sampleBase = (int)(r->spp + s0);
+                cp.nPaths = cp.npix * cp.samples;
+                cp.maxDepth = r->maxDepth;
+                cp.nee = r->nee;
+                cp.W = r->W;
+                cp.H = r->H;
+                cp.seed = r->seed;
+                r->lastChunks.push_back(cp);
+            }
+        } else {
+            for (int s0 = 0; s0 < spp; s0++)
+                for (int64_t p0 = 0; p0 < npix; p0 += pool) {
+                    ChunkParams cp;
+                    cp.pixBegin = (int)p0;
+                    cp.npix = (int)std::min<int64_t>(pool, npix - p0);
+                    cp.samples = 1;
+                    cp.sampleBase = (int)(r->spp + s0);
+                    cp.nPaths = cp.npix;
+                    cp.maxDepth = r->maxDepth;
+                    cp.nee = r->nee;
+                    cp.W = r->W;
+                    cp.H = r->H;
+                    cp.seed = r->seed;
+                    r->lastChunks.push_back(cp);
+                }
+        }
+        if (r->maxDepth > 0) {
+            for (const ChunkParams &cp : r->lastChunks) {
+                runChunk(r, cp);
+                r->pixelsAccumulated += cp.npix;
+                r->samplesAccumulated += (double)cp.npix * cp.samples;
+            }
+        }
+        r->spp += spp;
+        r->passSpp = spp;
+    }
+    r->lastPassPaths = (int64_t)npix * spp;
+    r->lastPassSingleChunk = r->lastChunks.size() == 1;
+    tonemap(r);
+    PT_HIP_TRY(hipEventRecord(r->evEnd, r->stream));
+    r->inFlight = true;
+    return PT_OK;
+    PT_API_END("pt_renderer_render_async")
+}
+
+int pt_renderer_poll(pt_renderer *r, int *finished, float *ms) {
+    if (guard("pt_renderer_poll", r)) return PT_ERR_INVALID;
+    PT_API_BEGIN
+    if (!r->inFlight) {
+        if (finished) *finished = 1;
+        if (ms) *ms = 0.f;
+        return PT_OK;
+    }
+    const hipError_t q = hipEventQuery(r->evEnd);
+    if (q == hipErrorNotReady) {
+        if (finished) *finished = 0;
+        return PT_OK;
+    }
+    if (q != hipSuccess) return fail(PT_ERR_DEVICE, std::string("pt_renderer_poll: ") + hipGetErrorString(q));
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, r->evBegin, r->evEnd);
+    finishPass(r);
+    if (finished) *finished = 1;
+    if (ms) *ms = t;
+    return PT_OK;
+    PT_API_END("pt_renderer_poll")
+}
+
+int pt_renderer_synchronize(pt_renderer *r) {
+    if (guard("pt_renderer_synchronize", r)) return PT_ERR_INVALID;
+    PT_API_BEGIN
+    PT_HIP_TRY(hipStreamSynchronize(r->stream));
+    if (r->inFlight) finishPass(r);
+    int ov = 0;
+    PT_HIP_TRY(hipMemcpy(&ov, r->overflow.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (ov != 0) return fail(PT_ERR_NUMERIC, "pt_renderer_synchronize: traversal stack overflowed");
+    return PT_OK;
+    PT_API_END("pt_renderer_synchronize")
+}
+
+int pt_renderer_reset(pt_renderer *r) {
+    if (guard("pt_renderer_reset", r)) return PT_ERR_INVALID;
+    r->resetPending = true;
+    return PT_OK;
+}
+
+int pt_renderer_get_spp(pt_renderer *r, int64_t *spp) {
+    if (guard("pt_renderer_get_spp", r) || !spp) return fail(PT_ERR_INVALID, "pt_renderer_get_spp: bad arguments");
+    *spp = r->resetPending ? 0 : r->spp;
+    return PT_OK;
+}
+
+int pt_renderer_get_radiance(pt_renderer *r, float *out) {
+    if (guard("pt_renderer_get_radiance", r) || !out) return fail(PT_ERR_INVALID, "pt_renderer_get_radiance: bad arguments");
+    PT_API_BEGIN
+    PT_HIP_TRY(hipStreamSynchronize(r->stream));
+    if (r->inFlight) finishPass(r);
+    PT_HIP_TRY(hipMemcpy(out, r->radiance.p, (size_t)r->W * r->H * 3 * sizeof(float), hipMemcpyDeviceToHost));
+    return PT_OK;
+    PT_API_END("pt_renderer_get_radiance")
+}
+
+int pt_renderer_get_radiance_sum(pt_renderer *r, double *out) {
+    if (guard("pt_renderer_get_radiance_sum", r) || !out) return fail(PT_ERR_INVALID, "pt_renderer_get_radiance_sum: bad arguments");
+    PT_API_BEGIN
+    PT_HIP_TRY(hipStreamSynchronize(r->stream));
+    if (r->inFlight) finishPass(r);
+    PT_HIP_TRY(hipMemcpy(out, r->accum.p, (size_t)r->W * r->H * 3 * sizeof(double), hipMemcpyDeviceToHost));
+    return PT_OK;
+    PT_API_END("pt_renderer_get_radiance_sum")
+}
+
+int pt_renderer_get_rgba8(pt_renderer *r, uint8_t *out) {
+    if (guard("pt_renderer_get_rgba8", r) || !out) return fail(PT_ERR_INVALID, "pt_renderer_get_rgba8: bad arguments");
+    PT_API_BEGIN
+    PT_HIP_TRY(hipStreamSynchronize(r->stream));
+    if (r->inFlight) finishPass(r);
+    PT_HIP_TRY(hipMemcpy(out, r->rgba.p, (size_t)r->W * r->H * 4, hipMemcpyDeviceToHost));
+    return PT_OK;
+    PT_API_END("pt_renderer_get_rgba8")
+}
+
+int pt_renderer_hit_test(pt_renderer *r, int x, int y, int *material_id, int *is_hit) {
+    if (guard("pt_renderer_hit_test", r)) return PT_ERR_INVALID;
+    if (!r->hasScene) return fail(PT_ERR_STATE, "pt_renderer_hit_test: no scene");
+    if (x < 0 || y < 0 || x >= r->W || y >= r->H) {
+        // hit_test.cu:25-33: no thread matches an off-image pixel; report a miss
+        if (material_id) *material_id = -1;
+        if (is_hit) *is_hit = 0;
+        return PT_OK;
+    }
+    if (r->inFlight) return fail(PT_ERR_STATE, "pt_renderer_hit_test: a pass is still running");
+    PT_API_BEGIN
+    ensurePool(r);
+    // one centre ray through the same extend kernel (hit_test.cu:25-47)
+    const float4 *dummy = nullptr;
+    (void)dummy;
+    float4 ro, rd;
+    {
+        // camera.cu:24-27 generateRay(row, col) with samples (0.5, 0.5)
+        const PtCamera &c = r->cam;
+        const float xc = (x + 0.5f) / c.W, yc = (y + 0.5f) / c.H;
+        const float py = yc * c.height - c.top, px = xc * c.width - c.right;
+        const float l = sqrtf(px * px + py * py + 1.f);
+        const float dl[3] = {px / l, py / l, -1.f / l};
+        float d[3];
+        for (int k = 0; k < 3; k++) d[k] = c.m[k][0] * dl[0] + c.m[k][1] * dl[1] + c.m[k][2] * dl[2];
+        ro = make_float4(c.m[0][3], c.m[1][3], c.m[2][3], 0.f);
+        rd = make_float4(d[0], d[1], d[2], FLT_MAX);
+    }
+    PT_HIP_TRY(hipMemcpyAsync(r->rayO[0].p, &ro, 16, hipMemcpyHostToDevice, r->stream));
+    PT_HIP_TRY(hipMemcpyAsync(r->rayD[0].p, &rd, 16, hipMemcpyHostToDevice, r->stream));
+    int *c = r->counters.as<int>();
+    k_set_counter<<<1, 64, 0, r->stream>>>(c, C_RAYS, 1);
+    launchExtend(r, &c[C_RAYS], &c[C_FETCH_EXT], r->rayO[0].as<float4>(), r->rayD[0].as<float4>(), r->hits.as<float4>());
+    float4 h;
+    PT_HIP_TRY(hipMemcpyAsync(&h, r->hits.p, 16, hipMemcpyDeviceToHost, r->stream));
+    PT_HIP_TRY(hipStreamSynchronize(r->stream));
+    const int slot = asInt(h.w);
+    if (slot == INT32_MIN) {
+        if (material_id) *material_id = -1;
+        if (is_hit) *is_hit = 0;
+    } else {
+        float4 p0;
+        PT_HIP_TRY(hipMemcpy(&p0, r->prims.as<float4>() + 3 * (size_t)slot, 16, hipMemcpyDeviceToHost));
+        if (material_id) *material_id = asInt(p0.w);
+        if (is_hit) *is_hit = 1;
+    }
+    return PT_OK;
+    PT_API_END("pt_renderer_hit_test")
+}
+
+int pt_renderer_get_stats(pt_renderer *r, pt_render_stats *out) {
+    if (guard("pt_renderer_get_stats", r) || !out) return fail(PT_ERR_INVALID, "pt_renderer_get_stats: bad arguments");
+    PT_API_BEGIN
+    PT_HIP_TRY(hipStreamSynchronize(r->stream));
+    if (r->inFlight) finishPass(r);
+    std::vector<unsigned long long> s(S_COUNT);
+    PT_HIP_TRY(hipMemcpy(s.data(), r->stats.p, S_COUNT * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    int ov = 0;
+    PT_HIP_TRY(hipMemcpy(&ov, r->overflow.p, sizeof(int), hipMemcpyDeviceToHost));
+    memset(out, 0, sizeof *out);
+    for (int b = 0; b < kMaxBounces; b++) {
+        out->extension_rays += (int64_t)s[S_RAYS + b];
+        out->shadow_rays += (int64_t)s[S_SHADOW + b];
+    }
+    out->rays_total = out->extension_rays + out->shadow_rays;
+    out->paths = (int64_t)s[S_PATHS];
+    out->passes = r->passes;
+    out->gpu_ms = r->gpuMs;
+    out->bvh_depth = r->bvh.depth;
+    out->bvh_nodes = r->bvh.numNodes;
+    out->stack_overflows = ov;
+    return PT_OK;
+    PT_API_END("pt_renderer_get_stats")
+}
+
+int pt_renderer_reset_stats(pt_renderer *r) {
+    if (guard("pt_renderer_reset_stats", r)) return PT_ERR_INVALID;
+    PT_API_BEGIN
+    PT_HIP_TRY(hipStreamSynchronize(r->stream));
+    if (r->inFlight) finishPass(r);
+    k_zero_u64<<<1, 512, 0, r->stream>>>(r->stats.as<unsigned long long>(), S_COUNT);
+    PT_HIP_TRY(hipStreamSynchronize(r->stream));
+    r->gpuMs = 0.0;
+    r->passes = 0;
+    for (int k = 0; k < K_NUM; k++) {
+        r->kstat[k].launches = 0;
+        r->kstat[k].ms = 0.0;
+        r->kstat[k].units = 0;
+        r->kstat[k].bytes = 0.0;
+    }
+    r->samplesAccumulated = 0.0;
+    r->pixelsAccumulated = 0.0;
+    return PT_OK;
+    PT_API_END("pt_renderer_reset_stats")
+}
+
+// Per-kernel statistics since the last pt_renderer_reset_stats: launch count and time from
+// the HIP events recorded around each launch (profiling on); units and algorithmic bytes from
+// the per-bounce queue counters the accumulate kernel folds into the device statistics.
+int pt_renderer_get_kernel_stats(pt_renderer *r, pt_kernel_stat *out, int cap, int *count) {
+    if (guard("pt_renderer_get_kernel_stats", r) || !out || !count) return fail(PT_ERR_INVALID, "pt_renderer_get_kernel_stats: bad arguments");
+    PT_API_BEGIN
+    PT_HIP_TRY(hipStreamSynchronize(r->stream));
+    if (r->inFlight) finishPass(r);
+    std::vector<unsigned long long> s(S_COUNT);
+    PT_HIP_TRY(hipMemcpy(s.data(), r->stats.p, S_COUNT * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    double rays = 0, shadows = 0, addShade = 0, addShadow = 0, nextRays = 0;
+    for (int b = 0; b < kMaxBounces; b++) {
+        rays += (double)s[S_RAYS + b];
+        shadows += (double)s[S_SHADOW + b];
+        addShade += (double)s[S_LADD_SHADE + b];
+        addShadow += (double)s[S_LADD_SHADOW + b];
+        if (b > 0) nextRays += (double)s[S_RAYS + b];
+    }
+    const double paths = (double)s[S_PATHS];
+    const double pixelsAcc = r->pixelsAccumulated, samplesAcc = r->samplesAccumulated;
+    const double npix = (double)r->W * r->H * (double)r->kstat[K_TONEMAP].launches;
+    // algorithmic bytes per unit (record sizes above)
+    pt_kernel_stat *k = r->kstat;
+    k[K_RAYGEN].units = (int64_t)paths;
+    k[K_RAYGEN].bytes = 80.0 * paths;                    // writes rayO, rayD, state, beta, Lpath
+    k[K_EXTEND].units = (int64_t)rays;
+    k[K_EXTEND].bytes = 48.0 * rays;                     // reads rayO, rayD; writes hit
+    k[K_SHADE].units = (int64_t)rays;
+    k[K_SHADE].bytes = 80.0 * rays + 64.0 * nextRays + 48.0 * shadows + 32.0 * addShade;
+    k[K_SHADOW].units = (int64_t)shadows;
+    k[K_SHADOW].bytes = 48.0 * shadows + 32.0 * addShadow;
+    k[K_ACCUM].units = (int64_t)pixelsAcc;
+    k[K_ACCUM].bytes = 16.0 * samplesAcc + 48.0 * pixelsAcc;
+    k[K_TONEMAP].units = (int64_t)npix;
+    k[K_TONEMAP].bytes = 40.0 * npix;
+    const int n = std::min(cap, (int)K_NUM);
+    for (int i = 0; i < n; i++) {
+        out[i] = k[i];
+        strncpy(out[i].name, kKernelNames[i], sizeof(out[i].name) - 1);
+        out[i].name[sizeof(out[i].name) - 1] = 0;
+    }
+    *count = n;
+    return PT_OK;
+    PT_API_END("pt_renderer_get_kernel_stats")
+}
+
+int pt_renderer_get_path_radiance(pt_renderer *r, float *out, int64_t cap, int64_t *count) {
+    if (guard("pt_renderer_get_path_radiance", r) || !out || !count) return fail(PT_ERR_INVALID, "pt_renderer_get_path_radiance: bad arguments");
+    if (!r->lastPassSingleChunk) return fail(PT_ERR_STATE, "pt_renderer_get_path_radiance: last pass did not fit in one chunk");
+    PT_API_BEGIN
+    PT_HIP_TRY(hipStreamSynchronize(r->stream));
+    if (r->inFlight) finishPass(r);
+    const int64_t n = std::min<int64_t>(cap, r->lastPassPaths);
+    std::vector<float4> tmp((size_t)n);
+    PT_HIP_TRY(hipMemcpy(tmp.data(), r->Lpath.p, (size_t)n * sizeof(float4), hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; i++) {
+        out[i * 3] = tmp[i].x;
+        out[i * 3 + 1] = tmp[i].y;
+        out[i * 3 + 2] = tmp[i].z;
+    }
+    *count = n;
+    return PT_OK;
+    PT_API_END("pt_renderer_get_path_radiance")
+}
+
+int pt_renderer_intersect(pt_renderer *r, const float *rays, int64_t n, int any_hit, float *out) {
+    if (guard("pt_renderer_intersect", r) || (n > 0 && (!rays || !out)) || n < 0) return fail(PT_ERR_INVALID, "pt_renderer_intersect: bad arguments");
+    if (!r->hasScene) return fail(PT_ERR_STATE, "pt_renderer_intersect: no scene");
+    if (r->inFlight) return fail(PT_ERR_STATE, "pt_renderer_intersect: a pass is still running");
+    PT_API_BEGIN
+    ensurePool(r);
+    int *c = r->counters.as<int>();
+    for (int64_t done = 0; done < n;) {
+        const int cnt = (int)std::min<int64_t>(n - done, r->pool);
+        std::vector<float4> o(cnt), d(cnt);
+        for (int i = 0; i < cnt; i++) {
+            const float *q = rays + (done + i) * 8;
+            o[i] = make_float4(q[0], q[1], q[2], q[3]);
+            d[i] = make_float4(q[4], q[5], q[6], q[7]);
+        }
+        if (!any_hit) {
+            PT_HIP_TRY(hipMemcpyAsync(r->rayO[0].p, o.data(), cnt * 16, hipMemcpyHostToDevice, r->stream));
+            PT_HIP_TRY(hipMemcpyAsync(r->rayD[0].p, d.data(), cnt * 16, hipMemcpyHostToDevice, r->stream));
+            k_set_counter<<<1, 64, 0, r->stream>>>(c, C_RAYS, cnt);
+            launchExtend(r, &c[C_RAYS], &c[C_FETCH_EXT], r->rayO[0].as<float4>(), r->rayD[0].as<float4>(), r->hits.as<float4>());
+            k_map_prims<<<(cnt + 255) / 256, 256, 0, r->stream>>>(r->view, r->hits.as<float4>(), cnt, r->sL.as<float4>());
+            std::vector<float4> h(cnt);
+            PT_HIP_TRY(hipMemcpyAsync(h.data(), r->sL.p, cnt * 16, hipMemcpyDeviceToHost, r->stream));
+            PT_HIP_TRY(hipStreamSynchronize(r->stream));
+            for (int i = 0; i < cnt; i++) memcpy(out + (done + i) * 4, &h[i], 16);
+        } else {
+            // shadow queue: occlusion over [tmin = 1e-4, tmax]; the contribution (1, 0, 0) lands
+            // in Lpath[i] when the ray is unoccluded
+            std::vector<float4> so(cnt), sd(cnt), sl(cnt, make_float4(1.f, 0.f, 0.f, 0.f)), z(cnt, make_float4(0.f, 0.f, 0.f, 0.f));
+            for (int i = 0; i < cnt; i++) {
+                so[i] = make_float4(o[i].x, o[i].y, o[i].z, d[i].w);
+                sd[i] = make_float4(d[i].x, d[i].y, d[i].z, asFloat(i));
+            }
+            PT_HIP_TRY(hipMemcpyAsync(r->sO.p, so.data(), cnt * 16, hipMemcpyHostToDevice, r->stream));
+            PT_HIP_TRY(hipMemcpyAsync(r->sD.p, sd.data(), cnt * 16, hipMemcpyHostToDevice, r->stream));
+            PT_HIP_TRY(hipMemcpyAsync(r->sL.p, sl.data(), cnt * 16, hipMemcpyHostToDevice, r->stream));
+            PT_HIP_TRY(hipMemcpyAsync(r->Lpath.p, z.data(), cnt * 16, hipMemcpyHostToDevice, r->stream));
+            k_set_counter<<<1, 64, 0, r->stream>>>(c, C_SHADOW, cnt);
+            launchShadow(r, &c[C_SHADOW], &c[C_FETCH_SHADOW], &c[C_LADD_SHADOW]);
+            std::vector<float4> h(cnt);
+            PT_HIP_TRY(hipMemcpyAsync(h.data(), r->Lpath.p, cnt * 16, hipMemcpyDeviceToHost, r->stream));
+            PT_HIP_TRY(hipStreamSynchronize(r->stream));
+            for (int i = 0; i < cnt; i++) {
+                const int prim = h[i].x > 0.5f ? INT32_MIN : 0;  // visible -> no occluder
+                out[(done + i) * 4] = h[i].x > 0.5f ? -1.f : 0.f;
+                out[(done + i) * 4 + 1] = 0.f;
+                out[(done + i) * 4 + 2] = 0.f;
+                memcpy(&out[(done + i) * 4 + 3], &prim, 4);
+            }
+        }
+        done += cnt;
+    }
+    return PT_OK;
+    PT_API_END("pt_renderer_intersect")
+}
+
+}  // extern "C"

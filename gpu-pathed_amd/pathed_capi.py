@@ -1,0 +1,393 @@
+"""ctypes binding of libpathed.so (include/pathed.h).
+
+Loads the in-tree library next to this file and fails loudly when it is missing: there is
+no CPU fallback for any GPU entry point.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpathed.so")
+
+PT_OK = 0
+LAMBERTIAN, MIRROR, GLASS, MICROFACET = 0, 1, 2, 3
+RENDERER_CUDA, RENDERER_OPTIX, RENDERER_NORMALS, RENDERER_GUIDED = 0, 1, 2, 3
+OPT_POOL_PATHS, OPT_TRACE_BLOCKS_PER_CU, OPT_SHADE_BLOCKS_PER_CU, OPT_PROFILE_KERNELS = 0, 1, 2, 3
+MISS = np.iinfo(np.int32).min
+
+# every symbol declared in include/pathed.h
+EXPORTS = [
+    "pt_last_error", "pt_abi_version", "pt_device_count",
+    "pt_scene_create", "pt_scene_destroy", "pt_scene_add_material", "pt_scene_add_triangles", "pt_scene_add_spheres",
+    "pt_scene_set_environment", "pt_scene_set_camera", "pt_scene_preset", "pt_scene_get_info", "pt_scene_export",
+    "pt_renderer_create", "pt_renderer_destroy", "pt_renderer_set_scene", "pt_renderer_set_camera", "pt_renderer_set_material",
+    "pt_renderer_set_max_depth", "pt_renderer_set_nee", "pt_renderer_set_type", "pt_renderer_set_seed", "pt_renderer_set_option",
+    "pt_renderer_get_option", "pt_renderer_render_async", "pt_renderer_poll", "pt_renderer_synchronize", "pt_renderer_reset",
+    "pt_renderer_get_spp", "pt_renderer_get_radiance", "pt_renderer_get_radiance_sum", "pt_renderer_get_rgba8",
+    "pt_renderer_hit_test", "pt_renderer_get_stats", "pt_renderer_reset_stats", "pt_renderer_get_kernel_stats",
+    "pt_renderer_get_path_radiance", "pt_renderer_intersect",
+    "pt_unit_triangle_test", "pt_unit_bsdf_eval", "pt_unit_bsdf_sample", "pt_unit_env_tables", "pt_unit_env_sample",
+    "pt_unit_path_uniforms",
+]
+
+
+class PtError(RuntimeError):
+    pass
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [
+        ("n_triangles", C.c_int64), ("n_spheres", C.c_int64), ("n_materials", C.c_int64), ("n_emitters", C.c_int64),
+        ("env_width", C.c_int), ("env_height", C.c_int), ("has_normals", C.c_int),
+        ("camera_origin", C.c_float * 3), ("camera_target", C.c_float * 3), ("camera_up", C.c_float * 3),
+        ("camera_vfov", C.c_float), ("camera_flip", C.c_int),
+        ("bounds_lo", C.c_float * 3), ("bounds_hi", C.c_float * 3),
+    ]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [
+        ("rays_total", C.c_int64), ("extension_rays", C.c_int64), ("shadow_rays", C.c_int64), ("paths", C.c_int64),
+        ("passes", C.c_int64), ("gpu_ms", C.c_double), ("bvh_depth", C.c_int), ("bvh_nodes", C.c_int64),
+        ("stack_overflows", C.c_int),
+    ]
+
+
+class KernelStat(C.Structure):
+    _fields_ = [("name", C.c_char * 16), ("launches", C.c_int64), ("units", C.c_int64), ("ms", C.c_double), ("bytes", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PtError(f"libpathed.so not built at {LIB_PATH}; run __graft_entry__.build()")
+        _lib = C.CDLL(LIB_PATH)
+        L = _lib
+        vp = C.c_void_p
+        L.pt_last_error.restype = C.c_char_p
+        for name in EXPORTS:
+            if name != "pt_last_error":
+                getattr(L, name).restype = C.c_int
+        L.pt_scene_create.argtypes = [C.POINTER(vp)]
+        L.pt_scene_destroy.argtypes = [vp]
+        L.pt_scene_add_material.argtypes = [vp, C.c_int, vp, vp, C.c_float, C.c_float, C.POINTER(C.c_int)]
+        L.pt_scene_add_triangles.argtypes = [vp, vp, vp, vp, C.c_int64]
+        L.pt_scene_add_spheres.argtypes = [vp, vp, vp, vp, C.c_int64]
+        L.pt_scene_set_environment.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int]
+        L.pt_scene_set_camera.argtypes = [vp, vp, vp, vp, C.c_float, C.c_int]
+        L.pt_scene_preset.argtypes = [vp, C.c_int, C.c_uint64, C.c_int64]
+        L.pt_scene_get_info.argtypes = [vp, C.POINTER(SceneInfo)]
+        L.pt_scene_export.argtypes = [vp] + [vp] * 11
+        L.pt_renderer_create.argtypes = [C.c_int, C.c_int, C.POINTER(vp)]
+        L.pt_renderer_destroy.argtypes = [vp]
+        L.pt_renderer_set_scene.argtypes = [vp, vp]
+        L.pt_renderer_set_camera.argtypes = [vp, vp, vp, vp, C.c_float, C.c_int]
+        L.pt_renderer_set_material.argtypes = [vp, C.c_int, C.c_int, vp, vp, C.c_float, C.c_float]
+        L.pt_renderer_set_max_depth.argtypes = [vp, C.c_int]
+        L.pt_renderer_set_nee.argtypes = [vp, C.c_int]
+        L.pt_renderer_set_type.argtypes = [vp, C.c_int]
+        L.pt_renderer_set_seed.argtypes = [vp, C.c_uint64]
+        L.pt_renderer_set_option.argtypes = [vp, C.c_int, C.c_int64]
+        L.pt_renderer_get_option.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
+        L.pt_renderer_render_async.argtypes = [vp, C.c_int]
+        L.pt_renderer_poll.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_float)]
+        L.pt_renderer_synchronize.argtypes = [vp]
+        L.pt_renderer_reset.argtypes = [vp]
+        L.pt_renderer_get_spp.argtypes = [vp, C.POINTER(C.c_int64)]
+        L.pt_renderer_get_radiance.argtypes = [vp, vp]
+        L.pt_renderer_get_radiance_sum.argtypes = [vp, vp]
+        L.pt_renderer_get_rgba8.argtypes = [vp, vp]
+        L.pt_renderer_hit_test.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.pt_renderer_get_stats.argtypes = [vp, C.POINTER(RenderStats)]
+        L.pt_renderer_reset_stats.argtypes = [vp]
+        L.pt_renderer_get_kernel_stats.argtypes = [vp, C.POINTER(KernelStat), C.c_int, C.POINTER(C.c_int)]
+        L.pt_renderer_get_path_radiance.argtypes = [vp, vp, C.c_int64, C.POINTER(C.c_int64)]
+        L.pt_renderer_intersect.argtypes = [vp, vp, C.c_int64, C.c_int, vp]
+        L.pt_unit_triangle_test.argtypes = [vp, vp, C.c_int64, C.c_int, vp]
+        L.pt_unit_bsdf_eval.argtypes = [vp, vp, vp, C.c_int64, vp, vp]
+        L.pt_unit_bsdf_sample.argtypes = [vp, vp, vp, C.c_int64, vp]
+        L.pt_unit_env_tables.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp]
+        L.pt_unit_env_sample.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, C.c_int64, vp, vp, vp, vp, C.c_int64, vp]
+        L.pt_unit_path_uniforms.argtypes = [C.c_uint64, vp, vp, C.c_int64, C.c_int, vp]
+        L.pt_device_count.argtypes = [C.POINTER(C.c_int)]
+    return _lib
+
+
+def check(status):
+    if status != PT_OK:
+        raise PtError(f"libpathed error {status}: {lib().pt_last_error().decode()}")
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def device_count():
+    n = C.c_int(0)
+    check(lib().pt_device_count(C.byref(n)))
+    return n.value
+
+
+class Scene:
+    """Host scene description (pt_scene)."""
+
+    def __init__(self):
+        self.h = C.c_void_p()
+        check(lib().pt_scene_create(C.byref(self.h)))
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().pt_scene_destroy(self.h)
+        except Exception:
+            pass
+
+    @classmethod
+    def preset(cls, preset, seed=0, param=0):
+        s = cls()
+        check(lib().pt_scene_preset(s.h, preset, seed, param))
+        return s
+
+    def add_material(self, mtype, albedo=(0, 0, 0), emit=(0, 0, 0), ior=1.5, alpha=0.1):
+        out = C.c_int()
+        a, e = _f32(albedo), _f32(emit)
+        check(lib().pt_scene_add_material(self.h, mtype, _p(a), _p(e), ior, alpha, C.byref(out)))
+        return out.value
+
+    def add_triangles(self, positions, material_ids, normals=None):
+        p = _f32(positions).reshape(-1)
+        m = np.ascontiguousarray(material_ids, dtype=np.int32)
+        n = None if normals is None else _f32(normals).reshape(-1)
+        check(lib().pt_scene_add_triangles(self.h, _p(p), _p(n), _p(m), m.size))
+
+    def add_spheres(self, centers, radii, material_ids):
+        c = _f32(centers).reshape(-1)
+        r = _f32(radii).reshape(-1)
+        m = np.ascontiguousarray(material_ids, dtype=np.int32)
+        check(lib().pt_scene_add_spheres(self.h, _p(c), _p(r), _p(m), m.size))
+
+    def set_environment(self, rgba, sin_theta_weight=False):
+        e = _f32(rgba)
+        check(lib().pt_scene_set_environment(self.h, _p(e), e.shape[1], e.shape[0], int(sin_theta_weight)))
+
+    def set_camera(self, origin, target, up, vfov, flip=False):
+        o, t, u = _f32(origin), _f32(target), _f32(up)
+        check(lib().pt_scene_set_camera(self.h, _p(o), _p(t), _p(u), vfov, int(flip)))
+
+    def info(self):
+        i = SceneInfo()
+        check(lib().pt_scene_get_info(self.h, C.byref(i)))
+        return i
+
+    def export(self):
+        """Flat arrays in the layout the oracle takes."""
+        i = self.info()
+        T, S, M = i.n_triangles, i.n_spheres, i.n_materials
+        pos = np.zeros((T, 9), np.float32)
+        nrm = np.zeros((T, 9), np.float32)
+        tm = np.zeros(T, np.int32)
+        sph = np.zeros((S, 4), np.float32)
+        sm = np.zeros(S, np.int32)
+        mt = np.zeros(M, np.int32)
+        ma = np.zeros((M, 3), np.float32)
+        me = np.zeros((M, 3), np.float32)
+        mi = np.zeros(M, np.float32)
+        mal = np.zeros(M, np.float32)
+        env = np.zeros((i.env_height, i.env_width, 4), np.float32) if i.env_width else None
+        check(lib().pt_scene_export(self.h, _p(pos), _p(nrm), _p(tm), _p(sph), _p(sm), _p(mt), _p(ma), _p(me), _p(mi), _p(mal), _p(env)))
+        return dict(
+            tri_pos=pos, tri_nrm=nrm if i.has_normals else None, tri_mat=tm, sph=sph, sph_mat=sm,
+            mat_type=mt, mat_albedo=ma, mat_emit=me, mat_ior=mi, mat_alpha=mal, env_rgba=env,
+            camera=dict(origin=tuple(i.camera_origin), target=tuple(i.camera_target), up=tuple(i.camera_up),
+                        vfov=i.camera_vfov, flip=i.camera_flip),
+        )
+
+
+class Renderer:
+    """Device renderer (pt_renderer)."""
+
+    def __init__(self, width, height):
+        self.W, self.H = width, height
+        self.h = C.c_void_p()
+        check(lib().pt_renderer_create(width, height, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().pt_renderer_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_scene(self, scene):
+        check(lib().pt_renderer_set_scene(self.h, scene.h))
+
+    def set_camera(self, origin, target, up, vfov, flip=False):
+        o, t, u = _f32(origin), _f32(target), _f32(up)
+        check(lib().pt_renderer_set_camera(self.h, _p(o), _p(t), _p(u), vfov, int(flip)))
+
+    def set_material(self, mid, mtype, albedo=(0, 0, 0), emit=(0, 0, 0), ior=1.5, alpha=0.1):
+        a, e = _f32(albedo), _f32(emit)
+        check(lib().pt_renderer_set_material(self.h, mid, mtype, _p(a), _p(e), ior, alpha))
+
+    def set_max_depth(self, d):
+        check(lib().pt_renderer_set_max_depth(self.h, d))
+
+    def set_nee(self, on):
+        check(lib().pt_renderer_set_nee(self.h, int(on)))
+
+    def set_type(self, t):
+        check(lib().pt_renderer_set_type(self.h, t))
+
+    def set_seed(self, s):
+        check(lib().pt_renderer_set_seed(self.h, s))
+
+    def set_option(self, opt, v):
+        check(lib().pt_renderer_set_option(self.h, opt, v))
+
+    def render_async(self, spp):
+        check(lib().pt_renderer_render_async(self.h, spp))
+
+    def poll(self):
+        f = C.c_int()
+        ms = C.c_float()
+        check(lib().pt_renderer_poll(self.h, C.byref(f), C.byref(ms)))
+        return bool(f.value), ms.value
+
+    def synchronize(self):
+        check(lib().pt_renderer_synchronize(self.h))
+
+    def render(self, spp):
+        self.render_async(spp)
+        self.synchronize()
+
+    def reset(self):
+        check(lib().pt_renderer_reset(self.h))
+
+    def spp(self):
+        v = C.c_int64()
+        check(lib().pt_renderer_get_spp(self.h, C.byref(v)))
+        return v.value
+
+    def radiance(self):
+        out = np.zeros((self.H, self.W, 3), np.float32)
+        check(lib().pt_renderer_get_radiance(self.h, _p(out)))
+        return out
+
+    def radiance_sum(self):
+        out = np.zeros((self.H, self.W, 3), np.float64)
+        check(lib().pt_renderer_get_radiance_sum(self.h, _p(out)))
+        return out
+
+    def rgba8(self):
+        out = np.zeros((self.H, self.W, 4), np.uint8)
+        check(lib().pt_renderer_get_rgba8(self.h, _p(out)))
+        return out
+
+    def hit_test(self, x, y):
+        m = C.c_int()
+        h = C.c_int()
+        check(lib().pt_renderer_hit_test(self.h, x, y, C.byref(m), C.byref(h)))
+        return m.value, bool(h.value)
+
+    def stats(self):
+        s = RenderStats()
+        check(lib().pt_renderer_get_stats(self.h, C.byref(s)))
+        return s
+
+    def reset_stats(self):
+        check(lib().pt_renderer_reset_stats(self.h))
+
+    def kernel_stats(self):
+        arr = (KernelStat * 16)()
+        n = C.c_int()
+        check(lib().pt_renderer_get_kernel_stats(self.h, arr, 16, C.byref(n)))
+        return {arr[i].name.decode(): dict(launches=arr[i].launches, units=arr[i].units, ms=arr[i].ms, bytes=arr[i].bytes) for i in range(n.value)}
+
+    def path_radiance(self, n):
+        out = np.zeros((n, 3), np.float32)
+        cnt = C.c_int64()
+        check(lib().pt_renderer_get_path_radiance(self.h, _p(out), n, C.byref(cnt)))
+        return out[: cnt.value]
+
+    def intersect(self, rays, any_hit=False):
+        r = _f32(rays).reshape(-1, 8)
+        out = np.zeros((r.shape[0], 4), np.float32)
+        check(lib().pt_renderer_intersect(self.h, _p(r), r.shape[0], int(any_hit), _p(out)))
+        prim = out[:, 3].copy().view(np.int32)
+        return out[:, 0], out[:, 1], out[:, 2], prim
+
+
+def triangle_test(tris, rays, mode):
+    t = _f32(tris).reshape(-1, 9)
+    r = _f32(rays).reshape(-1, 8)
+    out = np.zeros((t.shape[0], 4), np.float32)
+    check(lib().pt_unit_triangle_test(_p(t), _p(r), t.shape[0], mode, _p(out)))
+    return out
+
+
+def bsdf_eval(mat, wo, wi):
+    m = _f32(mat).reshape(-1, 6)
+    o = _f32(wo).reshape(-1, 3)
+    i = _f32(wi).reshape(-1, 3)
+    f = np.zeros((m.shape[0], 3), np.float32)
+    pdf = np.zeros(m.shape[0], np.float32)
+    check(lib().pt_unit_bsdf_eval(_p(m), _p(o), _p(i), m.shape[0], _p(f), _p(pdf)))
+    return f, pdf
+
+
+def bsdf_sample(mat, wo, xi):
+    m = _f32(mat).reshape(-1, 6)
+    o = _f32(wo).reshape(-1, 3)
+    x = _f32(xi).reshape(-1, 2)
+    out = np.zeros((m.shape[0], 8), np.float32)
+    check(lib().pt_unit_bsdf_sample(_p(m), _p(o), _p(x), m.shape[0], _p(out)))
+    return out
+
+
+def env_tables(rgba, sin_theta_weight=False):
+    e = _f32(rgba)
+    h, w = e.shape[:2]
+    rp = np.zeros((h, w), np.float64)
+    ra = np.zeros((h, w), np.int32)
+    mp = np.zeros(h, np.float64)
+    ma = np.zeros(h, np.int32)
+    tp = np.zeros((h, w), np.float32)
+    check(lib().pt_unit_env_tables(_p(e), w, h, int(sin_theta_weight), _p(rp), _p(ra), _p(mp), _p(ma), _p(tp)))
+    return rp, ra, mp, ma, tp
+
+
+def env_sample(rgba, xi, pdf_dirs=None, sin_theta_weight=False):
+    e = _f32(rgba)
+    h, w = e.shape[:2]
+    x = _f32(xi).reshape(-1, 2)
+    n = x.shape[0]
+    pd = _f32(pdf_dirs if pdf_dirs is not None else np.zeros((0, 3))).reshape(-1, 3)
+    m = pd.shape[0]
+    d = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    em = np.zeros((n, 3), np.float32)
+    po = np.zeros(m, np.float32)
+    check(lib().pt_unit_env_sample(_p(e), w, h, int(sin_theta_weight), _p(x), n, _p(d), _p(pdf), _p(em), _p(pd), m, _p(po)))
+    return d, pdf, em, po
+
+
+def path_uniforms(seed, pixels, samples, k):
+    p = np.ascontiguousarray(pixels, dtype=np.uint32)
+    s = np.ascontiguousarray(samples, dtype=np.uint32)
+    out = np.zeros((p.size, k), np.float32)
+    check(lib().pt_unit_path_uniforms(seed, _p(p), _p(s), p.size, k, _p(out)))
+    return out

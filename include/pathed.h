@@ -1,0 +1,258 @@
+/* pathed.h -- C ABI of the MI355X path tracer (libpathed.so).
+ *
+ * Plain pointers and sizes only; no torch or HIP types cross this boundary.  Every call
+ * returns PT_OK (0) or a negative pt_status; pt_last_error() describes the last failure of
+ * the calling thread.  Host pointers are host memory unless a name says otherwise.
+ *
+ * The renderer replaces the reference's Renderer plugin (include/renderers/renderer.h:14-35)
+ * and the device-side scene upload (src/cuda_globals.cu:10-139); the scene calls replace
+ * SceneData/SceneAdapter (include/scene_data.h:18-61, src/scene_data.cu:9-118); the C++
+ * RenderSession/SceneModel and the `path_tracer` Python module are built on top of this ABI.
+ *
+ * Threading: one host thread per renderer at a time.  Calls on different renderers may run
+ * concurrently.
+ */
+#ifndef PATHED_H
+#define PATHED_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PT_ABI_VERSION 1
+
+typedef enum {
+    PT_OK = 0,
+    PT_ERR_INVALID = -1,    /* bad argument (null pointer, out-of-range id, bad size) */
+    PT_ERR_DEVICE = -2,     /* HIP runtime error or no device */
+    PT_ERR_MEMORY = -3,     /* allocation failed */
+    PT_ERR_STATE = -4,      /* call not valid in the current state (no scene, pass running) */
+    PT_ERR_NUMERIC = -5     /* degenerate input (e.g. camera looks along its up vector) */
+} pt_status;
+
+/* Material kinds: values of the reference's MaterialType (include/materials/types.h:5-11). */
+typedef enum {
+    PT_LAMBERTIAN = 0,
+    PT_MIRROR = 1,
+    PT_GLASS = 2,
+    PT_MICROFACET = 3,
+    PT_MATERIAL_NONE = 4
+} pt_material_type;
+
+/* Renderer kinds: the reference's RendererType (include/scene_model.h:15-19) plus the
+ * guided wavefront.  CUDA and Optix both select the plain wavefront integrator. */
+typedef enum {
+    PT_RENDERER_CUDA = 0,
+    PT_RENDERER_OPTIX = 1,
+    PT_RENDERER_NORMALS = 2,
+    PT_RENDERER_GUIDED = 3
+} pt_renderer_type;
+
+typedef struct pt_scene pt_scene;
+typedef struct pt_renderer pt_renderer;
+
+/* ------------------------------------------------------------------------------------ */
+/* Errors and version                                                                   */
+/* ------------------------------------------------------------------------------------ */
+const char *pt_last_error(void);
+int pt_abi_version(void);
+/* Number of visible HIP devices (0 on a host without a GPU). */
+int pt_device_count(int *count);
+
+/* ------------------------------------------------------------------------------------ */
+/* Scene description (host memory).  Replaces SceneData + SceneAdapter::createSceneData   */
+/* (include/scene_data.h:18-61, src/scene_data.cu:9-118).                                 */
+/* ------------------------------------------------------------------------------------ */
+int pt_scene_create(pt_scene **out);
+int pt_scene_destroy(pt_scene *scene);
+
+/* Appends a material (params.h:15-107 LambertianParams/MirrorParams/GlassParams/
+ * MicrofacetParams); albedo/emit are used by Lambertian, ior by Glass, alpha by Microfacet.
+ * *out_id receives the material id (dense, in insertion order). */
+int pt_scene_add_material(pt_scene *scene, int type, const float albedo[3], const float emit[3],
+                          float ior, float alpha, int *out_id);
+
+/* Appends n triangles: positions[9n] (p0 p1 p2), normals[9n] per-vertex shading normals or
+ * NULL for flat normals from the winding (obj_parser.cpp:185-194), material_ids[n]. */
+int pt_scene_add_triangles(pt_scene *scene, const float *positions, const float *normals,
+                           const int *material_ids, int64_t n);
+
+/* Appends n spheres: centers[3n], radii[n], material_ids[n] (primitives/sphere.h:10-63). */
+int pt_scene_add_spheres(pt_scene *scene, const float *centers, const float *radii,
+                         const int *material_ids, int64_t n);
+
+/* Equirectangular y-up environment, RGBA float texels row-major (rgba[4*w*h]), as loaded
+ * by createEnvironmentLight (environment_light.cu:14-56).  sin_theta_weight = 0 keeps the
+ * reference's sampling weight r+g+b; 1 multiplies it by sin(theta). */
+int pt_scene_set_environment(pt_scene *scene, const float *rgba, int width, int height,
+                             int sin_theta_weight);
+
+/* Pinhole camera (core/camera.h:16-61, transform.cu:7-33).  vfov in radians. */
+int pt_scene_set_camera(pt_scene *scene, const float origin[3], const float target[3],
+                        const float up[3], float vfov, int flip_handedness);
+
+/* Procedural workloads (seeded):
+ *   0  Cornell box: 5 walls (10 triangles), 2-triangle ceiling light (17,12,4), diffuse and
+ *      glass (ior 1.5) spheres of radius 0.3; the reference's default camera (scene.cu:176-184)
+ *   1  displaced icosphere (subdivision level `param` (default 8 -> 1,310,720 triangles),
+ *      3-octave value noise of amplitude 0.05), per-face Beckmann alpha U[0.05,0.5] with 30%
+ *      Lambert faces, synthetic 1024x512 HDR environment (log-normal sigma 1 + sun lobe)
+ *   2  closed room of ~`param` triangles (cubes and ellipsoids, Beckmann alpha U[0.02,0.3])
+ *      lit through a 0.3-wide slot by an area light (1e3) and the environment, glass sphere
+ *      r = 0.5
+ *   3  synthetic environment only (width = param, height = param/2) for sampling tests
+ */
+int pt_scene_preset(pt_scene *scene, int preset, uint64_t seed, int64_t param);
+
+typedef struct {
+    int64_t n_triangles;
+    int64_t n_spheres;
+    int64_t n_materials;
+    int64_t n_emitters;
+    int env_width, env_height;
+    int has_normals;
+    float camera_origin[3], camera_target[3], camera_up[3];
+    float camera_vfov;
+    int camera_flip;
+    float bounds_lo[3], bounds_hi[3];
+} pt_scene_info;
+int pt_scene_get_info(const pt_scene *scene, pt_scene_info *info);
+
+/* Copies the scene arrays out (sizes from pt_scene_get_info; any pointer may be NULL):
+ * positions[9T], normals[9T] (filled with the flat normals if none were given),
+ * tri_mat[T], spheres[4S] (cx cy cz r), sph_mat[S], mat_type[M], mat_albedo[3M],
+ * mat_emit[3M], mat_ior[M], mat_alpha[M], env_rgba[4*W*H]. */
+int pt_scene_export(const pt_scene *scene, float *positions, float *normals, int *tri_mat,
+                    float *spheres, int *sph_mat, int *mat_type, float *mat_albedo,
+                    float *mat_emit, float *mat_ior, float *mat_alpha, float *env_rgba);
+
+/* ------------------------------------------------------------------------------------ */
+/* Renderer.  Replaces Renderer/PathTracer/OptixTracer (renderer.h:14-35, path_tracer.cu, */
+/* optix_tracer.cu) and CUDAGlobals (cuda_globals.cu:10-139).                            */
+/* ------------------------------------------------------------------------------------ */
+/* Renderer::init (renderer.cpp:10-18) + PathTracer::init (path_tracer.cu:266-283). */
+int pt_renderer_create(int width, int height, pt_renderer **out);
+int pt_renderer_destroy(pt_renderer *r);
+
+/* CUDAGlobals::initMaterials/mallocWorld/copySceneData (cuda_globals.cu:51-139): uploads the
+ * scene, builds the BVH and the environment alias tables on the GPU, resets accumulation. */
+int pt_renderer_set_scene(pt_renderer *r, const pt_scene *scene);
+
+/* Scene edits (render_session.cu:82-116 callback).  Each resets accumulation. */
+int pt_renderer_set_camera(pt_renderer *r, const float origin[3], const float target[3],
+                           const float up[3], float vfov, int flip_handedness);
+int pt_renderer_set_material(pt_renderer *r, int material_id, int type, const float albedo[3],
+                             const float emit[3], float ior, float alpha);
+int pt_renderer_set_max_depth(pt_renderer *r, int max_depth);
+int pt_renderer_set_nee(pt_renderer *r, int enabled);
+int pt_renderer_set_type(pt_renderer *r, int renderer_type);
+int pt_renderer_set_seed(pt_renderer *r, uint64_t seed);
+
+/* Tuning knobs (queue capacity in paths, block counts); values <= 0 restore defaults. */
+typedef enum {
+    PT_OPT_POOL_PATHS = 0,
+    PT_OPT_TRACE_BLOCKS_PER_CU = 1,
+    PT_OPT_SHADE_BLOCKS_PER_CU = 2,
+    PT_OPT_PROFILE_KERNELS = 3   /* 1: time each kernel launch with HIP events */
+} pt_option;
+int pt_renderer_set_option(pt_renderer *r, int option, int64_t value);
+int pt_renderer_get_option(pt_renderer *r, int option, int64_t *value);
+
+/* Renderer::renderAsync (path_tracer.cu:285-336): enqueues one progressive pass of `spp`
+ * samples per pixel on the renderer's stream and returns at once. */
+int pt_renderer_render_async(pt_renderer *r, int spp);
+/* Renderer::pollRender (path_tracer.cu:338-356): *finished = 1 once the pass completed;
+ * *milliseconds = its GPU time (begin/end events). */
+int pt_renderer_poll(pt_renderer *r, int *finished, float *milliseconds);
+int pt_renderer_synchronize(pt_renderer *r);
+/* Renderer::reset: accumulation restarts at the next pass. */
+int pt_renderer_reset(pt_renderer *r);
+/* Renderer::getSpp */
+int pt_renderer_get_spp(pt_renderer *r, int64_t *spp);
+
+/* Renderer::getRadianceBuffer (renderer.cpp:20-42): mean radiance, 3*W*H floats,
+ * row-major, row 0 = bottom. */
+int pt_renderer_get_radiance(pt_renderer *r, float *out);
+/* FP64 per-pixel radiance sums over all accumulated samples (3*W*H doubles). */
+int pt_renderer_get_radiance_sum(pt_renderer *r, double *out);
+/* The uchar4 framebuffer of framebuffer.cu:35-40 (sRGB, 4*W*H bytes). */
+int pt_renderer_get_rgba8(pt_renderer *r, uint8_t *out);
+
+/* hitTest (hit_test.cu:49-82): material id under pixel (x, y from the bottom). */
+int pt_renderer_hit_test(pt_renderer *r, int x, int y, int *material_id, int *is_hit);
+
+typedef struct {
+    int64_t rays_total;        /* extension + shadow rays since the last reset of stats */
+    int64_t extension_rays;
+    int64_t shadow_rays;
+    int64_t paths;
+    int64_t passes;
+    double gpu_ms;             /* sum of pass times */
+    int bvh_depth;
+    int64_t bvh_nodes;
+    int stack_overflows;       /* traversal stack overflow events (must stay 0) */
+} pt_render_stats;
+int pt_renderer_get_stats(pt_renderer *r, pt_render_stats *stats);
+int pt_renderer_reset_stats(pt_renderer *r);
+
+/* Per-kernel timing of the last pass when PT_OPT_PROFILE_KERNELS is on: for kernel k,
+ * launches, total ms and algorithmic HBM bytes.  Names: "raygen", "extend", "shade",
+ * "shadow", "accumulate", "tonemap". */
+typedef struct {
+    char name[16];
+    int64_t launches;
+    int64_t units;            /* rays / segments / paths / pixels processed */
+    double ms;
+    double bytes;             /* algorithmic bytes (per-unit constants x units) */
+} pt_kernel_stat;
+int pt_renderer_get_kernel_stats(pt_renderer *r, pt_kernel_stat *out, int capacity, int *count);
+
+/* Radiance of every path of the last pass in path order (sample-major: path = s*W*H + pixel),
+ * 3 floats each; only when the pass fit in one chunk (paths <= pool).  For parity tests. */
+int pt_renderer_get_path_radiance(pt_renderer *r, float *out, int64_t capacity, int64_t *count);
+
+/* ------------------------------------------------------------------------------------ */
+/* Unit-level entry points (parity harness).  Host pointers in and out.                  */
+/* ------------------------------------------------------------------------------------ */
+/* Closest hit (any_hit = 0) or occlusion (any_hit = 1) for n rays (8 floats each:
+ * o.xyz tmin d.xyz tmax) through the renderer's BVH (watertight triangle test).
+ * out[4n]: t, u, v, primitive (int bits: >= 0 triangle index, < 0 ~sphere index,
+ * INT32_MIN miss) -- the indexing of World::hit's PrimitiveIndex (world.cu:5-41). */
+int pt_renderer_intersect(pt_renderer *r, const float *rays, int64_t n, int any_hit, float *out);
+
+/* Single ray/triangle tests on the GPU (triangle.cu:7-48): tris[9n], rays[8n] -> out[4n]
+ * (hit flag, t, u, v).  mode 0 = Moeller-Trumbore in the reference's operation order,
+ * mode 1 = watertight.  Built without fused multiply-adds for bit-level comparison. */
+int pt_unit_triangle_test(const float *tris, const float *rays, int64_t n, int mode, float *out);
+
+/* BSDF f/pdf (bsdf.h:20-61) and sample (bsdf.h:87-115) on the GPU.  mat[6n] = type, albedo
+ * rgb, ior, alpha; wo/wi[3n]; xi[2n].  eval: f[3n], pdf[n]; sample: out[8n] = wi[3], pdf,
+ * f[3], is_delta. */
+int pt_unit_bsdf_eval(const float *mat, const float *wo, const float *wi, int64_t n, float *f,
+                      float *pdf);
+int pt_unit_bsdf_sample(const float *mat, const float *wo, const float *xi, int64_t n,
+                        float *out);
+
+/* Environment tables built on the GPU in FP64 (alias per row + marginal) for an RGBA map,
+ * copied back: row_prob[w*h] (double), row_alias[w*h], marg_prob[h], marg_alias[h],
+ * texel_pdf[w*h] (pmf * w*h / (2 pi^2), float). */
+int pt_unit_env_tables(const float *rgba, int width, int height, int sin_theta_weight,
+                       double *row_prob, int *row_alias, double *marg_prob, int *marg_alias,
+                       float *texel_pdf);
+/* Environment sampling and pdf on the GPU: xi[2n] -> dir[3n], pdf[n], emit[3n];
+ * pdf_dirs[3m] -> pdf_out[m]. */
+int pt_unit_env_sample(const float *rgba, int width, int height, int sin_theta_weight,
+                       const float *xi, int64_t n, float *dir, float *pdf, float *emit,
+                       const float *pdf_dirs, int64_t m, float *pdf_out);
+
+/* PCG32 path streams on the GPU: first k uniforms of (seed, pixel, sample) for n queries. */
+int pt_unit_path_uniforms(uint64_t seed, const uint32_t *pixel, const uint32_t *sample, int64_t n,
+                          int k, float *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PATHED_H */

@@ -982,6 +982,16 @@ template <class T> static LightSample<T> sampleDirectLights(const Scene<T> &s, c
     LightSample<T> ls;
     const int nl = (int)s.lights.size();
     const int choiceSize = nl + (s.env.present ? 1 : 0);
+    if (choiceSize == 0) {
+        // no emitter and no environment: the reference would index lightIndices[-1]
+        // (sampler.h:155-163); return a sample that the one-sidedness test rejects
+        ls.wi = V3<T>(T(0), T(0), T(1));
+        ls.normal = ls.wi;
+        ls.distance = T(0);
+        ls.pdf = T(0);
+        ls.emitted = V3<T>(T(0));
+        return ls;
+    }
     const int choice = (int)std::min((float)(choiceSize - 1), std::floor(xi1 * (float)choiceSize));
     const T choicePdf = T(1) / (T)choiceSize;
     if (choice < nl) {

@@ -1,0 +1,342 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the same seeded
+inputs.  Integer/bit-level work is compared exactly; floating-point work within the
+tolerances stated per test; rendered images within Monte Carlo confidence bands."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pathed_capi as P
+
+pytestmark = pytest.mark.gpu
+
+LAMB, MIRROR, GLASS, MICRO = 0, 1, 2, 3
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if P.device_count() == 0:
+        pytest.fail("no HIP device visible: the GPU tests need an MI355X")
+
+
+def sph_dirs(rng, n, upper=False):
+    v = rng.normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    if upper:
+        v[:, 2] = np.abs(v[:, 2])
+    return v.astype(np.float32)
+
+
+# --------------------------------------------------------------------------- RNG
+def test_path_uniforms_bit_exact():
+    say("case: PCG32 path streams")
+    rng = np.random.default_rng(0)
+    pix = rng.integers(0, 1 << 22, 2000).astype(np.uint32)
+    smp = rng.integers(0, 1 << 16, 2000).astype(np.uint32)
+    g = P.path_uniforms(0, pix, smp, 16)
+    for k in range(0, 2000, 97):
+        assert np.array_equal(g[k], O.path_uniforms(0, int(pix[k]), int(smp[k]), 16))
+    g7 = P.path_uniforms(7, pix[:50], smp[:50], 4)
+    assert not np.array_equal(g7, g[:50, :4])
+
+
+# --------------------------------------------------------------------------- triangles
+def _tri_rays(rng, n):
+    tris = rng.uniform(-1, 1, (n, 9)).astype(np.float32)
+    o = rng.uniform(-3, 3, (n, 3)).astype(np.float32)
+    b = rng.uniform(-0.2, 1.2, (n, 2))
+    tgt = tris[:, 0:3] * (1 - b[:, :1] - b[:, 1:]) + tris[:, 3:6] * b[:, :1] + tris[:, 6:9] * b[:, 1:]
+    d = (tgt - o).astype(np.float32)
+    tmin = rng.choice([0.0, 1e-4, 0.5], n)
+    tmax = rng.choice([1e30, 2.0, 0.99], n)
+    return tris, np.concatenate([o, tmin[:, None], d, tmax[:, None]], 1).astype(np.float32)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_triangle_tests_bit_exact(mode):
+    """Moeller-Trumbore (reference operation order) and watertight: hit/miss decisions and
+    t, u, v match the oracle bit for bit on seeded rays (both sides built without FMA)."""
+    say(f"case: triangle test mode {mode}")
+    rng = np.random.default_rng(10 + mode)
+    tris, rays = _tri_rays(rng, 200000)
+    g = P.triangle_test(tris, rays, mode)
+    c = O.tri_test(tris, rays, mode)
+    assert np.array_equal(g[:, 0], c[:, 0])
+    hit = g[:, 0] == 1
+    assert hit.mean() > 0.2
+    assert np.array_equal(g[hit, 1:].view(np.int32), c[hit, 1:].view(np.int32))
+
+
+# --------------------------------------------------------------------------- BSDFs
+def _mats(rng, n):
+    types = rng.integers(0, 4, n)
+    m = np.zeros((n, 6), np.float32)
+    m[:, 0] = types
+    m[:, 1:4] = rng.uniform(0, 1, (n, 3))
+    m[:, 4] = rng.uniform(1.1, 2.4, n)
+    m[:, 5] = rng.uniform(0.02, 0.6, n)
+    return m
+
+
+def test_bsdf_eval_within_1e5():
+    say("case: BSDF eval/pdf")
+    rng = np.random.default_rng(20)
+    n = 100000
+    m = _mats(rng, n)
+    wo = sph_dirs(rng, n)
+    wi = sph_dirs(rng, n)
+    wo[: n // 2, 2] = np.abs(wo[: n // 2, 2])
+    wi[: n // 2, 2] = np.abs(wi[: n // 2, 2])
+    gf, gp = P.bsdf_eval(m, wo, wi)
+    of, op = O.bsdf_eval(m, wo, wi, 32)
+    tol = 1e-5
+    assert np.all(np.abs(gf - of) <= tol * np.abs(of) + 1e-30)
+    assert np.all(np.abs(gp - op) <= tol * np.abs(op) + 1e-30)
+    # against the FP64 oracle on well-conditioned directions
+    df, dp = O.bsdf_eval(m, wo, wi, 64)
+    ok = (np.abs(wo[:, 2]) > 0.05) & (np.abs(wi[:, 2]) > 0.05)
+    assert np.all(np.abs(gf[ok] - df[ok]) <= 1e-4 * np.abs(df[ok]) + 1e-30)
+
+
+def test_bsdf_sample_matches_oracle():
+    say("case: BSDF sample")
+    rng = np.random.default_rng(21)
+    n = 100000
+    m = _mats(rng, n)
+    wo = sph_dirs(rng, n)
+    wo[: 3 * n // 4, 2] = np.abs(wo[: 3 * n // 4, 2])
+    xi = rng.random((n, 2)).astype(np.float32)
+    g = P.bsdf_sample(m, wo, xi)
+    c = O.bsdf_sample(m, wo, xi, 32)
+    # glass decisions at xi ~ F may flip between implementations: exclude a 1e-5 band
+    F = np.array([O.fresnel(abs(float(wo[k, 2])), *((1.0, float(m[k, 4])) if wo[k, 2] >= 0 else (float(m[k, 4]), 1.0)))
+                  if m[k, 0] == GLASS else 2.0 for k in range(n)])
+    sure = np.abs(xi[:, 0] - F) > 1e-5
+    assert np.array_equal(g[sure, 7], c[sure, 7].astype(np.float32))
+    fin = sure & np.isfinite(c[:, 3]) & (c[:, 3] > 1e-6) & (np.abs(c[:, 2]) > 1e-4)
+    assert np.allclose(g[fin, :3], c[fin, :3], atol=2e-5)
+    assert np.allclose(g[fin, 3], c[fin, 3], rtol=2e-4)
+    assert np.allclose(g[fin, 4:7], c[fin, 4:7], rtol=2e-4, atol=1e-30)
+
+
+# --------------------------------------------------------------------------- environment
+def _env(rng, h, w):
+    e = np.zeros((h, w, 4), np.float32)
+    e[..., :3] = rng.lognormal(0, 1, (h, w, 3))
+    e[h // 3, w // 5, :3] = 1e4
+    e[5:7, :, :3] = 0
+    return e
+
+
+def test_env_alias_tables_bit_exact():
+    """FP64 alias tables built on the GPU are identical to the oracle's (same sweep, same
+    operation order), and their pmf equals the reference CMF pmf."""
+    say("case: env alias tables")
+    rng = np.random.default_rng(30)
+    e = _env(rng, 128, 256)
+    g = P.env_tables(e)
+    c = O.env_alias(e)
+    for a, b in zip(g, c):
+        assert np.array_equal(a, b)
+
+
+def test_env_sampling_matches_oracle():
+    say("case: env sampling")
+    rng = np.random.default_rng(31)
+    e = _env(rng, 64, 128)
+    xi = rng.random((50000, 2)).astype(np.float32)
+    dirs = sph_dirs(rng, 20000)
+    gd, gp, ge, gpo = P.env_sample(e, xi, dirs)
+    cd, cp, ce = O.env_sample(e, xi, ref=False)
+    assert np.allclose(gd, cd, atol=2e-6)
+    same = np.array_equal
+    close = np.isclose(gp, cp, rtol=1e-5)
+    assert close.mean() > 0.999
+    assert (np.all(ge == ce, axis=1)).mean() > 0.999
+    op = O.env_pdf(e, dirs, ref=False)
+    assert np.isclose(gpo, op, rtol=1e-5).mean() > 0.999
+
+
+# --------------------------------------------------------------------------- BVH queries
+def _soup(rng, n):
+    c = rng.uniform(-2, 2, (n, 3))
+    return np.concatenate([c + rng.normal(size=(n, 3)) * 0.15 for _ in range(3)], 1).astype(np.float32)
+
+
+def test_bvh_closest_hit_matches_brute_force():
+    say("case: BVH closest hit vs brute force")
+    rng = np.random.default_rng(40)
+    tris = _soup(rng, 20000)
+    s = P.Scene()
+    m = s.add_material(LAMB, (0.5, 0.5, 0.5))
+    s.add_triangles(tris, np.full(len(tris), m))
+    s.add_spheres([[0, 0, 0], [1, 1, 1]], [0.5, 0.3], [m, m])
+    r = P.Renderer(16, 16)
+    r.set_scene(s)
+    o = O.Scene(s.export(), 16, 16, bvh=False)
+    n = 20000
+    org = rng.uniform(-4, 4, (n, 3))
+    d = rng.normal(size=(n, 3))
+    rays = np.concatenate([org, np.zeros((n, 1)), d, np.full((n, 1), 3e38)], 1).astype(np.float32)
+    rays[: n // 4, 7] = 1.0  # short rays
+    gt, gu, gv, gprim = r.intersect(rays)
+    ct, cu, cv, cprim = o.intersect(rays, tri_test=1)
+    agree = gprim == cprim
+    assert agree.mean() > 0.9995, agree.mean()
+    hit = agree & (gprim != P.MISS)
+    assert np.allclose(gt[hit], ct[hit], rtol=1e-5)
+    # occlusion queries agree with closest-hit existence
+    _, _, _, occ = r.intersect(rays, any_hit=True)
+    vis_ref = cprim == P.MISS
+    assert ((occ == P.MISS) == vis_ref).mean() > 0.999
+    st = r.stats()
+    assert st.stack_overflows == 0 and st.bvh_depth > 10
+    r.close()
+
+
+# --------------------------------------------------------------------------- rendering
+def _cornell(W=64, H=64):
+    s = P.Scene.preset(0)
+    r = P.Renderer(W, H)
+    r.set_scene(s)
+    return s, r
+
+
+def test_cornell_per_path_parity():
+    """Same PCG32 streams on both sides: the radiance of each path of a 64x64x4 Cornell pass
+    (NEE+MIS, depth 6) matches the oracle's product-mode integrator."""
+    say("case: Cornell per-path parity")
+    s, r = _cornell()
+    r.set_max_depth(6)
+    r.render(4)
+    gl = r.path_radiance(64 * 64 * 4)
+    o = O.Scene(s.export(), 64, 64)
+    pix = np.tile(np.arange(64 * 64), 4)
+    smp = np.repeat(np.arange(4), 64 * 64)
+    cl = o.trace_paths(pix, smp, max_depth=6, nee=1, mode=1, tri_test=1, ref_env=0, precision=32)
+    err = np.abs(gl - cl).max(1) / (np.abs(cl).max(1) + 1e-3)
+    frac = (err < 1e-3).mean()
+    say(f"per-path agreement {frac:.5f}")
+    assert frac > 0.99
+    assert np.isfinite(gl).all()
+    st = r.stats()
+    assert st.stack_overflows == 0
+    r.close()
+
+
+@pytest.mark.parametrize("nee", [1, 0])
+def test_cornell_image_within_confidence_band(nee):
+    """GPU image (FP64 accumulation) vs the FP64 oracle restating the reference integrator
+    (two traces per bounce, Moeller-Trumbore): 99% band per pixel."""
+    say(f"case: Cornell image band nee={nee}")
+    W = H = 32
+    s, r = _cornell(W, H)
+    r.set_max_depth(6)
+    r.set_nee(nee)
+    passes, per = 16, 64
+    imgs = []
+    prev = np.zeros((H, W, 3))
+    for k in range(passes):
+        r.render(per)
+        cur = r.radiance_sum()
+        imgs.append((cur - prev) / per)
+        prev = cur
+    imgs = np.array(imgs)
+    g_mean = imgs.mean(0)
+    g_var = imgs.var(0, ddof=1) / passes
+    o = O.Scene(s.export(), W, H)
+    spp = 2048
+    tot, sq, _ = o.render(want_sq=True, spp=spp, max_depth=6, nee=nee, mode=0, tri_test=0, ref_env=1, precision=64, seed=99)
+    o_mean = tot / spp
+    o_var = (sq / spp - o_mean**2) / spp
+    z = (g_mean - o_mean) / np.sqrt(g_var + o_var + 1e-12)
+    outside = (np.abs(z) > 2.576).mean()
+    say(f"fraction outside the 99% band: {outside:.4f}; rmse {np.sqrt(((g_mean - o_mean) ** 2).mean()):.5f}")
+    assert outside < 0.03
+    assert abs(z.mean()) < 0.1
+    r.close()
+
+
+def test_cornell_normals_view():
+    say("case: normals view")
+    s, r = _cornell(32, 32)
+    r.set_type(P.RENDERER_NORMALS)
+    r.render(1)
+    img = r.radiance()
+    # the back wall (normal +z) fills the image centre: colour (0.5, 0.5, 1)
+    assert np.allclose(img[20, 16], [0.5, 0.5, 1.0], atol=1e-5)
+    # the left wall (normal +x) at the left edge, mid height
+    assert np.allclose(img[20, 1], [1.0, 0.5, 0.5], atol=1e-5)
+    r.close()
+
+
+def test_hit_test_and_material_edit():
+    say("case: hit test + edit")
+    s, r = _cornell(64, 64)
+    mid, hit = r.hit_test(32, 40)
+    assert hit and mid == 0  # back wall: grey material 0
+    mid, hit = r.hit_test(1, 40)
+    assert hit and mid == 1  # left wall: red
+    r.set_max_depth(3)
+    r.render(64)
+    before = r.radiance()[30:50, 0:4].mean((0, 1))
+    r.set_material(1, LAMB, albedo=(0.05, 0.7, 0.05))
+    assert r.spp() == 0
+    r.render(64)
+    after = r.radiance()[30:50, 0:4].mean((0, 1))
+    assert before[0] > before[1] and after[1] > after[0]
+    # an emission edit puts the wall into the light list
+    r.set_material(1, LAMB, albedo=(0.5, 0.5, 0.5), emit=(5, 5, 5))
+    r.render(16)
+    assert r.radiance()[30:50, 0:4].mean() > 4.0
+    r.close()
+
+
+def test_session_api_on_gpu():
+    say("case: path_tracer session")
+    import path_tracer as pt
+    s = pt.RenderSession(64, 64)
+    st = s.init(11, 22)
+    assert st.isRendering is False and st.pbo == 11
+    model = s.getSceneModel()
+    seen = set()
+    for _ in range(8):
+        st = s.renderAsync()
+        assert st.isRendering
+        seen.add(st.pbo)
+        while s.pollRender().isRendering:
+            pass
+    assert model.getSpp() >= 8 and seen == {11, 22}
+    img = s.getRadianceBuffer()
+    assert img.shape == (64, 64, 3) and np.isfinite(img).all() and img.mean() > 0.05
+    fb = s.getFramebuffer()
+    assert fb.shape == (64, 64, 4) and np.all(fb[..., 3] == 255)
+    s.hitTest(32, 58)  # the ceiling light
+    assert model.getMaterialID() == 3 and model.getMaterialType() == pt.MaterialType.Lambertian
+    assert model.getEmit().r() == pytest.approx(17)
+    s.hitTest(32, 40)
+    assert model.getMaterialID() == 0
+    model.setMaterialType(pt.MaterialType.Glass)
+    assert model.getMaterialType() == pt.MaterialType.Glass and model.getIOR() == pytest.approx(1.4)
+    model.setIOR(1.7)
+    assert model.getIOR() == pytest.approx(1.7)
+    model.setNextEventEstimation(False)
+    s.renderAsync()
+    while s.pollRender().isRendering:
+        pass
+    assert model.getSpp() == 1  # accumulation restarted, the optimiser restarts at 1 spp
+    model.setRendererType(pt.RendererType.Normals)
+    s.renderAsync()
+    while s.pollRender().isRendering:
+        pass
+    s.renderAsync()
+    while s.pollRender().isRendering:
+        pass
+    assert abs(s.getRadianceBuffer()[40, 32, 2] - 1.0) < 1e-5
