@@ -213,6 +213,39 @@ PT_INLINE float bsdfPdf(const PtMaterial &m, f3 wo, f3 wi) {
     if (t == PT_MAT_MICROFACET) return microfacetPdf(m.b.w, wo, wi);
     return 0.f;
 }
+// Density of the directions sample() produces, evaluated at an arbitrary wi: for the
+// Beckmann lobe this is the sample() pdf form (widened alpha at |cos wo|, half vector taken
+// in the upper hemisphere), which differs from pdf() (alpha at |cos wi|, microfacet.h:124-134).
+// Used where MIS weights must sum to one (guided vertices, consistent-MIS option).
+PT_INLINE float bsdfSamplePdf(const PtMaterial &m, f3 wo, f3 wi) {
+    const int t = matType(m);
+    if (t == PT_MAT_LAMBERTIAN) return wi.z < 0.f ? 0.f : 0.15915494309189535f;
+    if (t == PT_MAT_MICROFACET) {
+        const f3 s = wo + wi;
+        if (isZero(s)) return 0.f;
+        f3 wh = normalize(s);
+        if (wh.z < 0.f) wh = -wh;
+        const float d = fabsf(dot(wi, wh));
+        if (!(d > 0.f)) return 0.f;
+        return mfD(m.b.w, wo, wh) * fabsf(wh.z) / (4.f * d);
+    }
+    return 0.f;
+}
+
+// von Mises-Fisher lobe around mu (Jakob 2012, "Numerically stable sampling of the von
+// Mises-Fisher distribution on S^2") and its density kappa / (2 pi (1 - e^{-2 kappa}))
+// exp(kappa (mu.w - 1)).
+PT_INLINE float vmfDensity(f3 mu, float kappa, f3 w) {
+    return kappa / (6.28318530718f * (1.f - __expf(-2.f * kappa))) * __expf(kappa * (dot(mu, w) - 1.f));
+}
+PT_INLINE f3 sampleVmf(f3 mu, float kappa, float u1, float u2) {
+    const float w = fminf(fmaxf(1.f + __logf(u1 + (1.f - u1) * __expf(-2.f * kappa)) / kappa, -1.f), 1.f);
+    const float s = sqrtf(fmaxf(0.f, 1.f - w * w));
+    float sp, cp;
+    sincosf(6.28318530717958647692f * u2, &sp, &cp);
+    return toWorld(makeFrame(mu), mk(s * cp, s * sp, w));
+}
+
 // Uniforms consumed by sample() per material (lambertian.h:55-59, mirror.h:37-43,
 // glass.h:35-37, microfacet.h:136-140)
 PT_INLINE int bsdfDims(int t) { return t == PT_MAT_MIRROR ? 0 : (t == PT_MAT_GLASS ? 1 : 2); }

@@ -1,0 +1,544 @@
+// pt_guide.hip -- BF16 MFMA kernels of the guiding network (gfx950).
+//
+// Every layer is 64 x 64 (the 48-output last layer is zero-padded to 64), weights BF16
+// ([out][in] for the forward pass, [in][out] for the backward pass), activations BF16,
+// accumulation FP32 with v_mfma_f32_32x32x16_bf16: one wave multiplies a 32-row tile by a
+// 64-wide layer as 2 output tiles x 4 K-steps.  Fragment layout of the 32x32x16 instruction
+// used throughout (A: M x K, B: K x N, lane l = 0..63):
+//   A: row (l & 31), 8 consecutive k starting at 8 * (l >> 5) of the 16-wide K step
+//   B: col (l & 31), the same 8 k
+//   C: 16 floats per lane, element r at row 8 * (r >> 2) + 4 * (l >> 5) + (r & 3), col (l & 31)
+// (checked on the device by pt_unit_mfma_tile).  Operands are staged so that each lane's 8
+// k-values are contiguous: activations row-major for the forward A operand, transposed copies
+// for the weight-gradient products.
+//
+// Inference: blocks of 2 waves, each wave loops over 32-row tiles of the ray queue, computes
+// the raw features of its rows, encodes them one-blob into LDS, runs the 5 layers and writes the
+// mixture (8 vMF lobes + selection probability) per row.
+// Training: one wave per block, persistent over 32-row tiles of the record batch; forward with
+// the layer inputs kept transposed in LDS, KL loss gradient at the output, backward with
+// weight gradients accumulated in a per-block FP32 partial (no atomics); a second kernel sums
+// the partials in block order (deterministic) and applies Adam, refreshing the BF16 copies.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include <random>
+#include <vector>
+
+#include "pt_device.h"
+#include "pt_gpu.h"
+#include "pt_guide.h"
+#include "pt_internal.h"
+
+namespace pt {
+
+typedef __bf16 bf16x8 __attribute__((__vector_size__(8 * sizeof(__bf16))));
+typedef float f32x16 __attribute__((__vector_size__(16 * sizeof(float))));
+
+constexpr int ACT_STRIDE = 72;   // bf16 per row of a row-major 32 x 64 tile (padded)
+constexpr int T_STRIDE = 40;     // bf16 per row of a transposed 64 x 32 tile (padded)
+
+#define WAVE_SYNC()                                         \
+    do {                                                    \
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); \
+        __builtin_amdgcn_wave_barrier();                    \
+    } while (0)
+
+__device__ __forceinline__ int cRow(int lane, int r) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
+
+__device__ __forceinline__ bf16x8 ld8(const __bf16 *p) { return *(const bf16x8 *)p; }
+
+__device__ __forceinline__ f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0); }
+
+__device__ __forceinline__ float luminance(float4 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+
+__device__ __forceinline__ float oneBlob(float s, int j) {
+    const float x = s * (float)GUIDE_BINS - ((float)j + 0.5f);
+    return __expf(-0.5f * x * x);
+}
+
+// forward through layer L for one 32-row tile: A from `act` (row-major), B from W[L]
+__device__ __forceinline__ f32x16 layerTile(const __bf16 *act, const __bf16 *W, int L, int ct, int lane) {
+    f32x16 acc = {};
+    const __bf16 *arow = act + (lane & 31) * ACT_STRIDE + (lane >> 5) * 8;
+    const __bf16 *wrow = W + (size_t)L * GUIDE_W * GUIDE_W + (ct * 32 + (lane & 31)) * GUIDE_W + (lane >> 5) * 8;
+#pragma unroll
+    for (int ks = 0; ks < GUIDE_W / 16; ks++) acc = mfma(ld8(arow + ks * 16), ld8(wrow + ks * 16), acc);
+    return acc;
+}
+
+// mixture parameters from the 41 used raw outputs
+struct Mixture {
+    f3 mu[GUIDE_LOBES];
+    float kappa[GUIDE_LOBES], pi[GUIDE_LOBES], mlen[GUIDE_LOBES];
+    float frac;
+};
+__device__ __forceinline__ void mixtureFromRaw(const float *o, Mixture &mx) {
+    float mxl = o[GO_LOGIT];
+    for (int k = 1; k < GUIDE_LOBES; k++) mxl = fmaxf(mxl, o[GO_LOGIT + k]);
+    float s = 0.f;
+    for (int k = 0; k < GUIDE_LOBES; k++) {
+        mx.pi[k] = __expf(o[GO_LOGIT + k] - mxl);
+        s += mx.pi[k];
+    }
+    for (int k = 0; k < GUIDE_LOBES; k++) {
+        mx.pi[k] /= s;
+        const f3 m = mk(o[GO_MEAN + 3 * k], o[GO_MEAN + 3 * k + 1], o[GO_MEAN + 3 * k + 2]);
+        const float l = sqrtf(dot(m, m));
+        mx.mlen[k] = l;
+        mx.mu[k] = l > 1e-12f ? m / l : mk(0.f, 1.f, 0.f);
+        mx.kappa[k] = __expf(fminf(fmaxf(o[GO_KAPPA + k], GUIDE_LOGK_MIN), GUIDE_LOGK_MAX));
+    }
+    const float c = 1.f / (1.f + __expf(-o[GO_FRAC]));
+    mx.frac = fminf(fmaxf(c, GUIDE_FRAC_MIN), GUIDE_FRAC_MAX);
+}
+
+// von Mises-Fisher density on S^2: kappa / (2 pi (1 - e^{-2 kappa})) exp(kappa (mu.w - 1))
+__device__ __forceinline__ float vmfPdf(f3 mu, float kappa, f3 w) {
+    return kappa / (6.28318530718f * (1.f - __expf(-2.f * kappa))) * __expf(kappa * (dot(mu, w) - 1.f));
+}
+
+// --------------------------------------------------------------------------------------------
+// inference
+// --------------------------------------------------------------------------------------------
+constexpr int INFER_WAVES = 2;
+
+__global__ __launch_bounds__(64 * INFER_WAVES) void k_guide_infer(PtSceneView sc, GuideSceneNorm nm, GuideNet net, const int *count,
+                                                                  const float4 *rayO, const float4 *rayD, const float4 *hits, int maxRows,
+                                                                  float4 *out, const float *rawFeats, float *rawOut) {
+    __shared__ float sRaw[INFER_WAVES][32][GUIDE_RAW];
+    __shared__ int sElig[INFER_WAVES][32];
+    __shared__ __bf16 sAct[INFER_WAVES][2][32 * ACT_STRIDE];
+    __shared__ float sOut[INFER_WAVES][32][GUIDE_OUT + 1];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int n = count ? min(*count, maxRows) : maxRows;
+    const int waveStride = gridDim.x * INFER_WAVES;
+    for (int tile = blockIdx.x * INFER_WAVES + wave; tile * 32 < n; tile += waveStride) {
+        const int rowBase = tile * 32;
+        if (lane < 32) {
+            const int i = rowBase + lane;
+            float f[GUIDE_RAW];
+            for (int k = 0; k < GUIDE_RAW; k++) f[k] = 0.f;
+            int elig = 0;
+            if (i < n) {
+                if (rawFeats) {
+                    for (int k = 0; k < GUIDE_RAW; k++) f[k] = rawFeats[(size_t)i * GUIDE_RAW + k];
+                    elig = 1;
+                } else {
+                    const float4 h = hits[i];
+                    const int slot = asInt(h.w);
+                    if (slot != INT32_MIN) {
+                        const float4 ro = rayO[i], rd = rayD[i];
+                        const f3 o = xyz(ro), d = xyz(rd);
+                        const float4 p0 = sc.prims[3 * (size_t)slot], p1 = sc.prims[3 * (size_t)slot + 1];
+                        const PtMaterial m = sc.materials[asInt(p0.w)];
+                        const int type = matType(m);
+                        if (type == PT_MAT_LAMBERTIAN || type == PT_MAT_MICROFACET) {
+                            f3 p, nn;
+                            const int tag = asInt(p1.w);
+                            if (tag == PT_TAG_SPHERE) {
+                                p = o + d * h.x;
+                                nn = normalize((p - xyz(p0)) / p1.x);
+                            } else {
+                                const float4 p2 = sc.prims[3 * (size_t)slot + 2];
+                                const f3 a = xyz(p0), b = xyz(p1), c = xyz(p2);
+                                p = a * (1.f - h.y - h.z) + b * h.y + c * h.z;
+                                if (tag >= 0) {
+                                    const f3 na = xyz(sc.normals[3 * (size_t)tag]), nb = xyz(sc.normals[3 * (size_t)tag + 1]),
+                                             nc = xyz(sc.normals[3 * (size_t)tag + 2]);
+                                    nn = normalize(na * (1.f - h.y - h.z) + nb * h.y + nc * h.z);
+                                } else {
+                                    nn = normalize(cross(b - a, c - a));
+                                }
+                            }
+                            const float rough = type == PT_MAT_LAMBERTIAN ? 1.f : fminf(m.b.w * (1.f / 0.6f), 1.f);
+                            guideFeatures(nm, p, nn, -d, rough, f);
+                            elig = 1;
+                        }
+                    }
+                }
+            }
+            for (int k = 0; k < GUIDE_RAW; k++) sRaw[wave][lane][k] = f[k];
+            sElig[wave][lane] = elig;
+        }
+        WAVE_SYNC();
+        {
+            // one-blob encoding: lane -> row (lane & 31), raw features 4 * (lane >> 5) .. + 3
+            const int row = lane & 31;
+            __bf16 *dst = sAct[wave][0] + row * ACT_STRIDE;
+            for (int kk = 0; kk < 4; kk++) {
+                const int k = 4 * (lane >> 5) + kk;
+                const float s = sRaw[wave][row][k];
+                for (int j = 0; j < GUIDE_BINS; j++) dst[k * GUIDE_BINS + j] = (__bf16)oneBlob(s, j);
+            }
+        }
+        WAVE_SYNC();
+        int cur = 0;
+        for (int L = 0; L < GUIDE_LAYERS; L++) {
+            for (int ct = 0; ct < 2; ct++) {
+                const f32x16 acc = layerTile(sAct[wave][cur], net.w, L, ct, lane);
+                const int col = ct * 32 + (lane & 31);
+                const float bias = net.b[L * GUIDE_W + col];
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    const int row = cRow(lane, r);
+                    const float v = acc[r] + bias;
+                    if (L < GUIDE_LAYERS - 1) sAct[wave][1 - cur][row * ACT_STRIDE + col] = (__bf16)fmaxf(v, 0.f);
+                    else if (col < GUIDE_OUT) sOut[wave][row][col] = v;
+                }
+            }
+            WAVE_SYNC();
+            cur = 1 - cur;
+        }
+        if (lane < 32) {
+            const int i = rowBase + lane;
+            if (i < n) {
+                if (rawOut) {
+                    for (int k = 0; k < GUIDE_OUT; k++) rawOut[(size_t)i * GUIDE_OUT + k] = sOut[wave][lane][k];
+                } else {
+                    Mixture mx;
+                    mixtureFromRaw(sOut[wave][lane], mx);
+                    float4 *o = out + (size_t)i * GUIDE_REC_F4;
+                    for (int k = 0; k < GUIDE_LOBES; k++) o[k] = f4(mx.mu[k], mx.kappa[k]);
+                    o[8] = make_float4(mx.pi[0], mx.pi[1], mx.pi[2], mx.pi[3]);
+                    o[9] = make_float4(mx.pi[4], mx.pi[5], mx.pi[6], mx.pi[7]);
+                    o[10] = make_float4(mx.frac, (float)sElig[wave][lane], 0.f, 0.f);
+                    o[11] = make_float4(0.f, 0.f, 0.f, 0.f);
+                }
+            }
+        }
+        WAVE_SYNC();
+    }
+}
+
+// --------------------------------------------------------------------------------------------
+// training: gradients
+// --------------------------------------------------------------------------------------------
+// d loss / d raw outputs for one record; loss = -w (log q + log(c q + (1 - c) p_bsdf))
+__device__ float lossGradient(const float *o, f3 w, float pb, float wt, float *g) {
+    for (int k = 0; k < GUIDE_W; k++) g[k] = 0.f;
+    if (!(wt > 0.f)) return 0.f;
+    Mixture mx;
+    mixtureFromRaw(o, mx);
+    float fk[GUIDE_LOBES];
+    float q = 0.f;
+    for (int k = 0; k < GUIDE_LOBES; k++) {
+        fk[k] = vmfPdf(mx.mu[k], mx.kappa[k], w);
+        q += mx.pi[k] * fk[k];
+    }
+    q = fmaxf(q, 1e-12f);
+    const float c = mx.frac;
+    const float mix = fmaxf(c * q + (1.f - c) * pb, 1e-12f);
+    const float loss = -wt * (__logf(q) + __logf(mix));
+    const float dq = -wt * (1.f / q + c / mix);
+    // selection probability (sigmoid, clamped)
+    const float craw = 1.f / (1.f + __expf(-o[GO_FRAC]));
+    if (craw > GUIDE_FRAC_MIN && craw < GUIDE_FRAC_MAX) g[GO_FRAC] = -wt * (q - pb) / mix * craw * (1.f - craw);
+    for (int k = 0; k < GUIDE_LOBES; k++) {
+        // softmax logits
+        g[GO_LOGIT + k] = dq * mx.pi[k] * (fk[k] - q);
+        const float df = dq * mx.pi[k] * fk[k];  // d loss / d log f_k
+        const float kap = mx.kappa[k];
+        const float cosw = dot(mx.mu[k], w);
+        // d log f / d kappa = 1/kappa - 2 e^{-2k} / (1 - e^{-2k}) + (mu.w - 1); kappa = exp(raw)
+        const float e2 = __expf(-2.f * kap);
+        const float dlk = 1.f / kap - 2.f * e2 / (1.f - e2) + (cosw - 1.f);
+        const float lk = o[GO_KAPPA + k];
+        if (lk > GUIDE_LOGK_MIN && lk < GUIDE_LOGK_MAX) g[GO_KAPPA + k] = df * dlk * kap;
+        // d log f / d m = kappa (w - (mu.w) mu) / |m|
+        if (mx.mlen[k] > 1e-12f) {
+            const float s = df * kap / mx.mlen[k];
+            g[GO_MEAN + 3 * k] = s * (w.x - cosw * mx.mu[k].x);
+            g[GO_MEAN + 3 * k + 1] = s * (w.y - cosw * mx.mu[k].y);
+            g[GO_MEAN + 3 * k + 2] = s * (w.z - cosw * mx.mu[k].z);
+        }
+    }
+    // keep single outliers from dominating a step
+    for (int k = 0; k < GUIDE_OUT; k++) g[k] = fminf(fmaxf(g[k], -1e3f), 1e3f);
+    return loss;
+}
+
+__global__ __launch_bounds__(64) void k_guide_train(GuideNet net, const float4 *recs, const int *countPtr, const float *wsumPtr, int base,
+                                                    int batch, float *partials, float *lossOut) {
+    __shared__ __bf16 sA[2][32 * ACT_STRIDE];
+    __shared__ __bf16 sAT[GUIDE_LAYERS][GUIDE_W * T_STRIDE];
+    __shared__ __bf16 sD[2][32 * ACT_STRIDE];
+    __shared__ __bf16 sDT[GUIDE_W * T_STRIDE];
+    __shared__ float sOut[32][GUIDE_OUT + 1];
+    __shared__ float sRec[32][8];
+    const int lane = threadIdx.x;
+    float *P = partials + (size_t)blockIdx.x * GUIDE_PARAMS;
+    for (int i = lane; i < GUIDE_PARAMS; i += 64) P[i] = 0.f;
+    const int count = *countPtr;
+    const float wsum = wsumPtr ? *wsumPtr : 0.f;
+    // normalise the weights to mean one over the record buffer
+    const float wscale = (wsumPtr == nullptr) ? 1.f : (wsum > 0.f ? (float)count / wsum : 0.f);
+    float lossAcc = 0.f;
+    const int ntiles = (batch + 31) / 32;
+    for (int tile = blockIdx.x; tile < ntiles && count > 0; tile += gridDim.x) {
+        if (lane < 32) {
+            const int r = tile * 32 + lane;
+            float f[GUIDE_RAW], om[3] = {0.f, 0.f, 0.f}, pb = 0.f, wt = 0.f;
+            for (int k = 0; k < GUIDE_RAW; k++) f[k] = 0.f;
+            if (r < batch) {
+                const size_t idx = (size_t)(((int64_t)base + r) % count);
+                const float4 a = recs[idx * GUIDE_TREC_F4], b = recs[idx * GUIDE_TREC_F4 + 1];
+                const float4 c = recs[idx * GUIDE_TREC_F4 + 2], d = recs[idx * GUIDE_TREC_F4 + 3];
+                f[0] = a.x; f[1] = a.y; f[2] = a.z; f[3] = a.w;
+                f[4] = b.x; f[5] = b.y; f[6] = b.z; f[7] = b.w;
+                om[0] = c.x; om[1] = c.y; om[2] = c.z;
+                pb = c.w;
+                wt = fminf(d.w * wscale, 64.f);
+            }
+            for (int k = 0; k < GUIDE_RAW; k++) sOut[lane][k] = f[k];
+            sRec[lane][0] = om[0]; sRec[lane][1] = om[1]; sRec[lane][2] = om[2];
+            sRec[lane][3] = pb;
+            sRec[lane][4] = wt;
+        }
+        WAVE_SYNC();
+        {
+            const int row = lane & 31;
+            for (int kk = 0; kk < 4; kk++) {
+                const int k = 4 * (lane >> 5) + kk;
+                const float s = sOut[row][k];
+                for (int j = 0; j < GUIDE_BINS; j++) {
+                    const __bf16 v = (__bf16)oneBlob(s, j);
+                    sA[0][row * ACT_STRIDE + k * GUIDE_BINS + j] = v;
+                    sAT[0][(k * GUIDE_BINS + j) * T_STRIDE + row] = v;
+                }
+            }
+        }
+        WAVE_SYNC();
+        int cur = 0;
+        for (int L = 0; L < GUIDE_LAYERS; L++) {
+            for (int ct = 0; ct < 2; ct++) {
+                const f32x16 acc = layerTile(sA[cur], net.w, L, ct, lane);
+                const int col = ct * 32 + (lane & 31);
+                const float bias = net.b[L * GUIDE_W + col];
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    const int row = cRow(lane, r);
+                    const float v = acc[r] + bias;
+                    if (L < GUIDE_LAYERS - 1) {
+                        const __bf16 h = (__bf16)fmaxf(v, 0.f);
+                        sA[1 - cur][row * ACT_STRIDE + col] = h;
+                        sAT[L + 1][col * T_STRIDE + row] = h;
+                    } else if (col < GUIDE_OUT) {
+                        sOut[row][col] = v;
+                    }
+                }
+            }
+            WAVE_SYNC();
+            cur = 1 - cur;
+        }
+        if (lane < 32) {
+            float g[GUIDE_W];
+            const f3 w = mk(sRec[lane][0], sRec[lane][1], sRec[lane][2]);
+            lossAcc += lossGradient(sOut[lane], w, sRec[lane][3], sRec[lane][4], g);
+            for (int k = 0; k < GUIDE_W; k++) {
+                const __bf16 v = (__bf16)g[k];
+                sD[0][lane * ACT_STRIDE + k] = v;
+                sDT[k * T_STRIDE + lane] = v;
+            }
+        }
+        WAVE_SYNC();
+        int dcur = 0;
+        for (int L = GUIDE_LAYERS - 1; L >= 0; L--) {
+            float *PL = P + (size_t)L * GUIDE_LAYER_PARAMS;
+            // dW[out][in] += sum_rows delta[row][out] * a_L[row][in]  (K = 32 rows)
+            for (int mt = 0; mt < 2; mt++)
+                for (int nt = 0; nt < 2; nt++) {
+                    f32x16 acc = {};
+                    const __bf16 *arow = sDT + (mt * 32 + (lane & 31)) * T_STRIDE + (lane >> 5) * 8;
+                    const __bf16 *brow = sAT[L] + (nt * 32 + (lane & 31)) * T_STRIDE + (lane >> 5) * 8;
+                    acc = mfma(ld8(arow), ld8(brow), acc);
+                    acc = mfma(ld8(arow + 16), ld8(brow + 16), acc);
+                    const int in = nt * 32 + (lane & 31);
+#pragma unroll
+                    for (int r = 0; r < 16; r++) PL[(mt * 32 + cRow(lane, r)) * GUIDE_W + in] += acc[r];
+                }
+            {
+                // db[out] = sum_rows delta[row][out]
+                float s = 0.f;
+                for (int row = 0; row < 32; row++) s += (float)sD[dcur][row * ACT_STRIDE + lane];
+                PL[GUIDE_W * GUIDE_W + lane] += s;
+            }
+            WAVE_SYNC();
+            if (L > 0) {
+                // delta_{L-1}[row][in] = sum_out delta[row][out] W_L[out][in], masked by relu'
+                for (int nt = 0; nt < 2; nt++) {
+                    f32x16 acc = {};
+                    const __bf16 *arow = sD[dcur] + (lane & 31) * ACT_STRIDE + (lane >> 5) * 8;
+                    const __bf16 *brow = net.wt + (size_t)L * GUIDE_W * GUIDE_W + (nt * 32 + (lane & 31)) * GUIDE_W + (lane >> 5) * 8;
+#pragma unroll
+                    for (int ks = 0; ks < GUIDE_W / 16; ks++) acc = mfma(ld8(arow + ks * 16), ld8(brow + ks * 16), acc);
+                    const int in = nt * 32 + (lane & 31);
+#pragma unroll
+                    for (int r = 0; r < 16; r++) {
+                        const int row = cRow(lane, r);
+                        const bool on = (float)sAT[L][in * T_STRIDE + row] > 0.f;
+                        const __bf16 v = (__bf16)(on ? acc[r] : 0.f);
+                        sD[1 - dcur][row * ACT_STRIDE + in] = v;
+                    }
+                }
+                WAVE_SYNC();
+                // transposed copy of the new delta for the next weight-gradient product
+                for (int e = lane; e < 32 * GUIDE_W; e += 64) {
+                    const int row = e >> 6, in = e & 63;
+                    sDT[in * T_STRIDE + row] = sD[1 - dcur][row * ACT_STRIDE + in];
+                }
+                WAVE_SYNC();
+                dcur = 1 - dcur;
+            }
+        }
+        WAVE_SYNC();
+    }
+    // loss of this block: lanes 0..31 hold partial sums
+    for (int off = 32; off >= 1; off >>= 1) lossAcc += __shfl_down(lossAcc, off);
+    if (lane == 0 && lossOut) lossOut[blockIdx.x] = lossAcc;
+}
+
+// --------------------------------------------------------------------------------------------
+// Adam (FP32 master weights) and the BF16 copies
+// --------------------------------------------------------------------------------------------
+__device__ __forceinline__ void storeParam(int p, float v, __bf16 *w, __bf16 *wt, float *b) {
+    const int L = p / GUIDE_LAYER_PARAMS, off = p - L * GUIDE_LAYER_PARAMS;
+    if (off < GUIDE_W * GUIDE_W) {
+        const int out = off / GUIDE_W, in = off % GUIDE_W;
+        const __bf16 h = (__bf16)v;
+        w[(size_t)L * GUIDE_W * GUIDE_W + out * GUIDE_W + in] = h;
+        wt[(size_t)L * GUIDE_W * GUIDE_W + in * GUIDE_W + out] = h;
+    } else {
+        b[L * GUIDE_W + (off - GUIDE_W * GUIDE_W)] = v;
+    }
+}
+__device__ __forceinline__ bool liveParam(int p) {
+    // rows >= 48 of the last layer are padding and stay zero
+    const int L = p / GUIDE_LAYER_PARAMS, off = p - L * GUIDE_LAYER_PARAMS;
+    if (L != GUIDE_LAYERS - 1) return true;
+    const int out = off < GUIDE_W * GUIDE_W ? off / GUIDE_W : off - GUIDE_W * GUIDE_W;
+    return out < GUIDE_OUT;
+}
+
+__global__ void k_guide_pack(const float *params, __bf16 *w, __bf16 *wt, float *b) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= GUIDE_PARAMS) return;
+    storeParam(p, liveParam(p) ? params[p] : 0.f, w, wt, b);
+}
+
+__global__ void k_guide_reduce(const float *partials, int nblocks, const float *lossPart, float invBatch, float *grad, float *lossOut) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < GUIDE_PARAMS) {
+        float g = 0.f;
+        for (int k = 0; k < nblocks; k++) g += partials[(size_t)k * GUIDE_PARAMS + p];
+        grad[p] = liveParam(p) ? g * invBatch : 0.f;
+    }
+    if (p == 0 && lossOut) {
+        float l = 0.f;
+        for (int k = 0; k < nblocks; k++) l += lossPart[k];
+        lossOut[0] = l * invBatch;
+    }
+}
+
+__global__ void k_guide_adam(float *params, float *m, float *v, const float *grad, float c1, float c2, float lr, __bf16 *w, __bf16 *wt,
+                             float *b) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= GUIDE_PARAMS) return;
+    if (!liveParam(p)) return;
+    const float g = grad[p];
+    const float mm = 0.9f * m[p] + 0.1f * g;
+    const float vv = 0.999f * v[p] + 0.001f * g * g;
+    m[p] = mm;
+    v[p] = vv;
+    const float val = params[p] - lr * (mm * c1) / (sqrtf(vv * c2) + 1e-8f);
+    params[p] = val;
+    storeParam(p, val, w, wt, b);
+}
+
+__global__ void k_mfma_tile(const float *A, const float *B, float *C) {
+    // one wave: C (32 x 32) = A (32 x 16) * B (16 x 32), all row-major FP32 in and out
+    const int lane = threadIdx.x;
+    if (lane >= 64) return;
+    bf16x8 a, b;
+    for (int i = 0; i < 8; i++) {
+        const int k = (lane >> 5) * 8 + i;
+        a[i] = (__bf16)A[(lane & 31) * 16 + k];
+        b[i] = (__bf16)B[k * 32 + (lane & 31)];
+    }
+    f32x16 acc = {};
+    acc = mfma(a, b, acc);
+    for (int r = 0; r < 16; r++) C[cRow(lane, r) * 32 + (lane & 31)] = acc[r];
+}
+
+}  // namespace pt
+
+// ============================================================================================
+// host side
+// ============================================================================================
+namespace pt {
+
+// He-uniform hidden layers; the last layer starts near zero with biases that give 8 lobes
+// along the cube diagonals, kappa = 2, and a selection probability of 0.18.
+void guideInitParams(float *params, uint64_t seed) {
+    std::mt19937_64 rng(seed * 0x9E3779B97F4A7C15ULL + 12345);
+    std::uniform_real_distribution<float> u(-1.f, 1.f);
+    for (int L = 0; L < GUIDE_LAYERS; L++) {
+        float *W = params + (size_t)L * GUIDE_LAYER_PARAMS;
+        float *b = W + GUIDE_W * GUIDE_W;
+        const float lim = L < GUIDE_LAYERS - 1 ? sqrtf(6.f / GUIDE_W) : 1e-3f;
+        for (int o = 0; o < GUIDE_W; o++)
+            for (int i = 0; i < GUIDE_W; i++) W[o * GUIDE_W + i] = (L == GUIDE_LAYERS - 1 && o >= GUIDE_OUT) ? 0.f : lim * u(rng);
+        for (int o = 0; o < GUIDE_W; o++) b[o] = 0.f;
+        if (L == GUIDE_LAYERS - 1) {
+            for (int k = 0; k < GUIDE_LOBES; k++) {
+                const float s = 0.57735027f;
+                b[GO_MEAN + 3 * k] = (k & 1) ? s : -s;
+                b[GO_MEAN + 3 * k + 1] = (k & 2) ? s : -s;
+                b[GO_MEAN + 3 * k + 2] = (k & 4) ? s : -s;
+                b[GO_KAPPA + k] = logf(2.f);
+            }
+            b[GO_FRAC] = -1.5f;
+        }
+    }
+}
+
+void guideUploadBf16(const float *dParams, __bf16 *dW, __bf16 *dWt, float *dB, hipStream_t s) {
+    k_guide_pack<<<(GUIDE_PARAMS + 255) / 256, 256, 0, s>>>(dParams, dW, dWt, dB);
+    PT_HIP_TRY(hipGetLastError());
+}
+
+void guideInfer(const PtSceneView &sc, GuideSceneNorm norm, GuideNet net, const int *count, const float4 *rayO, const float4 *rayD,
+                const float4 *hits, int maxRows, float4 *out, int cus, hipStream_t s) {
+    const int grid = std::max(1, std::min((maxRows + 32 * INFER_WAVES - 1) / (32 * INFER_WAVES), cus * 4));
+    k_guide_infer<<<grid, 64 * INFER_WAVES, 0, s>>>(sc, norm, net, count, rayO, rayD, hits, maxRows, out, nullptr, nullptr);
+    PT_HIP_TRY(hipGetLastError());
+}
+
+void guideInferRaw(GuideNet net, const float *feats, int n, float *rawOut, hipStream_t s) {
+    PtSceneView sc{};
+    GuideSceneNorm nm{};
+    const int grid = std::max(1, (n + 32 * INFER_WAVES - 1) / (32 * INFER_WAVES));
+    k_guide_infer<<<grid, 64 * INFER_WAVES, 0, s>>>(sc, nm, net, nullptr, nullptr, nullptr, nullptr, n, nullptr, feats, rawOut);
+    PT_HIP_TRY(hipGetLastError());
+}
+
+void guideGradients(GuideNet net, const float4 *records, const int *count, int base, int batch, const float *wsum, float *partials, int blocks,
+                    float *lossPart, float *grad, float *lossOut, hipStream_t s) {
+    k_guide_train<<<blocks, 64, 0, s>>>(net, records, count, wsum, base, batch, partials, lossPart);
+    k_guide_reduce<<<(GUIDE_PARAMS + 255) / 256, 256, 0, s>>>(partials, blocks, lossPart, 1.f / (float)std::max(batch, 1), grad, lossOut);
+    PT_HIP_TRY(hipGetLastError());
+}
+
+void guideAdam(float *params, float *m, float *v, const float *grad, int step, float lr, __bf16 *dW, __bf16 *dWt, float *dB, hipStream_t s) {
+    const float c1 = 1.f / (1.f - powf(0.9f, (float)step));
+    const float c2 = 1.f / (1.f - powf(0.999f, (float)step));
+    k_guide_adam<<<(GUIDE_PARAMS + 255) / 256, 256, 0, s>>>(params, m, v, grad, c1, c2, lr, dW, dWt, dB);
+    PT_HIP_TRY(hipGetLastError());
+}
+
+void mfmaTile(const float *A, const float *B, float *C, hipStream_t s) {
+    k_mfma_tile<<<1, 64, 0, s>>>(A, B, C);
+    PT_HIP_TRY(hipGetLastError());
+}
+
+}  // namespace pt

@@ -33,6 +33,7 @@
 
 #include "pt_device.h"
 #include "pt_gpu.h"
+#include "pt_guide.h"
 #include "pt_internal.h"
 
 namespace pt {
@@ -42,7 +43,10 @@ static constexpr int kMaxBounces = 64;
 // from shade, radiance adds from shadow
 enum { C_RAYS = 0, C_SHADOW = 1, C_FETCH_EXT = 2, C_FETCH_SHADOW = 3, C_LADD_SHADE = 4, C_LADD_SHADOW = 5, C_PER_BOUNCE = 6 };
 static constexpr int kNumCounters = C_PER_BOUNCE * kMaxBounces + 8;
-static constexpr int kOverflowCounter = C_PER_BOUNCE * kMaxBounces;
+// guided mode: training records appended, records kept (capped), sum of their weights (float)
+static constexpr int kRecCountSlot = C_PER_BOUNCE * kMaxBounces + 1;
+static constexpr int kRecValidSlot = C_PER_BOUNCE * kMaxBounces + 2;
+static constexpr int kWsumSlot = C_PER_BOUNCE * kMaxBounces + 3;
 // device statistics (uint64): per bounce rays, shadows, shade adds, shadow adds; totals
 enum { S_RAYS = 0, S_SHADOW = kMaxBounces, S_LADD_SHADE = 2 * kMaxBounces, S_LADD_SHADOW = 3 * kMaxBounces, S_PATHS = 4 * kMaxBounces, S_COUNT };
 
@@ -58,6 +62,7 @@ struct ChunkParams {
     int nPaths;       // npix * samples
     int maxDepth;
     int nee;
+    int misSample;    // 1: light-sample MIS weight uses the BSDF sampling density (consistent MIS)
     int W, H;
     uint64_t seed;
 };
@@ -361,9 +366,42 @@ __device__ __forceinline__ float pdfEnvLight(const PtSceneView &sc, f3 w) {
     return envPdf(sc.env, w) * (1.f / (float)(sc.numEmitters + 1));
 }
 
+// Guided-vertex inputs of the shade kernel (ignored by the plain integrator)
+struct GuideShadeArgs {
+    const float4 *mix;      // GUIDE_REC_F4 float4 per queue slot (inference output)
+    float4 *records;        // GUIDE_TREC_F4 float4 per training record
+    int recCap;
+    const int *recIn;       // training record of the vertex that spawned each queued ray (-1: none)
+    int *recOut;
+    GuideSceneNorm norm;
+};
+
+__device__ __forceinline__ float lum(float4 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+__device__ __forceinline__ float lum(f3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+
+__device__ __forceinline__ float mixturePdf(const float4 *G, f3 w) {
+    const float4 p0 = G[8], p1 = G[9];
+    const float pi[GUIDE_LOBES] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
+    float q = 0.f;
+#pragma unroll
+    for (int k = 0; k < GUIDE_LOBES; k++) {
+        const float4 l = G[k];
+        q += pi[k] * vmfDensity(xyz(l), l.w, w);
+    }
+    return q;
+}
+
+// One path vertex per lane.  Plain (GUIDED = false): the reference's NEE + MIS estimator
+// (path_tracer.cu:142-184) with the BSDF-sampled emitter term folded into the next vertex's
+// emission, or the naive estimator (:186-233).  Guided: at Lambert and Beckmann vertices the
+// direction comes from the one-sample mixture c q + (1 - c) p_bsdf of the learned vMF lobes and
+// the BSDF, the continuation is weighted by f cos / (c q + (1 - c) p_bsdf), and the same mixture
+// density is the directional technique's pdf in both MIS weights.
+template <bool GUIDED>
 __global__ __launch_bounds__(256) void k_shade(PtSceneView sc, ChunkParams cp, int bounce, int *counters, const float4 *rayO, const float4 *rayD,
                                                const uint4 *state, const float4 *beta, const float4 *hits, float4 *nRayO, float4 *nRayD,
-                                               uint4 *nState, float4 *nBeta, float4 *sO, float4 *sD, float4 *sL, float4 *Lpath) {
+                                               uint4 *nState, float4 *nBeta, float4 *sO, float4 *sD, float4 *sL, float4 *Lpath,
+                                               GuideShadeArgs ga) {
     int *c = counters + C_PER_BOUNCE * bounce;
     int *cNext = counters + C_PER_BOUNCE * (bounce + 1);
     const int n = c[C_RAYS];
@@ -371,8 +409,9 @@ __global__ __launch_bounds__(256) void k_shade(PtSceneView sc, ChunkParams cp, i
     for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n; base += waveStride) {
         const int i = base + laneId();
         const bool valid = i < n;
-        bool pushRay = false, pushShadow = false, addL = false;
+        bool pushRay = false, pushShadow = false, addL = false, wantRec = false;
         float4 outRO, outRD, outB, shO, shD, shL;
+        float4 rec0, rec1, rec2, rec3;
         uint4 outS;
         f3 L = mk(0.f);
         int pathId = 0;
@@ -383,6 +422,12 @@ __global__ __launch_bounds__(256) void k_shade(PtSceneView sc, ChunkParams cp, i
             const int depth = (int)st.w;
             const f3 o = xyz(ro), d = xyz(rd), thr = xyz(bp);
             const float prevPdf = bp.w;
+            if constexpr (GUIDED) {
+                // radiance of the path before this vertex: the baseline of the training record of
+                // the vertex that sampled this ray
+                const int rin = ga.recIn[i];
+                if (rin >= 0) ((float *)&ga.records[(size_t)rin * GUIDE_TREC_F4 + 3])[2] = lum(Lpath[pathId]);
+            }
             const int slot = asInt(h.w);
             if (slot == INT32_MIN) {
                 // miss: environment (camera miss: environment_light.h:68-83 as is)
@@ -407,10 +452,48 @@ __global__ __launch_bounds__(256) void k_shade(PtSceneView sc, ChunkParams cp, i
                     const int pixel = cp.pixBegin + (pathId - s * cp.npix);
                     rng.inc = streamInc(cp.seed, (uint32_t)pixel, (uint32_t)(cp.sampleBase + s));
                     rng.state = ((uint64_t)st.y << 32) | st.x;
-                    const int dims = bsdfDims(type);
-                    const float x1 = dims >= 1 ? rng.uniform() : 0.f;
-                    const float x2 = dims >= 2 ? rng.uniform() : 0.f;
-                    const BsdfSample bs = bsdfSample(m, wo, x1, x2);
+                    const float4 *G = nullptr;
+                    float frac = 0.f;
+                    if constexpr (GUIDED) {
+                        G = ga.mix + (size_t)i * GUIDE_REC_F4;
+                        const float4 g10 = G[10];
+                        if (g10.y > 0.f) frac = g10.x;
+                        else G = nullptr;
+                    }
+                    BsdfSample bs;
+                    float pdfDir;  // density of the direction sampling technique at bs.wi
+                    if (GUIDED && G) {
+                        const float usel = rng.uniform();
+                        if (usel < frac) {
+                            const float4 p0 = G[8], p1 = G[9];
+                            const float pi[GUIDE_LOBES] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
+                            const float ul = rng.uniform();
+                            int k = 0;
+                            float acc = pi[0];
+                            while (k < GUIDE_LOBES - 1 && ul >= acc) acc += pi[++k];
+                            const float u1 = rng.uniform(), u2 = rng.uniform();
+                            const float4 lobe = G[k];
+                            const f3 ww = sampleVmf(xyz(lobe), lobe.w, u1, u2);
+                            bs.wi = toLocal(fr, ww);
+                            bs.f = bsdfF(m, wo, bs.wi);
+                            bs.delta = false;
+                        } else {
+                            const int dims = bsdfDims(type);
+                            const float x1 = dims >= 1 ? rng.uniform() : 0.f;
+                            const float x2 = dims >= 2 ? rng.uniform() : 0.f;
+                            bs = bsdfSample(m, wo, x1, x2);
+                        }
+                        const float pb = bsdfSamplePdf(m, wo, bs.wi);
+                        pdfDir = frac * mixturePdf(G, toWorld(fr, bs.wi)) + (1.f - frac) * pb;
+                        bs.pdf = pdfDir;
+                        rec2.w = pb;
+                    } else {
+                        const int dims = bsdfDims(type);
+                        const float x1 = dims >= 1 ? rng.uniform() : 0.f;
+                        const float x2 = dims >= 2 ? rng.uniform() : 0.f;
+                        bs = bsdfSample(m, wo, x1, x2);
+                        pdfDir = bs.pdf;
+                    }
                     if (cp.nee && !bs.delta) {
                         // directSampleLights (path_tracer.cu:83-126); a scene without lights
                         // draws the three uniforms and adds nothing
@@ -420,7 +503,9 @@ __global__ __launch_bounds__(256) void k_shade(PtSceneView sc, ChunkParams cp, i
                         if (anyLight) ls = sampleLights(sc, sp.p, l1, l2, l3);
                         if (anyLight && dot(ls.n, ls.wi) < 0.f) {
                             const f3 wiL = toLocal(fr, ls.wi);
-                            const float pB = bsdfPdf(m, wo, wiL);
+                            float pB;
+                            if (GUIDED && G) pB = frac * mixturePdf(G, ls.wi) + (1.f - frac) * bsdfSamplePdf(m, wo, wiL);
+                            else pB = cp.misSample ? bsdfSamplePdf(m, wo, wiL) : bsdfPdf(m, wo, wiL);
                             const float w = balance(ls.pdf, pB);
                             const f3 contrib = thr * (ls.emitted * w * bsdfF(m, wo, wiL) * fabsf(dot(sp.n, ls.wi)) / ls.pdf);
                             if (!isZero(contrib) && !(contrib.x != contrib.x)) {
@@ -433,12 +518,26 @@ __global__ __launch_bounds__(256) void k_shade(PtSceneView sc, ChunkParams cp, i
                     }
                     if (!isZero(bs.f) && bs.pdf > 0.f) {
                         const f3 nb = thr * (bs.f * fabsf(bs.wi.z) / bs.pdf);
-                        if (!isZero(nb)) {
+                        if (!isZero(nb) && nb.x == nb.x) {
                             pushRay = true;
+                            const f3 wworld = toWorld(fr, bs.wi);
                             outRO = f4(sp.p, kBounceEps);
-                            outRD = f4(toWorld(fr, bs.wi), FLT_MAX);
+                            outRD = f4(wworld, FLT_MAX);
                             outS = make_uint4((uint32_t)rng.state, (uint32_t)(rng.state >> 32), (uint32_t)pathId, (uint32_t)(depth + 1));
-                            outB = f4(nb, bs.delta ? -1.f : bs.pdf);
+                            outB = f4(nb, bs.delta ? -1.f : pdfDir);
+                            if (GUIDED && G) {
+                                const float lt = lum(thr);
+                                if (lt > 0.f) {
+                                    float f[GUIDE_RAW];
+                                    const float rough = type == PT_MAT_LAMBERTIAN ? 1.f : fminf(m.b.w * (1.f / 0.6f), 1.f);
+                                    guideFeatures(ga.norm, sp.p, sp.n, -d, rough, f);
+                                    rec0 = make_float4(f[0], f[1], f[2], f[3]);
+                                    rec1 = make_float4(f[4], f[5], f[6], f[7]);
+                                    rec2 = f4(wworld, rec2.w);
+                                    rec3 = make_float4(1.f / lt, asFloat(pathId), -1.f, 0.f);
+                                    wantRec = true;
+                                }
+                            }
                         }
                     }
                 }
@@ -457,6 +556,21 @@ __global__ __launch_bounds__(256) void k_shade(PtSceneView sc, ChunkParams cp, i
             nState[rs] = outS;
             nBeta[rs] = outB;
         }
+        if constexpr (GUIDED) {
+            const int slotRec = waveAppend(wantRec, &counters[kRecCountSlot]);
+            if (pushRay) {
+                int ridx = -1;
+                if (wantRec && slotRec < ga.recCap) {
+                    ridx = slotRec;
+                    float4 *R = ga.records + (size_t)slotRec * GUIDE_TREC_F4;
+                    R[0] = rec0;
+                    R[1] = rec1;
+                    R[2] = rec2;
+                    R[3] = rec3;
+                }
+                ga.recOut[rs] = ridx;
+            }
+        }
         const int ss = waveAppend(pushShadow, &c[C_SHADOW]);
         if (pushShadow) {
             sO[ss] = shO;
@@ -465,6 +579,25 @@ __global__ __launch_bounds__(256) void k_shade(PtSceneView sc, ChunkParams cp, i
         }
         waveCount(addL, &c[C_LADD_SHADE]);
     }
+}
+
+// Training targets: weight = (radiance after the vertex's sampled direction) / luminance of the
+// throughput at the vertex, i.e. the MC estimate of f cos L_i / p for the sampled direction.
+__global__ void k_guide_finalize(float4 *records, int recCap, int *counters, const float4 *Lpath) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = min(counters[kRecCountSlot], recCap);
+    if (i == 0) counters[kRecValidSlot] = n;
+    float w = 0.f;
+    if (i < n) {
+        float4 *r = records + (size_t)i * GUIDE_TREC_F4 + 3;
+        const float4 v = *r;
+        const int pid = asInt(v.y);
+        if (v.z >= 0.f) w = fmaxf(0.f, lum(Lpath[pid]) - v.z) * v.x;
+        if (!(w == w) || isinf(w)) w = 0.f;
+        r->w = w;
+    }
+    for (int off = 32; off >= 1; off >>= 1) w += __shfl_down(w, off);
+    if (laneId() == 0 && w != 0.f) atomicAdd((float *)&counters[kWsumSlot], w);
 }
 
 // Per pixel of the chunk: FP64 sum over the chunk's sample rows (framebuffer.cu:7-41 keeps a
@@ -561,6 +694,10 @@ __global__ void k_zero_d(double *p, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0;
 }
+__global__ void k_fill_i32(int *p, int n, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
 __global__ void k_zero_u64(unsigned long long *p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0ull;
@@ -574,8 +711,8 @@ __global__ void k_zero_u64(unsigned long long *p, int n) {
 using namespace pt;
 
 namespace {
-enum { K_RAYGEN = 0, K_EXTEND, K_SHADE, K_SHADOW, K_ACCUM, K_TONEMAP, K_NUM };
-const char *kKernelNames[K_NUM] = {"raygen", "extend", "shade", "shadow", "accumulate", "tonemap"};
+enum { K_RAYGEN = 0, K_EXTEND, K_SHADE, K_SHADOW, K_ACCUM, K_TONEMAP, K_GUIDE_INFER, K_GUIDE_TRAIN, K_NUM };
+const char *kKernelNames[K_NUM] = {"raygen", "extend", "shade", "shadow", "accumulate", "tonemap", "guide_infer", "guide_train"};
 }  // namespace
 
 struct pt_renderer {
@@ -622,6 +759,19 @@ struct pt_renderer {
     pt_kernel_stat kstat[K_NUM] = {};
     double samplesAccumulated = 0.0, pixelsAccumulated = 0.0;
     std::vector<ChunkParams> lastChunks;
+    int misSample = 0;
+    // guiding network and its training state
+    bool guideReady = false;
+    DevBuf gParams, gM, gV, gGrad, gPartials, gLossPart, gLoss, gW, gWt, gB, gMix, gRecords, gRecIdx[2];
+    int64_t gRecCap = 0;
+    int gTrainSteps = 2;
+    int gBatch = 1 << 18;
+    float gLr = 1e-2f;
+    int gAdamStep = 0;
+    int64_t gTrainTotal = 0;
+    uint64_t gSeed = 1;
+    float sceneLo[3] = {0.f, 0.f, 0.f}, sceneHi[3] = {1.f, 1.f, 1.f};
+    double trainMs = 0.0;
 };
 
 namespace {
@@ -756,13 +906,69 @@ void launchShadow(pt_renderer *r, const int *count, int *fetch, int *ladd) {
     PT_HIP_TRY(hipGetLastError());
 }
 
+void ensureGuide(pt_renderer *r) {
+    if (r->guideReady && r->gRecCap == 2 * r->pool) return;
+    const size_t np = GUIDE_PARAMS;
+    if (!r->guideReady) {
+        r->gParams.alloc(np * 4);
+        r->gM.alloc(np * 4);
+        r->gV.alloc(np * 4);
+        r->gGrad.alloc(np * 4);
+        r->gPartials.alloc((size_t)r->cus * np * 4);
+        r->gLossPart.alloc((size_t)r->cus * 4);
+        r->gLoss.alloc(4);
+        r->gW.alloc(GUIDE_LAYERS * GUIDE_W * GUIDE_W * 2);
+        r->gWt.alloc(GUIDE_LAYERS * GUIDE_W * GUIDE_W * 2);
+        r->gB.alloc(GUIDE_LAYERS * GUIDE_W * 4);
+        std::vector<float> init(np);
+        guideInitParams(init.data(), r->gSeed);
+        std::vector<float> zero(np, 0.f);
+        PT_HIP_TRY(hipMemcpyAsync(r->gParams.p, init.data(), np * 4, hipMemcpyHostToDevice, r->stream));
+        PT_HIP_TRY(hipMemcpyAsync(r->gM.p, zero.data(), np * 4, hipMemcpyHostToDevice, r->stream));
+        PT_HIP_TRY(hipMemcpyAsync(r->gV.p, zero.data(), np * 4, hipMemcpyHostToDevice, r->stream));
+        guideUploadBf16(r->gParams.as<float>(), r->gW.as<__bf16>(), r->gWt.as<__bf16>(), r->gB.as<float>(), r->stream);
+        PT_HIP_TRY(hipStreamSynchronize(r->stream));
+        r->gAdamStep = 0;
+    }
+    r->gMix.alloc((size_t)r->pool * GUIDE_REC_F4 * sizeof(float4));
+    r->gRecCap = 2 * r->pool;
+    r->gRecords.alloc((size_t)r->gRecCap * GUIDE_TREC_F4 * sizeof(float4));
+    r->gRecIdx[0].alloc((size_t)r->pool * 4);
+    r->gRecIdx[1].alloc((size_t)r->pool * 4);
+    r->guideReady = true;
+}
+
+GuideNet guideNet(pt_renderer *r) {
+    GuideNet net;
+    net.w = r->gW.as<__bf16>();
+    net.wt = r->gWt.as<__bf16>();
+    net.b = r->gB.as<float>();
+    return net;
+}
+
+GuideSceneNorm guideNorm(pt_renderer *r) {
+    GuideSceneNorm nm;
+    for (int k = 0; k < 3; k++) {
+        const float e = r->sceneHi[k] - r->sceneLo[k];
+        nm.lo[k] = r->sceneLo[k];
+        nm.invExt[k] = e > 0.f ? 1.f / e : 0.f;
+    }
+    return nm;
+}
+
 void runChunk(pt_renderer *r, const ChunkParams &cp) {
     int *ctr = r->counters.as<int>();
+    const bool guided = r->type == PT_RENDERER_GUIDED;
     {
         KTimer t(r, K_RAYGEN);
         k_raygen<<<(cp.nPaths + 255) / 256, 256, 0, r->stream>>>(cp, r->cam, r->rayO[0].as<float4>(), r->rayD[0].as<float4>(), r->st[0].as<uint4>(),
                                                                 r->beta[0].as<float4>(), r->Lpath.as<float4>(), ctr);
         PT_HIP_TRY(hipGetLastError());
+        if (guided) {
+            // camera rays were not spawned by a guided vertex
+            k_fill_i32<<<(cp.nPaths + 255) / 256, 256, 0, r->stream>>>(r->gRecIdx[0].as<int>(), cp.nPaths, -1);
+            PT_HIP_TRY(hipGetLastError());
+        }
     }
     const int shadeGrid = (int)std::min<int64_t>((cp.nPaths + 255) / 256, (int64_t)r->cus * r->shadeBlocksPerCU);
     int cur = 0;
@@ -772,12 +978,32 @@ void runChunk(pt_renderer *r, const ChunkParams &cp) {
             KTimer t(r, K_EXTEND);
             launchExtend(r, &c[C_RAYS], &c[C_FETCH_EXT], r->rayO[cur].as<float4>(), r->rayD[cur].as<float4>(), r->hits.as<float4>());
         }
+        GuideShadeArgs ga{};
+        if (guided && b + 1 < cp.maxDepth) {
+            KTimer t(r, K_GUIDE_INFER);
+            guideInfer(r->view, guideNorm(r), guideNet(r), &c[C_RAYS], r->rayO[cur].as<float4>(), r->rayD[cur].as<float4>(), r->hits.as<float4>(),
+                       cp.nPaths, r->gMix.as<float4>(), r->cus, r->stream);
+        }
+        if (guided) {
+            ga.mix = r->gMix.as<float4>();
+            ga.records = r->gRecords.as<float4>();
+            ga.recCap = (int)r->gRecCap;
+            ga.recIn = r->gRecIdx[cur].as<int>();
+            ga.recOut = r->gRecIdx[1 - cur].as<int>();
+            ga.norm = guideNorm(r);
+        }
         {
             KTimer t(r, K_SHADE);
-            k_shade<<<shadeGrid, 256, 0, r->stream>>>(r->view, cp, b, ctr, r->rayO[cur].as<float4>(), r->rayD[cur].as<float4>(), r->st[cur].as<uint4>(),
-                                                      r->beta[cur].as<float4>(), r->hits.as<float4>(), r->rayO[1 - cur].as<float4>(),
-                                                      r->rayD[1 - cur].as<float4>(), r->st[1 - cur].as<uint4>(), r->beta[1 - cur].as<float4>(),
-                                                      r->sO.as<float4>(), r->sD.as<float4>(), r->sL.as<float4>(), r->Lpath.as<float4>());
+            if (guided)
+                k_shade<true><<<shadeGrid, 256, 0, r->stream>>>(
+                    r->view, cp, b, ctr, r->rayO[cur].as<float4>(), r->rayD[cur].as<float4>(), r->st[cur].as<uint4>(), r->beta[cur].as<float4>(),
+                    r->hits.as<float4>(), r->rayO[1 - cur].as<float4>(), r->rayD[1 - cur].as<float4>(), r->st[1 - cur].as<uint4>(),
+                    r->beta[1 - cur].as<float4>(), r->sO.as<float4>(), r->sD.as<float4>(), r->sL.as<float4>(), r->Lpath.as<float4>(), ga);
+            else
+                k_shade<false><<<shadeGrid, 256, 0, r->stream>>>(
+                    r->view, cp, b, ctr, r->rayO[cur].as<float4>(), r->rayD[cur].as<float4>(), r->st[cur].as<uint4>(), r->beta[cur].as<float4>(),
+                    r->hits.as<float4>(), r->rayO[1 - cur].as<float4>(), r->rayD[1 - cur].as<float4>(), r->st[1 - cur].as<uint4>(),
+                    r->beta[1 - cur].as<float4>(), r->sO.as<float4>(), r->sD.as<float4>(), r->sL.as<float4>(), r->Lpath.as<float4>(), ga);
             PT_HIP_TRY(hipGetLastError());
         }
         if (cp.nee) {
@@ -791,6 +1017,20 @@ void runChunk(pt_renderer *r, const ChunkParams &cp) {
         k_accumulate<<<(cp.npix + 255) / 256, 256, 0, r->stream>>>(cp, r->Lpath.as<float4>(), r->accum.as<double>(), ctr,
                                                                   r->stats.as<unsigned long long>());
         PT_HIP_TRY(hipGetLastError());
+    }
+    if (guided && r->gTrainSteps > 0) {
+        KTimer t(r, K_GUIDE_TRAIN);
+        const int nrec = (int)r->gRecCap;
+        k_guide_finalize<<<(nrec + 255) / 256, 256, 0, r->stream>>>(r->gRecords.as<float4>(), nrec, ctr, r->Lpath.as<float4>());
+        PT_HIP_TRY(hipGetLastError());
+        for (int s = 0; s < r->gTrainSteps; s++) {
+            const int base = (int)((r->gTrainTotal * (int64_t)r->gBatch) & 0x3fffffff);
+            guideGradients(guideNet(r), r->gRecords.as<float4>(), &ctr[kRecValidSlot], base, r->gBatch, (const float *)&ctr[kWsumSlot],
+                           r->gPartials.as<float>(), r->cus, r->gLossPart.as<float>(), r->gGrad.as<float>(), r->gLoss.as<float>(), r->stream);
+            guideAdam(r->gParams.as<float>(), r->gM.as<float>(), r->gV.as<float>(), r->gGrad.as<float>(), ++r->gAdamStep, r->gLr,
+                      r->gW.as<__bf16>(), r->gWt.as<__bf16>(), r->gB.as<float>(), r->stream);
+            r->gTrainTotal++;
+        }
     }
 }
 
@@ -957,6 +1197,10 @@ int pt_renderer_set_scene(pt_renderer *r, const pt_scene *s) {
     r->nodes.alloc((size_t)std::max<int64_t>(n - 1, 1) * 4 * sizeof(float4));
     r->normals.alloc(std::max<size_t>(nrm.size(), 1) * sizeof(float4));
     if (!nrm.empty()) PT_HIP_TRY(hipMemcpyAsync(r->normals.p, nrm.data(), nrm.size() * sizeof(float4), hipMemcpyHostToDevice, r->stream));
+    for (int k = 0; k < 3; k++) {
+        r->sceneLo[k] = std::isfinite(lo[k]) ? lo[k] : 0.f;
+        r->sceneHi[k] = std::isfinite(hi[k]) ? hi[k] : 1.f;
+    }
     buildBvh(r->primsIn.as<float4>(), (int)n, lo, hi, r->nodes.as<float4>(), r->prims.as<float4>(), r->stream, &r->bvh);
     r->primsIn.release();
     const int need = r->bvh.depth + 1;
@@ -1058,7 +1302,6 @@ int pt_renderer_set_nee(pt_renderer *r, int on) {
 int pt_renderer_set_type(pt_renderer *r, int t) {
     if (guard("pt_renderer_set_type", r)) return PT_ERR_INVALID;
     if (t < PT_RENDERER_CUDA || t > PT_RENDERER_GUIDED) return fail(PT_ERR_INVALID, "pt_renderer_set_type: unknown renderer type");
-    if (t == PT_RENDERER_GUIDED) return fail(PT_ERR_STATE, "pt_renderer_set_type: the guided integrator is not built into this library");
     r->type = t;
     r->resetPending = true;
     return PT_OK;
@@ -1087,6 +1330,24 @@ int pt_renderer_set_option(pt_renderer *r, int opt, int64_t v) {
     case PT_OPT_TRACE_BLOCKS_PER_CU: r->traceBlocksPerCU = v > 0 ? (int)std::min<int64_t>(v, 64) : 8; break;
     case PT_OPT_SHADE_BLOCKS_PER_CU: r->shadeBlocksPerCU = v > 0 ? (int)std::min<int64_t>(v, 64) : 8; break;
     case PT_OPT_PROFILE_KERNELS: r->profile = v != 0; break;
+    case PT_OPT_CONSISTENT_MIS: r->misSample = v != 0; r->resetPending = true; break;
+    case PT_OPT_GUIDE_TRAIN_STEPS: r->gTrainSteps = (int)std::max<int64_t>(0, std::min<int64_t>(v, 1024)); break;
+    case PT_OPT_GUIDE_BATCH:
+        if (v < 32 || v > (1 << 22)) return fail(PT_ERR_INVALID, "pt_renderer_set_option: guide batch must be in [32, 2^22]");
+        r->gBatch = (int)v;
+        break;
+    case PT_OPT_GUIDE_LR_MICRO:
+        if (v <= 0) return fail(PT_ERR_INVALID, "pt_renderer_set_option: learning rate must be positive");
+        r->gLr = (float)v * 1e-6f;
+        break;
+    case PT_OPT_GUIDE_RESET:
+        // re-initialise the network (seed = value) at the next guided pass
+        PT_HIP_TRY(hipStreamSynchronize(r->stream));
+        r->gSeed = (uint64_t)v;
+        r->guideReady = false;
+        r->gTrainTotal = 0;
+        r->resetPending = true;
+        break;
     default: return fail(PT_ERR_INVALID, "pt_renderer_set_option: unknown option");
     }
     return PT_OK;
@@ -1100,6 +1361,11 @@ int pt_renderer_get_option(pt_renderer *r, int opt, int64_t *v) {
     case PT_OPT_TRACE_BLOCKS_PER_CU: *v = r->traceBlocksPerCU; break;
     case PT_OPT_SHADE_BLOCKS_PER_CU: *v = r->shadeBlocksPerCU; break;
     case PT_OPT_PROFILE_KERNELS: *v = r->profile ? 1 : 0; break;
+    case PT_OPT_CONSISTENT_MIS: *v = r->misSample; break;
+    case PT_OPT_GUIDE_TRAIN_STEPS: *v = r->gTrainSteps; break;
+    case PT_OPT_GUIDE_BATCH: *v = r->gBatch; break;
+    case PT_OPT_GUIDE_LR_MICRO: *v = (int64_t)llroundf(r->gLr * 1e6f); break;
+    case PT_OPT_GUIDE_RESET: *v = (int64_t)r->gTrainTotal; break;
     default: return fail(PT_ERR_INVALID, "pt_renderer_get_option: unknown option");
     }
     return PT_OK;
@@ -1130,6 +1396,7 @@ int pt_renderer_render_async(pt_renderer *r, int spp) {
         r->passSpp = spp;
     } else {
         ensurePool(r);
+        if (r->type == PT_RENDERER_GUIDED) ensureGuide(r);
         const int64_t pool = r->pool;
         if (npix <= pool) {
             const int rowsPerChunk = (int)std::min<int64_t>(spp, pool / npix);
@@ -1142,6 +1409,7 @@ int pt_renderer_render_async(pt_renderer *r, int spp) {
                 cp.nPaths = cp.npix * cp.samples;
                 cp.maxDepth = r->maxDepth;
                 cp.nee = r->nee;
+                cp.misSample = r->misSample;
                 cp.W = r->W;
                 cp.H = r->H;
                 cp.seed = r->seed;
@@ -1158,6 +1426,7 @@ int pt_renderer_render_async(pt_renderer *r, int spp) {
                     cp.nPaths = cp.npix;
                     cp.maxDepth = r->maxDepth;
                     cp.nee = r->nee;
+                    cp.misSample = r->misSample;
                     cp.W = r->W;
                     cp.H = r->H;
                     cp.seed = r->seed;
@@ -1331,6 +1600,12 @@ int pt_renderer_get_stats(pt_renderer *r, pt_render_stats *out) {
     out->bvh_depth = r->bvh.depth;
     out->bvh_nodes = r->bvh.numNodes;
     out->stack_overflows = ov;
+    out->guide_steps = r->gTrainTotal;
+    if (r->guideReady && r->gTrainTotal > 0) {
+        float l = 0.f;
+        PT_HIP_TRY(hipMemcpy(&l, r->gLoss.p, sizeof(float), hipMemcpyDeviceToHost));
+        out->guide_loss = l;
+    }
     return PT_OK;
     PT_API_END("pt_renderer_get_stats")
 }

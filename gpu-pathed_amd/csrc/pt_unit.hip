@@ -13,6 +13,7 @@
 
 #include "pt_device.h"
 #include "pt_gpu.h"
+#include "pt_guide.h"
 #include "pt_internal.h"
 
 namespace pt {
@@ -271,6 +272,89 @@ int pt_unit_path_uniforms(uint64_t seed, const uint32_t *pixel, const uint32_t *
     st.sync();
     return PT_OK;
     PT_UNIT_END("pt_unit_path_uniforms")
+}
+
+namespace {
+// BF16 copies of a parameter vector on the device
+struct NetStage {
+    DevBuf params, w, wt, b;
+    GuideNet net;
+    NetStage(const float *host, hipStream_t s) {
+        params.alloc(GUIDE_PARAMS * sizeof(float));
+        w.alloc(GUIDE_LAYERS * GUIDE_W * GUIDE_W * sizeof(__bf16));
+        wt.alloc(GUIDE_LAYERS * GUIDE_W * GUIDE_W * sizeof(__bf16));
+        b.alloc(GUIDE_LAYERS * GUIDE_W * sizeof(float));
+        PT_HIP_TRY(hipMemcpyAsync(params.p, host, GUIDE_PARAMS * sizeof(float), hipMemcpyHostToDevice, s));
+        guideUploadBf16(params.as<float>(), w.as<__bf16>(), wt.as<__bf16>(), b.as<float>(), s);
+        net.w = w.as<__bf16>();
+        net.wt = wt.as<__bf16>();
+        net.b = b.as<float>();
+    }
+};
+}  // namespace
+
+int pt_unit_guide_forward(const float *params, const float *features, int64_t n, float *out) {
+    if (!params || n < 0 || (n > 0 && (!features || !out)) || n > (1 << 24)) return fail(PT_ERR_INVALID, "pt_unit_guide_forward: bad arguments");
+    PT_UNIT_BEGIN("pt_unit_guide_forward")
+    Stage st;
+    NetStage ns(params, st.s);
+    const float *df = st.in(features, (size_t)n * GUIDE_RAW);
+    float *dout = st.out<float>((size_t)n * GUIDE_OUT);
+    if (n > 0) guideInferRaw(ns.net, df, (int)n, dout, st.s);
+    st.back(out, dout, (size_t)n * GUIDE_OUT);
+    st.sync();
+    return PT_OK;
+    PT_UNIT_END("pt_unit_guide_forward")
+}
+
+int pt_unit_guide_gradients(const float *params, const float *records, int64_t n, float *grad, float *loss) {
+    if (!params || !grad || n <= 0 || !records || n > (1 << 24)) return fail(PT_ERR_INVALID, "pt_unit_guide_gradients: bad arguments");
+    PT_UNIT_BEGIN("pt_unit_guide_gradients")
+    Stage st;
+    NetStage ns(params, st.s);
+    const float4 *dr = (const float4 *)st.in(records, (size_t)n * 16);
+    const int cnt = (int)n;
+    const int *dcount = st.in(&cnt, 1);
+    const int blocks = std::min(deviceCUs(), (int)((n + 31) / 32));
+    float *partials = st.out<float>((size_t)blocks * GUIDE_PARAMS);
+    float *lossPart = st.out<float>((size_t)blocks);
+    float *dgrad = st.out<float>(GUIDE_PARAMS);
+    float *dloss = st.out<float>(1);
+    guideGradients(ns.net, dr, dcount, 0, cnt, nullptr, partials, blocks, lossPart, dgrad, dloss, st.s);
+    st.back(grad, dgrad, GUIDE_PARAMS);
+    if (loss) st.back(loss, dloss, 1);
+    st.sync();
+    return PT_OK;
+    PT_UNIT_END("pt_unit_guide_gradients")
+}
+
+int pt_unit_guide_adam(float *params, float *m, float *v, const float *grad, int step, float lr) {
+    if (!params || !m || !v || !grad || step < 1) return fail(PT_ERR_INVALID, "pt_unit_guide_adam: bad arguments");
+    PT_UNIT_BEGIN("pt_unit_guide_adam")
+    Stage st;
+    NetStage ns(params, st.s);
+    float *dm = st.in(m, GUIDE_PARAMS), *dv = st.in(v, GUIDE_PARAMS);
+    const float *dg = st.in(grad, GUIDE_PARAMS);
+    guideAdam(ns.params.as<float>(), dm, dv, dg, step, lr, ns.w.as<__bf16>(), ns.wt.as<__bf16>(), ns.b.as<float>(), st.s);
+    st.back(params, ns.params.as<float>(), GUIDE_PARAMS);
+    st.back(m, dm, GUIDE_PARAMS);
+    st.back(v, dv, GUIDE_PARAMS);
+    st.sync();
+    return PT_OK;
+    PT_UNIT_END("pt_unit_guide_adam")
+}
+
+int pt_unit_mfma_tile(const float *a, const float *b, float *c) {
+    if (!a || !b || !c) return fail(PT_ERR_INVALID, "pt_unit_mfma_tile: bad arguments");
+    PT_UNIT_BEGIN("pt_unit_mfma_tile")
+    Stage st;
+    const float *da = st.in(a, 32 * 16), *db = st.in(b, 16 * 32);
+    float *dc = st.out<float>(32 * 32);
+    mfmaTile(da, db, dc, st.s);
+    st.back(c, dc, 32 * 32);
+    st.sync();
+    return PT_OK;
+    PT_UNIT_END("pt_unit_mfma_tile")
 }
 
 }  // extern "C"

@@ -15,6 +15,7 @@ PT_OK = 0
 LAMBERTIAN, MIRROR, GLASS, MICROFACET = 0, 1, 2, 3
 RENDERER_CUDA, RENDERER_OPTIX, RENDERER_NORMALS, RENDERER_GUIDED = 0, 1, 2, 3
 OPT_POOL_PATHS, OPT_TRACE_BLOCKS_PER_CU, OPT_SHADE_BLOCKS_PER_CU, OPT_PROFILE_KERNELS = 0, 1, 2, 3
+OPT_CONSISTENT_MIS, OPT_GUIDE_TRAIN_STEPS, OPT_GUIDE_BATCH, OPT_GUIDE_LR_MICRO, OPT_GUIDE_RESET = 4, 5, 6, 7, 8
 MISS = np.iinfo(np.int32).min
 
 # every symbol declared in include/pathed.h
@@ -29,8 +30,9 @@ EXPORTS = [
     "pt_renderer_hit_test", "pt_renderer_get_stats", "pt_renderer_reset_stats", "pt_renderer_get_kernel_stats",
     "pt_renderer_get_path_radiance", "pt_renderer_intersect",
     "pt_unit_triangle_test", "pt_unit_bsdf_eval", "pt_unit_bsdf_sample", "pt_unit_env_tables", "pt_unit_env_sample",
-    "pt_unit_path_uniforms",
+    "pt_unit_path_uniforms", "pt_unit_guide_forward", "pt_unit_guide_gradients", "pt_unit_guide_adam", "pt_unit_mfma_tile",
 ]
+GUIDE_PARAMS = 20800
 
 
 class PtError(RuntimeError):
@@ -51,7 +53,7 @@ class RenderStats(C.Structure):
     _fields_ = [
         ("rays_total", C.c_int64), ("extension_rays", C.c_int64), ("shadow_rays", C.c_int64), ("paths", C.c_int64),
         ("passes", C.c_int64), ("gpu_ms", C.c_double), ("bvh_depth", C.c_int), ("bvh_nodes", C.c_int64),
-        ("stack_overflows", C.c_int),
+        ("stack_overflows", C.c_int), ("guide_steps", C.c_int64), ("guide_loss", C.c_double),
     ]
 
 
@@ -116,6 +118,10 @@ def lib():
         L.pt_unit_env_sample.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, C.c_int64, vp, vp, vp, vp, C.c_int64, vp]
         L.pt_unit_path_uniforms.argtypes = [C.c_uint64, vp, vp, C.c_int64, C.c_int, vp]
         L.pt_device_count.argtypes = [C.POINTER(C.c_int)]
+        L.pt_unit_guide_forward.argtypes = [vp, vp, C.c_int64, vp]
+        L.pt_unit_guide_gradients.argtypes = [vp, vp, C.c_int64, vp, vp]
+        L.pt_unit_guide_adam.argtypes = [vp, vp, vp, vp, C.c_int, C.c_float]
+        L.pt_unit_mfma_tile.argtypes = [vp, vp, vp]
     return _lib
 
 
@@ -258,6 +264,11 @@ class Renderer:
     def set_option(self, opt, v):
         check(lib().pt_renderer_set_option(self.h, opt, v))
 
+    def get_option(self, opt):
+        v = C.c_int64()
+        check(lib().pt_renderer_get_option(self.h, opt, C.byref(v)))
+        return v.value
+
     def render_async(self, spp):
         check(lib().pt_renderer_render_async(self.h, spp))
 
@@ -391,3 +402,37 @@ def path_uniforms(seed, pixels, samples, k):
     out = np.zeros((p.size, k), np.float32)
     check(lib().pt_unit_path_uniforms(seed, _p(p), _p(s), p.size, k, _p(out)))
     return out
+
+
+def guide_forward(params, feats):
+    p = _f32(params)
+    f = _f32(feats).reshape(-1, 8)
+    out = np.zeros((f.shape[0], 48), np.float32)
+    check(lib().pt_unit_guide_forward(_p(p), _p(f), f.shape[0], _p(out)))
+    return out
+
+
+def guide_gradients(params, records):
+    p = _f32(params)
+    r = _f32(records).reshape(-1, 16)
+    g = np.zeros(GUIDE_PARAMS, np.float32)
+    loss = np.zeros(1, np.float32)
+    check(lib().pt_unit_guide_gradients(_p(p), _p(r), r.shape[0], _p(g), _p(loss)))
+    return float(loss[0]), g
+
+
+def guide_adam(params, m, v, grad, step, lr):
+    p = _f32(params).copy()
+    mm = _f32(m).copy()
+    vv = _f32(v).copy()
+    g = _f32(grad)
+    check(lib().pt_unit_guide_adam(_p(p), _p(mm), _p(vv), _p(g), step, lr))
+    return p, mm, vv
+
+
+def mfma_tile(a, b):
+    a = _f32(a).reshape(32, 16)
+    b = _f32(b).reshape(16, 32)
+    c = np.zeros((32, 32), np.float32)
+    check(lib().pt_unit_mfma_tile(_p(a), _p(b), _p(c)))
+    return c

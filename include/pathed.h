@@ -155,7 +155,15 @@ typedef enum {
     PT_OPT_POOL_PATHS = 0,
     PT_OPT_TRACE_BLOCKS_PER_CU = 1,
     PT_OPT_SHADE_BLOCKS_PER_CU = 2,
-    PT_OPT_PROFILE_KERNELS = 3   /* 1: time each kernel launch with HIP events */
+    PT_OPT_PROFILE_KERNELS = 3,  /* 1: time each kernel launch with HIP events */
+    PT_OPT_CONSISTENT_MIS = 4,   /* 1: the light-sample MIS weight uses the BSDF sampling density
+                                    (the reference uses Microfacet::pdf, microfacet.h:124-134,
+                                    whose widened alpha differs from sample()'s) */
+    PT_OPT_GUIDE_TRAIN_STEPS = 5,/* guided: Adam steps after each chunk (default 2) */
+    PT_OPT_GUIDE_BATCH = 6,      /* guided: training records per step (default 2^18) */
+    PT_OPT_GUIDE_LR_MICRO = 7,   /* guided: Adam learning rate in units of 1e-6 (default 10000) */
+    PT_OPT_GUIDE_RESET = 8       /* guided: re-initialise the network with seed = value;
+                                    get: number of training steps taken */
 } pt_option;
 int pt_renderer_set_option(pt_renderer *r, int option, int64_t value);
 int pt_renderer_get_option(pt_renderer *r, int option, int64_t *value);
@@ -193,6 +201,8 @@ typedef struct {
     int bvh_depth;
     int64_t bvh_nodes;
     int stack_overflows;       /* traversal stack overflow events (must stay 0) */
+    int64_t guide_steps;       /* guided: training steps taken since the network was created */
+    double guide_loss;         /* guided: mean loss of the last training step */
 } pt_render_stats;
 int pt_renderer_get_stats(pt_renderer *r, pt_render_stats *stats);
 int pt_renderer_reset_stats(pt_renderer *r);
@@ -250,6 +260,20 @@ int pt_unit_env_sample(const float *rgba, int width, int height, int sin_theta_w
 /* PCG32 path streams on the GPU: first k uniforms of (seed, pixel, sample) for n queries. */
 int pt_unit_path_uniforms(uint64_t seed, const uint32_t *pixel, const uint32_t *sample, int64_t n,
                           int k, float *out);
+
+/* Guiding network on the GPU (BF16 MFMA, FP32 accumulate).  params: PT_GUIDE_PARAMS floats,
+ * 5 layers of (W[64 out][64 in], b[64]); the last layer's rows 48..63 are padding.
+ * forward: features[8n] raw inputs in [0, 1] -> out[48n] raw outputs.
+ * gradients: records[16n] = features[8], omega[3], bsdf pdf, -, -, -, weight -> grad of the
+ * mean loss (PT_GUIDE_PARAMS floats) and the mean loss.
+ * adam: one Adam step (0.9, 0.999, 1e-8) in place on params/m/v.
+ * mfma_tile: c[32x32] = a[32x16] x b[16x32] (row-major) through one v_mfma_f32_32x32x16_bf16,
+ * the lane layout the network kernels rely on. */
+#define PT_GUIDE_PARAMS 20800
+int pt_unit_guide_forward(const float *params, const float *features, int64_t n, float *out);
+int pt_unit_guide_gradients(const float *params, const float *records, int64_t n, float *grad, float *loss);
+int pt_unit_guide_adam(float *params, float *m, float *v, const float *grad, int step, float lr);
+int pt_unit_mfma_tile(const float *a, const float *b, float *c);
 
 #ifdef __cplusplus
 }

@@ -48,6 +48,7 @@ class RenderParams(C.Structure):
         ("use_bvh", C.c_int), ("precision", C.c_int), ("threads", C.c_int),
         ("seed", C.c_uint64), ("sample_start", C.c_int64), ("spp", C.c_int),
         ("row_begin", C.c_int), ("row_end", C.c_int), ("pixel_stride", C.c_int),
+        ("mis_sample", C.c_int),
     ]
 
 
@@ -145,7 +146,7 @@ class Scene:
             pass
 
     def params(self, spp=1, max_depth=6, nee=1, mode=0, tri_test=0, ref_env=1, precision=32,
-               threads=0, seed=0, sample_start=0, row_begin=0, row_end=0, pixel_stride=1):
+               threads=0, seed=0, sample_start=0, row_begin=0, row_end=0, pixel_stride=1, mis_sample=0):
         p = RenderParams()
         p.width, p.height = self.width, self.height
         p.max_depth, p.nee, p.mode, p.tri_test, p.ref_env = max_depth, nee, mode, tri_test, ref_env
@@ -153,6 +154,7 @@ class Scene:
         p.precision, p.threads, p.seed = precision, threads, seed
         p.sample_start, p.spp = sample_start, spp
         p.row_begin, p.row_end, p.pixel_stride = row_begin, row_end, pixel_stride
+        p.mis_sample = mis_sample
         return p
 
     def render(self, want_sq=False, **kw):
@@ -189,7 +191,10 @@ def tri_test(tris, rays, mode):
     return out
 
 
-def bsdf_eval(mat, wo, wi, precision=32):
+def bsdf_eval(mat, wo, wi, precision=32, sample_pdf=False):
+    """f and pdf() at (wo, wi); sample_pdf=True returns the density sample() draws with instead."""
+    if sample_pdf:
+        precision += 1
     m = _f32(mat).reshape(-1, 6)
     o = _f32(wo).reshape(-1, 3)
     i = _f32(wi).reshape(-1, 3)

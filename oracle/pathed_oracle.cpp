@@ -276,6 +276,26 @@ template <class T> inline T bsdfPdf(const Material<T> &m, const V3<T> &wo, const
     default: return T(0);
     }
 }
+// Density of the directions bsdfSample() draws, at an arbitrary wi.  For the Beckmann lobe
+// sample() draws the half vector from the alpha widened at |cos wo| in the upper hemisphere
+// and reflects wo about it (microfacet.h:53-63, 148-158), so p(wi) = D(wh) |cos wh| / (4 |wi.wh|)
+// with wh = +-normalize(wo + wi) taken upward; this differs from pdf() (microfacet.h:124-134),
+// which widens alpha at |cos wi|.  The consistent-MIS option weights light samples with it.
+template <class T> inline T bsdfSamplePdf(const Material<T> &m, const V3<T> &wo, const V3<T> &wi) {
+    switch (m.type) {
+    case MAT_LAMBERTIAN: return uniformHemispherePdf(wi);
+    case MAT_MICROFACET: {
+        const V3<T> s = wo + wi;
+        if (s.isZero()) return T(0);
+        V3<T> wh = normalized(s);
+        if (wh.z < T(0)) wh = -wh;
+        const T d = absDot(wi, wh);
+        if (!(d > T(0))) return T(0);
+        return mfD(m.alpha, wo, wh) * std::fabs(wh.z) / (T(4) * d);
+    }
+    default: return T(0);
+    }
+}
 // Number of uniforms each material's sample() consumes (lambertian.h:55-59, mirror.h:37-43,
 // glass.h:35-37, microfacet.h:136-140).
 inline int bsdfSampleDims(int type) {
@@ -1048,6 +1068,7 @@ struct Opts {
     int mode;      // 0 = reference (re-traced BSDF ray), 1 = merged (product wavefront)
     int triTest;   // 0 = MT, 1 = watertight
     int refEnv;    // 1 = CMF texel-centre env sampling (reference), 0 = alias (product)
+    int misSample; // 1 = light-sample MIS weight uses bsdfSamplePdf (weights sum to one)
 };
 
 struct RayCount { int64_t rays = 0; };
@@ -1086,7 +1107,7 @@ template <class T> static V3<T> Li(const Scene<T> &s, V3<T> o, V3<T> d, Pcg32 &r
                     rc.rays++;
                     if (!worldHit(s, rec.point, ls.wi, (T)1e-4, ls.distance - (T)2e-4, op.triTest, true, &oh)) {
                         const V3<T> wiL = fr.toLocal(ls.wi);
-                        const T pB = bsdfPdf(m, rec.wo, wiL);
+                        const T pB = op.misSample ? bsdfSamplePdf(m, rec.wo, wiL) : bsdfPdf(m, rec.wo, wiL);
                         const T w = balance(ls.pdf, pB);
                         direct += V3<T>(T(1)) * ls.emitted * w * bsdfF(m, rec.wo, wiL) * absDot(rec.normal, ls.wi) / ls.pdf;
                     }
@@ -1198,6 +1219,7 @@ typedef struct {
     int spp;
     int row_begin, row_end;  // rows [begin, end) (end <= 0: all)
     int pixel_stride;        // render every k-th pixel of each row (>= 1)
+    int mis_sample;          // 1: consistent MIS (light weight from the sampling density)
 } or_render_params;
 
 }  // extern "C"
@@ -1271,7 +1293,7 @@ extern "C" void *or_scene_create(const or_scene_desc *d, int width, int height, 
 extern "C" void or_scene_destroy(void *h) { delete (OrHandle *)h; }
 
 template <class T> static void renderImpl(const Scene<T> &s, const or_render_params *p, double *sum, double *sumsq, int64_t *rays) {
-    Opts op{p->max_depth, p->nee, p->mode, p->tri_test, p->ref_env};
+    Opts op{p->max_depth, p->nee, p->mode, p->tri_test, p->ref_env, p->mis_sample};
     const int W = p->width, H = p->height;
     const int rb = p->row_begin, re = p->row_end > 0 ? p->row_end : H;
     const int stride = p->pixel_stride > 0 ? p->pixel_stride : 1;
@@ -1326,7 +1348,7 @@ int or_render(void *hv, const or_render_params *p, double *sum, double *sumsq, i
 // Radiance of single paths (pixel, sample) in FP32/FP64 -> out[3n]
 int or_trace_paths(void *hv, const or_render_params *p, const int *pix, const int *smp, int64_t n, double *out) {
     OrHandle *h = (OrHandle *)hv;
-    Opts op{p->max_depth, p->nee, p->mode, p->tri_test, p->ref_env};
+    Opts op{p->max_depth, p->nee, p->mode, p->tri_test, p->ref_env, p->mis_sample};
     RayCount rc;
     for (int64_t i = 0; i < n; i++) {
         const int row = pix[i] / p->width, col = pix[i] % p->width;
@@ -1384,23 +1406,24 @@ int or_tri_test(const float *tris, const float *rays, int64_t n, int tri_test, f
     return 0;
 }
 
-// BSDF queries in FP32 (precision 32) or FP64.  mat: type, albedo[3], ior, alpha per entry
-// (6 floats).  wo/wi: 3 floats each.  out_f[3n], out_pdf[n].
+// BSDF queries in FP32 (precision 32) or FP64 (64).  mat: type, albedo[3], ior, alpha per entry
+// (6 floats).  wo/wi: 3 floats each.  out_f[3n], out_pdf[n]: pdf() for precision 32/64, the
+// sampling density bsdfSamplePdf for precision 33/65.
 int or_bsdf_eval(const float *mat, const float *wo, const float *wi, int64_t n, int precision, double *out_f, double *out_pdf) {
     for (int64_t i = 0; i < n; i++) {
         const float *m = mat + i * 6;
-        if (precision == 64) {
+        if (precision == 64 || precision == 65) {
             Material<double> M{(int)m[0], V3<double>(m[1], m[2], m[3]), V3<double>(0.0), (double)m[4], (double)m[5]};
             const V3<double> o(wo[i * 3], wo[i * 3 + 1], wo[i * 3 + 2]), w(wi[i * 3], wi[i * 3 + 1], wi[i * 3 + 2]);
             const V3<double> f = bsdfF(M, o, w);
             out_f[i * 3] = f.x; out_f[i * 3 + 1] = f.y; out_f[i * 3 + 2] = f.z;
-            out_pdf[i] = bsdfPdf(M, o, w);
+            out_pdf[i] = precision == 65 ? bsdfSamplePdf(M, o, w) : bsdfPdf(M, o, w);
         } else {
             Material<float> M{(int)m[0], V3<float>(m[1], m[2], m[3]), V3<float>(0.f), m[4], m[5]};
             const V3<float> o(wo[i * 3], wo[i * 3 + 1], wo[i * 3 + 2]), w(wi[i * 3], wi[i * 3 + 1], wi[i * 3 + 2]);
             const V3<float> f = bsdfF(M, o, w);
             out_f[i * 3] = f.x; out_f[i * 3 + 1] = f.y; out_f[i * 3 + 2] = f.z;
-            out_pdf[i] = bsdfPdf(M, o, w);
+            out_pdf[i] = precision == 33 ? bsdfSamplePdf(M, o, w) : bsdfPdf(M, o, w);
         }
     }
     return 0;

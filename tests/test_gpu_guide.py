@@ -1,0 +1,101 @@
+"""GPU parity of the guiding network's BF16 MFMA kernels against the numpy oracle."""
+import numpy as np
+import pytest
+
+import guide_oracle as G
+import pathed_capi as P
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if P.device_count() == 0:
+        pytest.fail("no HIP device visible: the GPU tests need an MI355X")
+
+
+def test_mfma_lane_layout():
+    """One v_mfma_f32_32x32x16_bf16 with the operand/result lane layout the kernels use."""
+    print("case: mfma tile", flush=True)
+    rng = np.random.default_rng(0)
+    a = G.bf16(rng.normal(size=(32, 16)))
+    b = G.bf16(rng.normal(size=(16, 32)))
+    c = P.mfma_tile(a, b)
+    assert np.allclose(c, a @ b, rtol=1e-5, atol=1e-5)
+
+
+def _params(rng):
+    p = G.init_params(5)
+    last = (G.LAYERS - 1) * G.LAYER_PARAMS
+    Wl = p[last:last + 64 * 64].reshape(64, 64)
+    Wl[:48] = rng.uniform(-0.3, 0.3, (48, 64))
+    return p.astype(np.float32).astype(np.float64)
+
+
+def test_forward_matches_oracle():
+    print("case: guide forward", flush=True)
+    rng = np.random.default_rng(1)
+    p = _params(rng)
+    feats = rng.random((1000, 8)).astype(np.float32)
+    g = P.guide_forward(p, feats)
+    ref, _ = G.forward(p, feats.astype(np.float64), emulate_bf16=True)
+    ref32, _ = G.forward(p, feats.astype(np.float64))
+    scale = np.abs(ref32).max()
+    # BF16 activations: within the oracle's own BF16 emulation up to accumulation order
+    assert np.abs(g - ref).max() <= 2e-2 * scale
+    assert np.abs(g - ref32).max() <= 5e-2 * scale
+
+
+def test_gradients_match_oracle():
+    print("case: guide gradients", flush=True)
+    rng = np.random.default_rng(2)
+    p = _params(rng)
+    n = 4096
+    feats = rng.random((n, 8))
+    w = rng.normal(size=(n, 3))
+    w /= np.linalg.norm(w, axis=1, keepdims=True)
+    pb = rng.uniform(0.05, 0.5, n)
+    wt = rng.uniform(0.1, 3.0, n)
+    rec = np.zeros((n, 16), np.float32)
+    rec[:, :8] = feats
+    rec[:, 8:11] = w
+    rec[:, 11] = pb
+    rec[:, 15] = wt
+    loss, g = P.guide_gradients(p, rec)
+    lref, gref = G.gradients(p, rec[:, :8].astype(np.float64), rec[:, 8:11].astype(np.float64), rec[:, 11].astype(np.float64),
+                             rec[:, 15].astype(np.float64), emulate_bf16=True)
+    assert loss == pytest.approx(lref, rel=1e-2)
+    # per layer: relative L2 error of the gradient and cosine similarity
+    for L in range(G.LAYERS):
+        sl = slice(L * G.LAYER_PARAMS, (L + 1) * G.LAYER_PARAMS)
+        a, b = g[sl].astype(np.float64), gref[sl]
+        rel = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
+        cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30)
+        print(f"layer {L}: rel L2 {rel:.4f} cos {cos:.6f}", flush=True)
+        assert rel < 0.03 and cos > 0.999
+    # and against the exact FP64 gradient
+    _, g64 = G.gradients(p, rec[:, :8].astype(np.float64), rec[:, 8:11].astype(np.float64), rec[:, 11].astype(np.float64),
+                         rec[:, 15].astype(np.float64))
+    cos = g @ g64 / (np.linalg.norm(g) * np.linalg.norm(g64))
+    assert cos > 0.995
+
+
+def test_adam_matches_oracle():
+    print("case: guide adam", flush=True)
+    rng = np.random.default_rng(3)
+    p = _params(rng)
+    m = np.zeros(G.PARAMS)
+    v = np.zeros(G.PARAMS)
+    for step in (1, 2, 3):
+        grad = rng.normal(size=G.PARAMS) * 0.1
+        last = (G.LAYERS - 1) * G.LAYER_PARAMS
+        grad[last + 48 * 64:last + 64 * 64] = 0
+        grad[last + 64 * 64 + 48:] = 0
+        gp, gm, gv = P.guide_adam(p, m, v, grad, step, 1e-2)
+        rp, rm, rv = G.adam(p, m, v, grad.astype(np.float32).astype(np.float64), step, 1e-2)
+        live = np.ones(G.PARAMS, bool)
+        live[last + 48 * 64:last + 64 * 64] = False
+        live[last + 64 * 64 + 48:] = False
+        assert np.allclose(gp[live], rp[live], rtol=1e-5, atol=1e-6)
+        assert np.allclose(gm[live], rm[live], rtol=1e-5, atol=1e-9)
+        p, m, v = gp.astype(np.float64), gm.astype(np.float64), gv.astype(np.float64)

@@ -233,6 +233,63 @@ def test_microfacet_sample_matches_its_pdf(alpha, cos_o):
     assert np.sum(Dn * c * np.sin(th)) * (np.pi / 2 / 200000) * 2 * np.pi == pytest.approx(1.0, rel=1e-6)
 
 
+@pytest.mark.parametrize("alpha,cos_o", [(0.1, 0.9), (0.3, 0.5), (0.5, 0.2)])
+def test_microfacet_sampling_density(alpha, cos_o):
+    """The density the consistent-MIS weights use: at the directions sample() draws it equals
+    the pdf sample() returns, it integrates to one over the sphere (sample() may reflect below
+    the horizon), and it differs from pdf(), which widens alpha at |cos wi|."""
+    rng = np.random.default_rng(7)
+    n = 20000
+    wo1 = np.array([math.sqrt(1 - cos_o**2), 0, cos_o])
+    wo = np.tile(wo1, (n, 1)).astype(np.float32)
+    m = np.array([mat(MICRO, alpha=alpha)] * n, np.float32)
+    s = O.bsdf_sample(m, wo, rng.random((n, 2)).astype(np.float32), 64)
+    _, ps = O.bsdf_eval(m, wo, s[:, :3], precision=64, sample_pdf=True)
+    # the half vector is ill-conditioned where wi approaches -wo (|wo + wi| -> 0)
+    ok = (s[:, 3] > 1e-6) & (np.linalg.norm(s[:, :3] + wo1, axis=1) > 0.05)
+    assert np.allclose(ps[ok], s[ok, 3], rtol=1e-4)
+    # integral over the sphere by stratified uniform directions: E[4 pi p(w)] = 1
+    k = 1000
+    u = (np.arange(k)[:, None] + rng.random((k, 400))) / k
+    z = (1 - 2 * u).reshape(-1)
+    ph = rng.random(z.size) * 2 * np.pi
+    r = np.sqrt(1 - z * z)
+    w = np.stack([r * np.cos(ph), r * np.sin(ph), z], 1)
+    mm = np.array([mat(MICRO, alpha=alpha)] * z.size, np.float32)
+    wwo = np.tile(wo1, (z.size, 1))
+    _, pa = O.bsdf_eval(mm, wwo, w, precision=64, sample_pdf=True)
+    integral = 4 * np.pi * pa.mean()
+    se = 4 * np.pi * pa.std() / math.sqrt(pa.size)
+    assert abs(integral - 1) < max(5 * se, 2e-3), (integral, se)
+    _, pr = O.bsdf_eval(m, wo, s[:, :3], precision=64)
+    assert np.abs(pr[ok] / ps[ok] - 1).max() > 1e-3
+
+
+def test_consistent_mis_matches_naive_on_glossy_floor():
+    """With the light-sample MIS weight taken from the sampling density the weights of the two
+    techniques sum to one, so NEE+MIS converges to the naive estimator's image on a Beckmann
+    floor lit by a small quad."""
+    tris = []
+    f = np.array([(-2, 0, -2), (2, 0, -2), (2, 0, 2), (-2, 0, 2)], np.float64)
+    tris += [np.concatenate([f[0], f[2], f[1]]), np.concatenate([f[0], f[3], f[2]])]
+    L = np.array([(-0.25, 1.0, -0.25), (0.25, 1.0, -0.25), (0.25, 1.0, 0.25), (-0.25, 1.0, 0.25)])
+    tris += [np.concatenate([L[0], L[1], L[2]]), np.concatenate([L[0], L[2], L[3]])]
+    arr = _scene_arrays(tris=np.array(tris),
+                        mats=[dict(type=3, alpha=0.3), dict(type=0, albedo=(0.5, 0.5, 0.5), emit=(8, 8, 8))],
+                        cam=dict(origin=(0, 1.2, 3.0), target=(0, 0, 0), up=(0, 1, 0), vfov=0.6))
+    arr["tri_mat"] = np.array([0, 0, 1, 1], np.int32)
+    s = O.Scene(arr, 16, 16)
+    spp = 256
+    r = {}
+    for nee, mis in ((0, 0), (1, 1)):
+        tot, sq, _ = s.render(want_sq=True, spp=spp, max_depth=3, nee=nee, mode=1, tri_test=1, precision=64, mis_sample=mis)
+        r[nee] = (tot / spp, (sq / spp - (tot / spp) ** 2) / spp)
+    d = r[1][0].sum(axis=(0, 1)) - r[0][0].sum(axis=(0, 1))
+    se = np.sqrt(r[1][1].sum(axis=(0, 1)) + r[0][1].sum(axis=(0, 1)))
+    assert np.all(np.abs(d) < 4.5 * se), (d, se)
+    assert r[0][0].sum() > 0
+
+
 # ----------------------------------------------------------------------------- intersection
 def _tri_rays(rng, n):
     tris = rng.uniform(-1, 1, (n, 9)).astype(np.float32)
