@@ -33,38 +33,114 @@ def log(msg):
 
 def cpu_baseline(scene_arrays, budget_s):
     """The oracle (scalar C++ restatement of the reference's shading and intersection, with
-    a median-split BVH) on the host cores: 1 spp over a band of rows of the same image,
-    sized to about `budget_s` seconds."""
+    a median-split BVH) on the host cores: a band of rows of the same image at as many
+    samples per pixel as fit in about `budget_s` seconds."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np  # noqa
     import oracle as O
 
     threads = max(1, min(16, os.cpu_count() or 1))
     t0 = time.time()
     o = O.Scene(scene_arrays, WIDTH, HEIGHT, bvh=True)
     log(f"cpu baseline: oracle scene + BVH built in {time.time() - t0:.1f}s")
-    kw = dict(spp=1, max_depth=DEPTH, nee=1, mode=1, tri_test=1, ref_env=0, precision=32, threads=threads)
-    # probe 8 rows of the image centre, then size the band to the budget
+    kw = dict(max_depth=DEPTH, nee=1, mode=1, tri_test=1, ref_env=0, precision=32, threads=threads)
+    # probe 16 rows of the image centre at 1 spp, then size rows x spp to the budget
     t0 = time.time()
-    _, _, rays = o.render(row_begin=508, row_end=516, **kw)
-    dt = max(time.time() - t0, 1e-3)
-    rows = int(min(HEIGHT, max(8, 8 * budget_s / dt)))
+    o.render(row_begin=504, row_end=520, spp=1, **kw)
+    per_row = max(time.time() - t0, 1e-3) / 16
+    rows = int(min(HEIGHT, max(16, budget_s / per_row)))
+    spp = int(max(1, min(SPP, budget_s / (per_row * rows))))
     rb = max(0, HEIGHT // 2 - rows // 2)
     t0 = time.time()
-    _, _, rays = o.render(row_begin=rb, row_end=rb + rows, **kw)
+    _, _, rays = o.render(row_begin=rb, row_end=rb + rows, spp=spp, **kw)
     dt = time.time() - t0
     return {
         "value": rays / dt / 1e6,
         "unit": "Mrays/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"1 spp over rows [{rb}, {rb + rows}) of the 1024x1024 image ({rows * WIDTH} paths, {rays} rays, {dt:.1f} s), "
-                  f"same scene, depth 8, NEE+MIS, FP32, median-split BVH",
+        "sample": f"{spp} spp over rows [{rb}, {rb + rows}) of the {WIDTH}x{HEIGHT} image ({rows * WIDTH * spp} paths, {rays} rays, "
+                  f"{dt:.1f} s), same scene, depth {DEPTH}, NEE+MIS, FP32, median-split BVH, {threads} threads",
+    }
+
+
+ROOM_W, ROOM_H, ROOM_DEPTH = 2048, 1024, 16
+ROOM_REFERENCE = os.path.join(ROOT, "tests", "golden", "room_reference_256px.npz")
+
+
+def room_equal_time(P, budget_s, spp_per_pass):
+    """Hard indirect lighting, equal wall time: the closed room of 4M triangles lit through a
+    0.3-wide slot (2048x1024, depth 16, NEE + consistent MIS) rendered for `budget_s` seconds
+    by the plain and by the guided renderer (network trained from scratch inside the budget).
+    Mean squared error at a fixed seeded subset of 256 pixels against the FP64 oracle's
+    65,536-sample estimate there (the committed reference fixture)."""
+    import numpy as np
+
+    ref = np.load(ROOM_REFERENCE)
+    scene = P.Scene.preset(2, seed=0)
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    from make_room_reference import scene_fingerprint
+
+    fp = scene_fingerprint(scene.export())
+    if not np.allclose(fp, ref["fingerprint"], rtol=1e-9):
+        raise SystemExit("bench.py: the room scene differs from the one the reference fixture was rendered for")
+    pix = ref["pixels"]
+    ref_mean = ref["sum"] / ref["spp"]
+    ref_var = (ref["sumsq"] / ref["spp"] - ref_mean**2) / ref["spp"]
+    res = {}
+    for mode in ("plain", "guided"):
+        r = P.Renderer(ROOM_W, ROOM_H)
+        r.set_scene(scene)
+        r.set_max_depth(ROOM_DEPTH)
+        r.set_nee(True)
+        r.set_option(P.OPT_CONSISTENT_MIS, 1)
+        if mode == "guided":
+            r.set_type(P.RENDERER_GUIDED)
+        # load the kernels, then start again from nothing: fresh accumulation, fresh network
+        r.render(1)
+        if mode == "guided":
+            r.set_option(P.OPT_GUIDE_RESET, 1)
+        r.reset()
+        r.reset_stats()
+        r.synchronize()
+        t0 = time.perf_counter()
+        curve = []
+        while True:
+            r.render(spp_per_pass)
+            el = time.perf_counter() - t0
+            if el >= budget_s:
+                break
+            if len(curve) < 64 and r.spp() & (r.spp() - 1) == 0:
+                img = r.radiance_sum().reshape(-1, 3)[pix] / r.spp()
+                curve.append((round(el, 3), r.spp(), float(((img - ref_mean) ** 2).mean())))
+                t0 += time.perf_counter() - t0 - el  # the error readback is not render time
+        el = time.perf_counter() - t0
+        spp = r.spp()
+        img = r.radiance_sum().reshape(-1, 3)[pix] / spp
+        st = r.stats()
+        mse = float(((img - ref_mean) ** 2).mean())
+        res[mode] = {"mse": mse, "rel_mse": float((((img - ref_mean) ** 2) / (ref_mean**2 + 1e-2)).mean()),
+                     "spp": spp, "seconds": round(el, 3), "mrays_s": round(st.rays_total / el / 1e6, 2),
+                     "guide_steps": int(st.guide_steps), "curve": curve}
+        log(f"room {mode}: {spp} spp in {el:.1f}s, MSE {mse:.5g}, {res[mode]['mrays_s']} Mrays/s")
+        r.close()
+    return {
+        "metric": "equal-time MSE ratio (plain / guided)", "value": round(res["plain"]["mse"] / res["guided"]["mse"], 3),
+        "unit": "x", "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"closed room, {scene.info().n_triangles} tris, 0.3-wide slot, light 1e3, glass sphere r 0.5, "
+                               f"2048x1024 env, {ROOM_W}x{ROOM_H}, depth {ROOM_DEPTH}, NEE + consistent MIS, "
+                               f"{budget_s:.0f} s per renderer",
+                   "pixels": int(pix.size), "reference_spp": int(ref["spp"]),
+                   "reference_noise_mse": float(ref_var.mean())},
+        "plain": res["plain"], "guided": res["guided"],
     }
 
 
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=["icosphere", "room"], default="icosphere",
+                    help="icosphere: Mrays/s of the 1M-triangle render (default); room: equal-time MSE, guided vs plain")
+    ap.add_argument("--budget", type=float, default=20.0, help="room: seconds per renderer")
+    ap.add_argument("--spp-per-pass", type=int, default=1, help="room: samples per pixel per progressive pass")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
@@ -94,6 +170,9 @@ def main():
     if P.device_count() == 0:
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "room":
+        print(json.dumps(room_equal_time(P, args.budget, args.spp_per_pass)), flush=True)
+        return
     t0 = time.time()
     scene = P.Scene.preset(1, seed=0, param=args.level)
     info = scene.info()

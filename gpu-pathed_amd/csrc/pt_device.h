@@ -337,7 +337,11 @@ PT_INLINE float envPdf(const PtEnv &e, f3 w) {
     envAngles(w, &phi, &theta);
     int x, y;
     envTexel(e, phi, theta, &x, &y);
-    return e.texelPdf[(size_t)y * e.width + x] / sinf(theta);
+    // sin(theta) from the xz components keeps its precision next to the poles, where y rounds
+    // to +-1; on a pole the (phi, theta) map degenerates: density 0 there, and a light sample
+    // that lands on one is dropped (pbrt-v3 InfiniteAreaLight::Pdf_Li)
+    const float st = sqrtf(w.x * w.x + w.z * w.z);
+    return st > 0.f ? e.texelPdf[(size_t)y * e.width + x] / st : 0.f;
 }
 PT_INLINE int aliasDraw(const float2 *tab, int n, float xi, float *rem) {
     const float s = xi * (float)n;

@@ -501,7 +501,7 @@ __global__ __launch_bounds__(256) void k_shade(PtSceneView sc, ChunkParams cp, i
                         const bool anyLight = sc.numEmitters > 0 || sc.env.present;
                         LightSample ls;
                         if (anyLight) ls = sampleLights(sc, sp.p, l1, l2, l3);
-                        if (anyLight && dot(ls.n, ls.wi) < 0.f) {
+                        if (anyLight && dot(ls.n, ls.wi) < 0.f && ls.pdf > 0.f) {
                             const f3 wiL = toLocal(fr, ls.wi);
                             float pB;
                             if (GUIDED && G) pB = frac * mixturePdf(G, ls.wi) + (1.f - frac) * bsdfSamplePdf(m, wo, wiL);

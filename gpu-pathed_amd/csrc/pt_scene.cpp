@@ -305,12 +305,15 @@ static void presetRoom(pt_scene &s, uint64_t seed, int64_t target) {
     // emitter strip above the slot facing down (x in [-0.6, 0.6], z in [-1, 1], y = 3.6)
     { const float a[3] = {-0.6f, 3.6f, 1.f}, b[3] = {-0.6f, 3.6f, -1.f}, c[3] = {0.6f, 3.6f, -1.f}, d[3] = {0.6f, 3.6f, 1.f}; addQuad(s, a, b, c, d, light); }
     addSphere(s, 0.8f, 0.5f, 0.6f, 0.5f, glass);
-    // clutter
+    // clutter, kept clear of the camera (-2.4, 1.4, 2.6) and of the glass sphere
     int64_t budget = target - (int64_t)s.triMat.size();
     int obj = 0;
     while (budget > 0) {
-        const int mat = addMat(s, PT_MICROFACET, 0, 0, 0, 0, 0, 0, 1.f, 0.02f + 0.28f * u01(rng));
+        const float alpha = 0.02f + 0.28f * u01(rng);
         const float cx = -2.6f + 5.2f * u01(rng), cz = -2.6f + 5.2f * u01(rng);
+        const float dcx = cx + 2.4f, dcz = cz - 2.6f, dsx = cx - 0.8f, dsz = cz - 0.6f;
+        if (dcx * dcx + dcz * dcz < 1.5f * 1.5f || dsx * dsx + dsz * dsz < 0.9f * 0.9f) continue;
+        const int mat = addMat(s, PT_MICROFACET, 0, 0, 0, 0, 0, 0, 1.f, alpha);
         if ((obj % 4) == 3) {
             const float sx = 0.1f + 0.3f * u01(rng), sy = 0.1f + 0.6f * u01(rng), sz = 0.1f + 0.3f * u01(rng);
             const float lo[3] = {cx - sx, 0.f, cz - sz}, hi[3] = {cx + sx, 2 * sy, cz + sz};

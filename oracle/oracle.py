@@ -59,6 +59,7 @@ def _setup(L):
     L.or_scene_destroy.argtypes = [vp]
     L.or_render.argtypes = [vp, C.POINTER(RenderParams), vp, vp, vp]
     L.or_trace_paths.argtypes = [vp, C.POINTER(RenderParams), vp, vp, C.c_int64, vp]
+    L.or_render_pixels.argtypes = [vp, C.POINTER(RenderParams), vp, C.c_int64, vp, vp, vp]
     L.or_intersect.argtypes = [vp, vp, C.c_int64, C.c_int, C.c_int, vp]
     L.or_tri_test.argtypes = [vp, vp, C.c_int64, C.c_int, vp]
     L.or_bsdf_eval.argtypes = [vp, vp, vp, C.c_int64, C.c_int, vp, vp]
@@ -166,6 +167,17 @@ class Scene:
         rays = np.zeros(1, np.int64)
         lib().or_render(self.handle, C.byref(p), _p(s), _p(sq), _p(rays))
         return s.reshape(self.height, self.width, 3), (None if sq is None else sq.reshape(self.height, self.width, 3)), int(rays[0])
+
+    def render_pixels(self, pixels, **kw):
+        """FP64 sums and sums of squares (n, 3) of `spp` samples at each listed pixel index
+        (row * width + col); returns (sum, sumsq, rays)."""
+        p = self.params(**kw)
+        pix = np.ascontiguousarray(pixels, dtype=np.int32)
+        s = np.zeros((pix.size, 3), np.float64)
+        sq = np.zeros((pix.size, 3), np.float64)
+        rays = np.zeros(1, np.int64)
+        lib().or_render_pixels(self.handle, C.byref(p), _p(pix), pix.size, _p(s), _p(sq), _p(rays))
+        return s, sq, int(rays[0])
 
     def trace_paths(self, pixels, samples, **kw):
         p = self.params(**kw)

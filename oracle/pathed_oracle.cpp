@@ -587,7 +587,11 @@ template <class T> inline T envPdf(const EnvTables &e, const V3<T> &wi, bool ref
     }
     int x, y;
     envTexel(e, (double)phi, (double)theta, &x, &y);
-    return (T)e.texelPdf[(size_t)y * e.w + x] / std::sin(theta);
+    // sin(theta) from the xz components keeps its precision next to the poles, where y
+    // rounds to +-1; on a pole the (phi, theta) map degenerates: density 0 there, and a
+    // light sample that lands on one is dropped (pbrt-v3 InfiniteAreaLight::Pdf_Li)
+    const T st = std::sqrt(wi.x * wi.x + wi.z * wi.z);
+    return st > T(0) ? (T)e.texelPdf[(size_t)y * e.w + x] / st : T(0);
 }
 
 // Alias draw: bucket = floor(xi*n); the remainder decides prob vs alias and is rescaled to a
@@ -1102,7 +1106,7 @@ template <class T> static V3<T> Li(const Scene<T> &s, V3<T> o, V3<T> d, Pcg32 &r
             if (!bs.isDelta) {
                 const float l1 = rng.uniform(), l2 = rng.uniform(), l3 = rng.uniform();
                 const LightSample<T> ls = sampleDirectLights(s, rec.point, l1, l2, l3, op.refEnv != 0);
-                if (dot(ls.normal, ls.wi) < T(0)) {
+                if (dot(ls.normal, ls.wi) < T(0) && ls.pdf > T(0)) {
                     Hit<T> oh;
                     rc.rays++;
                     if (!worldHit(s, rec.point, ls.wi, (T)1e-4, ls.distance - (T)2e-4, op.triTest, true, &oh)) {
@@ -1366,6 +1370,70 @@ int or_trace_paths(void *hv, const or_render_params *p, const int *pix, const in
             out[i * 3] = L.x; out[i * 3 + 1] = L.y; out[i * 3 + 2] = L.z;
         }
     }
+    return 0;
+}
+
+// Per-pixel FP64 sums (and sums of squares) of samples [sample_start, sample_start + spp) for
+// a list of pixels: sum[3n], sumsq[3n].  Work is split into blocks of up to 1024 samples that
+// the threads take in turn; block partials are reduced in a fixed order, so the result does
+// not depend on the thread count.
+int or_render_pixels(void *hv, const or_render_params *p, const int *pix, int64_t n, double *sum, double *sumsq, int64_t *rays) {
+    OrHandle *h = (OrHandle *)hv;
+    const Opts op{p->max_depth, p->nee, p->mode, p->tri_test, p->ref_env, p->mis_sample};
+    const int blk = 1024;
+    const int64_t nb = (p->spp + blk - 1) / blk;
+    const int64_t items = n * nb;
+    std::vector<double> part((size_t)items * 6, 0.0);
+    int nthreads = p->threads > 0 ? p->threads : (int)std::thread::hardware_concurrency();
+    if (nthreads < 1) nthreads = 1;
+    std::atomic<int64_t> next(0), rayTotal(0);
+    auto work = [&]() {
+        RayCount rc;
+        for (;;) {
+            const int64_t it = next.fetch_add(1);
+            if (it >= items) break;
+            const int64_t i = it / nb, b = it % nb;
+            const int row = pix[i] / p->width, col = pix[i] % p->width;
+            const int k0 = (int)(b * blk), k1 = std::min(p->spp, k0 + blk);
+            double *acc = &part[(size_t)it * 6];
+            for (int k = k0; k < k1; k++) {
+                Pcg32 rng = pathRng(p->seed, (uint32_t)pix[i], (uint32_t)(p->sample_start + k));
+                const float sx = rng.uniform(), sy = rng.uniform();
+                double L[3];
+                if (p->precision == 64) {
+                    V3<double> o, d;
+                    cameraRay(h->sd.cam, row, col, (double)sx, (double)sy, &o, &d);
+                    const V3<double> v = Li(h->sd, o, d, rng, op, rc);
+                    L[0] = v.x; L[1] = v.y; L[2] = v.z;
+                } else {
+                    V3<float> o, d;
+                    cameraRay(h->sf.cam, row, col, sx, sy, &o, &d);
+                    const V3<float> v = Li(h->sf, o, d, rng, op, rc);
+                    L[0] = v.x; L[1] = v.y; L[2] = v.z;
+                }
+                for (int c = 0; c < 3; c++) {
+                    acc[c] += L[c];
+                    acc[3 + c] += L[c] * L[c];
+                }
+            }
+        }
+        rayTotal += rc.rays;
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; t++) th.emplace_back(work);
+    for (auto &t : th) t.join();
+    for (int64_t i = 0; i < n; i++) {
+        for (int c = 0; c < 3; c++) {
+            double a = 0.0, q = 0.0;
+            for (int64_t b = 0; b < nb; b++) {
+                a += part[(size_t)(i * nb + b) * 6 + c];
+                q += part[(size_t)(i * nb + b) * 6 + 3 + c];
+            }
+            sum[i * 3 + c] = a;
+            if (sumsq) sumsq[i * 3 + c] = q;
+        }
+    }
+    if (rays) *rays = rayTotal.load();
     return 0;
 }
 

@@ -476,7 +476,12 @@ def test_env_sampling_matches_pdf():
     th1 = (np.arange(h) + 1) / h * np.pi
     area = (np.cos(th0) - np.cos(th1)) * 2 * np.pi / w
     exact = (rgba[..., 0].astype(np.float64) * area[:, None]).sum()
-    est = np.mean(em[:, 0] / pdf)
+    # a sample that lands exactly on a pole has density 0 and contributes nothing; next to
+    # the poles (y rounded to +-1) the density stays finite
+    zero = pdf == 0
+    assert zero.sum() <= 2 and np.all(np.abs(d[zero, 1]) == 1.0)
+    assert np.isfinite(pdf).all()
+    est = np.mean(np.where(zero, 0.0, em[:, 0] / np.where(zero, 1.0, pdf)))
     assert est == pytest.approx(exact, rel=0.01)
     # samples land on the texel whose emission they return
     assert np.allclose(pdf, O.env_pdf(rgba, d, ref=False))
