@@ -7,8 +7,10 @@
 //    index)
 // 4. bottom-up AABB refit (atomic arrival counter per node; the second child to arrive
 //    merges) and depth
-// 5. node write: 64-byte nodes holding both child boxes; subtrees of <= PT_LEAF_MAX
-//    primitives become leaf ranges of the sorted primitive array
+// 5. collapse into a 4-wide BVH of 128-byte nodes: subtrees of <= PT_LEAF_MAX primitives
+//    become leaf ranges of the sorted primitive array, and every other level of the
+//    remaining internal nodes is skipped (a node at even depth takes its grandchildren as
+//    children), which halves the number of dependent node fetches per ray
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -262,6 +264,21 @@ __global__ void k_depth(const int *parentNode, const int *parentLeaf, int n, int
     atomicMax(maxDepth, d);
 }
 
+__device__ __forceinline__ bool liveNode(const int2 *ranges, int c) { return c >= 0 && ranges[c].y - ranges[c].x + 1 > PT_LEAF_MAX; }
+
+// An internal node whose subtree holds more than PT_LEAF_MAX primitives and that sits at an
+// even depth (root = 0) becomes a 4-wide node.  flag[i] = 1 for those; maxLevel receives the
+// deepest 4-wide level.
+__global__ void k_flag4(const int2 *ranges, const int *parentNode, int ni, uint32_t *flag, int *maxLevel) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ni) return;
+    int d = 0;
+    for (int p = parentNode[i]; p >= 0 && d < ni; p = parentNode[p]) d++;
+    const bool f = liveNode(ranges, i) && (d & 1) == 0;
+    flag[i] = f ? 1u : 0u;
+    if (f) atomicMax(maxLevel, d / 2 + 1);
+}
+
 __device__ __forceinline__ int encodeChild(int c, const int2 *ranges) {
     if (c < 0) return ~((0 << PT_LEAF_COUNT_SHIFT) | (~c));
     const int2 r = ranges[c];
@@ -270,16 +287,66 @@ __device__ __forceinline__ int encodeChild(int c, const int2 *ranges) {
     return c;
 }
 
-__global__ void k_write_nodes(const int2 *children, const int2 *ranges, const float *leafBoxes, const float *nodeBoxes, int n, float4 *nodes) {
+// One thread per flagged node: gathers up to four child slots (a child that is itself a live
+// internal node contributes its two children instead) and writes the 128-byte node at its
+// compacted index.
+__global__ void k_write4(const int2 *children, const int2 *ranges, const float *leafBoxes, const float *nodeBoxes, const uint32_t *flag,
+                         const uint32_t *index, int ni, float4 *nodes) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n - 1) return;
+    if (i >= ni || !flag[i]) return;
+    float lo[3][4], hi[3][4];
+    int ref[4];
+    int cnt = 0;
+    auto slot = [&](int c) {
+        const float *b = c >= 0 ? &nodeBoxes[6 * c] : &leafBoxes[6 * (~c)];
+        for (int k = 0; k < 3; k++) {
+            lo[k][cnt] = b[k];
+            hi[k][cnt] = b[3 + k];
+        }
+        const int e = encodeChild(c, ranges);
+        ref[cnt] = e >= 0 ? (int)index[e] : e;
+        cnt++;
+    };
     const int2 c = children[i];
-    const float *a = c.x >= 0 ? &nodeBoxes[6 * c.x] : &leafBoxes[6 * (~c.x)];
-    const float *b = c.y >= 0 ? &nodeBoxes[6 * c.y] : &leafBoxes[6 * (~c.y)];
-    nodes[4 * i + 0] = make_float4(a[0], a[3], a[1], a[4]);
-    nodes[4 * i + 1] = make_float4(b[0], b[3], b[1], b[4]);
-    nodes[4 * i + 2] = make_float4(a[2], a[5], b[2], b[5]);
-    nodes[4 * i + 3] = make_float4(asFloat(encodeChild(c.x, ranges)), asFloat(encodeChild(c.y, ranges)), 0.f, 0.f);
+    const int cc[2] = {c.x, c.y};
+    for (int k = 0; k < 2; k++) {
+        if (liveNode(ranges, cc[k])) {
+            const int2 g = children[cc[k]];
+            slot(g.x);
+            slot(g.y);
+        } else {
+            slot(cc[k]);
+        }
+    }
+    for (; cnt < 4; cnt++) {
+        for (int k = 0; k < 3; k++) lo[k][cnt] = hi[k][cnt] = 1e30f;
+        ref[cnt] = PT_EMPTY_CHILD;
+    }
+    float4 *o = nodes + (size_t)PT_NODE_F4 * index[i];
+    for (int k = 0; k < 3; k++) {
+        o[2 * k] = make_float4(lo[k][0], lo[k][1], lo[k][2], lo[k][3]);
+        o[2 * k + 1] = make_float4(hi[k][0], hi[k][1], hi[k][2], hi[k][3]);
+    }
+    o[6] = make_float4(asFloat(ref[0]), asFloat(ref[1]), asFloat(ref[2]), asFloat(ref[3]));
+    o[7] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// All primitives in one leaf below a single 4-wide root (n <= PT_LEAF_MAX).
+__global__ void k_write_single(const float *leafBoxes, int n, float4 *nodes) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) {
+            lo[k] = fminf(lo[k], leafBoxes[6 * i + k]);
+            hi[k] = fmaxf(hi[k], leafBoxes[6 * i + 3 + k]);
+        }
+    for (int k = 0; k < 3; k++) {
+        nodes[2 * k] = make_float4(lo[k], 1e30f, 1e30f, 1e30f);
+        nodes[2 * k + 1] = make_float4(hi[k], 1e30f, 1e30f, 1e30f);
+    }
+    nodes[6] = make_float4(asFloat(~(((n - 1) << PT_LEAF_COUNT_SHIFT) | 0)), asFloat(PT_EMPTY_CHILD), asFloat(PT_EMPTY_CHILD),
+                           asFloat(PT_EMPTY_CHILD));
+    nodes[7] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 __global__ void k_fill_int(int *p, int n, int v) {
@@ -299,10 +366,11 @@ int deviceCUs() {
     return prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
 }
 
-void buildBvh(const float4 *primsIn, int n, const float lo[3], const float hi[3], float4 *nodes, float4 *primsSorted, hipStream_t stream,
+void buildBvh(const float4 *primsIn, int n, const float lo[3], const float hi[3], DevBuf &nodes, float4 *primsSorted, hipStream_t stream,
               BvhInfo *info) {
     info->numNodes = 0;
     info->depth = 0;
+    info->binaryDepth = 0;
     if (n <= 0) return;
     DevBuf keys, keys2, vals, vals2, boxes, boxes2, hist;
     keys.alloc(sizeof(uint64_t) * n);
@@ -331,43 +399,53 @@ void buildBvh(const float4 *primsIn, int n, const float lo[3], const float hi[3]
     // after 8 passes the sorted data is back in keys/vals
     k_gather<<<blocks(n, 256), 256, 0, stream>>>(primsIn, boxes.as<float>(), vin, n, primsSorted, boxes2.as<float>());
     PT_HIP_TRY(hipGetLastError());
-    if (n == 1) {
-        // one primitive: a root whose two children are the same one-primitive leaf
-        std::vector<float> b(6);
-        PT_HIP_TRY(hipMemcpyAsync(b.data(), boxes2.p, 24, hipMemcpyDeviceToHost, stream));
-        PT_HIP_TRY(hipStreamSynchronize(stream));
-        const int leaf = ~0;
-        float4 h[4] = {make_float4(b[0], b[3], b[1], b[4]), make_float4(b[0], b[3], b[1], b[4]), make_float4(b[2], b[5], b[2], b[5]),
-                       make_float4(asFloat(leaf), asFloat(leaf), 0.f, 0.f)};
-        PT_HIP_TRY(hipMemcpyAsync(nodes, h, sizeof h, hipMemcpyHostToDevice, stream));
+    if (n <= PT_LEAF_MAX) {
+        nodes.alloc(sizeof(float4) * PT_NODE_F4);
+        k_write_single<<<1, 64, 0, stream>>>(boxes2.as<float>(), n, nodes.as<float4>());
+        PT_HIP_TRY(hipGetLastError());
         PT_HIP_TRY(hipStreamSynchronize(stream));
         info->numNodes = 1;
         info->depth = 1;
+        info->binaryDepth = n > 1 ? 1 : 0;
         return;
     }
     const int ni = n - 1;
-    DevBuf children, ranges, parentNode, parentLeaf, nodeBoxes, arrivals, depth;
+    DevBuf children, ranges, parentNode, parentLeaf, nodeBoxes, arrivals, counters, flag, index;
     children.alloc(sizeof(int2) * ni);
     ranges.alloc(sizeof(int2) * ni);
     parentNode.alloc(sizeof(int) * ni);
     parentLeaf.alloc(sizeof(int) * n);
     nodeBoxes.alloc(sizeof(float) * 6 * ni);
     arrivals.alloc(sizeof(int) * ni);
-    depth.alloc(sizeof(int));
+    counters.alloc(sizeof(int) * 2);
+    flag.alloc(sizeof(uint32_t) * ni);
+    index.alloc(sizeof(uint32_t) * ni);
     k_fill_int<<<blocks(ni, 256), 256, 0, stream>>>(parentNode.as<int>(), ni, -1);
     k_fill_int<<<blocks(ni, 256), 256, 0, stream>>>(arrivals.as<int>(), ni, 0);
-    k_fill_int<<<1, 64, 0, stream>>>(depth.as<int>(), 1, 0);
+    k_fill_int<<<1, 64, 0, stream>>>(counters.as<int>(), 2, 0);
     k_karras<<<blocks(ni, 256), 256, 0, stream>>>(kin, n, children.as<int2>(), ranges.as<int2>(), parentNode.as<int>(), parentLeaf.as<int>());
     k_refit<<<blocks(n, 256), 256, 0, stream>>>(children.as<int2>(), parentNode.as<int>(), parentLeaf.as<int>(), boxes2.as<float>(),
                                                  nodeBoxes.as<float>(), arrivals.as<int>(), n);
-    k_depth<<<blocks(n, 256), 256, 0, stream>>>(parentNode.as<int>(), parentLeaf.as<int>(), n, depth.as<int>());
-    k_write_nodes<<<blocks(ni, 256), 256, 0, stream>>>(children.as<int2>(), ranges.as<int2>(), boxes2.as<float>(), nodeBoxes.as<float>(), n, nodes);
+    k_depth<<<blocks(n, 256), 256, 0, stream>>>(parentNode.as<int>(), parentLeaf.as<int>(), n, counters.as<int>());
+    k_flag4<<<blocks(ni, 256), 256, 0, stream>>>(ranges.as<int2>(), parentNode.as<int>(), ni, flag.as<uint32_t>(), counters.as<int>() + 1);
+    PT_HIP_TRY(hipMemcpyAsync(index.p, flag.p, sizeof(uint32_t) * ni, hipMemcpyDeviceToDevice, stream));
+    k_sort_scan<<<1, 1024, 0, stream>>>(index.as<uint32_t>(), ni);
     PT_HIP_TRY(hipGetLastError());
-    int hd = 0;
-    PT_HIP_TRY(hipMemcpyAsync(&hd, depth.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+    uint32_t tail[2] = {0, 0};
+    int hc[2] = {0, 0};
+    PT_HIP_TRY(hipMemcpyAsync(&tail[0], index.as<uint32_t>() + ni - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    PT_HIP_TRY(hipMemcpyAsync(&tail[1], flag.as<uint32_t>() + ni - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    PT_HIP_TRY(hipMemcpyAsync(hc, counters.p, sizeof hc, hipMemcpyDeviceToHost, stream));
     PT_HIP_TRY(hipStreamSynchronize(stream));
-    info->numNodes = ni;
-    info->depth = hd;
+    const int count = (int)(tail[0] + tail[1]);
+    nodes.alloc(sizeof(float4) * PT_NODE_F4 * (size_t)std::max(count, 1));
+    k_write4<<<blocks(ni, 256), 256, 0, stream>>>(children.as<int2>(), ranges.as<int2>(), boxes2.as<float>(), nodeBoxes.as<float>(),
+                                                  flag.as<uint32_t>(), index.as<uint32_t>(), ni, nodes.as<float4>());
+    PT_HIP_TRY(hipGetLastError());
+    PT_HIP_TRY(hipStreamSynchronize(stream));
+    info->numNodes = count;
+    info->depth = hc[1];
+    info->binaryDepth = hc[0];
 }
 
 }  // namespace pt

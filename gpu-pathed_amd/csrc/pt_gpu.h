@@ -47,14 +47,15 @@ struct DevBuf {
 };
 
 struct BvhInfo {
-    int numNodes = 0;
-    int depth = 0;
+    int numNodes = 0;      // 4-wide nodes
+    int depth = 0;         // levels of 4-wide nodes
+    int binaryDepth = 0;   // levels of internal nodes of the binary radix tree
 };
 
-// Builds the LBVH over `n` primitive records (3 float4 each, input order) on `stream`:
-// writes 4 float4 per internal node to `nodes` (capacity max(n - 1, 1) nodes) and the
-// records in BVH order to `primsSorted`.  lo/hi bound all primitive centroids.
-void buildBvh(const float4 *primsIn, int n, const float lo[3], const float hi[3], float4 *nodes,
+// Builds the LBVH over `n` primitive records (3 float4 each, input order) on `stream` and
+// collapses it into a 4-wide BVH: allocates `nodes` (PT_NODE_F4 float4 per node) and writes
+// the records in BVH order to `primsSorted`.  lo/hi bound all primitive centroids.
+void buildBvh(const float4 *primsIn, int n, const float lo[3], const float hi[3], DevBuf &nodes,
               float4 *primsSorted, hipStream_t stream, BvhInfo *info);
 
 // Environment tables in FP64 on the GPU: per-row and marginal alias tables packed as

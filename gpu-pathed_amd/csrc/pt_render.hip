@@ -126,14 +126,25 @@ __global__ __launch_bounds__(256) void k_raygen(ChunkParams cp, PtCamera cam, fl
 }
 
 // ------------------------------------------------------------------------------------------
-// BVH traversal (closest hit / any hit).  Ordered traversal of the binary BVH: both child
-// boxes per 64-byte node, nearer child first, farther one pushed on a per-lane stack in LDS
-// (stack[level * blockDim + lane]).
+// BVH traversal (closest hit / any hit) over the 4-wide BVH: one 128-byte node fetch tests
+// four child boxes; the hit children are ordered near to far (a 5-exchange sorting network),
+// the nearest is visited next and the others are pushed far-first on a per-lane stack in LDS
+// (stack[level * blockDim + lane]).  Any-hit traversal skips the ordering.
 // ------------------------------------------------------------------------------------------
 struct TraceHit {
     float t, u, v;
     int prim;
 };
+
+__device__ __forceinline__ void cswap(float &ta, int &ra, float &tb, int &rb) {
+    const bool s = tb < ta;
+    const float t = s ? tb : ta;
+    const int r = s ? rb : ra;
+    tb = s ? ta : tb;
+    rb = s ? ra : rb;
+    ta = t;
+    ra = r;
+}
 
 template <bool ANY, int STACK>
 __device__ __forceinline__ bool traverse(const PtSceneView &sc, f3 o, f3 d, float tmin, float tmax, int *stack, TraceHit *out, int *overflow) {
@@ -147,34 +158,59 @@ __device__ __forceinline__ bool traverse(const PtSceneView &sc, f3 o, f3 d, floa
     int sp = 0;
     int node = 0;
     bool hit = false;
+    auto push = [&](int ref) {
+        if (sp < STACK) {
+            stack[sp * stride] = ref;
+            sp++;
+        } else {
+            atomicAdd(overflow, 1);
+        }
+    };
     while (true) {
         if (node >= 0) {
-            const float4 *n = sc.nodes + 4 * (size_t)node;
-            const float4 n0 = n[0], n1 = n[1], n2 = n[2], n3 = n[3];
-            const float a0 = n0.x * ix - ox, a1 = n0.y * ix - ox, a2 = n0.z * iy - oy, a3 = n0.w * iy - oy;
-            const float a4 = n2.x * iz - oz, a5 = n2.y * iz - oz;
-            const float b0 = n1.x * ix - ox, b1 = n1.y * ix - ox, b2 = n1.z * iy - oy, b3 = n1.w * iy - oy;
-            const float b4 = n2.z * iz - oz, b5 = n2.w * iz - oz;
-            const float c0lo = fmaxf(fmaxf(fminf(a0, a1), fminf(a2, a3)), fmaxf(fminf(a4, a5), tmin));
-            const float c0hi = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fminf(fmaxf(a4, a5), tmax));
-            const float c1lo = fmaxf(fmaxf(fminf(b0, b1), fminf(b2, b3)), fmaxf(fminf(b4, b5), tmin));
-            const float c1hi = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fminf(fmaxf(b4, b5), tmax));
-            const bool h0 = c0lo <= c0hi, h1 = c1lo <= c1hi;
-            const int ch0 = asInt(n3.x), ch1 = asInt(n3.y);
-            if (h0 && h1) {
-                int nearN = ch0, farN = ch1;
-                if (c1lo < c0lo) { nearN = ch1; farN = ch0; }
-                if (sp < STACK) {
-                    stack[sp * stride] = farN;
-                    sp++;
-                } else {
-                    atomicAdd(overflow, 1);
-                }
-                node = nearN;
-                continue;
+            const float4 *n = sc.nodes + (size_t)PT_NODE_F4 * node;
+            const float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5], rf = n[6];
+            float tn[4];
+            int ref[4] = {asInt(rf.x), asInt(rf.y), asInt(rf.z), asInt(rf.w)};
+            const float blx[4] = {lx.x, lx.y, lx.z, lx.w}, bhx[4] = {hx.x, hx.y, hx.z, hx.w};
+            const float bly[4] = {ly.x, ly.y, ly.z, ly.w}, bhy[4] = {hy.x, hy.y, hy.z, hy.w};
+            const float blz[4] = {lz.x, lz.y, lz.z, lz.w}, bhz[4] = {hz.x, hz.y, hz.z, hz.w};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const float x0 = blx[k] * ix - ox, x1 = bhx[k] * ix - ox;
+                const float y0 = bly[k] * iy - oy, y1 = bhy[k] * iy - oy;
+                const float z0 = blz[k] * iz - oz, z1 = bhz[k] * iz - oz;
+                const float lo = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+                const float hi = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
+                tn[k] = (lo <= hi && ref[k] != PT_EMPTY_CHILD) ? lo : INFINITY;
             }
-            if (h0) { node = ch0; continue; }
-            if (h1) { node = ch1; continue; }
+            if constexpr (!ANY) {
+                cswap(tn[0], ref[0], tn[1], ref[1]);
+                cswap(tn[2], ref[2], tn[3], ref[3]);
+                cswap(tn[0], ref[0], tn[2], ref[2]);
+                cswap(tn[1], ref[1], tn[3], ref[3]);
+                cswap(tn[1], ref[1], tn[2], ref[2]);
+                if (tn[0] != INFINITY) {
+                    if (tn[3] != INFINITY) push(ref[3]);
+                    if (tn[2] != INFINITY) push(ref[2]);
+                    if (tn[1] != INFINITY) push(ref[1]);
+                    node = ref[0];
+                    continue;
+                }
+            } else {
+                int next = PT_EMPTY_CHILD;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    if (tn[k] != INFINITY) {
+                        if (next == PT_EMPTY_CHILD) next = ref[k];
+                        else push(ref[k]);
+                    }
+                }
+                if (next != PT_EMPTY_CHILD) {
+                    node = next;
+                    continue;
+                }
+            }
         } else {
             const int leaf = ~node;
             const int first = leaf & ((1 << PT_LEAF_COUNT_SHIFT) - 1);
@@ -888,6 +924,7 @@ void launchExtend(pt_renderer *r, const int *count, int *fetch, const float4 *ro
     int *ov = r->overflow.as<int>();
     switch (r->stackSize) {
     case 32: k_extend<32><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, ro, rd, hits, ov); break;
+    case 48: k_extend<48><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, ro, rd, hits, ov); break;
     case 64: k_extend<64><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, ro, rd, hits, ov); break;
     default: k_extend<96><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, ro, rd, hits, ov); break;
     }
@@ -900,6 +937,7 @@ void launchShadow(pt_renderer *r, const int *count, int *fetch, int *ladd) {
     float4 *L = r->Lpath.as<float4>();
     switch (r->stackSize) {
     case 32: k_shadow<32><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, o, d, l, L, ladd, ov); break;
+    case 48: k_shadow<48><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, o, d, l, L, ladd, ov); break;
     case 64: k_shadow<64><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, o, d, l, L, ladd, ov); break;
     default: k_shadow<96><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, o, d, l, L, ladd, ov); break;
     }
@@ -1194,18 +1232,18 @@ int pt_renderer_set_scene(pt_renderer *r, const pt_scene *s) {
     r->primsIn.alloc(recs.size() * sizeof(float4));
     PT_HIP_TRY(hipMemcpyAsync(r->primsIn.p, recs.data(), recs.size() * sizeof(float4), hipMemcpyHostToDevice, r->stream));
     r->prims.alloc(recs.size() * sizeof(float4));
-    r->nodes.alloc((size_t)std::max<int64_t>(n - 1, 1) * 4 * sizeof(float4));
     r->normals.alloc(std::max<size_t>(nrm.size(), 1) * sizeof(float4));
     if (!nrm.empty()) PT_HIP_TRY(hipMemcpyAsync(r->normals.p, nrm.data(), nrm.size() * sizeof(float4), hipMemcpyHostToDevice, r->stream));
     for (int k = 0; k < 3; k++) {
         r->sceneLo[k] = std::isfinite(lo[k]) ? lo[k] : 0.f;
         r->sceneHi[k] = std::isfinite(hi[k]) ? hi[k] : 1.f;
     }
-    buildBvh(r->primsIn.as<float4>(), (int)n, lo, hi, r->nodes.as<float4>(), r->prims.as<float4>(), r->stream, &r->bvh);
+    buildBvh(r->primsIn.as<float4>(), (int)n, lo, hi, r->nodes, r->prims.as<float4>(), r->stream, &r->bvh);
     r->primsIn.release();
-    const int need = r->bvh.depth + 1;
-    if (need > 96) return fail(PT_ERR_NUMERIC, "pt_renderer_set_scene: BVH deeper than 96 levels");
-    r->stackSize = need <= 32 ? 32 : (need <= 64 ? 64 : 96);
+    // ordered 4-wide traversal pushes at most three references per level
+    const int need = 3 * r->bvh.depth + 1;
+    if (need > 96) return fail(PT_ERR_NUMERIC, "pt_renderer_set_scene: BVH deeper than 31 4-wide levels");
+    r->stackSize = need <= 32 ? 32 : (need <= 48 ? 48 : (need <= 64 ? 64 : 96));
     r->view.nodes = r->nodes.as<float4>();
     r->view.prims = r->prims.as<float4>();
     r->view.normals = r->normals.as<float4>();

@@ -40,9 +40,15 @@ struct PtEnv {
 //        child < 0: leaf ~((count-1) << 28 | first primitive)
 static constexpr int PT_LEAF_COUNT_SHIFT = 28;
 static constexpr int PT_LEAF_MAX = 4;
+// 4-wide BVH node: 8 float4 = 128 bytes (one L2 line): lo.x, hi.x, lo.y, hi.y, lo.z, hi.z of
+// the four child boxes (one float4 per bound and axis), then the four child references as int
+// bits (>= 0 node index, < 0 leaf ~((count - 1) << PT_LEAF_COUNT_SHIFT | first), or
+// PT_EMPTY_CHILD for an unused slot), then one unused float4.
+static constexpr int PT_NODE_F4 = 8;
+static constexpr int PT_EMPTY_CHILD = 0x7fffffff;
 
 struct PtSceneView {
-    const float4 *nodes;      // 4 per internal node
+    const float4 *nodes;      // PT_NODE_F4 per 4-wide node
     const float4 *prims;      // 3 per primitive, BVH order
     const float4 *normals;    // 3 per triangle with per-vertex normals
     const PtMaterial *materials;

@@ -209,7 +209,8 @@ def test_bvh_closest_hit_matches_brute_force():
     vis_ref = cprim == P.MISS
     assert ((occ == P.MISS) == vis_ref).mean() > 0.999
     st = r.stats()
-    assert st.stack_overflows == 0 and st.bvh_depth > 10
+    # a real 4-wide tree: 20k primitives in leaves of <= 4 need > 1250 leaf slots
+    assert st.stack_overflows == 0 and st.bvh_depth > 5 and st.bvh_nodes > 400
     r.close()
 
 
