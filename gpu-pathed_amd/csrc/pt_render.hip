@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <chrono>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -51,6 +52,9 @@ static constexpr int kWsumSlot = C_PER_BOUNCE * kMaxBounces + 3;
 enum { S_RAYS = 0, S_SHADOW = kMaxBounces, S_LADD_SHADE = 2 * kMaxBounces, S_LADD_SHADOW = 3 * kMaxBounces, S_PATHS = 4 * kMaxBounces, S_COUNT };
 
 static constexpr int kTraceThreads = 128;
+// Default path-pool capacity: 2^26 paths (about 13 GB of SoA path state), a whole 1024^2 x
+// 64 spp pass in one chunk, so that every bounce's queues stay long.
+static constexpr int64_t kDefaultPool = 1ll << 26;
 static constexpr float kShadowEps = 1e-4f;   // path_tracer.cu:104-110
 static constexpr float kBounceEps = 1e-3f;   // path_tracer.cu:177
 
@@ -128,12 +132,24 @@ __global__ __launch_bounds__(256) void k_raygen(ChunkParams cp, PtCamera cam, fl
 // ------------------------------------------------------------------------------------------
 // BVH traversal (closest hit / any hit) over the 4-wide BVH: one 128-byte node fetch tests
 // four child boxes; the hit children are ordered near to far (a 5-exchange sorting network),
-// the nearest is visited next and the others are pushed far-first on a per-lane stack in LDS
-// (stack[level * blockDim + lane]).  Any-hit traversal skips the ordering.
+// the nearest is visited next and the others are pushed far-first on a per-lane stack.  Any-
+// hit traversal skips the ordering.
+//
+// The stack has two levels: the top LDS entries of each lane (stack[level * blockDim + lane],
+// LDS entries per lane) and, below them, a spill area in global memory interleaved by lane
+// (spill[(level - LDS) * lanes + lane]), sized for the deepest stack the tree can need.  The
+// small LDS part keeps the trace kernels' occupancy high; deep stacks are rare.
 // ------------------------------------------------------------------------------------------
 struct TraceHit {
     float t, u, v;
     int prim;
+};
+
+struct TraceStack {
+    int *lds;       // this lane's first LDS entry; entries blockDim.x apart
+    int *spill;     // this lane's first spill entry; entries spillStride apart
+    int spillStride;
+    int spillCap;   // spill entries per lane
 };
 
 __device__ __forceinline__ void cswap(float &ta, int &ra, float &tb, int &rb) {
@@ -147,7 +163,8 @@ __device__ __forceinline__ void cswap(float &ta, int &ra, float &tb, int &rb) {
 }
 
 template <bool ANY, int STACK>
-__device__ __forceinline__ bool traverse(const PtSceneView &sc, f3 o, f3 d, float tmin, float tmax, int *stack, TraceHit *out, int *overflow) {
+__device__ __forceinline__ bool traverse(const PtSceneView &sc, f3 o, f3 d, float tmin, float tmax, const TraceStack &ts, TraceHit *out,
+                                         int *overflow) {
     if (sc.numNodes == 0) return false;
     const WtRay wr = wtSetup(o, d);
     const float ix = 1.f / (fabsf(d.x) > 1e-20f ? d.x : copysignf(1e-20f, d.x));
@@ -160,7 +177,10 @@ __device__ __forceinline__ bool traverse(const PtSceneView &sc, f3 o, f3 d, floa
     bool hit = false;
     auto push = [&](int ref) {
         if (sp < STACK) {
-            stack[sp * stride] = ref;
+            ts.lds[sp * stride] = ref;
+            sp++;
+        } else if (sp - STACK < ts.spillCap) {
+            ts.spill[(size_t)(sp - STACK) * ts.spillStride] = ref;
             sp++;
         } else {
             atomicAdd(overflow, 1);
@@ -238,17 +258,26 @@ __device__ __forceinline__ bool traverse(const PtSceneView &sc, f3 o, f3 d, floa
         }
         if (sp == 0) break;
         sp--;
-        node = stack[sp * stride];
+        node = sp < STACK ? ts.lds[sp * stride] : ts.spill[(size_t)(sp - STACK) * ts.spillStride];
     }
     return hit;
+}
+
+__device__ __forceinline__ TraceStack traceStack(int *sstack, int *spill, int spillCap) {
+    TraceStack ts;
+    ts.lds = sstack + threadIdx.x;
+    ts.spillStride = gridDim.x * blockDim.x;
+    ts.spill = spill + blockIdx.x * blockDim.x + threadIdx.x;
+    ts.spillCap = spillCap;
+    return ts;
 }
 
 // Persistent closest-hit kernel: each wave takes 64 rays per atomic until the queue is empty.
 template <int STACK>
 __global__ __launch_bounds__(kTraceThreads) void k_extend(PtSceneView sc, const int *count, int *fetch, const float4 *rayO, const float4 *rayD,
-                                                         float4 *hits, int *overflow) {
+                                                         float4 *hits, int *spill, int spillCap, int *overflow) {
     __shared__ int sstack[STACK * kTraceThreads];
-    int *stack = sstack + threadIdx.x;
+    const TraceStack ts = traceStack(sstack, spill, spillCap);
     const int n = *count;
     while (true) {
         int base = 0;
@@ -260,7 +289,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_extend(PtSceneView sc, const 
             const float4 ro = rayO[i], rd = rayD[i];
             TraceHit th;
             th.t = FLT_MAX; th.u = 0.f; th.v = 0.f; th.prim = INT32_MIN;
-            traverse<false, STACK>(sc, xyz(ro), xyz(rd), ro.w, rd.w, stack, &th, overflow);
+            traverse<false, STACK>(sc, xyz(ro), xyz(rd), ro.w, rd.w, ts, &th, overflow);
             hits[i] = make_float4(th.t, th.u, th.v, asFloat(th.prim));
         }
     }
@@ -269,9 +298,10 @@ __global__ __launch_bounds__(kTraceThreads) void k_extend(PtSceneView sc, const 
 // Persistent any-hit kernel for shadow rays (interval [1e-4, tmax]).
 template <int STACK>
 __global__ __launch_bounds__(kTraceThreads) void k_shadow(PtSceneView sc, const int *count, int *fetch, const float4 *sO, const float4 *sD,
-                                                         const float4 *sL, float4 *Lpath, int *laddCounter, int *overflow) {
+                                                         const float4 *sL, float4 *Lpath, int *laddCounter, int *spill, int spillCap,
+                                                         int *overflow) {
     __shared__ int sstack[STACK * kTraceThreads];
-    int *stack = sstack + threadIdx.x;
+    const TraceStack ts = traceStack(sstack, spill, spillCap);
     const int n = *count;
     while (true) {
         int base = 0;
@@ -283,7 +313,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_shadow(PtSceneView sc, const 
         if (i < n) {
             const float4 o = sO[i], d = sD[i];
             TraceHit th;
-            const bool occluded = traverse<true, STACK>(sc, xyz(o), xyz(d), kShadowEps, o.w, stack, &th, overflow);
+            const bool occluded = traverse<true, STACK>(sc, xyz(o), xyz(d), kShadowEps, o.w, ts, &th, overflow);
             if (!occluded) {
                 const int pid = asInt(d.w);
                 const float4 l = sL[i];
@@ -739,6 +769,12 @@ __global__ void k_zero_u64(unsigned long long *p, int n) {
     if (i < n) p[i] = 0ull;
 }
 
+// Completion stamp of the work queued before it on the stream: the device's constant-rate
+// wall clock (hipDeviceAttributeWallClockRate kHz).
+__global__ void k_stamp(unsigned long long *slot) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *slot = (unsigned long long)wall_clock64();
+}
+
 }  // namespace pt
 
 // ============================================================================================
@@ -751,15 +787,23 @@ enum { K_RAYGEN = 0, K_EXTEND, K_SHADE, K_SHADOW, K_ACCUM, K_TONEMAP, K_GUIDE_IN
 const char *kKernelNames[K_NUM] = {"raygen", "extend", "shade", "shadow", "accumulate", "tonemap", "guide_infer", "guide_train"};
 }  // namespace
 
+// SoA path state, traversal stack spill area and queue counters of up to `pool` paths.
+struct PathPool {
+    int64_t pool = 0;
+    DevBuf rayO[2], rayD[2], st[2], beta[2], hits, sO, sD, sL, Lpath, counters, overflow, spill;
+};
+
 struct pt_renderer {
     int W = 0, H = 0;
     hipStream_t stream = nullptr;
+    bool ownsStream = true;  // false for the renders of a batch, which share the batch's stream
     int cus = 256;
     // scene on the device
     DevBuf nodes, prims, primsIn, normals, materials, emitters, envRgba, envRows, envMarg, envPdf;
     PtSceneView view{};
     BvhInfo bvh;
-    int stackSize = 32;
+    int stackNeed = 1;    // deepest traversal stack the tree can need (3 per 4-wide level)
+    int ldsStack = 16;    // stack entries per lane kept in LDS; the rest spill to the pool's `spill`
     bool hasScene = false;
     // host copies needed for edits
     std::vector<HostMaterial> mats;
@@ -771,9 +815,9 @@ struct pt_renderer {
     int nee = 1;
     int type = PT_RENDERER_OPTIX;
     uint64_t seed = 0;
-    // path pool
-    int64_t pool = 0, poolWanted = 1 << 22;
-    DevBuf rayO[2], rayD[2], st[2], beta[2], hits, sO, sD, sL, Lpath, counters, overflow;
+    // path pool (owned, or shared by the renders of a batch) and its capacity
+    std::shared_ptr<PathPool> pp = std::make_shared<PathPool>();
+    int64_t poolWanted = kDefaultPool;
     int traceBlocksPerCU = 8, shadeBlocksPerCU = 8;
     // accumulation
     DevBuf accum, radiance, rgba, stats;
@@ -893,22 +937,26 @@ void uploadEmitters(pt_renderer *r) {
     r->view.numEmitters = (int)(e.size() / 3);
 }
 
-void ensurePool(pt_renderer *r) {
-    const int64_t want = std::max<int64_t>(4096, r->poolWanted);
-    if (r->pool == want) return;
+// Path pool sized to the work at hand: `need` paths rounded up to a power of two, at least
+// 4096, at most the configured capacity; it grows, it does not shrink.
+void ensurePool(pt_renderer *r, int64_t need = 4096) {
+    int64_t want = 4096;
+    while (want < need) want <<= 1;
+    want = std::min<int64_t>(want, std::max<int64_t>(4096, r->poolWanted));
+    if (r->pp->pool >= want) return;
     const size_t n = (size_t)want;
     for (int k = 0; k < 2; k++) {
-        r->rayO[k].alloc(n * 16);
-        r->rayD[k].alloc(n * 16);
-        r->st[k].alloc(n * 16);
-        r->beta[k].alloc(n * 16);
+        r->pp->rayO[k].alloc(n * 16);
+        r->pp->rayD[k].alloc(n * 16);
+        r->pp->st[k].alloc(n * 16);
+        r->pp->beta[k].alloc(n * 16);
     }
-    r->hits.alloc(n * 16);
-    r->sO.alloc(n * 16);
-    r->sD.alloc(n * 16);
-    r->sL.alloc(n * 16);
-    r->Lpath.alloc(n * 16);
-    r->pool = want;
+    r->pp->hits.alloc(n * 16);
+    r->pp->sO.alloc(n * 16);
+    r->pp->sD.alloc(n * 16);
+    r->pp->sL.alloc(n * 16);
+    r->pp->Lpath.alloc(n * 16);
+    r->pp->pool = want;
 }
 
 void zeroAccum(pt_renderer *r) {
@@ -919,33 +967,37 @@ void zeroAccum(pt_renderer *r) {
     r->resetPending = false;
 }
 
+// Spill area of the traversal stacks: entries beyond the LDS part for every lane of the grid.
+int *ensureSpill(pt_renderer *r, int grid, int *cap) {
+    *cap = std::max(0, r->stackNeed - r->ldsStack);
+    const size_t bytes = (size_t)grid * kTraceThreads * std::max(*cap, 1) * sizeof(int);
+    if (r->pp->spill.bytes < bytes) r->pp->spill.alloc(bytes);
+    return r->pp->spill.as<int>();
+}
+
 void launchExtend(pt_renderer *r, const int *count, int *fetch, const float4 *ro, const float4 *rd, float4 *hits) {
     const int grid = r->cus * r->traceBlocksPerCU;
-    int *ov = r->overflow.as<int>();
-    switch (r->stackSize) {
-    case 32: k_extend<32><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, ro, rd, hits, ov); break;
-    case 48: k_extend<48><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, ro, rd, hits, ov); break;
-    case 64: k_extend<64><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, ro, rd, hits, ov); break;
-    default: k_extend<96><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, ro, rd, hits, ov); break;
-    }
+    int *ov = r->pp->overflow.as<int>();
+    int cap = 0;
+    int *sp = ensureSpill(r, grid, &cap);
+    if (r->ldsStack == 32) k_extend<32><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, ro, rd, hits, sp, cap, ov);
+    else k_extend<16><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, ro, rd, hits, sp, cap, ov);
     PT_HIP_TRY(hipGetLastError());
 }
 void launchShadow(pt_renderer *r, const int *count, int *fetch, int *ladd) {
     const int grid = r->cus * r->traceBlocksPerCU;
-    int *ov = r->overflow.as<int>();
-    const float4 *o = r->sO.as<float4>(), *d = r->sD.as<float4>(), *l = r->sL.as<float4>();
-    float4 *L = r->Lpath.as<float4>();
-    switch (r->stackSize) {
-    case 32: k_shadow<32><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, o, d, l, L, ladd, ov); break;
-    case 48: k_shadow<48><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, o, d, l, L, ladd, ov); break;
-    case 64: k_shadow<64><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, o, d, l, L, ladd, ov); break;
-    default: k_shadow<96><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, o, d, l, L, ladd, ov); break;
-    }
+    int *ov = r->pp->overflow.as<int>();
+    const float4 *o = r->pp->sO.as<float4>(), *d = r->pp->sD.as<float4>(), *l = r->pp->sL.as<float4>();
+    float4 *L = r->pp->Lpath.as<float4>();
+    int cap = 0;
+    int *sp = ensureSpill(r, grid, &cap);
+    if (r->ldsStack == 32) k_shadow<32><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, o, d, l, L, ladd, sp, cap, ov);
+    else k_shadow<16><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, o, d, l, L, ladd, sp, cap, ov);
     PT_HIP_TRY(hipGetLastError());
 }
 
 void ensureGuide(pt_renderer *r) {
-    if (r->guideReady && r->gRecCap == 2 * r->pool) return;
+    if (r->guideReady && r->gRecCap == 2 * r->pp->pool) return;
     const size_t np = GUIDE_PARAMS;
     if (!r->guideReady) {
         r->gParams.alloc(np * 4);
@@ -968,11 +1020,11 @@ void ensureGuide(pt_renderer *r) {
         PT_HIP_TRY(hipStreamSynchronize(r->stream));
         r->gAdamStep = 0;
     }
-    r->gMix.alloc((size_t)r->pool * GUIDE_REC_F4 * sizeof(float4));
-    r->gRecCap = 2 * r->pool;
+    r->gMix.alloc((size_t)r->pp->pool * GUIDE_REC_F4 * sizeof(float4));
+    r->gRecCap = 2 * r->pp->pool;
     r->gRecords.alloc((size_t)r->gRecCap * GUIDE_TREC_F4 * sizeof(float4));
-    r->gRecIdx[0].alloc((size_t)r->pool * 4);
-    r->gRecIdx[1].alloc((size_t)r->pool * 4);
+    r->gRecIdx[0].alloc((size_t)r->pp->pool * 4);
+    r->gRecIdx[1].alloc((size_t)r->pp->pool * 4);
     r->guideReady = true;
 }
 
@@ -995,12 +1047,12 @@ GuideSceneNorm guideNorm(pt_renderer *r) {
 }
 
 void runChunk(pt_renderer *r, const ChunkParams &cp) {
-    int *ctr = r->counters.as<int>();
+    int *ctr = r->pp->counters.as<int>();
     const bool guided = r->type == PT_RENDERER_GUIDED;
     {
         KTimer t(r, K_RAYGEN);
-        k_raygen<<<(cp.nPaths + 255) / 256, 256, 0, r->stream>>>(cp, r->cam, r->rayO[0].as<float4>(), r->rayD[0].as<float4>(), r->st[0].as<uint4>(),
-                                                                r->beta[0].as<float4>(), r->Lpath.as<float4>(), ctr);
+        k_raygen<<<(cp.nPaths + 255) / 256, 256, 0, r->stream>>>(cp, r->cam, r->pp->rayO[0].as<float4>(), r->pp->rayD[0].as<float4>(), r->pp->st[0].as<uint4>(),
+                                                                r->pp->beta[0].as<float4>(), r->pp->Lpath.as<float4>(), ctr);
         PT_HIP_TRY(hipGetLastError());
         if (guided) {
             // camera rays were not spawned by a guided vertex
@@ -1014,12 +1066,12 @@ void runChunk(pt_renderer *r, const ChunkParams &cp) {
         int *c = ctr + C_PER_BOUNCE * b;
         {
             KTimer t(r, K_EXTEND);
-            launchExtend(r, &c[C_RAYS], &c[C_FETCH_EXT], r->rayO[cur].as<float4>(), r->rayD[cur].as<float4>(), r->hits.as<float4>());
+            launchExtend(r, &c[C_RAYS], &c[C_FETCH_EXT], r->pp->rayO[cur].as<float4>(), r->pp->rayD[cur].as<float4>(), r->pp->hits.as<float4>());
         }
         GuideShadeArgs ga{};
         if (guided && b + 1 < cp.maxDepth) {
             KTimer t(r, K_GUIDE_INFER);
-            guideInfer(r->view, guideNorm(r), guideNet(r), &c[C_RAYS], r->rayO[cur].as<float4>(), r->rayD[cur].as<float4>(), r->hits.as<float4>(),
+            guideInfer(r->view, guideNorm(r), guideNet(r), &c[C_RAYS], r->pp->rayO[cur].as<float4>(), r->pp->rayD[cur].as<float4>(), r->pp->hits.as<float4>(),
                        cp.nPaths, r->gMix.as<float4>(), r->cus, r->stream);
         }
         if (guided) {
@@ -1034,14 +1086,14 @@ void runChunk(pt_renderer *r, const ChunkParams &cp) {
             KTimer t(r, K_SHADE);
             if (guided)
                 k_shade<true><<<shadeGrid, 256, 0, r->stream>>>(
-                    r->view, cp, b, ctr, r->rayO[cur].as<float4>(), r->rayD[cur].as<float4>(), r->st[cur].as<uint4>(), r->beta[cur].as<float4>(),
-                    r->hits.as<float4>(), r->rayO[1 - cur].as<float4>(), r->rayD[1 - cur].as<float4>(), r->st[1 - cur].as<uint4>(),
-                    r->beta[1 - cur].as<float4>(), r->sO.as<float4>(), r->sD.as<float4>(), r->sL.as<float4>(), r->Lpath.as<float4>(), ga);
+                    r->view, cp, b, ctr, r->pp->rayO[cur].as<float4>(), r->pp->rayD[cur].as<float4>(), r->pp->st[cur].as<uint4>(), r->pp->beta[cur].as<float4>(),
+                    r->pp->hits.as<float4>(), r->pp->rayO[1 - cur].as<float4>(), r->pp->rayD[1 - cur].as<float4>(), r->pp->st[1 - cur].as<uint4>(),
+                    r->pp->beta[1 - cur].as<float4>(), r->pp->sO.as<float4>(), r->pp->sD.as<float4>(), r->pp->sL.as<float4>(), r->pp->Lpath.as<float4>(), ga);
             else
                 k_shade<false><<<shadeGrid, 256, 0, r->stream>>>(
-                    r->view, cp, b, ctr, r->rayO[cur].as<float4>(), r->rayD[cur].as<float4>(), r->st[cur].as<uint4>(), r->beta[cur].as<float4>(),
-                    r->hits.as<float4>(), r->rayO[1 - cur].as<float4>(), r->rayD[1 - cur].as<float4>(), r->st[1 - cur].as<uint4>(),
-                    r->beta[1 - cur].as<float4>(), r->sO.as<float4>(), r->sD.as<float4>(), r->sL.as<float4>(), r->Lpath.as<float4>(), ga);
+                    r->view, cp, b, ctr, r->pp->rayO[cur].as<float4>(), r->pp->rayD[cur].as<float4>(), r->pp->st[cur].as<uint4>(), r->pp->beta[cur].as<float4>(),
+                    r->pp->hits.as<float4>(), r->pp->rayO[1 - cur].as<float4>(), r->pp->rayD[1 - cur].as<float4>(), r->pp->st[1 - cur].as<uint4>(),
+                    r->pp->beta[1 - cur].as<float4>(), r->pp->sO.as<float4>(), r->pp->sD.as<float4>(), r->pp->sL.as<float4>(), r->pp->Lpath.as<float4>(), ga);
             PT_HIP_TRY(hipGetLastError());
         }
         if (cp.nee) {
@@ -1052,14 +1104,14 @@ void runChunk(pt_renderer *r, const ChunkParams &cp) {
     }
     {
         KTimer t(r, K_ACCUM);
-        k_accumulate<<<(cp.npix + 255) / 256, 256, 0, r->stream>>>(cp, r->Lpath.as<float4>(), r->accum.as<double>(), ctr,
+        k_accumulate<<<(cp.npix + 255) / 256, 256, 0, r->stream>>>(cp, r->pp->Lpath.as<float4>(), r->accum.as<double>(), ctr,
                                                                   r->stats.as<unsigned long long>());
         PT_HIP_TRY(hipGetLastError());
     }
     if (guided && r->gTrainSteps > 0) {
         KTimer t(r, K_GUIDE_TRAIN);
         const int nrec = (int)r->gRecCap;
-        k_guide_finalize<<<(nrec + 255) / 256, 256, 0, r->stream>>>(r->gRecords.as<float4>(), nrec, ctr, r->Lpath.as<float4>());
+        k_guide_finalize<<<(nrec + 255) / 256, 256, 0, r->stream>>>(r->gRecords.as<float4>(), nrec, ctr, r->pp->Lpath.as<float4>());
         PT_HIP_TRY(hipGetLastError());
         for (int s = 0; s < r->gTrainSteps; s++) {
             const int base = (int)((r->gTrainTotal * (int64_t)r->gBatch) & 0x3fffffff);
@@ -1069,6 +1121,36 @@ void runChunk(pt_renderer *r, const ChunkParams &cp) {
                       r->gW.as<__bf16>(), r->gWt.as<__bf16>(), r->gB.as<float>(), r->stream);
             r->gTrainTotal++;
         }
+    }
+}
+
+// A pass of `spp` samples per pixel cut into chunks of at most the pool's capacity: whole
+// sample rows of the image when a row fits, else runs of pixels of one sample.
+void planChunks(const pt_renderer *r, int spp, std::vector<ChunkParams> &out) {
+    const int npix = r->W * r->H;
+    // a shared pool may be larger than this renderer's configured capacity
+    const int64_t pool = std::min<int64_t>(r->pp->pool, std::max<int64_t>(4096, r->poolWanted));
+    auto chunk = [&](int pixBegin, int n, int samples, int sampleBase) {
+        ChunkParams cp;
+        cp.pixBegin = pixBegin;
+        cp.npix = n;
+        cp.samples = samples;
+        cp.sampleBase = sampleBase;
+        cp.nPaths = n * samples;
+        cp.maxDepth = r->maxDepth;
+        cp.nee = r->nee;
+        cp.misSample = r->misSample;
+        cp.W = r->W;
+        cp.H = r->H;
+        cp.seed = r->seed;
+        out.push_back(cp);
+    };
+    if (npix <= pool) {
+        const int rowsPerChunk = (int)std::min<int64_t>(spp, pool / npix);
+        for (int s0 = 0; s0 < spp; s0 += rowsPerChunk) chunk(0, npix, std::min(rowsPerChunk, spp - s0), (int)(r->spp + s0));
+    } else {
+        for (int s0 = 0; s0 < spp; s0++)
+            for (int64_t p0 = 0; p0 < npix; p0 += pool) chunk((int)p0, (int)std::min<int64_t>(pool, npix - p0), 1, (int)(r->spp + s0));
     }
 }
 
@@ -1145,9 +1227,9 @@ int pt_renderer_create(int width, int height, pt_renderer **out) {
     r->radiance.alloc(npix * 3 * sizeof(float));
     r->rgba.alloc(npix * 4);
     r->stats.alloc(S_COUNT * sizeof(unsigned long long));
-    r->counters.alloc(kNumCounters * sizeof(int));
-    r->overflow.alloc(sizeof(int));
-    PT_HIP_TRY(hipMemsetAsync(r->overflow.p, 0, sizeof(int), r->stream));
+    r->pp->counters.alloc(kNumCounters * sizeof(int));
+    r->pp->overflow.alloc(sizeof(int));
+    PT_HIP_TRY(hipMemsetAsync(r->pp->overflow.p, 0, sizeof(int), r->stream));
     k_zero_u64<<<1, 512, 0, r->stream>>>(r->stats.as<unsigned long long>(), S_COUNT);
     zeroAccum(r);
     computeCamera(r);
@@ -1167,7 +1249,7 @@ int pt_renderer_destroy(pt_renderer *r) {
     for (hipEvent_t e : r->eventPool) (void)hipEventDestroy(e);
     if (r->evBegin) (void)hipEventDestroy(r->evBegin);
     if (r->evEnd) (void)hipEventDestroy(r->evEnd);
-    hipStream_t s = r->stream;
+    hipStream_t s = r->ownsStream ? r->stream : nullptr;
     delete r;
     if (s) (void)hipStreamDestroy(s);
     return PT_OK;
@@ -1241,9 +1323,7 @@ int pt_renderer_set_scene(pt_renderer *r, const pt_scene *s) {
     buildBvh(r->primsIn.as<float4>(), (int)n, lo, hi, r->nodes, r->prims.as<float4>(), r->stream, &r->bvh);
     r->primsIn.release();
     // ordered 4-wide traversal pushes at most three references per level
-    const int need = 3 * r->bvh.depth + 1;
-    if (need > 96) return fail(PT_ERR_NUMERIC, "pt_renderer_set_scene: BVH deeper than 31 4-wide levels");
-    r->stackSize = need <= 32 ? 32 : (need <= 48 ? 48 : (need <= 64 ? 64 : 96));
+    r->stackNeed = 3 * r->bvh.depth + 1;
     r->view.nodes = r->nodes.as<float4>();
     r->view.prims = r->prims.as<float4>();
     r->view.normals = r->normals.as<float4>();
@@ -1359,11 +1439,14 @@ int pt_renderer_set_option(pt_renderer *r, int opt, int64_t v) {
     switch (opt) {
     case PT_OPT_POOL_PATHS:
         if (v > (1ll << 27)) return fail(PT_ERR_INVALID, "pt_renderer_set_option: pool larger than 2^27 paths");
-        r->poolWanted = v > 0 ? v : (1 << 22);
-        if (r->hasScene) {
-            PT_HIP_TRY(hipStreamSynchronize(r->stream));
-            ensurePool(r);
-        }
+        PT_HIP_TRY(hipStreamSynchronize(r->stream));
+        r->poolWanted = v > 0 ? v : kDefaultPool;
+        // shrink: the next pass allocates at most the new capacity
+        if (r->pp->pool > r->poolWanted) r->pp->pool = 0;
+        break;
+    case PT_OPT_TRACE_LDS_STACK:
+        if (v != 0 && v != 16 && v != 32) return fail(PT_ERR_INVALID, "pt_renderer_set_option: LDS stack entries must be 16 or 32");
+        r->ldsStack = v == 0 ? 16 : (int)v;
         break;
     case PT_OPT_TRACE_BLOCKS_PER_CU: r->traceBlocksPerCU = v > 0 ? (int)std::min<int64_t>(v, 64) : 8; break;
     case PT_OPT_SHADE_BLOCKS_PER_CU: r->shadeBlocksPerCU = v > 0 ? (int)std::min<int64_t>(v, 64) : 8; break;
@@ -1397,6 +1480,7 @@ int pt_renderer_get_option(pt_renderer *r, int opt, int64_t *v) {
     switch (opt) {
     case PT_OPT_POOL_PATHS: *v = r->poolWanted; break;
     case PT_OPT_TRACE_BLOCKS_PER_CU: *v = r->traceBlocksPerCU; break;
+    case PT_OPT_TRACE_LDS_STACK: *v = r->ldsStack; break;
     case PT_OPT_SHADE_BLOCKS_PER_CU: *v = r->shadeBlocksPerCU; break;
     case PT_OPT_PROFILE_KERNELS: *v = r->profile ? 1 : 0; break;
     case PT_OPT_CONSISTENT_MIS: *v = r->misSample; break;
@@ -1423,54 +1507,19 @@ int pt_renderer_render_async(pt_renderer *r, int spp) {
         // one centre ray per pixel, added once per pass (the reference's framebuffer sees
         // spp = 1 each pass, g_buffer.cu:108-117)
         if (npix > r->poolWanted) r->poolWanted = npix;
-        ensurePool(r);
-        int *c = r->counters.as<int>();
-        k_centre_rays<<<(npix + 255) / 256, 256, 0, r->stream>>>(r->cam, npix, r->W, r->rayO[0].as<float4>(), r->rayD[0].as<float4>(), c);
-        launchExtend(r, &c[C_RAYS], &c[C_FETCH_EXT], r->rayO[0].as<float4>(), r->rayD[0].as<float4>(), r->hits.as<float4>());
-        k_normals_shade<<<(npix + 255) / 256, 256, 0, r->stream>>>(r->view, npix, r->rayO[0].as<float4>(), r->rayD[0].as<float4>(),
-                                                                  r->hits.as<float4>(), r->accum.as<double>());
+        ensurePool(r, npix);
+        int *c = r->pp->counters.as<int>();
+        k_centre_rays<<<(npix + 255) / 256, 256, 0, r->stream>>>(r->cam, npix, r->W, r->pp->rayO[0].as<float4>(), r->pp->rayD[0].as<float4>(), c);
+        launchExtend(r, &c[C_RAYS], &c[C_FETCH_EXT], r->pp->rayO[0].as<float4>(), r->pp->rayD[0].as<float4>(), r->pp->hits.as<float4>());
+        k_normals_shade<<<(npix + 255) / 256, 256, 0, r->stream>>>(r->view, npix, r->pp->rayO[0].as<float4>(), r->pp->rayD[0].as<float4>(),
+                                                                  r->pp->hits.as<float4>(), r->accum.as<double>());
         PT_HIP_TRY(hipGetLastError());
         r->spp += 1;
         r->passSpp = spp;
     } else {
-        ensurePool(r);
+        ensurePool(r, (int64_t)npix * spp);
         if (r->type == PT_RENDERER_GUIDED) ensureGuide(r);
-        const int64_t pool = r->pool;
-        if (npix <= pool) {
-            const int rowsPerChunk = (int)std::min<int64_t>(spp, pool / npix);
-            for (int s0 = 0; s0 < spp; s0 += rowsPerChunk) {
-                ChunkParams cp;
-                cp.pixBegin = 0;
-                cp.npix = npix;
-                cp.samples = std::min(rowsPerChunk, spp - s0);
-                cp.sampleBase = (int)(r->spp + s0);
-                cp.nPaths = cp.npix * cp.samples;
-                cp.maxDepth = r->maxDepth;
-                cp.nee = r->nee;
-                cp.misSample = r->misSample;
-                cp.W = r->W;
-                cp.H = r->H;
-                cp.seed = r->seed;
-                r->lastChunks.push_back(cp);
-            }
-        } else {
-            for (int s0 = 0; s0 < spp; s0++)
-                for (int64_t p0 = 0; p0 < npix; p0 += pool) {
-                    ChunkParams cp;
-                    cp.pixBegin = (int)p0;
-                    cp.npix = (int)std::min<int64_t>(pool, npix - p0);
-                    cp.samples = 1;
-                    cp.sampleBase = (int)(r->spp + s0);
-                    cp.nPaths = cp.npix;
-                    cp.maxDepth = r->maxDepth;
-                    cp.nee = r->nee;
-                    cp.misSample = r->misSample;
-                    cp.W = r->W;
-                    cp.H = r->H;
-                    cp.seed = r->seed;
-                    r->lastChunks.push_back(cp);
-                }
-        }
+        planChunks(r, spp, r->lastChunks);
         if (r->maxDepth > 0) {
             for (const ChunkParams &cp : r->lastChunks) {
                 runChunk(r, cp);
@@ -1519,7 +1568,7 @@ int pt_renderer_synchronize(pt_renderer *r) {
     PT_HIP_TRY(hipStreamSynchronize(r->stream));
     if (r->inFlight) finishPass(r);
     int ov = 0;
-    PT_HIP_TRY(hipMemcpy(&ov, r->overflow.p, sizeof(int), hipMemcpyDeviceToHost));
+    PT_HIP_TRY(hipMemcpy(&ov, r->pp->overflow.p, sizeof(int), hipMemcpyDeviceToHost));
     if (ov != 0) return fail(PT_ERR_NUMERIC, "pt_renderer_synchronize: traversal stack overflowed");
     return PT_OK;
     PT_API_END("pt_renderer_synchronize")
@@ -1595,13 +1644,13 @@ int pt_renderer_hit_test(pt_renderer *r, int x, int y, int *material_id, int *is
         ro = make_float4(c.m[0][3], c.m[1][3], c.m[2][3], 0.f);
         rd = make_float4(d[0], d[1], d[2], FLT_MAX);
     }
-    PT_HIP_TRY(hipMemcpyAsync(r->rayO[0].p, &ro, 16, hipMemcpyHostToDevice, r->stream));
-    PT_HIP_TRY(hipMemcpyAsync(r->rayD[0].p, &rd, 16, hipMemcpyHostToDevice, r->stream));
-    int *c = r->counters.as<int>();
+    PT_HIP_TRY(hipMemcpyAsync(r->pp->rayO[0].p, &ro, 16, hipMemcpyHostToDevice, r->stream));
+    PT_HIP_TRY(hipMemcpyAsync(r->pp->rayD[0].p, &rd, 16, hipMemcpyHostToDevice, r->stream));
+    int *c = r->pp->counters.as<int>();
     k_set_counter<<<1, 64, 0, r->stream>>>(c, C_RAYS, 1);
-    launchExtend(r, &c[C_RAYS], &c[C_FETCH_EXT], r->rayO[0].as<float4>(), r->rayD[0].as<float4>(), r->hits.as<float4>());
+    launchExtend(r, &c[C_RAYS], &c[C_FETCH_EXT], r->pp->rayO[0].as<float4>(), r->pp->rayD[0].as<float4>(), r->pp->hits.as<float4>());
     float4 h;
-    PT_HIP_TRY(hipMemcpyAsync(&h, r->hits.p, 16, hipMemcpyDeviceToHost, r->stream));
+    PT_HIP_TRY(hipMemcpyAsync(&h, r->pp->hits.p, 16, hipMemcpyDeviceToHost, r->stream));
     PT_HIP_TRY(hipStreamSynchronize(r->stream));
     const int slot = asInt(h.w);
     if (slot == INT32_MIN) {
@@ -1625,7 +1674,7 @@ int pt_renderer_get_stats(pt_renderer *r, pt_render_stats *out) {
     std::vector<unsigned long long> s(S_COUNT);
     PT_HIP_TRY(hipMemcpy(s.data(), r->stats.p, S_COUNT * sizeof(unsigned long long), hipMemcpyDeviceToHost));
     int ov = 0;
-    PT_HIP_TRY(hipMemcpy(&ov, r->overflow.p, sizeof(int), hipMemcpyDeviceToHost));
+    PT_HIP_TRY(hipMemcpy(&ov, r->pp->overflow.p, sizeof(int), hipMemcpyDeviceToHost));
     memset(out, 0, sizeof *out);
     for (int b = 0; b < kMaxBounces; b++) {
         out->extension_rays += (int64_t)s[S_RAYS + b];
@@ -1723,7 +1772,7 @@ int pt_renderer_get_path_radiance(pt_renderer *r, float *out, int64_t cap, int64
     if (r->inFlight) finishPass(r);
     const int64_t n = std::min<int64_t>(cap, r->lastPassPaths);
     std::vector<float4> tmp((size_t)n);
-    PT_HIP_TRY(hipMemcpy(tmp.data(), r->Lpath.p, (size_t)n * sizeof(float4), hipMemcpyDeviceToHost));
+    PT_HIP_TRY(hipMemcpy(tmp.data(), r->pp->Lpath.p, (size_t)n * sizeof(float4), hipMemcpyDeviceToHost));
     for (int64_t i = 0; i < n; i++) {
         out[i * 3] = tmp[i].x;
         out[i * 3 + 1] = tmp[i].y;
@@ -1740,9 +1789,9 @@ int pt_renderer_intersect(pt_renderer *r, const float *rays, int64_t n, int any_
     if (r->inFlight) return fail(PT_ERR_STATE, "pt_renderer_intersect: a pass is still running");
     PT_API_BEGIN
     ensurePool(r);
-    int *c = r->counters.as<int>();
+    int *c = r->pp->counters.as<int>();
     for (int64_t done = 0; done < n;) {
-        const int cnt = (int)std::min<int64_t>(n - done, r->pool);
+        const int cnt = (int)std::min<int64_t>(n - done, r->pp->pool);
         std::vector<float4> o(cnt), d(cnt);
         for (int i = 0; i < cnt; i++) {
             const float *q = rays + (done + i) * 8;
@@ -1750,13 +1799,13 @@ int pt_renderer_intersect(pt_renderer *r, const float *rays, int64_t n, int any_
             d[i] = make_float4(q[4], q[5], q[6], q[7]);
         }
         if (!any_hit) {
-            PT_HIP_TRY(hipMemcpyAsync(r->rayO[0].p, o.data(), cnt * 16, hipMemcpyHostToDevice, r->stream));
-            PT_HIP_TRY(hipMemcpyAsync(r->rayD[0].p, d.data(), cnt * 16, hipMemcpyHostToDevice, r->stream));
+            PT_HIP_TRY(hipMemcpyAsync(r->pp->rayO[0].p, o.data(), cnt * 16, hipMemcpyHostToDevice, r->stream));
+            PT_HIP_TRY(hipMemcpyAsync(r->pp->rayD[0].p, d.data(), cnt * 16, hipMemcpyHostToDevice, r->stream));
             k_set_counter<<<1, 64, 0, r->stream>>>(c, C_RAYS, cnt);
-            launchExtend(r, &c[C_RAYS], &c[C_FETCH_EXT], r->rayO[0].as<float4>(), r->rayD[0].as<float4>(), r->hits.as<float4>());
-            k_map_prims<<<(cnt + 255) / 256, 256, 0, r->stream>>>(r->view, r->hits.as<float4>(), cnt, r->sL.as<float4>());
+            launchExtend(r, &c[C_RAYS], &c[C_FETCH_EXT], r->pp->rayO[0].as<float4>(), r->pp->rayD[0].as<float4>(), r->pp->hits.as<float4>());
+            k_map_prims<<<(cnt + 255) / 256, 256, 0, r->stream>>>(r->view, r->pp->hits.as<float4>(), cnt, r->pp->sL.as<float4>());
             std::vector<float4> h(cnt);
-            PT_HIP_TRY(hipMemcpyAsync(h.data(), r->sL.p, cnt * 16, hipMemcpyDeviceToHost, r->stream));
+            PT_HIP_TRY(hipMemcpyAsync(h.data(), r->pp->sL.p, cnt * 16, hipMemcpyDeviceToHost, r->stream));
             PT_HIP_TRY(hipStreamSynchronize(r->stream));
             for (int i = 0; i < cnt; i++) memcpy(out + (done + i) * 4, &h[i], 16);
         } else {
@@ -1767,14 +1816,14 @@ int pt_renderer_intersect(pt_renderer *r, const float *rays, int64_t n, int any_
                 so[i] = make_float4(o[i].x, o[i].y, o[i].z, d[i].w);
                 sd[i] = make_float4(d[i].x, d[i].y, d[i].z, asFloat(i));
             }
-            PT_HIP_TRY(hipMemcpyAsync(r->sO.p, so.data(), cnt * 16, hipMemcpyHostToDevice, r->stream));
-            PT_HIP_TRY(hipMemcpyAsync(r->sD.p, sd.data(), cnt * 16, hipMemcpyHostToDevice, r->stream));
-            PT_HIP_TRY(hipMemcpyAsync(r->sL.p, sl.data(), cnt * 16, hipMemcpyHostToDevice, r->stream));
-            PT_HIP_TRY(hipMemcpyAsync(r->Lpath.p, z.data(), cnt * 16, hipMemcpyHostToDevice, r->stream));
+            PT_HIP_TRY(hipMemcpyAsync(r->pp->sO.p, so.data(), cnt * 16, hipMemcpyHostToDevice, r->stream));
+            PT_HIP_TRY(hipMemcpyAsync(r->pp->sD.p, sd.data(), cnt * 16, hipMemcpyHostToDevice, r->stream));
+            PT_HIP_TRY(hipMemcpyAsync(r->pp->sL.p, sl.data(), cnt * 16, hipMemcpyHostToDevice, r->stream));
+            PT_HIP_TRY(hipMemcpyAsync(r->pp->Lpath.p, z.data(), cnt * 16, hipMemcpyHostToDevice, r->stream));
             k_set_counter<<<1, 64, 0, r->stream>>>(c, C_SHADOW, cnt);
             launchShadow(r, &c[C_SHADOW], &c[C_FETCH_SHADOW], &c[C_LADD_SHADOW]);
             std::vector<float4> h(cnt);
-            PT_HIP_TRY(hipMemcpyAsync(h.data(), r->Lpath.p, cnt * 16, hipMemcpyDeviceToHost, r->stream));
+            PT_HIP_TRY(hipMemcpyAsync(h.data(), r->pp->Lpath.p, cnt * 16, hipMemcpyDeviceToHost, r->stream));
             PT_HIP_TRY(hipStreamSynchronize(r->stream));
             for (int i = 0; i < cnt; i++) {
                 const int prim = h[i].x > 0.5f ? INT32_MIN : 0;  // visible -> no occluder
@@ -1788,6 +1837,430 @@ int pt_renderer_intersect(pt_renderer *r, const float *rays, int64_t n, int any_
     }
     return PT_OK;
     PT_API_END("pt_renderer_intersect")
+}
+
+}  // extern "C"
+
+// ============================================================================================
+// Autotuner: coordinate descent over the path-pool capacity, trace and shade blocks per CU
+// and the LDS part of the traversal stack, each candidate timed on a probe render.
+// ============================================================================================
+namespace {
+
+struct TuneCfg {
+    int64_t pool;
+    int trace, shade, lds;
+};
+
+template <class Probe>
+TuneCfg coordinateDescent(TuneCfg cur, const std::vector<int64_t> &pools, Probe probe, int *probes, double *bestRate) {
+    double best = probe(cur);
+    int n = 1;
+    const std::vector<int> traces = {6, 8, 12, 14}, shades = {4, 8, 16}, ldss = {16, 32};
+    for (int dim = 0; dim < 4; dim++) {
+        const size_t count = dim == 0 ? pools.size() : dim == 1 ? traces.size() : dim == 2 ? ldss.size() : shades.size();
+        for (size_t k = 0; k < count; k++) {
+            TuneCfg c = cur;
+            if (dim == 0) c.pool = pools[k];
+            if (dim == 1) c.trace = traces[k];
+            if (dim == 2) c.lds = ldss[k];
+            if (dim == 3) c.shade = shades[k];
+            if (c.pool == cur.pool && c.trace == cur.trace && c.lds == cur.lds && c.shade == cur.shade) continue;
+            const double rate = probe(c);
+            n++;
+            if (rate > best) {
+                best = rate;
+                cur = c;
+            }
+        }
+    }
+    *probes = n;
+    *bestRate = best;
+    return cur;
+}
+
+void applyCfg(pt_renderer *r, const TuneCfg &c) {
+    r->poolWanted = c.pool;
+    r->traceBlocksPerCU = c.trace;
+    r->shadeBlocksPerCU = c.shade;
+    r->ldsStack = c.lds;
+}
+
+std::vector<int64_t> poolCandidates(int64_t maxPaths) {
+    std::vector<int64_t> v;
+    for (int64_t p = 1 << 20; p <= kDefaultPool; p <<= 2) {
+        v.push_back(p);
+        if (p >= maxPaths) break;
+    }
+    return v;
+}
+
+}  // namespace
+
+// ============================================================================================
+// Batch of independent renders in one captured graph
+// ============================================================================================
+struct pt_batch {
+    hipStream_t stream = nullptr;
+    std::shared_ptr<PathPool> pp = std::make_shared<PathPool>();
+    std::vector<pt_renderer *> jobs;
+    std::vector<int> spp;
+    TuneCfg cfg{kDefaultPool, 8, 8, 16};
+    bool useGraph = true;
+    DevBuf stamps;
+    double clockKHz = 100000.0;
+    // last run
+    std::vector<double> latencyMs;
+    double wallMs = 0.0;
+    int chunks = 0;
+    size_t graphNodes = 0;
+    std::vector<int> order;
+    // captured graph of the last plan
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graphScale = -1;
+    TuneCfg graphCfg{0, 0, 0, 0};
+    size_t graphJobs = 0;
+    void dropGraph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        graphScale = -1;
+    }
+};
+
+namespace {
+
+bool sameCfg(const TuneCfg &a, const TuneCfg &b) { return a.pool == b.pool && a.trace == b.trace && a.shade == b.shade && a.lds == b.lds; }
+
+// Queues every render of the batch (shortest expected first: pixels x spp x log2 triangles)
+// on the batch stream: accumulators and counters cleared, chunks traced, then a completion
+// stamp per render.  spp of each render is divided by `scale` (autotuner probes).
+// Device memory the batch's kernels need (shared pool, stack spill area, stamps) and the
+// render order; allocates, so it runs before any stream capture.
+void prepareBatch(pt_batch *b, int scale) {
+    for (pt_renderer *r : b->jobs) applyCfg(r, b->cfg);
+    int64_t need = 0;
+    int deepest = 0;
+    for (size_t j = 0; j < b->jobs.size(); j++) {
+        pt_renderer *r = b->jobs[j];
+        need = std::max<int64_t>(need, (int64_t)r->W * r->H * std::max(1, b->spp[j] / scale));
+        if (r->stackNeed > b->jobs[deepest]->stackNeed) deepest = (int)j;
+    }
+    ensurePool(b->jobs[0], need);
+    int cap = 0;
+    ensureSpill(b->jobs[deepest], b->jobs[0]->cus * b->cfg.trace, &cap);
+    b->stamps.alloc((b->jobs.size() + 1) * sizeof(unsigned long long));
+    b->order.resize(b->jobs.size());
+    for (size_t j = 0; j < b->jobs.size(); j++) b->order[j] = (int)j;
+    auto cost = [&](int j) {
+        const pt_renderer *r = b->jobs[j];
+        return (double)r->W * r->H * b->spp[j] * std::log2(2.0 + (double)r->triMat.size() + (double)r->sphMat.size());
+    };
+    std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) { return cost(x) < cost(y); });
+}
+
+void enqueueBatch(pt_batch *b, int scale) {
+    unsigned long long *st = b->stamps.as<unsigned long long>();
+    k_stamp<<<1, 64, 0, b->stream>>>(st);
+    b->chunks = 0;
+    for (size_t k = 0; k < b->order.size(); k++) {
+        pt_renderer *r = b->jobs[b->order[k]];
+        const int spp = std::max(1, b->spp[b->order[k]] / scale);
+        zeroAccum(r);
+        k_zero_u64<<<1, 512, 0, b->stream>>>(r->stats.as<unsigned long long>(), S_COUNT);
+        std::vector<ChunkParams> plan;
+        planChunks(r, spp, plan);
+        for (const ChunkParams &cp : plan) runChunk(r, cp);
+        b->chunks += (int)plan.size();
+        r->spp = spp;
+        k_stamp<<<1, 64, 0, b->stream>>>(st + 1 + k);
+        PT_HIP_TRY(hipGetLastError());
+    }
+}
+
+// Runs the batch once (scale as in enqueueBatch); returns wall milliseconds of the launch.
+double runBatch(pt_batch *b, int scale) {
+    PT_HIP_TRY(hipStreamSynchronize(b->stream));
+    double ms = 0.0;
+    if (b->useGraph) {
+        if (!b->exec || b->graphScale != scale || !sameCfg(b->graphCfg, b->cfg) || b->graphJobs != b->jobs.size()) {
+            b->dropGraph();
+            prepareBatch(b, scale);
+            PT_HIP_TRY(hipStreamSynchronize(b->stream));
+            PT_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            try {
+                enqueueBatch(b, scale);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(b->stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            PT_HIP_TRY(hipStreamEndCapture(b->stream, &b->graph));
+            PT_HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+            size_t n = 0;
+            PT_HIP_TRY(hipGraphGetNodes(b->graph, nullptr, &n));
+            b->graphNodes = n;
+            b->graphScale = scale;
+            b->graphCfg = b->cfg;
+            b->graphJobs = b->jobs.size();
+        }
+        const auto t0 = std::chrono::steady_clock::now();
+        PT_HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+        PT_HIP_TRY(hipStreamSynchronize(b->stream));
+        ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    } else {
+        prepareBatch(b, scale);
+        PT_HIP_TRY(hipStreamSynchronize(b->stream));
+        const auto t0 = std::chrono::steady_clock::now();
+        enqueueBatch(b, scale);
+        PT_HIP_TRY(hipStreamSynchronize(b->stream));
+        ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        b->graphNodes = 0;
+    }
+    std::vector<unsigned long long> st(b->jobs.size() + 1);
+    PT_HIP_TRY(hipMemcpy(st.data(), b->stamps.p, st.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    b->latencyMs.assign(b->jobs.size(), 0.0);
+    for (size_t k = 0; k < b->order.size(); k++) b->latencyMs[b->order[k]] = (double)(st[1 + k] - st[0]) / b->clockKHz;
+    b->wallMs = ms;
+    return ms;
+}
+
+int64_t jobRays(pt_renderer *r) {
+    std::vector<unsigned long long> s(S_COUNT);
+    PT_HIP_TRY(hipMemcpy(s.data(), r->stats.p, S_COUNT * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    int64_t n = 0;
+    for (int k = 0; k < kMaxBounces; k++) n += (int64_t)(s[S_RAYS + k] + s[S_SHADOW + k]);
+    return n;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pt_batch_create(pt_batch **out) {
+    if (!out) return fail(PT_ERR_INVALID, "pt_batch_create: out is NULL");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(PT_ERR_DEVICE, "pt_batch_create: no HIP device");
+    pt_batch *b = new pt_batch();
+    PT_API_BEGIN
+    PT_HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    b->pp->counters.alloc(kNumCounters * sizeof(int));
+    b->pp->overflow.alloc(sizeof(int));
+    PT_HIP_TRY(hipMemsetAsync(b->pp->overflow.p, 0, sizeof(int), b->stream));
+    int dev = 0, khz = 0;
+    PT_HIP_TRY(hipGetDevice(&dev));
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0) b->clockKHz = khz;
+    PT_HIP_TRY(hipStreamSynchronize(b->stream));
+    *out = b;
+    return PT_OK;
+    }
+    catch (const HipFailure &e) {
+        delete b;
+        return fail(PT_ERR_DEVICE, "pt_batch_create: " + e.msg);
+    }
+}
+
+int pt_batch_destroy(pt_batch *b) {
+    if (!b) return PT_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    b->dropGraph();
+    for (pt_renderer *r : b->jobs) pt_renderer_destroy(r);
+    hipStream_t s = b->stream;
+    delete b;
+    if (s) (void)hipStreamDestroy(s);
+    return PT_OK;
+}
+
+int pt_batch_add(pt_batch *b, const pt_scene *scene, int width, int height, int spp, int max_depth, int *index) {
+    if (!b || !scene) return fail(PT_ERR_INVALID, "pt_batch_add: bad arguments");
+    if (spp <= 0 || max_depth < 0 || max_depth > kMaxBounces) return fail(PT_ERR_INVALID, "pt_batch_add: spp must be positive and depth in [0, 64]");
+    pt_renderer *r = nullptr;
+    int rc = pt_renderer_create(width, height, &r);
+    if (rc != PT_OK) return rc;
+    rc = pt_renderer_set_scene(r, scene);
+    if (rc != PT_OK) {
+        pt_renderer_destroy(r);
+        return rc;
+    }
+    PT_API_BEGIN
+    PT_HIP_TRY(hipStreamSynchronize(r->stream));
+    PT_HIP_TRY(hipStreamDestroy(r->stream));
+    r->stream = b->stream;
+    r->ownsStream = false;
+    r->pp = b->pp;
+    r->maxDepth = max_depth;
+    r->nee = 1;
+    r->type = PT_RENDERER_OPTIX;
+    b->jobs.push_back(r);
+    b->spp.push_back(spp);
+    b->dropGraph();
+    if (index) *index = (int)b->jobs.size() - 1;
+    return PT_OK;
+    PT_API_END("pt_batch_add")
+}
+
+int pt_batch_set_option(pt_batch *b, int option, int64_t v) {
+    if (!b) return fail(PT_ERR_INVALID, "pt_batch_set_option: batch is NULL");
+    switch (option) {
+    case PT_OPT_POOL_PATHS:
+        if (v > (1ll << 27)) return fail(PT_ERR_INVALID, "pt_batch_set_option: pool larger than 2^27 paths");
+        b->cfg.pool = v > 0 ? v : kDefaultPool;
+        break;
+    case PT_OPT_TRACE_BLOCKS_PER_CU: b->cfg.trace = v > 0 ? (int)std::min<int64_t>(v, 64) : 8; break;
+    case PT_OPT_SHADE_BLOCKS_PER_CU: b->cfg.shade = v > 0 ? (int)std::min<int64_t>(v, 64) : 8; break;
+    case PT_OPT_TRACE_LDS_STACK:
+        if (v != 0 && v != 16 && v != 32) return fail(PT_ERR_INVALID, "pt_batch_set_option: LDS stack entries must be 16 or 32");
+        b->cfg.lds = v == 0 ? 16 : (int)v;
+        break;
+    case PT_OPT_BATCH_GRAPH: b->useGraph = v != 0; break;
+    default: return fail(PT_ERR_INVALID, "pt_batch_set_option: unknown option");
+    }
+    return PT_OK;
+}
+
+int pt_batch_get_option(pt_batch *b, int option, int64_t *v) {
+    if (!b || !v) return fail(PT_ERR_INVALID, "pt_batch_get_option: bad arguments");
+    switch (option) {
+    case PT_OPT_POOL_PATHS: *v = b->cfg.pool; break;
+    case PT_OPT_TRACE_BLOCKS_PER_CU: *v = b->cfg.trace; break;
+    case PT_OPT_SHADE_BLOCKS_PER_CU: *v = b->cfg.shade; break;
+    case PT_OPT_TRACE_LDS_STACK: *v = b->cfg.lds; break;
+    case PT_OPT_BATCH_GRAPH: *v = b->useGraph ? 1 : 0; break;
+    default: return fail(PT_ERR_INVALID, "pt_batch_get_option: unknown option");
+    }
+    return PT_OK;
+}
+
+int pt_batch_run(pt_batch *b) {
+    if (!b) return fail(PT_ERR_INVALID, "pt_batch_run: batch is NULL");
+    if (b->jobs.empty()) return fail(PT_ERR_STATE, "pt_batch_run: no renders");
+    PT_API_BEGIN
+    runBatch(b, 1);
+    return PT_OK;
+    PT_API_END("pt_batch_run")
+}
+
+int pt_batch_autotune(pt_batch *b, int probe_divisor, pt_tune_result *out) {
+    if (!b || probe_divisor < 1) return fail(PT_ERR_INVALID, "pt_batch_autotune: bad arguments");
+    if (b->jobs.empty()) return fail(PT_ERR_STATE, "pt_batch_autotune: no renders");
+    PT_API_BEGIN
+    int64_t maxPaths = 0;
+    for (size_t j = 0; j < b->jobs.size(); j++)
+        maxPaths = std::max<int64_t>(maxPaths, (int64_t)b->jobs[j]->W * b->jobs[j]->H * std::max(1, b->spp[j] / probe_divisor));
+    auto probe = [&](const TuneCfg &c) {
+        b->cfg = c;
+        runBatch(b, probe_divisor);  // capture (or first enqueue) and warm-up
+        const double ms = runBatch(b, probe_divisor);
+        int64_t rays = 0;
+        for (pt_renderer *r : b->jobs) rays += jobRays(r);
+        return (double)rays / (ms * 1e3);
+    };
+    int probes = 0;
+    double rate = 0.0;
+    b->cfg = coordinateDescent(b->cfg, poolCandidates(maxPaths), probe, &probes, &rate);
+    b->dropGraph();
+    if (out) {
+        out->pool_paths = b->cfg.pool;
+        out->trace_blocks_per_cu = b->cfg.trace;
+        out->shade_blocks_per_cu = b->cfg.shade;
+        out->lds_stack = b->cfg.lds;
+        out->mrays_per_s = rate;
+        out->probes = probes;
+    }
+    return PT_OK;
+    PT_API_END("pt_batch_autotune")
+}
+
+int pt_batch_get_radiance(pt_batch *b, int index, float *out) {
+    if (!b || !out || index < 0 || index >= (int)b->jobs.size()) return fail(PT_ERR_INVALID, "pt_batch_get_radiance: bad arguments");
+    PT_API_BEGIN
+    pt_renderer *r = b->jobs[index];
+    PT_HIP_TRY(hipStreamSynchronize(b->stream));
+    const size_t n = (size_t)r->W * r->H * 3;
+    std::vector<double> acc(n);
+    PT_HIP_TRY(hipMemcpy(acc.data(), r->accum.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    const double inv = r->spp > 0 ? 1.0 / (double)r->spp : 0.0;
+    for (size_t i = 0; i < n; i++) out[i] = (float)(acc[i] * inv);
+    return PT_OK;
+    PT_API_END("pt_batch_get_radiance")
+}
+
+int pt_batch_get_render_stats(pt_batch *b, int index, pt_batch_render_stats *out) {
+    if (!b || !out || index < 0 || index >= (int)b->jobs.size()) return fail(PT_ERR_INVALID, "pt_batch_get_render_stats: bad arguments");
+    PT_API_BEGIN
+    PT_HIP_TRY(hipStreamSynchronize(b->stream));
+    pt_renderer *r = b->jobs[index];
+    memset(out, 0, sizeof *out);
+    out->width = r->W;
+    out->height = r->H;
+    out->spp = (int)r->spp;
+    out->triangles = (int64_t)r->triMat.size();
+    out->rays = jobRays(r);
+    out->latency_ms = index < (int)b->latencyMs.size() ? b->latencyMs[index] : 0.0;
+    return PT_OK;
+    PT_API_END("pt_batch_get_render_stats")
+}
+
+int pt_batch_get_stats(pt_batch *b, pt_batch_stats *out) {
+    if (!b || !out) return fail(PT_ERR_INVALID, "pt_batch_get_stats: bad arguments");
+    PT_API_BEGIN
+    PT_HIP_TRY(hipStreamSynchronize(b->stream));
+    memset(out, 0, sizeof *out);
+    out->renders = (int)b->jobs.size();
+    out->chunks = b->chunks;
+    out->graph_nodes = (int64_t)b->graphNodes;
+    out->wall_ms = b->wallMs;
+    int64_t rays = 0;
+    for (pt_renderer *r : b->jobs) rays += jobRays(r);
+    out->rays = rays;
+    int ov = 0;
+    PT_HIP_TRY(hipMemcpy(&ov, b->pp->overflow.p, sizeof(int), hipMemcpyDeviceToHost));
+    out->stack_overflows = ov;
+    return PT_OK;
+    PT_API_END("pt_batch_get_stats")
+}
+
+int pt_renderer_autotune(pt_renderer *r, int spp, pt_tune_result *out) {
+    if (guard("pt_renderer_autotune", r) || spp <= 0) return fail(PT_ERR_INVALID, "pt_renderer_autotune: bad arguments");
+    if (!r->hasScene) return fail(PT_ERR_STATE, "pt_renderer_autotune: no scene");
+    if (r->inFlight) return fail(PT_ERR_STATE, "pt_renderer_autotune: a pass is still running");
+    PT_API_BEGIN
+    const bool prof = r->profile;
+    r->profile = false;
+    auto probe = [&](const TuneCfg &c) {
+        applyCfg(r, c);
+        if (r->pp->pool > r->poolWanted) r->pp->pool = 0;
+        int rc = pt_renderer_render_async(r, spp);  // allocation and warm-up
+        if (rc == PT_OK) rc = pt_renderer_synchronize(r);
+        if (rc != PT_OK) throw HipFailure(pt_last_error());
+        const int64_t before = jobRays(r);
+        const auto t0 = std::chrono::steady_clock::now();
+        rc = pt_renderer_render_async(r, spp);
+        if (rc == PT_OK) rc = pt_renderer_synchronize(r);
+        if (rc != PT_OK) throw HipFailure(pt_last_error());
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return (double)(jobRays(r) - before) / (ms * 1e3);
+    };
+    TuneCfg start{r->poolWanted, r->traceBlocksPerCU, r->shadeBlocksPerCU, r->ldsStack};
+    int probes = 0;
+    double rate = 0.0;
+    const TuneCfg best = coordinateDescent(start, poolCandidates((int64_t)r->W * r->H * spp), probe, &probes, &rate);
+    applyCfg(r, best);
+    r->profile = prof;
+    r->resetPending = true;  // the probe samples are not part of the image
+    if (out) {
+        out->pool_paths = best.pool;
+        out->trace_blocks_per_cu = best.trace;
+        out->shade_blocks_per_cu = best.shade;
+        out->lds_stack = best.lds;
+        out->mrays_per_s = rate;
+        out->probes = probes;
+    }
+    return PT_OK;
+    PT_API_END("pt_renderer_autotune")
 }
 
 }  // extern "C"

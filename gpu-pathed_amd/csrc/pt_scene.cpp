@@ -190,17 +190,71 @@ static void syntheticEnvironment(pt_scene &s, int W, int H, uint64_t seed) {
     }
 }
 
+static void finishIcosphere(pt_scene &s, std::vector<float> &V, const std::vector<int> &F, uint64_t seed);
+
+static void icosahedron(std::vector<float> &V, std::vector<int> &F) {
+    const float t = (1.f + sqrtf(5.f)) / 2.f;
+    V = {-1, t, 0, 1, t, 0, -1, -t, 0, 1, -t, 0, 0, -1, t, 0, 1, t,
+         0, -1, -t, 0, 1, -t, t, 0, -1, t, 0, 1, -t, 0, -1, -t, 0, 1};
+    F = {0, 11, 5, 0, 5, 1, 0, 1, 7, 0, 7, 10, 0, 10, 11, 1, 5, 9, 5, 11, 4,
+         11, 10, 2, 10, 7, 6, 7, 1, 8, 3, 9, 4, 3, 4, 2, 3, 2, 6, 3, 6, 8,
+         3, 8, 9, 4, 9, 5, 2, 4, 11, 6, 2, 10, 8, 6, 7, 9, 8, 1};
+    for (size_t i = 0; i < V.size(); i += 3) norm3(&V[i]);
+}
+
+// Geodesic displaced sphere of about `target` triangles: each icosahedron face split into
+// n^2 triangles (n = round(sqrt(target / 20))), points projected to the unit sphere, then
+// displaced and given materials as the subdivided icosphere.  A point's position is summed
+// over the face corners in global-index order, so points on a shared edge come out bitwise
+// equal from both faces and the mesh stays closed.
+static void presetGeodesic(pt_scene &s, uint64_t seed, int64_t target) {
+    std::vector<float> C;
+    std::vector<int> CF;
+    icosahedron(C, CF);
+    const int n = std::max(1, (int)llround(sqrt((double)target / 20.0)));
+    std::vector<float> V;
+    std::vector<int> F;
+    V.reserve((size_t)20 * (n + 1) * (n + 2) / 2 * 3);
+    F.reserve((size_t)20 * n * n * 3);
+    for (int f = 0; f < 20; f++) {
+        const int ci[3] = {CF[3 * f], CF[3 * f + 1], CF[3 * f + 2]};
+        int order[3] = {0, 1, 2};
+        std::sort(order, order + 3, [&](int a, int b) { return ci[a] < ci[b]; });
+        const int base = (int)(V.size() / 3);
+        // point (i, j): weights (n - i - j, i, j) on corners (A, B, C) of the face
+        for (int i = 0; i <= n; i++)
+            for (int j = 0; j <= n - i; j++) {
+                const int w[3] = {n - i - j, i, j};
+                float p[3] = {0.f, 0.f, 0.f};
+                for (int k = 0; k < 3; k++) {
+                    const int c = order[k];
+                    const float wf = (float)w[c] / (float)n;
+                    for (int a = 0; a < 3; a++) p[a] += wf * C[3 * ci[c] + a];
+                }
+                norm3(p);
+                V.insert(V.end(), p, p + 3);
+            }
+        auto idx = [&](int i, int j) { return base + i * (n + 1) - i * (i - 1) / 2 + j; };
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n - i; j++) {
+                const int t0[3] = {idx(i, j), idx(i + 1, j), idx(i, j + 1)};
+                F.insert(F.end(), t0, t0 + 3);
+                if (i + j < n - 1) {
+                    const int t1[3] = {idx(i + 1, j), idx(i + 1, j + 1), idx(i, j + 1)};
+                    F.insert(F.end(), t1, t1 + 3);
+                }
+            }
+    }
+    finishIcosphere(s, V, F, seed);
+}
+
 // Displaced icosphere: level-`level` subdivision of the icosahedron (20 * 4^level faces),
 // vertices on the unit sphere pushed out radially by 0.05 * fbm3; per-face materials:
 // 30% Lambert (albedo U[0.2, 0.8]^3), 70% Beckmann alpha U[0.05, 0.5].
 static void presetIcosphere(pt_scene &s, uint64_t seed, int level) {
-    const float t = (1.f + sqrtf(5.f)) / 2.f;
-    std::vector<float> V = {-1, t, 0, 1, t, 0, -1, -t, 0, 1, -t, 0, 0, -1, t, 0, 1, t,
-                            0, -1, -t, 0, 1, -t, t, 0, -1, t, 0, 1, -t, 0, -1, -t, 0, 1};
-    std::vector<int> F = {0, 11, 5, 0, 5, 1, 0, 1, 7, 0, 7, 10, 0, 10, 11, 1, 5, 9, 5, 11, 4,
-                          11, 10, 2, 10, 7, 6, 7, 1, 8, 3, 9, 4, 3, 4, 2, 3, 2, 6, 3, 6, 8,
-                          3, 8, 9, 4, 9, 5, 2, 4, 11, 6, 2, 10, 8, 6, 7, 9, 8, 1};
-    for (size_t i = 0; i < V.size(); i += 3) norm3(&V[i]);
+    std::vector<float> V;
+    std::vector<int> F;
+    icosahedron(V, F);
     for (int l = 0; l < level; l++) {
         std::unordered_map<uint64_t, int> mid;
         mid.reserve(F.size());
@@ -225,6 +279,11 @@ static void presetIcosphere(pt_scene &s, uint64_t seed, int level) {
         }
         F.swap(F2);
     }
+    finishIcosphere(s, V, F, seed);
+}
+
+// Displacement, per-face materials, environment and camera shared by the icosphere workloads.
+static void finishIcosphere(pt_scene &s, std::vector<float> &V, const std::vector<int> &F, uint64_t seed) {
     const size_t nv = V.size() / 3;
     for (size_t i = 0; i < nv; i++) {
         float *p = &V[i * 3];
@@ -472,6 +531,10 @@ int pt_scene_preset(pt_scene *s, int preset, uint64_t seed, int64_t param) {
             break;
         }
         case 2: presetRoom(*s, seed, param > 0 ? param : 4000000); break;
+        case 4:
+            if (param > 50000000) return fail(PT_ERR_INVALID, "pt_scene_preset: geodesic sphere limited to 5e7 triangles");
+            presetGeodesic(*s, seed, param > 0 ? param : 1000000);
+            break;
         case 3: {
             const int w = param > 0 ? (int)param : 1024;
             syntheticEnvironment(*s, w, std::max(1, w / 2), seed);

@@ -114,6 +114,17 @@ __global__ void k_path_uniforms(uint64_t seed, const uint32_t *pix, const uint32
     for (int j = 0; j < k; j++) out[i * k + j] = r.uniform();
 }
 
+__global__ void k_vmf_sample(const float *mu, const float *kappa, const float *xi, int64_t n, float *dir, float *pdf) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const f3 m = mk(mu[3 * i], mu[3 * i + 1], mu[3 * i + 2]);
+    const f3 w = sampleVmf(m, kappa[i], xi[2 * i], xi[2 * i + 1]);
+    dir[3 * i] = w.x;
+    dir[3 * i + 1] = w.y;
+    dir[3 * i + 2] = w.z;
+    pdf[i] = vmfDensity(m, kappa[i], w);
+}
+
 // Small RAII helper for the unit calls: device copies of host arrays on a private stream.
 struct Stage {
     hipStream_t s = nullptr;
@@ -272,6 +283,23 @@ int pt_unit_path_uniforms(uint64_t seed, const uint32_t *pixel, const uint32_t *
     st.sync();
     return PT_OK;
     PT_UNIT_END("pt_unit_path_uniforms")
+}
+
+int pt_unit_vmf_sample(const float *mu, const float *kappa, const float *xi, int64_t n, float *dir, float *pdf) {
+    if (n < 0 || (n > 0 && (!mu || !kappa || !xi || !dir || !pdf))) return fail(PT_ERR_INVALID, "pt_unit_vmf_sample: bad arguments");
+    for (int64_t i = 0; i < n; i++)
+        if (!(kappa[i] > 0.f) || std::isinf(kappa[i])) return fail(PT_ERR_INVALID, "pt_unit_vmf_sample: kappa must be positive and finite");
+    PT_UNIT_BEGIN("pt_unit_vmf_sample")
+    Stage st;
+    const float *dm = st.in(mu, (size_t)n * 3), *dk = st.in(kappa, (size_t)n), *dx = st.in(xi, (size_t)n * 2);
+    float *dd = st.out<float>((size_t)n * 3), *dp = st.out<float>((size_t)n);
+    k_vmf_sample<<<nblk(n), 256, 0, st.s>>>(dm, dk, dx, n, dd, dp);
+    PT_HIP_TRY(hipGetLastError());
+    st.back(dir, dd, (size_t)n * 3);
+    st.back(pdf, dp, (size_t)n);
+    st.sync();
+    return PT_OK;
+    PT_UNIT_END("pt_unit_vmf_sample")
 }
 
 namespace {

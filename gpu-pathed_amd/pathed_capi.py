@@ -16,6 +16,7 @@ LAMBERTIAN, MIRROR, GLASS, MICROFACET = 0, 1, 2, 3
 RENDERER_CUDA, RENDERER_OPTIX, RENDERER_NORMALS, RENDERER_GUIDED = 0, 1, 2, 3
 OPT_POOL_PATHS, OPT_TRACE_BLOCKS_PER_CU, OPT_SHADE_BLOCKS_PER_CU, OPT_PROFILE_KERNELS = 0, 1, 2, 3
 OPT_CONSISTENT_MIS, OPT_GUIDE_TRAIN_STEPS, OPT_GUIDE_BATCH, OPT_GUIDE_LR_MICRO, OPT_GUIDE_RESET = 4, 5, 6, 7, 8
+OPT_TRACE_LDS_STACK, OPT_BATCH_GRAPH = 9, 10
 MISS = np.iinfo(np.int32).min
 
 # every symbol declared in include/pathed.h
@@ -31,6 +32,9 @@ EXPORTS = [
     "pt_renderer_get_path_radiance", "pt_renderer_intersect",
     "pt_unit_triangle_test", "pt_unit_bsdf_eval", "pt_unit_bsdf_sample", "pt_unit_env_tables", "pt_unit_env_sample",
     "pt_unit_path_uniforms", "pt_unit_guide_forward", "pt_unit_guide_gradients", "pt_unit_guide_adam", "pt_unit_mfma_tile",
+    "pt_unit_vmf_sample",
+    "pt_renderer_autotune", "pt_batch_create", "pt_batch_destroy", "pt_batch_add", "pt_batch_set_option", "pt_batch_get_option",
+    "pt_batch_autotune", "pt_batch_run", "pt_batch_get_radiance", "pt_batch_get_render_stats", "pt_batch_get_stats",
 ]
 GUIDE_PARAMS = 20800
 
@@ -59,6 +63,24 @@ class RenderStats(C.Structure):
 
 class KernelStat(C.Structure):
     _fields_ = [("name", C.c_char * 16), ("launches", C.c_int64), ("units", C.c_int64), ("ms", C.c_double), ("bytes", C.c_double)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("pool_paths", C.c_int64), ("trace_blocks_per_cu", C.c_int), ("shade_blocks_per_cu", C.c_int),
+                ("lds_stack", C.c_int), ("mrays_per_s", C.c_double), ("probes", C.c_int)]
+
+    def asdict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BatchRenderStats(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("spp", C.c_int), ("triangles", C.c_int64), ("rays", C.c_int64),
+                ("latency_ms", C.c_double)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("renders", C.c_int), ("chunks", C.c_int), ("graph_nodes", C.c_int64), ("rays", C.c_int64), ("wall_ms", C.c_double),
+                ("stack_overflows", C.c_int)]
 
 
 _lib = None
@@ -111,6 +133,17 @@ def lib():
         L.pt_renderer_get_kernel_stats.argtypes = [vp, C.POINTER(KernelStat), C.c_int, C.POINTER(C.c_int)]
         L.pt_renderer_get_path_radiance.argtypes = [vp, vp, C.c_int64, C.POINTER(C.c_int64)]
         L.pt_renderer_intersect.argtypes = [vp, vp, C.c_int64, C.c_int, vp]
+        L.pt_renderer_autotune.argtypes = [vp, C.c_int, C.POINTER(TuneResult)]
+        L.pt_batch_create.argtypes = [C.POINTER(vp)]
+        L.pt_batch_destroy.argtypes = [vp]
+        L.pt_batch_add.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]
+        L.pt_batch_set_option.argtypes = [vp, C.c_int, C.c_int64]
+        L.pt_batch_get_option.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
+        L.pt_batch_autotune.argtypes = [vp, C.c_int, C.POINTER(TuneResult)]
+        L.pt_batch_run.argtypes = [vp]
+        L.pt_batch_get_radiance.argtypes = [vp, C.c_int, vp]
+        L.pt_batch_get_render_stats.argtypes = [vp, C.c_int, C.POINTER(BatchRenderStats)]
+        L.pt_batch_get_stats.argtypes = [vp, C.POINTER(BatchStats)]
         L.pt_unit_triangle_test.argtypes = [vp, vp, C.c_int64, C.c_int, vp]
         L.pt_unit_bsdf_eval.argtypes = [vp, vp, vp, C.c_int64, vp, vp]
         L.pt_unit_bsdf_sample.argtypes = [vp, vp, vp, C.c_int64, vp]
@@ -122,6 +155,7 @@ def lib():
         L.pt_unit_guide_gradients.argtypes = [vp, vp, C.c_int64, vp, vp]
         L.pt_unit_guide_adam.argtypes = [vp, vp, vp, vp, C.c_int, C.c_float]
         L.pt_unit_mfma_tile.argtypes = [vp, vp, vp]
+        L.pt_unit_vmf_sample.argtypes = [vp, vp, vp, C.c_int64, vp, vp]
     return _lib
 
 
@@ -341,6 +375,69 @@ class Renderer:
         prim = out[:, 3].copy().view(np.int32)
         return out[:, 0], out[:, 1], out[:, 2], prim
 
+    def autotune(self, spp):
+        t = TuneResult()
+        check(lib().pt_renderer_autotune(self.h, spp, C.byref(t)))
+        return t.asdict()
+
+
+class Batch:
+    """Independent renders sharing one path pool, run as one captured graph (pt_batch)."""
+
+    def __init__(self):
+        self.h = C.c_void_p()
+        check(lib().pt_batch_create(C.byref(self.h)))
+        self.sizes = []
+
+    def close(self):
+        if self.h:
+            lib().pt_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, scene, width, height, spp, max_depth=8):
+        i = C.c_int()
+        check(lib().pt_batch_add(self.h, scene.h, width, height, spp, max_depth, C.byref(i)))
+        self.sizes.append((width, height))
+        return i.value
+
+    def set_option(self, opt, v):
+        check(lib().pt_batch_set_option(self.h, opt, v))
+
+    def get_option(self, opt):
+        v = C.c_int64()
+        check(lib().pt_batch_get_option(self.h, opt, C.byref(v)))
+        return v.value
+
+    def autotune(self, probe_divisor=8):
+        t = TuneResult()
+        check(lib().pt_batch_autotune(self.h, probe_divisor, C.byref(t)))
+        return t.asdict()
+
+    def run(self):
+        check(lib().pt_batch_run(self.h))
+
+    def radiance(self, i):
+        w, h = self.sizes[i]
+        out = np.zeros((h, w, 3), np.float32)
+        check(lib().pt_batch_get_radiance(self.h, i, _p(out)))
+        return out
+
+    def render_stats(self, i):
+        s = BatchRenderStats()
+        check(lib().pt_batch_get_render_stats(self.h, i, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def stats(self):
+        s = BatchStats()
+        check(lib().pt_batch_get_stats(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
 
 def triangle_test(tris, rays, mode):
     t = _f32(tris).reshape(-1, 9)
@@ -436,3 +533,14 @@ def mfma_tile(a, b):
     c = np.zeros((32, 32), np.float32)
     check(lib().pt_unit_mfma_tile(_p(a), _p(b), _p(c)))
     return c
+
+
+def vmf_sample(mu, kappa, xi):
+    """Directions drawn from vMF lobes on the GPU and the lobe density at them."""
+    m = _f32(mu).reshape(-1, 3)
+    k = _f32(kappa).reshape(-1)
+    x = _f32(xi).reshape(-1, 2)
+    d = np.zeros((k.size, 3), np.float32)
+    p = np.zeros(k.size, np.float32)
+    check(lib().pt_unit_vmf_sample(_p(m), _p(k), _p(x), k.size, _p(d), _p(p)))
+    return d, p

@@ -103,6 +103,8 @@ int pt_scene_set_camera(pt_scene *scene, const float origin[3], const float targ
  *      lit through a 0.3-wide slot by an area light (1e3) and the environment, glass sphere
  *      r = 0.5
  *   3  synthetic environment only (width = param, height = param/2) for sampling tests
+ *   4  displaced geodesic sphere of about `param` triangles (default 1e6; 20 n^2 with
+ *      n = round(sqrt(param / 20))), displacement, materials, environment and camera as 1
  */
 int pt_scene_preset(pt_scene *scene, int preset, uint64_t seed, int64_t param);
 
@@ -150,7 +152,8 @@ int pt_renderer_set_nee(pt_renderer *r, int enabled);
 int pt_renderer_set_type(pt_renderer *r, int renderer_type);
 int pt_renderer_set_seed(pt_renderer *r, uint64_t seed);
 
-/* Tuning knobs (queue capacity in paths, block counts); values <= 0 restore defaults. */
+/* Tuning knobs (queue capacity in paths, default 2^26; block counts); values <= 0 restore
+ * defaults. */
 typedef enum {
     PT_OPT_POOL_PATHS = 0,
     PT_OPT_TRACE_BLOCKS_PER_CU = 1,
@@ -162,8 +165,12 @@ typedef enum {
     PT_OPT_GUIDE_TRAIN_STEPS = 5,/* guided: Adam steps after each chunk (default 2) */
     PT_OPT_GUIDE_BATCH = 6,      /* guided: training records per step (default 2^18) */
     PT_OPT_GUIDE_LR_MICRO = 7,   /* guided: Adam learning rate in units of 1e-6 (default 10000) */
-    PT_OPT_GUIDE_RESET = 8       /* guided: re-initialise the network with seed = value;
+    PT_OPT_GUIDE_RESET = 8,      /* guided: re-initialise the network with seed = value;
                                     get: number of training steps taken */
+    PT_OPT_TRACE_LDS_STACK = 9,  /* traversal stack entries per lane kept in LDS (16 or 32);
+                                    deeper entries spill to global memory */
+    PT_OPT_BATCH_GRAPH = 10      /* batches only: 1 (default) = capture the whole batch in one
+                                    hipGraph and launch it once; 0 = launch kernels directly */
 } pt_option;
 int pt_renderer_set_option(pt_renderer *r, int option, int64_t value);
 int pt_renderer_get_option(pt_renderer *r, int option, int64_t *value);
@@ -274,6 +281,72 @@ int pt_unit_guide_forward(const float *params, const float *features, int64_t n,
 int pt_unit_guide_gradients(const float *params, const float *records, int64_t n, float *grad, float *loss);
 int pt_unit_guide_adam(float *params, float *m, float *v, const float *grad, int step, float lr);
 int pt_unit_mfma_tile(const float *a, const float *b, float *c);
+/* von Mises-Fisher lobes as the guided vertices use them: for n queries (mean mu[3n], unit;
+ * concentration kappa[n]; uniforms xi[2n]) -> sampled directions dir[3n] and the density
+ * of the lobe at them, pdf[n]. */
+int pt_unit_vmf_sample(const float *mu, const float *kappa, const float *xi, int64_t n, float *dir, float *pdf);
+
+/* ---------------------------------------------------------------------------------------
+ * Autotuning and batches of renders.  No counterpart in the reference (which renders one
+ * session at a time, render_session.h:59-69, with a fixed megakernel launch shape,
+ * path_tracer.cu:295-311); these serve callers that render many views or scenes at once.
+ * ------------------------------------------------------------------------------------- */
+
+/* Tuned launch configuration: path-pool capacity, trace/shade blocks per CU, LDS stack
+ * entries, and the rate (primary + extension + shadow rays per microsecond = Mrays/s) of
+ * the best probe. */
+typedef struct {
+    int64_t pool_paths;
+    int trace_blocks_per_cu;
+    int shade_blocks_per_cu;
+    int lds_stack;
+    double mrays_per_s;
+    int probes;
+} pt_tune_result;
+
+/* Coordinate descent over the four knobs, each candidate timed on a pass of `spp` samples
+ * (the probe samples are discarded: accumulation restarts at the next pass); the best
+ * configuration stays set on the renderer. */
+int pt_renderer_autotune(pt_renderer *r, int spp, pt_tune_result *out);
+
+/* A batch: independent renders (own scene, size, spp, depth; plain NEE + MIS) that share one
+ * path pool and one stream.  pt_batch_run queues them shortest-first (pixels x spp x log2
+ * triangles) and, with PT_OPT_BATCH_GRAPH on, captures the whole batch (per render:
+ * accumulator clear, per chunk raygen, depth x (extend, shade, shadow), accumulate, and a
+ * completion stamp from the device wall clock) in one hipGraph that is launched once and
+ * re-launched while the batch and its options are unchanged. */
+typedef struct pt_batch pt_batch;
+int pt_batch_create(pt_batch **out);
+int pt_batch_destroy(pt_batch *b);
+/* The scene is copied to the device; it may be destroyed afterwards.  index: position of the
+ * render in the batch. */
+int pt_batch_add(pt_batch *b, const pt_scene *scene, int width, int height, int spp, int max_depth, int *index);
+/* PT_OPT_POOL_PATHS, PT_OPT_TRACE_BLOCKS_PER_CU, PT_OPT_SHADE_BLOCKS_PER_CU,
+ * PT_OPT_TRACE_LDS_STACK, PT_OPT_BATCH_GRAPH */
+int pt_batch_set_option(pt_batch *b, int option, int64_t value);
+int pt_batch_get_option(pt_batch *b, int option, int64_t *value);
+/* Tunes the batch's options on probe runs of the same batch with spp / probe_divisor. */
+int pt_batch_autotune(pt_batch *b, int probe_divisor, pt_tune_result *out);
+/* Renders every render of the batch from scratch; blocks until done. */
+int pt_batch_run(pt_batch *b);
+/* Mean radiance of render `index` after a run: 3 floats per pixel, row 0 = bottom. */
+int pt_batch_get_radiance(pt_batch *b, int index, float *out);
+typedef struct {
+    int width, height, spp;
+    int64_t triangles;
+    int64_t rays;        /* primary + extension + shadow rays of the last run */
+    double latency_ms;   /* device time from the start of the last run to this render's end */
+} pt_batch_render_stats;
+int pt_batch_get_render_stats(pt_batch *b, int index, pt_batch_render_stats *out);
+typedef struct {
+    int renders;
+    int chunks;
+    int64_t graph_nodes;   /* nodes of the captured graph (0: direct launches) */
+    int64_t rays;
+    double wall_ms;        /* host wall time of the last run's launch to completion */
+    int stack_overflows;
+} pt_batch_stats;
+int pt_batch_get_stats(pt_batch *b, pt_batch_stats *out);
 
 #ifdef __cplusplus
 }

@@ -1,4 +1,6 @@
 """GPU parity of the guiding network's BF16 MFMA kernels against the numpy oracle."""
+import math
+
 import numpy as np
 import pytest
 
@@ -99,3 +101,35 @@ def test_adam_matches_oracle():
         assert np.allclose(gp[live], rp[live], rtol=1e-5, atol=1e-6)
         assert np.allclose(gm[live], rm[live], rtol=1e-5, atol=1e-9)
         p, m, v = gp.astype(np.float64), gm.astype(np.float64), gv.astype(np.float64)
+
+
+@pytest.mark.parametrize("kappa", [0.05, 2.0, 60.0, 5000.0])
+def test_vmf_sampler_matches_density(kappa):
+    """Jakob's vMF sampler: cos(theta) to the mean has the CDF
+    (exp(kappa (w - 1)) - exp(-2 kappa)) / (1 - exp(-2 kappa)), the azimuth is uniform, and the
+    returned density is kappa / (2 pi (1 - exp(-2 kappa))) exp(kappa (w - 1))."""
+    from scipy import stats
+
+    print(f"case: vmf sampler kappa={kappa}", flush=True)
+    rng = np.random.default_rng(int(kappa * 10))
+    n = 100000
+    mu = rng.normal(size=3)
+    mu /= np.linalg.norm(mu)
+    d, pdf = P.vmf_sample(np.tile(mu, (n, 1)), np.full(n, kappa), rng.random((n, 2)))
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=2e-6)
+    w = np.clip(d.astype(np.float64) @ mu, -1, 1)
+    e = math.exp(-2 * kappa)
+
+    def cdf(x):
+        return (np.exp(kappa * (np.asarray(x) - 1)) - e) / (1 - e)
+
+    assert stats.kstest(w, cdf).pvalue > 1e-3
+    # azimuth about mu in a frame of our own
+    a = np.cross(mu, [1.0, 0.0, 0.0] if abs(mu[0]) < 0.9 else [0.0, 1.0, 0.0])
+    a /= np.linalg.norm(a)
+    b = np.cross(mu, a)
+    phi = (np.arctan2(d @ b, d @ a) + np.pi) / (2 * np.pi)
+    ok = w < 1 - 1e-6
+    assert stats.kstest(phi[ok], "uniform").pvalue > 1e-3
+    ref = kappa / (2 * np.pi * (1 - e)) * np.exp(kappa * (w - 1))
+    assert np.allclose(pdf, ref, rtol=2e-3 if kappa < 1000 else 3e-2)

@@ -34,6 +34,31 @@ def test_vmf_normalised():
         assert f.mean() * 4 * np.pi == pytest.approx(1.0, rel=0.02)
 
 
+def test_mixture_is_a_normalised_density():
+    """Softmax weights sum to one, means are unit vectors, kappa and the selection fraction
+    respect their clamps, and the weighted vMF mixture integrates to one over the sphere."""
+    rng = np.random.default_rng(4)
+    out = rng.normal(scale=3.0, size=(1, G.OUT))
+    out[0, G.GO_KAPPA:G.GO_KAPPA + G.LOBES] = [-9, -3, 0, 1, 2, 4, 9, 20]
+    pi, mu, _, kappa, _, _, c = G.mixture(out)
+    assert pi.sum() == pytest.approx(1.0, abs=1e-12)
+    assert np.allclose(np.linalg.norm(mu, axis=2), 1.0)
+    assert np.allclose(kappa[0], np.exp(np.clip([-9, -3, 0, 1, 2, 4, 9, 20], G.LOGK_MIN, G.LOGK_MAX)))
+    assert G.FRAC_MIN <= c[0] <= G.FRAC_MAX
+    # stratified uniform directions (equal-area bands in z), the sharp lobes need many
+    n = 2_000_000
+    k = 4000
+    z = 1 - 2 * ((np.arange(k)[:, None] + rng.random((k, n // k))) / k).reshape(-1)
+    ph = rng.random(n) * 2 * np.pi
+    r = np.sqrt(1 - z * z)
+    v = np.stack([r * np.cos(ph), r * np.sin(ph), z], 1)
+    q = np.zeros(n)
+    for j in range(G.LOBES):
+        f, _ = G.vmf(np.tile(mu[:, j:j + 1], (n, 1, 1)), np.full((n, 1), kappa[0, j]), v)
+        q += pi[0, j] * f[:, 0]
+    assert q.mean() * 4 * np.pi == pytest.approx(1.0, rel=0.03)
+
+
 def test_analytic_gradient_matches_finite_differences():
     rng = np.random.default_rng(1)
     p = G.init_params(3)
