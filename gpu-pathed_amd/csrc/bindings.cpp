@@ -50,7 +50,9 @@ PYBIND11_MODULE(path_tracer, m) {
              },
              "sRGB RGBA8 framebuffer, shape (height, width, 4)")
         .def("getAccumulatedSpp", &RenderSession::getAccumulatedSpp)
-        .def("loadPreset", &RenderSession::loadPreset, py::arg("preset"), py::arg("seed") = 0, py::arg("param") = 0);
+        .def("loadPreset", &RenderSession::loadPreset, py::arg("preset"), py::arg("seed") = 0, py::arg("param") = 0)
+        .def("loadReferenceScene", &RenderSession::loadReferenceScene, py::arg("index"), py::arg("scenes_dir"))
+        .def("setExrDumpPath", &RenderSession::setExrDumpPath, py::arg("path"));
 
     py::enum_<RendererType>(m, "RendererType")
         .value("CUDA", RendererType::CUDA)

@@ -284,6 +284,16 @@ RenderState RenderSession::pollRender() {
         int64_t spp = 0;
         check(pt_renderer_get_spp(m_renderer, &spp));
         m_sceneModel->updateSpp((int)spp);
+        // render_session.h:87-101: dump the radiance when this pass crossed a power of two
+        if (!m_exrDumpPath.empty()) {
+            for (int64_t t = 2; t <= (1 << 20); t <<= 1) {
+                if (spp >= t && spp - m_currentSpp < t) {
+                    const std::vector<float> radiance = getRadianceBuffer();
+                    check(pt_exr_write_rgb(m_exrDumpPath.c_str(), radiance.data(), m_width, m_height, 1));
+                    break;
+                }
+            }
+        }
         if (m_resetRenderer) {
             // render_session.h:103-114: switch the integrator and restart accumulation
             check(pt_renderer_set_type(m_renderer, (int)m_rendererType));
@@ -314,6 +324,23 @@ int64_t RenderSession::getAccumulatedSpp() {
     int64_t s = 0;
     check(pt_renderer_get_spp(m_renderer, &s));
     return s;
+}
+
+void RenderSession::loadReferenceScene(int index, const std::string &scenesDir) {
+    if (m_rendering) throw std::runtime_error("loadReferenceScene: a pass is still running");
+    pt_scene *fresh = nullptr;
+    check(pt_scene_create(&fresh));
+    const int rc = pt_scene_reference(fresh, index, scenesDir.c_str());
+    if (rc != PT_OK) {
+        const std::string msg = pt_last_error();
+        pt_scene_destroy(fresh);
+        throw std::runtime_error(msg);
+    }
+    pt_scene_destroy(m_hostScene);
+    m_hostScene = fresh;
+    m_dirtyMaterials.clear();
+    buildSceneObjects();
+    if (m_renderer) init(m_pbo1, m_pbo2);
 }
 
 void RenderSession::loadPreset(int preset, uint64_t seed, int64_t param) {

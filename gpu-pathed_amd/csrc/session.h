@@ -219,6 +219,11 @@ public:
     std::vector<uint8_t> getFramebuffer();   // uchar4 sRGB of framebuffer.cu:35-40
     int64_t getAccumulatedSpp();
     void loadPreset(int preset, uint64_t seed, int64_t param);
+    // the reference's scene `index` (scene.cu:14-176) from its scenes/ folder layout
+    void loadReferenceScene(int index, const std::string &scenesDir);
+    // render_session.h:87-101 writes out.exr whenever the spp count crosses a power of two;
+    // here that dump is opt-in: a non-empty path enables it
+    void setExrDumpPath(const std::string &path) { m_exrDumpPath = path; }
     pt_renderer *renderer() { return m_renderer; }
 
 private:
@@ -238,6 +243,7 @@ private:
     bool m_resetRenderer = false;
     RendererType m_rendererType;
     std::vector<int> m_dirtyMaterials;  // edited before init()
+    std::string m_exrDumpPath;
     pt_scene *m_hostScene = nullptr;
     pt_renderer *m_renderer = nullptr;
     std::unique_ptr<Scene> m_scene;

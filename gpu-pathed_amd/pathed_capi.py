@@ -33,6 +33,8 @@ EXPORTS = [
     "pt_unit_triangle_test", "pt_unit_bsdf_eval", "pt_unit_bsdf_sample", "pt_unit_env_tables", "pt_unit_env_sample",
     "pt_unit_path_uniforms", "pt_unit_guide_forward", "pt_unit_guide_gradients", "pt_unit_guide_adam", "pt_unit_mfma_tile",
     "pt_unit_vmf_sample",
+    "pt_scene_load_obj", "pt_scene_load_ply", "pt_scene_load_exr_environment", "pt_scene_reference", "pt_exr_info",
+    "pt_exr_read_rgba", "pt_exr_write_rgb",
     "pt_renderer_autotune", "pt_batch_create", "pt_batch_destroy", "pt_batch_add", "pt_batch_set_option", "pt_batch_get_option",
     "pt_batch_autotune", "pt_batch_run", "pt_batch_get_radiance", "pt_batch_get_render_stats", "pt_batch_get_stats",
 ]
@@ -134,6 +136,13 @@ def lib():
         L.pt_renderer_get_path_radiance.argtypes = [vp, vp, C.c_int64, C.POINTER(C.c_int64)]
         L.pt_renderer_intersect.argtypes = [vp, vp, C.c_int64, C.c_int, vp]
         L.pt_renderer_autotune.argtypes = [vp, C.c_int, C.POINTER(TuneResult)]
+        L.pt_scene_load_obj.argtypes = [vp, C.c_char_p, C.c_int, C.POINTER(C.c_int)]
+        L.pt_scene_load_ply.argtypes = [vp, C.c_char_p, C.c_int]
+        L.pt_scene_load_exr_environment.argtypes = [vp, C.c_char_p, C.c_int]
+        L.pt_scene_reference.argtypes = [vp, C.c_int, C.c_char_p]
+        L.pt_exr_info.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.pt_exr_read_rgba.argtypes = [C.c_char_p, vp, C.c_int64]
+        L.pt_exr_write_rgb.argtypes = [C.c_char_p, vp, C.c_int, C.c_int, C.c_int]
         L.pt_batch_create.argtypes = [C.POINTER(vp)]
         L.pt_batch_destroy.argtypes = [vp]
         L.pt_batch_add.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]
@@ -215,6 +224,24 @@ class Scene:
         r = _f32(radii).reshape(-1)
         m = np.ascontiguousarray(material_ids, dtype=np.int32)
         check(lib().pt_scene_add_spheres(self.h, _p(c), _p(r), _p(m), m.size))
+
+    @classmethod
+    def reference(cls, index, scenes_dir):
+        """The reference's built-in scene `index` (0..3) read from its scenes/ folder layout."""
+        s = cls()
+        check(lib().pt_scene_reference(s.h, index, os.fsencode(scenes_dir)))
+        return s
+
+    def load_obj(self, path, default_material=0):
+        first = C.c_int()
+        check(lib().pt_scene_load_obj(self.h, os.fsencode(path), default_material, C.byref(first)))
+        return first.value
+
+    def load_ply(self, path, material):
+        check(lib().pt_scene_load_ply(self.h, os.fsencode(path), material))
+
+    def load_exr_environment(self, path, sin_theta_weight=False):
+        check(lib().pt_scene_load_exr_environment(self.h, os.fsencode(path), int(sin_theta_weight)))
 
     def set_environment(self, rgba, sin_theta_weight=False):
         e = _f32(rgba)
@@ -544,3 +571,19 @@ def vmf_sample(mu, kappa, xi):
     p = np.zeros(k.size, np.float32)
     check(lib().pt_unit_vmf_sample(_p(m), _p(k), _p(x), k.size, _p(d), _p(p)))
     return d, p
+
+
+def exr_read(path):
+    """RGBA float image (H, W, 4) of an OpenEXR file, scanline order (row 0 = top)."""
+    w, h = C.c_int(), C.c_int()
+    check(lib().pt_exr_info(os.fsencode(path), C.byref(w), C.byref(h)))
+    out = np.zeros((h.value, w.value, 4), np.float32)
+    check(lib().pt_exr_read_rgba(os.fsencode(path), _p(out), out.size))
+    return out
+
+
+def exr_write(path, radiance, half=True):
+    """Writes an (H, W, 3) radiance buffer (row 0 = bottom) as an OpenEXR file."""
+    r = _f32(radiance)
+    h, w = r.shape[0], r.shape[1]
+    check(lib().pt_exr_write_rgb(os.fsencode(path), _p(r), w, h, int(half)))

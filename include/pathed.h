@@ -29,7 +29,8 @@ typedef enum {
     PT_ERR_DEVICE = -2,     /* HIP runtime error or no device */
     PT_ERR_MEMORY = -3,     /* allocation failed */
     PT_ERR_STATE = -4,      /* call not valid in the current state (no scene, pass running) */
-    PT_ERR_NUMERIC = -5     /* degenerate input (e.g. camera looks along its up vector) */
+    PT_ERR_NUMERIC = -5,    /* degenerate input (e.g. camera looks along its up vector) */
+    PT_ERR_IO = -6          /* a file could not be read, parsed or written */
 } pt_status;
 
 /* Material kinds: values of the reference's MaterialType (include/materials/types.h:5-11). */
@@ -285,6 +286,36 @@ int pt_unit_mfma_tile(const float *a, const float *b, float *c);
  * concentration kappa[n]; uniforms xi[2n]) -> sampled directions dir[3n] and the density
  * of the lobe at them, pdf[n]. */
 int pt_unit_vmf_sample(const float *mu, const float *kappa, const float *xi, int64_t n, float *dir, float *pdf);
+
+/* ---------------------------------------------------------------------------------------
+ * Scene and image files (host only; no device needed).
+ * ------------------------------------------------------------------------------------- */
+
+/* Wavefront OBJ (+ MTL) appended to the scene, as src/parsers/obj_parser.cpp:20-269 and
+ * src/scene_data.cu:20-52 read it: v, vn, f (quads of plain indices split (0,1,2) (0,2,3);
+ * triangles of v/t/n or v//n corners; triangles of plain indices are read as well), negative
+ * indices relative to the end, mtllib / usemtl.  The file's materials (newmtl, Kd, Ke;
+ * src/parsers/mtl_parser.cpp:15-109) become Lambertians appended in name order; faces of a
+ * file without materials take default_material.  first_material (may be NULL): id of the
+ * first appended material, or -1. */
+int pt_scene_load_obj(pt_scene *scene, const char *path, int default_material, int *first_material);
+/* Binary little-endian PLY (float x y z vertices, uchar-count int-index triangles), every
+ * face with `material` (src/parsers/ply_parser.cpp:9-112). */
+int pt_scene_load_ply(pt_scene *scene, const char *path, int material);
+/* Environment map from an OpenEXR file (src/lights/environment_light.cu:14-56): single-part
+ * scanline files with NONE, ZIPS or ZIP compression and HALF / FLOAT / UINT R, G, B [, A]
+ * channels (PIZ and other codecs return PT_ERR_IO). */
+int pt_scene_load_exr_environment(pt_scene *scene, const char *path, int sin_theta_weight);
+/* The reference's built-in scenes 0..3 (src/scene.cu:14-176: Cornell box OBJ, glossy Cornell
+ * box with a mirror box and a glass ball, plane + 1-pixel environment, Veach MIS plates) read
+ * from `scenes_dir` (the reference's scenes/ folder layout); the scene must be empty. */
+int pt_scene_reference(pt_scene *scene, int index, const char *scenes_dir);
+/* OpenEXR images: size; RGBA floats (4*w*h, scanline order, alpha 1 when absent);
+ * writing an RGB radiance buffer (3*w*h, row 0 = bottom, as pt_renderer_get_radiance) as
+ * uncompressed B, G, R scanlines, HALF when `half` else FLOAT (src/io/image.cpp:10-78). */
+int pt_exr_info(const char *path, int *width, int *height);
+int pt_exr_read_rgba(const char *path, float *rgba, int64_t capacity);
+int pt_exr_write_rgb(const char *path, const float *radiance, int width, int height, int half);
 
 /* ---------------------------------------------------------------------------------------
  * Autotuning and batches of renders.  No counterpart in the reference (which renders one
