@@ -463,8 +463,9 @@ __device__ __forceinline__ float mixturePdf(const float4 *G, f3 w) {
 // direction comes from the one-sample mixture c q + (1 - c) p_bsdf of the learned vMF lobes and
 // the BSDF, the continuation is weighted by f cos / (c q + (1 - c) p_bsdf), and the same mixture
 // density is the directional technique's pdf in both MIS weights.
+// Plain shading fits 96 VGPRs without spilling: 5 waves per SIMD.
 template <bool GUIDED>
-__global__ __launch_bounds__(256) void k_shade(PtSceneView sc, ChunkParams cp, int bounce, int *counters, const float4 *rayO, const float4 *rayD,
+__global__ __launch_bounds__(256, GUIDED ? 3 : 5) void k_shade(PtSceneView sc, ChunkParams cp, int bounce, int *counters, const float4 *rayO, const float4 *rayD,
                                                const uint4 *state, const float4 *beta, const float4 *hits, float4 *nRayO, float4 *nRayD,
                                                uint4 *nState, float4 *nBeta, float4 *sO, float4 *sD, float4 *sL, float4 *Lpath,
                                                GuideShadeArgs ga) {
@@ -472,22 +473,20 @@ __global__ __launch_bounds__(256) void k_shade(PtSceneView sc, ChunkParams cp, i
     int *cNext = counters + C_PER_BOUNCE * (bounce + 1);
     const int n = c[C_RAYS];
     const int waveStride = gridDim.x * blockDim.x;
+    // Queue appends happen where each record is complete (one atomic per wave and branch:
+    // the lanes taking a branch append together), so no output record stays live across the
+    // rest of the vertex.
     for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n; base += waveStride) {
         const int i = base + laneId();
-        const bool valid = i < n;
-        bool pushRay = false, pushShadow = false, addL = false, wantRec = false;
-        float4 outRO, outRD, outB, shO, shD, shL;
-        float4 rec0, rec1, rec2, rec3;
-        uint4 outS;
-        f3 L = mk(0.f);
-        int pathId = 0;
-        if (valid) {
+        bool addL = false;
+        if (i < n) {
             const float4 ro = rayO[i], rd = rayD[i], h = hits[i], bp = beta[i];
             const uint4 st = state[i];
-            pathId = (int)st.z;
+            const int pathId = (int)st.z;
             const int depth = (int)st.w;
             const f3 o = xyz(ro), d = xyz(rd), thr = xyz(bp);
             const float prevPdf = bp.w;
+            f3 L = mk(0.f);
             if constexpr (GUIDED) {
                 // radiance of the path before this vertex: the baseline of the training record of
                 // the vertex that sampled this ray
@@ -528,6 +527,7 @@ __global__ __launch_bounds__(256) void k_shade(PtSceneView sc, ChunkParams cp, i
                     }
                     BsdfSample bs;
                     float pdfDir;  // density of the direction sampling technique at bs.wi
+                    float pbRec = 0.f;
                     if (GUIDED && G) {
                         const float usel = rng.uniform();
                         if (usel < frac) {
@@ -549,10 +549,9 @@ __global__ __launch_bounds__(256) void k_shade(PtSceneView sc, ChunkParams cp, i
                             const float x2 = dims >= 2 ? rng.uniform() : 0.f;
                             bs = bsdfSample(m, wo, x1, x2);
                         }
-                        const float pb = bsdfSamplePdf(m, wo, bs.wi);
-                        pdfDir = frac * mixturePdf(G, toWorld(fr, bs.wi)) + (1.f - frac) * pb;
+                        pbRec = bsdfSamplePdf(m, wo, bs.wi);
+                        pdfDir = frac * mixturePdf(G, toWorld(fr, bs.wi)) + (1.f - frac) * pbRec;
                         bs.pdf = pdfDir;
-                        rec2.w = pb;
                     } else {
                         const int dims = bsdfDims(type);
                         const float x1 = dims >= 1 ? rng.uniform() : 0.f;
@@ -565,6 +564,8 @@ __global__ __launch_bounds__(256) void k_shade(PtSceneView sc, ChunkParams cp, i
                         // draws the three uniforms and adds nothing
                         const float l1 = rng.uniform(), l2 = rng.uniform(), l3 = rng.uniform();
                         const bool anyLight = sc.numEmitters > 0 || sc.env.present;
+                        bool pushShadow = false;
+                        f3 contrib = mk(0.f);
                         LightSample ls;
                         if (anyLight) ls = sampleLights(sc, sp.p, l1, l2, l3);
                         if (anyLight && dot(ls.n, ls.wi) < 0.f && ls.pdf > 0.f) {
@@ -573,37 +574,49 @@ __global__ __launch_bounds__(256) void k_shade(PtSceneView sc, ChunkParams cp, i
                             if (GUIDED && G) pB = frac * mixturePdf(G, ls.wi) + (1.f - frac) * bsdfSamplePdf(m, wo, wiL);
                             else pB = cp.misSample ? bsdfSamplePdf(m, wo, wiL) : bsdfPdf(m, wo, wiL);
                             const float w = balance(ls.pdf, pB);
-                            const f3 contrib = thr * (ls.emitted * w * bsdfF(m, wo, wiL) * fabsf(dot(sp.n, ls.wi)) / ls.pdf);
-                            if (!isZero(contrib) && !(contrib.x != contrib.x)) {
-                                pushShadow = true;
-                                shO = f4(sp.p, ls.dist - 2e-4f);
-                                shD = f4(ls.wi, asFloat(pathId));
-                                shL = f4(contrib, 0.f);
-                            }
+                            contrib = thr * (ls.emitted * w * bsdfF(m, wo, wiL) * fabsf(dot(sp.n, ls.wi)) / ls.pdf);
+                            pushShadow = !isZero(contrib) && !(contrib.x != contrib.x);
+                        }
+                        const int ss = waveAppend(pushShadow, &c[C_SHADOW]);
+                        if (pushShadow) {
+                            sO[ss] = f4(sp.p, ls.dist - 2e-4f);
+                            sD[ss] = f4(ls.wi, asFloat(pathId));
+                            sL[ss] = f4(contrib, 0.f);
                         }
                     }
+                    bool pushRay = false;
+                    f3 nb = mk(0.f);
                     if (!isZero(bs.f) && bs.pdf > 0.f) {
-                        const f3 nb = thr * (bs.f * fabsf(bs.wi.z) / bs.pdf);
-                        if (!isZero(nb) && nb.x == nb.x) {
-                            pushRay = true;
-                            const f3 wworld = toWorld(fr, bs.wi);
-                            outRO = f4(sp.p, kBounceEps);
-                            outRD = f4(wworld, FLT_MAX);
-                            outS = make_uint4((uint32_t)rng.state, (uint32_t)(rng.state >> 32), (uint32_t)pathId, (uint32_t)(depth + 1));
-                            outB = f4(nb, bs.delta ? -1.f : pdfDir);
-                            if (GUIDED && G) {
-                                const float lt = lum(thr);
-                                if (lt > 0.f) {
-                                    float f[GUIDE_RAW];
-                                    const float rough = type == PT_MAT_LAMBERTIAN ? 1.f : fminf(m.b.w * (1.f / 0.6f), 1.f);
-                                    guideFeatures(ga.norm, sp.p, sp.n, -d, rough, f);
-                                    rec0 = make_float4(f[0], f[1], f[2], f[3]);
-                                    rec1 = make_float4(f[4], f[5], f[6], f[7]);
-                                    rec2 = f4(wworld, rec2.w);
-                                    rec3 = make_float4(1.f / lt, asFloat(pathId), -1.f, 0.f);
-                                    wantRec = true;
-                                }
+                        nb = thr * (bs.f * fabsf(bs.wi.z) / bs.pdf);
+                        pushRay = !isZero(nb) && nb.x == nb.x;
+                    }
+                    const int rs = waveAppend(pushRay, &cNext[C_RAYS]);
+                    const f3 wworld = toWorld(fr, bs.wi);
+                    if (pushRay) {
+                        nRayO[rs] = f4(sp.p, kBounceEps);
+                        nRayD[rs] = f4(wworld, FLT_MAX);
+                        nState[rs] = make_uint4((uint32_t)rng.state, (uint32_t)(rng.state >> 32), (uint32_t)pathId, (uint32_t)(depth + 1));
+                        nBeta[rs] = f4(nb, bs.delta ? -1.f : pdfDir);
+                    }
+                    if constexpr (GUIDED) {
+                        // a training record per guided vertex whose path continues
+                        const float lt = lum(thr);
+                        const bool wantRec = pushRay && G != nullptr && lt > 0.f;
+                        const int slotRec = waveAppend(wantRec, &counters[kRecCountSlot]);
+                        if (pushRay) {
+                            int ridx = -1;
+                            if (wantRec && slotRec < ga.recCap) {
+                                ridx = slotRec;
+                                float f[GUIDE_RAW];
+                                const float rough = type == PT_MAT_LAMBERTIAN ? 1.f : fminf(m.b.w * (1.f / 0.6f), 1.f);
+                                guideFeatures(ga.norm, sp.p, sp.n, -d, rough, f);
+                                float4 *R = ga.records + (size_t)slotRec * GUIDE_TREC_F4;
+                                R[0] = make_float4(f[0], f[1], f[2], f[3]);
+                                R[1] = make_float4(f[4], f[5], f[6], f[7]);
+                                R[2] = f4(wworld, pbRec);
+                                R[3] = make_float4(1.f / lt, asFloat(pathId), -1.f, 0.f);
                             }
+                            ga.recOut[rs] = ridx;
                         }
                     }
                 }
@@ -614,34 +627,6 @@ __global__ __launch_bounds__(256) void k_shade(PtSceneView sc, ChunkParams cp, i
                 acc.x += L.x; acc.y += L.y; acc.z += L.z;
                 Lpath[pathId] = acc;
             }
-        }
-        const int rs = waveAppend(pushRay, &cNext[C_RAYS]);
-        if (pushRay) {
-            nRayO[rs] = outRO;
-            nRayD[rs] = outRD;
-            nState[rs] = outS;
-            nBeta[rs] = outB;
-        }
-        if constexpr (GUIDED) {
-            const int slotRec = waveAppend(wantRec, &counters[kRecCountSlot]);
-            if (pushRay) {
-                int ridx = -1;
-                if (wantRec && slotRec < ga.recCap) {
-                    ridx = slotRec;
-                    float4 *R = ga.records + (size_t)slotRec * GUIDE_TREC_F4;
-                    R[0] = rec0;
-                    R[1] = rec1;
-                    R[2] = rec2;
-                    R[3] = rec3;
-                }
-                ga.recOut[rs] = ridx;
-            }
-        }
-        const int ss = waveAppend(pushShadow, &c[C_SHADOW]);
-        if (pushShadow) {
-            sO[ss] = shO;
-            sD[ss] = shD;
-            sL[ss] = shL;
         }
         waveCount(addL, &c[C_LADD_SHADE]);
     }
