@@ -32,6 +32,23 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "wavefront_icosphere1M_1024sq_64spp_hbm_traffic.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary of this workload (lines 'traffic_bytes_per_dispatch: <kernel> <bytes>'), or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                parts = line.strip().strip("`").split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_dispatch:" and parts[1] == kernel:
+                    return int(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(scene_arrays, budget_s):
     """The oracle (scalar C++ restatement of the reference's shading and intersection, with
     a median-split BVH) on the host cores: a band of rows of the same image at as many
@@ -68,12 +85,13 @@ ROOM_W, ROOM_H, ROOM_DEPTH = 2048, 1024, 16
 ROOM_REFERENCE = os.path.join(ROOT, "tests", "golden", "room_reference_256px.npz")
 
 
-def room_equal_time(P, budget_s, spp_per_pass):
+def room_equal_time(P, budget_s, spp_per_pass, repeats=1):
     """Hard indirect lighting, equal wall time: the closed room of 4M triangles lit through a
     0.3-wide slot (2048x1024, depth 16, NEE + consistent MIS) rendered for `budget_s` seconds
     by the plain and by the guided renderer (network trained from scratch inside the budget).
     Mean squared error at a fixed seeded subset of 256 pixels against the FP64 oracle's
-    65,536-sample estimate there (the committed reference fixture)."""
+    65,536-sample estimate there (the committed reference fixture).  With repeats > 1 each
+    renderer runs that many times with different PCG32 seeds and the MSEs are averaged."""
     import numpy as np
 
     ref = np.load(ROOM_REFERENCE)
@@ -89,51 +107,64 @@ def room_equal_time(P, budget_s, spp_per_pass):
     ref_var = (ref["sumsq"] / ref["spp"] - ref_mean**2) / ref["spp"]
     res = {}
     for mode in ("plain", "guided"):
-        r = P.Renderer(ROOM_W, ROOM_H)
-        r.set_scene(scene)
-        r.set_max_depth(ROOM_DEPTH)
-        r.set_nee(True)
-        r.set_option(P.OPT_CONSISTENT_MIS, 1)
-        if mode == "guided":
-            r.set_type(P.RENDERER_GUIDED)
-        # load the kernels, then start again from nothing: fresh accumulation, fresh network
-        r.render(1)
-        if mode == "guided":
-            r.set_option(P.OPT_GUIDE_RESET, 1)
-        r.reset()
-        r.reset_stats()
-        r.synchronize()
-        t0 = time.perf_counter()
-        curve = []
-        while True:
-            r.render(spp_per_pass)
-            el = time.perf_counter() - t0
-            if el >= budget_s:
-                break
-            if len(curve) < 64 and r.spp() & (r.spp() - 1) == 0:
-                img = r.radiance_sum().reshape(-1, 3)[pix] / r.spp()
-                curve.append((round(el, 3), r.spp(), float(((img - ref_mean) ** 2).mean())))
-                t0 += time.perf_counter() - t0 - el  # the error readback is not render time
-        el = time.perf_counter() - t0
-        spp = r.spp()
-        img = r.radiance_sum().reshape(-1, 3)[pix] / spp
-        st = r.stats()
-        mse = float(((img - ref_mean) ** 2).mean())
-        res[mode] = {"mse": mse, "rel_mse": float((((img - ref_mean) ** 2) / (ref_mean**2 + 1e-2)).mean()),
-                     "spp": spp, "seconds": round(el, 3), "mrays_s": round(st.rays_total / el / 1e6, 2),
-                     "guide_steps": int(st.guide_steps), "curve": curve}
-        log(f"room {mode}: {spp} spp in {el:.1f}s, MSE {mse:.5g}, {res[mode]['mrays_s']} Mrays/s")
-        r.close()
+        runs = [room_run(P, scene, mode, budget_s, spp_per_pass, seed, pix, ref_mean) for seed in range(repeats)]
+        res[mode] = runs[0] if repeats == 1 else {
+            "mse": float(np.mean([x["mse"] for x in runs])), "rel_mse": float(np.mean([x["rel_mse"] for x in runs])),
+            "spp": [x["spp"] for x in runs], "seconds": [x["seconds"] for x in runs], "mrays_s": [x["mrays_s"] for x in runs],
+            "mse_runs": [x["mse"] for x in runs], "guide_steps": [x["guide_steps"] for x in runs]}
     return {
         "metric": "equal-time MSE ratio (plain / guided)", "value": round(res["plain"]["mse"] / res["guided"]["mse"], 3),
         "unit": "x", "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"closed room, {scene.info().n_triangles} tris, 0.3-wide slot, light 1e3, glass sphere r 0.5, "
                                f"2048x1024 env, {ROOM_W}x{ROOM_H}, depth {ROOM_DEPTH}, NEE + consistent MIS, "
-                               f"{budget_s:.0f} s per renderer",
+                               f"{budget_s:.0f} s per renderer, {repeats} seed(s)",
                    "pixels": int(pix.size), "reference_spp": int(ref["spp"]),
                    "reference_noise_mse": float(ref_var.mean())},
         "plain": res["plain"], "guided": res["guided"],
     }
+
+
+def room_run(P, scene, mode, budget_s, spp_per_pass, seed, pix, ref_mean):
+    """One equal-time render of the room by the plain or the guided renderer."""
+    import numpy as np
+
+    r = P.Renderer(ROOM_W, ROOM_H)
+    r.set_scene(scene)
+    r.set_max_depth(ROOM_DEPTH)
+    r.set_nee(True)
+    r.set_option(P.OPT_CONSISTENT_MIS, 1)
+    r.set_seed(seed)
+    if mode == "guided":
+        r.set_type(P.RENDERER_GUIDED)
+    # load the kernels, then start again from nothing: fresh accumulation, fresh network
+    r.render(1)
+    if mode == "guided":
+        r.set_option(P.OPT_GUIDE_RESET, 1)
+    r.reset()
+    r.reset_stats()
+    r.synchronize()
+    t0 = time.perf_counter()
+    curve = []
+    while True:
+        r.render(spp_per_pass)
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            break
+        if len(curve) < 64 and r.spp() & (r.spp() - 1) == 0:
+            img = r.radiance_sum().reshape(-1, 3)[pix] / r.spp()
+            curve.append((round(el, 3), r.spp(), float(((img - ref_mean) ** 2).mean())))
+            t0 += time.perf_counter() - t0 - el  # the error readback is not render time
+    el = time.perf_counter() - t0
+    spp = r.spp()
+    img = r.radiance_sum().reshape(-1, 3)[pix] / spp
+    st = r.stats()
+    mse = float(((img - ref_mean) ** 2).mean())
+    out = {"mse": mse, "rel_mse": float((((img - ref_mean) ** 2) / (ref_mean**2 + 1e-2)).mean()),
+           "spp": spp, "seconds": round(el, 3), "mrays_s": round(st.rays_total / el / 1e6, 2),
+           "guide_steps": int(st.guide_steps), "curve": curve}
+    log(f"room {mode} seed {seed}: {spp} spp in {el:.1f}s, MSE {mse:.5g}, {out['mrays_s']} Mrays/s")
+    r.close()
+    return out
 
 
 def batch_renders(P, steps, probe_divisor, seed=2024):
@@ -198,6 +229,7 @@ def main():
     ap.add_argument("--probe-divisor", type=int, default=16, help="batch: autotuner probes run spp / this")
     ap.add_argument("--budget", type=float, default=20.0, help="room: seconds per renderer")
     ap.add_argument("--spp-per-pass", type=int, default=1, help="room: samples per pixel per progressive pass")
+    ap.add_argument("--repeats", type=int, default=1, help="room: seeds per renderer (MSE averaged)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
@@ -228,7 +260,7 @@ def main():
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
     if args.workload == "room":
-        print(json.dumps(room_equal_time(P, args.budget, args.spp_per_pass)), flush=True)
+        print(json.dumps(room_equal_time(P, args.budget, args.spp_per_pass, args.repeats)), flush=True)
         return
     if args.workload == "batch":
         print(json.dumps(batch_renders(P, args.steps, args.probe_divisor)), flush=True)
@@ -276,9 +308,15 @@ def main():
     d = kernels[dom]
     roof = {
         "kernel": dom, "bound": "hbm", "achieved": round(d["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+        "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
         "bytes_per_launch": d["bytes_per_launch"], "avg_launch_us": d["avg_us"],
     }
+    # queue / path-state kernels (no traversal): algorithmic bytes over their summed time
+    qk = [k for k in ("raygen", "shade", "accumulate", "tonemap") if k in kernels]
+    qbytes = sum(kernels[k]["bytes_per_launch"] * kernels[k]["launches"] for k in qk)
+    qms = sum(kernels[k]["total_ms"] for k in qk)
+    queue_roof = {"kernels": qk, "achieved": round(qbytes / (qms * 1e-3) / 1e9, 1) if qms > 0 else 0.0, "peak": HBM_PEAK_GBS,
+                  "unit": "GB/s", "frac": round(qbytes / (qms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if qms > 0 else 0.0}
     out = {
         "metric": "Mrays/s", "value": round(mrays, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 2), "higher_is_better": True,
@@ -288,6 +326,7 @@ def main():
                    "rays_per_step": st.rays_total // args.steps, "samples_per_s": round(samples),
                    "bvh_depth": st.bvh_depth, "parallelism": "1 GPU"},
         "roofline": roof,
+        "queue_roofline": queue_roof,
         "kernels": kernels,
     }
     if args.cpu_budget > 0:

@@ -55,6 +55,9 @@ static constexpr int kTraceThreads = 128;
 // Default path-pool capacity: 2^26 paths (about 13 GB of SoA path state), a whole 1024^2 x
 // 64 spp pass in one chunk, so that every bounce's queues stay long.
 static constexpr int64_t kDefaultPool = 1ll << 26;
+// Trace blocks of 128 threads per CU: 14 fill the 7 waves per SIMD that the trace kernels'
+// 71 VGPRs and 8 KiB of LDS stack per block allow.
+static constexpr int kDefaultTraceBlocks = 14;
 static constexpr float kShadowEps = 1e-4f;   // path_tracer.cu:104-110
 static constexpr float kBounceEps = 1e-3f;   // path_tracer.cu:177
 
@@ -803,7 +806,7 @@ struct pt_renderer {
     // path pool (owned, or shared by the renders of a batch) and its capacity
     std::shared_ptr<PathPool> pp = std::make_shared<PathPool>();
     int64_t poolWanted = kDefaultPool;
-    int traceBlocksPerCU = 8, shadeBlocksPerCU = 8;
+    int traceBlocksPerCU = kDefaultTraceBlocks, shadeBlocksPerCU = 8;
     // accumulation
     DevBuf accum, radiance, rgba, stats;
     int64_t spp = 0;
@@ -1433,7 +1436,7 @@ int pt_renderer_set_option(pt_renderer *r, int opt, int64_t v) {
         if (v != 0 && v != 16 && v != 32) return fail(PT_ERR_INVALID, "pt_renderer_set_option: LDS stack entries must be 16 or 32");
         r->ldsStack = v == 0 ? 16 : (int)v;
         break;
-    case PT_OPT_TRACE_BLOCKS_PER_CU: r->traceBlocksPerCU = v > 0 ? (int)std::min<int64_t>(v, 64) : 8; break;
+    case PT_OPT_TRACE_BLOCKS_PER_CU: r->traceBlocksPerCU = v > 0 ? (int)std::min<int64_t>(v, 64) : kDefaultTraceBlocks; break;
     case PT_OPT_SHADE_BLOCKS_PER_CU: r->shadeBlocksPerCU = v > 0 ? (int)std::min<int64_t>(v, 64) : 8; break;
     case PT_OPT_PROFILE_KERNELS: r->profile = v != 0; break;
     case PT_OPT_CONSISTENT_MIS: r->misSample = v != 0; r->resetPending = true; break;
@@ -1841,7 +1844,7 @@ template <class Probe>
 TuneCfg coordinateDescent(TuneCfg cur, const std::vector<int64_t> &pools, Probe probe, int *probes, double *bestRate) {
     double best = probe(cur);
     int n = 1;
-    const std::vector<int> traces = {6, 8, 12, 14}, shades = {4, 8, 16}, ldss = {16, 32};
+    const std::vector<int> traces = {8, 12, 14, 16}, shades = {4, 8, 16}, ldss = {16, 32};
     for (int dim = 0; dim < 4; dim++) {
         const size_t count = dim == 0 ? pools.size() : dim == 1 ? traces.size() : dim == 2 ? ldss.size() : shades.size();
         for (size_t k = 0; k < count; k++) {
@@ -1890,7 +1893,7 @@ struct pt_batch {
     std::shared_ptr<PathPool> pp = std::make_shared<PathPool>();
     std::vector<pt_renderer *> jobs;
     std::vector<int> spp;
-    TuneCfg cfg{kDefaultPool, 8, 8, 16};
+    TuneCfg cfg{kDefaultPool, kDefaultTraceBlocks, 8, 16};
     bool useGraph = true;
     DevBuf stamps;
     double clockKHz = 100000.0;
@@ -2094,7 +2097,7 @@ int pt_batch_set_option(pt_batch *b, int option, int64_t v) {
         if (v > (1ll << 27)) return fail(PT_ERR_INVALID, "pt_batch_set_option: pool larger than 2^27 paths");
         b->cfg.pool = v > 0 ? v : kDefaultPool;
         break;
-    case PT_OPT_TRACE_BLOCKS_PER_CU: b->cfg.trace = v > 0 ? (int)std::min<int64_t>(v, 64) : 8; break;
+    case PT_OPT_TRACE_BLOCKS_PER_CU: b->cfg.trace = v > 0 ? (int)std::min<int64_t>(v, 64) : kDefaultTraceBlocks; break;
     case PT_OPT_SHADE_BLOCKS_PER_CU: b->cfg.shade = v > 0 ? (int)std::min<int64_t>(v, 64) : 8; break;
     case PT_OPT_TRACE_LDS_STACK:
         if (v != 0 && v != 16 && v != 32) return fail(PT_ERR_INVALID, "pt_batch_set_option: LDS stack entries must be 16 or 32");
