@@ -91,9 +91,11 @@ __device__ __forceinline__ int waveAppend(bool want, int *counter) {
     base = __shfl(base, leader);
     return want ? base + (int)lanesBelow(m) : -1;
 }
-__device__ __forceinline__ void waveCount(bool c, int *counter) {
-    const uint64_t m = __ballot(c);
-    if (m != 0 && laneId() == __ffsll((unsigned long long)m) - 1) atomicAdd(counter, __popcll(m));
+// Adds a per-lane count once per wave (statistics counted over a persistent wave's lifetime
+// cost one atomic instead of one per iteration).
+__device__ __forceinline__ void waveSum(int v, int *counter) {
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    if (laneId() == 0 && v != 0) atomicAdd(counter, v);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -306,6 +308,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_shadow(PtSceneView sc, const 
     __shared__ int sstack[STACK * kTraceThreads];
     const TraceStack ts = traceStack(sstack, spill, spillCap);
     const int n = *count;
+    int adds = 0;
     while (true) {
         int base = 0;
         if (laneId() == 0) base = atomicAdd(fetch, 64);
@@ -326,8 +329,9 @@ __global__ __launch_bounds__(kTraceThreads) void k_shadow(PtSceneView sc, const 
                 add = true;
             }
         }
-        waveCount(add, laddCounter);
+        adds += add ? 1 : 0;
     }
+    waveSum(adds, laddCounter);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -443,6 +447,7 @@ struct GuideShadeArgs {
     const int *recIn;       // training record of the vertex that spawned each queued ray (-1: none)
     int *recOut;
     GuideSceneNorm norm;
+    float fracCap;          // renderer-side cap on the guided-sampling probability (defensive share)
 };
 
 __device__ __forceinline__ float lum(float4 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
@@ -479,6 +484,7 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 5) void k_shade(PtSceneView sc, C
     // Queue appends happen where each record is complete (one atomic per wave and branch:
     // the lanes taking a branch append together), so no output record stays live across the
     // rest of the vertex.
+    int adds = 0;
     for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n; base += waveStride) {
         const int i = base + laneId();
         bool addL = false;
@@ -525,7 +531,7 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 5) void k_shade(PtSceneView sc, C
                     if constexpr (GUIDED) {
                         G = ga.mix + (size_t)i * GUIDE_REC_F4;
                         const float4 g10 = G[10];
-                        if (g10.y > 0.f) frac = g10.x;
+                        if (g10.y > 0.f) frac = fminf(g10.x, ga.fracCap);
                         else G = nullptr;
                     }
                     BsdfSample bs;
@@ -631,8 +637,9 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 5) void k_shade(PtSceneView sc, C
                 Lpath[pathId] = acc;
             }
         }
-        waveCount(addL, &c[C_LADD_SHADE]);
+        adds += addL ? 1 : 0;
     }
+    waveSum(adds, &c[C_LADD_SHADE]);
 }
 
 // Training targets: weight = (radiance after the vertex's sampled direction) / luminance of the
@@ -835,6 +842,7 @@ struct pt_renderer {
     int gTrainSteps = 2;
     int gBatch = 1 << 18;
     float gLr = 1e-2f;
+    float gFracCap = 1.f;
     int gAdamStep = 0;
     int64_t gTrainTotal = 0;
     uint64_t gSeed = 1;
@@ -1069,6 +1077,7 @@ void runChunk(pt_renderer *r, const ChunkParams &cp) {
             ga.recIn = r->gRecIdx[cur].as<int>();
             ga.recOut = r->gRecIdx[1 - cur].as<int>();
             ga.norm = guideNorm(r);
+            ga.fracCap = r->gFracCap;
         }
         {
             KTimer t(r, K_SHADE);
@@ -1445,6 +1454,10 @@ int pt_renderer_set_option(pt_renderer *r, int opt, int64_t v) {
         if (v < 32 || v > (1 << 22)) return fail(PT_ERR_INVALID, "pt_renderer_set_option: guide batch must be in [32, 2^22]");
         r->gBatch = (int)v;
         break;
+    case PT_OPT_GUIDE_MAX_FRACTION_MILLI:
+        if (v <= 0 || v > 1000) return fail(PT_ERR_INVALID, "pt_renderer_set_option: guided fraction cap must be in (0, 1000] thousandths");
+        r->gFracCap = (float)v * 1e-3f;
+        break;
     case PT_OPT_GUIDE_LR_MICRO:
         if (v <= 0) return fail(PT_ERR_INVALID, "pt_renderer_set_option: learning rate must be positive");
         r->gLr = (float)v * 1e-6f;
@@ -1475,6 +1488,7 @@ int pt_renderer_get_option(pt_renderer *r, int opt, int64_t *v) {
     case PT_OPT_GUIDE_TRAIN_STEPS: *v = r->gTrainSteps; break;
     case PT_OPT_GUIDE_BATCH: *v = r->gBatch; break;
     case PT_OPT_GUIDE_LR_MICRO: *v = (int64_t)llroundf(r->gLr * 1e6f); break;
+    case PT_OPT_GUIDE_MAX_FRACTION_MILLI: *v = (int64_t)llroundf(r->gFracCap * 1e3f); break;
     case PT_OPT_GUIDE_RESET: *v = (int64_t)r->gTrainTotal; break;
     default: return fail(PT_ERR_INVALID, "pt_renderer_get_option: unknown option");
     }

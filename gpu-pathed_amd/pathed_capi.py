@@ -1,7 +1,7 @@
 """ctypes binding of libpathed.so (include/pathed.h).
 
-Loads the in-tree library next to this file and fails loudly when it is missing: there is
-no CPU fallback for any GPU entry point.
+Loads the in-tree library next to this file (or the one PATHED_LIB names) and fails loudly
+when it is missing: there is no CPU fallback for any GPU entry point.
 """
 import ctypes as C
 import os
@@ -9,14 +9,14 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpathed.so")
+LIB_PATH = os.environ.get("PATHED_LIB") or os.path.join(HERE, "libpathed.so")
 
 PT_OK = 0
 LAMBERTIAN, MIRROR, GLASS, MICROFACET = 0, 1, 2, 3
 RENDERER_CUDA, RENDERER_OPTIX, RENDERER_NORMALS, RENDERER_GUIDED = 0, 1, 2, 3
 OPT_POOL_PATHS, OPT_TRACE_BLOCKS_PER_CU, OPT_SHADE_BLOCKS_PER_CU, OPT_PROFILE_KERNELS = 0, 1, 2, 3
 OPT_CONSISTENT_MIS, OPT_GUIDE_TRAIN_STEPS, OPT_GUIDE_BATCH, OPT_GUIDE_LR_MICRO, OPT_GUIDE_RESET = 4, 5, 6, 7, 8
-OPT_TRACE_LDS_STACK, OPT_BATCH_GRAPH = 9, 10
+OPT_TRACE_LDS_STACK, OPT_BATCH_GRAPH, OPT_GUIDE_MAX_FRACTION_MILLI = 9, 10, 11
 MISS = np.iinfo(np.int32).min
 
 # every symbol declared in include/pathed.h

@@ -170,8 +170,11 @@ typedef enum {
                                     get: number of training steps taken */
     PT_OPT_TRACE_LDS_STACK = 9,  /* traversal stack entries per lane kept in LDS (16 or 32);
                                     deeper entries spill to global memory */
-    PT_OPT_BATCH_GRAPH = 10      /* batches only: 1 (default) = capture the whole batch in one
+    PT_OPT_BATCH_GRAPH = 10,     /* batches only: 1 (default) = capture the whole batch in one
                                     hipGraph and launch it once; 0 = launch kernels directly */
+    PT_OPT_GUIDE_MAX_FRACTION_MILLI = 11 /* guided: cap, in thousandths, on the probability of
+                                    sampling the learned lobes (the rest samples the BSDF);
+                                    default 1000 = the network's own clamp (0.98) */
 } pt_option;
 int pt_renderer_set_option(pt_renderer *r, int option, int64_t value);
 int pt_renderer_get_option(pt_renderer *r, int option, int64_t *value);
