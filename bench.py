@@ -82,14 +82,14 @@ def cpu_baseline(scene_arrays, budget_s):
 
 
 ROOM_W, ROOM_H, ROOM_DEPTH = 2048, 1024, 16
-ROOM_REFERENCE = os.path.join(ROOT, "tests", "golden", "room_reference_256px.npz")
+ROOM_REFERENCE = os.path.join(ROOT, "tests", "golden", "room_reference_1024px.npz")
 
 
 def room_equal_time(P, budget_s, spp_per_pass, repeats=1):
     """Hard indirect lighting, equal wall time: the closed room of 4M triangles lit through a
     0.3-wide slot (2048x1024, depth 16, NEE + consistent MIS) rendered for `budget_s` seconds
     by the plain and by the guided renderer (network trained from scratch inside the budget).
-    Mean squared error at a fixed seeded subset of 256 pixels against the FP64 oracle's
+    Mean squared error at a fixed seeded subset of 1024 pixels against the FP64 oracle's
     65,536-sample estimate there (the committed reference fixture).  With repeats > 1 each
     renderer runs that many times with different PCG32 seeds and the MSEs are averaged."""
     import numpy as np
@@ -229,7 +229,7 @@ def main():
     ap.add_argument("--probe-divisor", type=int, default=16, help="batch: autotuner probes run spp / this")
     ap.add_argument("--budget", type=float, default=20.0, help="room: seconds per renderer")
     ap.add_argument("--spp-per-pass", type=int, default=1, help="room: samples per pixel per progressive pass")
-    ap.add_argument("--repeats", type=int, default=1, help="room: seeds per renderer (MSE averaged)")
+    ap.add_argument("--repeats", type=int, default=3, help="room: seeds per renderer (MSE averaged)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)

@@ -163,7 +163,7 @@ typedef enum {
     PT_OPT_CONSISTENT_MIS = 4,   /* 1: the light-sample MIS weight uses the BSDF sampling density
                                     (the reference uses Microfacet::pdf, microfacet.h:124-134,
                                     whose widened alpha differs from sample()'s) */
-    PT_OPT_GUIDE_TRAIN_STEPS = 5,/* guided: Adam steps after each chunk (default 2) */
+    PT_OPT_GUIDE_TRAIN_STEPS = 5,/* guided: Adam steps after each chunk (default 16) */
     PT_OPT_GUIDE_BATCH = 6,      /* guided: training records per step (default 2^18) */
     PT_OPT_GUIDE_LR_MICRO = 7,   /* guided: Adam learning rate in units of 1e-6 (default 10000) */
     PT_OPT_GUIDE_RESET = 8,      /* guided: re-initialise the network with seed = value;
@@ -174,7 +174,7 @@ typedef enum {
                                     hipGraph and launch it once; 0 = launch kernels directly */
     PT_OPT_GUIDE_MAX_FRACTION_MILLI = 11 /* guided: cap, in thousandths, on the probability of
                                     sampling the learned lobes (the rest samples the BSDF);
-                                    default 1000 = the network's own clamp (0.98) */
+                                    default 800; 1000 leaves the network's own clamp (0.98) */
 } pt_option;
 int pt_renderer_set_option(pt_renderer *r, int option, int64_t value);
 int pt_renderer_get_option(pt_renderer *r, int option, int64_t *value);
