@@ -176,10 +176,17 @@ __device__ __forceinline__ bool traverse(const PtSceneView &sc, f3 o, f3 d, floa
     const float iy = 1.f / (fabsf(d.y) > 1e-20f ? d.y : copysignf(1e-20f, d.y));
     const float iz = 1.f / (fabsf(d.z) > 1e-20f ? d.z : copysignf(1e-20f, d.z));
     const float ox = o.x * ix, oy = o.y * iy, oz = o.z * iz;
+    // Per axis the near plane of every box is its lower bound when the direction is positive:
+    // the slab test reads the near and far float4 of each axis directly (byte offsets within
+    // the 128-byte node), which gives the same entry and exit distances as min/max of both.
+    const uint32_t nX = ix < 0.f ? 16u : 0u, nY = iy < 0.f ? 48u : 32u, nZ = iz < 0.f ? 80u : 64u;
+    const uint32_t fX = 16u - nX, fY = 80u - nY, fZ = 144u - nZ;
+    const char *nodes = (const char *)sc.nodes;
     const int stride = blockDim.x;
     int sp = 0;
     int node = 0;
     bool hit = false;
+    // general push (LDS, then the spill area); the common case below writes LDS directly
     auto push = [&](int ref) {
         if (sp < STACK) {
             ts.lds[sp * stride] = ref;
@@ -193,20 +200,21 @@ __device__ __forceinline__ bool traverse(const PtSceneView &sc, f3 o, f3 d, floa
     };
     while (true) {
         if (node >= 0) {
-            const float4 *n = sc.nodes + (size_t)PT_NODE_F4 * node;
-            const float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5], rf = n[6];
+            const uint32_t base = (uint32_t)node * (uint32_t)(PT_NODE_F4 * 16);
+            // 32-bit offsets from the array base (a node array stays below 4 GiB)
+            const float4 nx = *(const float4 *)(nodes + (base + nX)), fx = *(const float4 *)(nodes + (base + fX));
+            const float4 ny = *(const float4 *)(nodes + (base + nY)), fy = *(const float4 *)(nodes + (base + fY));
+            const float4 nz = *(const float4 *)(nodes + (base + nZ)), fz = *(const float4 *)(nodes + (base + fZ));
+            const float4 rf = *(const float4 *)(nodes + (base + 96u));
             float tn[4];
             int ref[4] = {asInt(rf.x), asInt(rf.y), asInt(rf.z), asInt(rf.w)};
-            const float blx[4] = {lx.x, lx.y, lx.z, lx.w}, bhx[4] = {hx.x, hx.y, hx.z, hx.w};
-            const float bly[4] = {ly.x, ly.y, ly.z, ly.w}, bhy[4] = {hy.x, hy.y, hy.z, hy.w};
-            const float blz[4] = {lz.x, lz.y, lz.z, lz.w}, bhz[4] = {hz.x, hz.y, hz.z, hz.w};
+            const float bnx[4] = {nx.x, nx.y, nx.z, nx.w}, bfx[4] = {fx.x, fx.y, fx.z, fx.w};
+            const float bny[4] = {ny.x, ny.y, ny.z, ny.w}, bfy[4] = {fy.x, fy.y, fy.z, fy.w};
+            const float bnz[4] = {nz.x, nz.y, nz.z, nz.w}, bfz[4] = {fz.x, fz.y, fz.z, fz.w};
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                const float x0 = blx[k] * ix - ox, x1 = bhx[k] * ix - ox;
-                const float y0 = bly[k] * iy - oy, y1 = bhy[k] * iy - oy;
-                const float z0 = blz[k] * iz - oz, z1 = bhz[k] * iz - oz;
-                const float lo = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
-                const float hi = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
+                const float lo = fmaxf(fmaxf(bnx[k] * ix - ox, bny[k] * iy - oy), fmaxf(bnz[k] * iz - oz, tmin));
+                const float hi = fminf(fminf(bfx[k] * ix - ox, bfy[k] * iy - oy), fminf(bfz[k] * iz - oz, tmax));
                 tn[k] = (lo <= hi && ref[k] != PT_EMPTY_CHILD) ? lo : INFINITY;
             }
             if constexpr (!ANY) {
@@ -216,9 +224,19 @@ __device__ __forceinline__ bool traverse(const PtSceneView &sc, f3 o, f3 d, floa
                 cswap(tn[1], ref[1], tn[3], ref[3]);
                 cswap(tn[1], ref[1], tn[2], ref[2]);
                 if (tn[0] != INFINITY) {
-                    if (tn[3] != INFINITY) push(ref[3]);
-                    if (tn[2] != INFINITY) push(ref[2]);
-                    if (tn[1] != INFINITY) push(ref[1]);
+                    // hit children are a prefix of the sorted slots: push all but the nearest,
+                    // the second nearest on top
+                    const int more = (tn[1] != INFINITY) + (tn[2] != INFINITY) + (tn[3] != INFINITY);
+                    if (sp + 3 <= STACK) {
+                        if (more >= 3) ts.lds[sp * stride] = ref[3];
+                        if (more >= 2) ts.lds[(sp + more - 2) * stride] = ref[2];
+                        if (more >= 1) ts.lds[(sp + more - 1) * stride] = ref[1];
+                        sp += more;
+                    } else {
+                        if (more >= 3) push(ref[3]);
+                        if (more >= 2) push(ref[2]);
+                        if (more >= 1) push(ref[1]);
+                    }
                     node = ref[0];
                     continue;
                 }
@@ -277,18 +295,27 @@ __device__ __forceinline__ TraceStack traceStack(int *sstack, int *spill, int sp
     return ts;
 }
 
-// Persistent closest-hit kernel: each wave takes 64 rays per atomic until the queue is empty.
+// Work distribution of the persistent trace kernels: wave w first takes rays [64 w, 64 w + 64)
+// without any atomic, then grabs 64 rays at a time from the shared cursor (offset by the
+// statically assigned part).  Waves whose first batch is already past the queue end leave at
+// once, so short queues cost few atomics; 64 rays per grab keeps the rays in flight on a CU
+// adjacent in the queue and so coherent (256 per grab measured 10% slower), and the cursor is
+// never read without an atomic (plain reads of it queue behind the atomics in its L2 channel).
+__device__ __forceinline__ int firstRays() { return (int)((blockIdx.x * blockDim.x + threadIdx.x) & ~63u); }
+__device__ __forceinline__ int nextRays(int *fetch) {
+    int base = 0;
+    if (laneId() == 0) base = atomicAdd(fetch, 64);
+    return __shfl(base, 0) + (int)(gridDim.x * blockDim.x);
+}
+
+// Persistent closest-hit kernel: each wave traces 64 rays at a time until the queue is empty.
 template <int STACK>
-__global__ __launch_bounds__(kTraceThreads) void k_extend(PtSceneView sc, const int *count, int *fetch, const float4 *rayO, const float4 *rayD,
+__global__ __launch_bounds__(kTraceThreads, 7) void k_extend(PtSceneView sc, const int *count, int *fetch, const float4 *rayO, const float4 *rayD,
                                                          float4 *hits, int *spill, int spillCap, int *overflow) {
     __shared__ int sstack[STACK * kTraceThreads];
     const TraceStack ts = traceStack(sstack, spill, spillCap);
     const int n = *count;
-    while (true) {
-        int base = 0;
-        if (laneId() == 0) base = atomicAdd(fetch, 64);
-        base = __shfl(base, 0);
-        if (base >= n) break;
+    for (int base = firstRays(); base < n; base = nextRays(fetch)) {
         const int i = base + laneId();
         if (i < n) {
             const float4 ro = rayO[i], rd = rayD[i];
@@ -302,20 +329,15 @@ __global__ __launch_bounds__(kTraceThreads) void k_extend(PtSceneView sc, const 
 
 // Persistent any-hit kernel for shadow rays (interval [1e-4, tmax]).
 template <int STACK>
-__global__ __launch_bounds__(kTraceThreads) void k_shadow(PtSceneView sc, const int *count, int *fetch, const float4 *sO, const float4 *sD,
+__global__ __launch_bounds__(kTraceThreads, 7) void k_shadow(PtSceneView sc, const int *count, int *fetch, const float4 *sO, const float4 *sD,
                                                          const float4 *sL, float4 *Lpath, int *laddCounter, int *spill, int spillCap,
                                                          int *overflow) {
     __shared__ int sstack[STACK * kTraceThreads];
     const TraceStack ts = traceStack(sstack, spill, spillCap);
     const int n = *count;
     int adds = 0;
-    while (true) {
-        int base = 0;
-        if (laneId() == 0) base = atomicAdd(fetch, 64);
-        base = __shfl(base, 0);
-        if (base >= n) break;
+    for (int base = firstRays(); base < n; base = nextRays(fetch)) {
         const int i = base + laneId();
-        bool add = false;
         if (i < n) {
             const float4 o = sO[i], d = sD[i];
             TraceHit th;
@@ -326,10 +348,9 @@ __global__ __launch_bounds__(kTraceThreads) void k_shadow(PtSceneView sc, const 
                 float4 acc = Lpath[pid];
                 acc.x += l.x; acc.y += l.y; acc.z += l.z;
                 Lpath[pid] = acc;
-                add = true;
+                adds++;
             }
         }
-        adds += add ? 1 : 0;
     }
     waveSum(adds, laddCounter);
 }
@@ -1681,6 +1702,10 @@ int pt_renderer_get_stats(pt_renderer *r, pt_render_stats *out) {
     for (int b = 0; b < kMaxBounces; b++) {
         out->extension_rays += (int64_t)s[S_RAYS + b];
         out->shadow_rays += (int64_t)s[S_SHADOW + b];
+        if (b < 16) {
+            out->bounce_rays[b] = (int64_t)s[S_RAYS + b];
+            out->bounce_shadow[b] = (int64_t)s[S_SHADOW + b];
+        }
     }
     out->rays_total = out->extension_rays + out->shadow_rays;
     out->paths = (int64_t)s[S_PATHS];

@@ -60,6 +60,7 @@ class RenderStats(C.Structure):
         ("rays_total", C.c_int64), ("extension_rays", C.c_int64), ("shadow_rays", C.c_int64), ("paths", C.c_int64),
         ("passes", C.c_int64), ("gpu_ms", C.c_double), ("bvh_depth", C.c_int), ("bvh_nodes", C.c_int64),
         ("stack_overflows", C.c_int), ("guide_steps", C.c_int64), ("guide_loss", C.c_double),
+        ("bounce_rays", C.c_int64 * 16), ("bounce_shadow", C.c_int64 * 16),
     ]
 
 

@@ -214,6 +214,8 @@ typedef struct {
     int stack_overflows;       /* traversal stack overflow events (must stay 0) */
     int64_t guide_steps;       /* guided: training steps taken since the network was created */
     double guide_loss;         /* guided: mean loss of the last training step */
+    int64_t bounce_rays[16];   /* extension rays traced at bounce 0..15 (0 = camera rays) */
+    int64_t bounce_shadow[16]; /* shadow rays traced at bounce 0..15 */
 } pt_render_stats;
 int pt_renderer_get_stats(pt_renderer *r, pt_render_stats *stats);
 int pt_renderer_reset_stats(pt_renderer *r);
