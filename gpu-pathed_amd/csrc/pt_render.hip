@@ -310,7 +310,7 @@ __device__ __forceinline__ int nextRays(int *fetch) {
 
 // Persistent closest-hit kernel: each wave traces 64 rays at a time until the queue is empty.
 template <int STACK>
-__global__ __launch_bounds__(kTraceThreads, 7) void k_extend(PtSceneView sc, const int *count, int *fetch, const float4 *rayO, const float4 *rayD,
+__global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 7 : 5) void k_extend(PtSceneView sc, const int *count, int *fetch, const float4 *rayO, const float4 *rayD,
                                                          float4 *hits, int *spill, int spillCap, int *overflow) {
     __shared__ int sstack[STACK * kTraceThreads];
     const TraceStack ts = traceStack(sstack, spill, spillCap);
@@ -329,7 +329,7 @@ __global__ __launch_bounds__(kTraceThreads, 7) void k_extend(PtSceneView sc, con
 
 // Persistent any-hit kernel for shadow rays (interval [1e-4, tmax]).
 template <int STACK>
-__global__ __launch_bounds__(kTraceThreads, 7) void k_shadow(PtSceneView sc, const int *count, int *fetch, const float4 *sO, const float4 *sD,
+__global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 7 : 5) void k_shadow(PtSceneView sc, const int *count, int *fetch, const float4 *sO, const float4 *sD,
                                                          const float4 *sL, float4 *Lpath, int *laddCounter, int *spill, int spillCap,
                                                          int *overflow) {
     __shared__ int sstack[STACK * kTraceThreads];
