@@ -268,7 +268,12 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideNet net, const float4 *
     __shared__ float sRec[32][8];
     const int lane = threadIdx.x;
     float *P = partials + (size_t)blockIdx.x * GUIDE_PARAMS;
-    for (int i = lane; i < GUIDE_PARAMS; i += 64) P[i] = 0.f;
+    // The block's weight and bias gradients accumulate in LDS across its tiles (one wave per
+    // block: every lane owns its addresses, no atomics) and reach the block's partial in
+    // global memory once, at the end.
+    __shared__ float sG[GUIDE_PARAMS];
+    for (int i = lane; i < GUIDE_PARAMS; i += 64) sG[i] = 0.f;
+    WAVE_SYNC();
     const int count = *countPtr;
     const float wsum = wsumPtr ? *wsumPtr : 0.f;
     // normalise the weights to mean one over the record buffer
@@ -344,7 +349,7 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideNet net, const float4 *
         WAVE_SYNC();
         int dcur = 0;
         for (int L = GUIDE_LAYERS - 1; L >= 0; L--) {
-            float *PL = P + (size_t)L * GUIDE_LAYER_PARAMS;
+            float *GL = sG + L * GUIDE_LAYER_PARAMS;
             // dW[out][in] += sum_rows delta[row][out] * a_L[row][in]  (K = 32 rows)
             for (int mt = 0; mt < 2; mt++)
                 for (int nt = 0; nt < 2; nt++) {
@@ -355,13 +360,13 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideNet net, const float4 *
                     acc = mfma(ld8(arow + 16), ld8(brow + 16), acc);
                     const int in = nt * 32 + (lane & 31);
 #pragma unroll
-                    for (int r = 0; r < 16; r++) PL[(mt * 32 + cRow(lane, r)) * GUIDE_W + in] += acc[r];
+                    for (int r = 0; r < 16; r++) GL[(mt * 32 + cRow(lane, r)) * GUIDE_W + in] += acc[r];
                 }
             {
                 // db[out] = sum_rows delta[row][out]
                 float s = 0.f;
                 for (int row = 0; row < 32; row++) s += (float)sD[dcur][row * ACT_STRIDE + lane];
-                PL[GUIDE_W * GUIDE_W + lane] += s;
+                GL[GUIDE_W * GUIDE_W + lane] += s;
             }
             WAVE_SYNC();
             if (L > 0) {
@@ -393,6 +398,8 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideNet net, const float4 *
         }
         WAVE_SYNC();
     }
+    WAVE_SYNC();
+    for (int i = lane; i < GUIDE_PARAMS; i += 64) P[i] = sG[i];
     // loss of this block: lanes 0..31 hold partial sums
     for (int off = 32; off >= 1; off >>= 1) lossAcc += __shfl_down(lossAcc, off);
     if (lane == 0 && lossOut) lossOut[blockIdx.x] = lossAcc;
