@@ -268,12 +268,7 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideNet net, const float4 *
     __shared__ float sRec[32][8];
     const int lane = threadIdx.x;
     float *P = partials + (size_t)blockIdx.x * GUIDE_PARAMS;
-    // The block's weight and bias gradients accumulate in LDS across its tiles (one wave per
-    // block: every lane owns its addresses, no atomics) and reach the block's partial in
-    // global memory once, at the end.
-    __shared__ float sG[GUIDE_PARAMS];
-    for (int i = lane; i < GUIDE_PARAMS; i += 64) sG[i] = 0.f;
-    WAVE_SYNC();
+    for (int i = lane; i < GUIDE_PARAMS; i += 64) P[i] = 0.f;
     const int count = *countPtr;
     const float wsum = wsumPtr ? *wsumPtr : 0.f;
     // normalise the weights to mean one over the record buffer
@@ -349,7 +344,7 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideNet net, const float4 *
         WAVE_SYNC();
         int dcur = 0;
         for (int L = GUIDE_LAYERS - 1; L >= 0; L--) {
-            float *GL = sG + L * GUIDE_LAYER_PARAMS;
+            float *GL = P + (size_t)L * GUIDE_LAYER_PARAMS;
             // dW[out][in] += sum_rows delta[row][out] * a_L[row][in]  (K = 32 rows)
             for (int mt = 0; mt < 2; mt++)
                 for (int nt = 0; nt < 2; nt++) {
@@ -398,8 +393,6 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideNet net, const float4 *
         }
         WAVE_SYNC();
     }
-    WAVE_SYNC();
-    for (int i = lane; i < GUIDE_PARAMS; i += 64) P[i] = sG[i];
     // loss of this block: lanes 0..31 hold partial sums
     for (int off = 32; off >= 1; off >>= 1) lossAcc += __shfl_down(lossAcc, off);
     if (lane == 0 && lossOut) lossOut[blockIdx.x] = lossAcc;
@@ -433,14 +426,25 @@ __global__ void k_guide_pack(const float *params, __bf16 *w, __bf16 *wt, float *
     storeParam(p, liveParam(p) ? params[p] : 0.f, w, wt, b);
 }
 
-__global__ void k_guide_reduce(const float *partials, int nblocks, const float *lossPart, float invBatch, float *grad, float *lossOut) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < GUIDE_PARAMS) {
-        float g = 0.f;
-        for (int k = 0; k < nblocks; k++) g += partials[(size_t)k * GUIDE_PARAMS + p];
-        grad[p] = liveParam(p) ? g * invBatch : 0.f;
+// Sum of the per-block partial gradients: 64 parameters per block, four slices of the
+// partials per parameter (k = slice, slice + 4, ...), the slices added in a fixed order, so
+// the result does not depend on scheduling.
+__global__ __launch_bounds__(256) void k_guide_reduce(const float *partials, int nblocks, const float *lossPart, float invBatch, float *grad,
+                                                      float *lossOut) {
+    __shared__ float part[4][64];
+    const int p = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int slice = threadIdx.x >> 6;
+    float g = 0.f;
+    if (p < GUIDE_PARAMS)
+        for (int k = slice; k < nblocks; k += 4) g += partials[(size_t)k * GUIDE_PARAMS + p];
+    part[slice][threadIdx.x & 63] = g;
+    __syncthreads();
+    if (slice == 0 && p < GUIDE_PARAMS) {
+        const int j = threadIdx.x;
+        const float t = (part[0][j] + part[1][j]) + (part[2][j] + part[3][j]);
+        grad[p] = liveParam(p) ? t * invBatch : 0.f;
     }
-    if (p == 0 && lossOut) {
+    if (p == 0 && slice == 0 && lossOut) {
         float l = 0.f;
         for (int k = 0; k < nblocks; k++) l += lossPart[k];
         lossOut[0] = l * invBatch;
@@ -532,7 +536,7 @@ void guideInferRaw(GuideNet net, const float *feats, int n, float *rawOut, hipSt
 void guideGradients(GuideNet net, const float4 *records, const int *count, int base, int batch, const float *wsum, float *partials, int blocks,
                     float *lossPart, float *grad, float *lossOut, hipStream_t s) {
     k_guide_train<<<blocks, 64, 0, s>>>(net, records, count, wsum, base, batch, partials, lossPart);
-    k_guide_reduce<<<(GUIDE_PARAMS + 255) / 256, 256, 0, s>>>(partials, blocks, lossPart, 1.f / (float)std::max(batch, 1), grad, lossOut);
+    k_guide_reduce<<<(GUIDE_PARAMS + 63) / 64, 256, 0, s>>>(partials, blocks, lossPart, 1.f / (float)std::max(batch, 1), grad, lossOut);
     PT_HIP_TRY(hipGetLastError());
 }
 

@@ -58,6 +58,9 @@ static constexpr int64_t kDefaultPool = 1ll << 26;
 // Trace blocks of 128 threads per CU: 14 fill the 7 waves per SIMD that the trace kernels'
 // 71 VGPRs and 8 KiB of LDS stack per block allow.
 static constexpr int kDefaultTraceBlocks = 14;
+// Guide-training blocks per CU: two one-wave blocks of 56 KiB LDS each fit a CU; the second
+// hides the first one's global-memory latency (1.3x faster steps than one block per CU)
+static constexpr int kTrainBlocksPerCU = 2;
 static constexpr float kShadowEps = 1e-4f;   // path_tracer.cu:104-110
 static constexpr float kBounceEps = 1e-3f;   // path_tracer.cu:177
 
@@ -665,21 +668,30 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 5) void k_shade(PtSceneView sc, C
 
 // Training targets: weight = (radiance after the vertex's sampled direction) / luminance of the
 // throughput at the vertex, i.e. the MC estimate of f cos L_i / p for the sampled direction.
-__global__ void k_guide_finalize(float4 *records, int recCap, int *counters, const float4 *Lpath) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_guide_finalize(float4 *records, int recCap, int *counters, const float4 *Lpath) {
     const int n = min(counters[kRecCountSlot], recCap);
-    if (i == 0) counters[kRecValidSlot] = n;
+    if (blockIdx.x == 0 && threadIdx.x == 0) counters[kRecValidSlot] = n;
+    // grid-stride over the records; the weight sum is reduced per block so that the shared
+    // accumulator takes one atomic per block
     float w = 0.f;
-    if (i < n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         float4 *r = records + (size_t)i * GUIDE_TREC_F4 + 3;
         const float4 v = *r;
         const int pid = asInt(v.y);
-        if (v.z >= 0.f) w = fmaxf(0.f, lum(Lpath[pid]) - v.z) * v.x;
-        if (!(w == w) || isinf(w)) w = 0.f;
-        r->w = w;
+        float wi = 0.f;
+        if (v.z >= 0.f) wi = fmaxf(0.f, lum(Lpath[pid]) - v.z) * v.x;
+        if (!(wi == wi) || isinf(wi)) wi = 0.f;
+        r->w = wi;
+        w += wi;
     }
     for (int off = 32; off >= 1; off >>= 1) w += __shfl_down(w, off);
-    if (laneId() == 0 && w != 0.f) atomicAdd((float *)&counters[kWsumSlot], w);
+    __shared__ float part[4];
+    if (laneId() == 0) part[threadIdx.x >> 6] = w;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float t = part[0] + part[1] + part[2] + part[3];
+        if (t != 0.f) atomicAdd((float *)&counters[kWsumSlot], t);
+    }
 }
 
 // Per pixel of the chunk: FP64 sum over the chunk's sample rows (framebuffer.cu:7-41 keeps a
@@ -1021,8 +1033,8 @@ void ensureGuide(pt_renderer *r) {
         r->gM.alloc(np * 4);
         r->gV.alloc(np * 4);
         r->gGrad.alloc(np * 4);
-        r->gPartials.alloc((size_t)r->cus * np * 4);
-        r->gLossPart.alloc((size_t)r->cus * 4);
+        r->gPartials.alloc((size_t)r->cus * kTrainBlocksPerCU * np * 4);
+        r->gLossPart.alloc((size_t)r->cus * kTrainBlocksPerCU * 4);
         r->gLoss.alloc(4);
         r->gW.alloc(GUIDE_LAYERS * GUIDE_W * GUIDE_W * 2);
         r->gWt.alloc(GUIDE_LAYERS * GUIDE_W * GUIDE_W * 2);
@@ -1129,12 +1141,13 @@ void runChunk(pt_renderer *r, const ChunkParams &cp) {
     if (guided && r->gTrainSteps > 0) {
         KTimer t(r, K_GUIDE_TRAIN);
         const int nrec = (int)r->gRecCap;
-        k_guide_finalize<<<(nrec + 255) / 256, 256, 0, r->stream>>>(r->gRecords.as<float4>(), nrec, ctr, r->pp->Lpath.as<float4>());
+        const int fblocks = (int)std::min<int64_t>((nrec + 255) / 256, (int64_t)r->cus * 4);
+        k_guide_finalize<<<fblocks, 256, 0, r->stream>>>(r->gRecords.as<float4>(), nrec, ctr, r->pp->Lpath.as<float4>());
         PT_HIP_TRY(hipGetLastError());
         for (int s = 0; s < r->gTrainSteps; s++) {
             const int base = (int)((r->gTrainTotal * (int64_t)r->gBatch) & 0x3fffffff);
             guideGradients(guideNet(r), r->gRecords.as<float4>(), &ctr[kRecValidSlot], base, r->gBatch, (const float *)&ctr[kWsumSlot],
-                           r->gPartials.as<float>(), r->cus, r->gLossPart.as<float>(), r->gGrad.as<float>(), r->gLoss.as<float>(), r->stream);
+                           r->gPartials.as<float>(), kTrainBlocksPerCU * r->cus, r->gLossPart.as<float>(), r->gGrad.as<float>(), r->gLoss.as<float>(), r->stream);
             guideAdam(r->gParams.as<float>(), r->gM.as<float>(), r->gV.as<float>(), r->gGrad.as<float>(), ++r->gAdamStep, r->gLr,
                       r->gW.as<__bf16>(), r->gWt.as<__bf16>(), r->gB.as<float>(), r->stream);
             r->gTrainTotal++;
