@@ -172,9 +172,13 @@ typedef enum {
                                     deeper entries spill to global memory */
     PT_OPT_BATCH_GRAPH = 10,     /* batches only: 1 (default) = capture the whole batch in one
                                     hipGraph and launch it once; 0 = launch kernels directly */
-    PT_OPT_GUIDE_MAX_FRACTION_MILLI = 11 /* guided: cap, in thousandths, on the probability of
+    PT_OPT_GUIDE_MAX_FRACTION_MILLI = 11, /* guided: cap, in thousandths, on the probability of
                                     sampling the learned lobes (the rest samples the BSDF);
                                     default 800; 1000 leaves the network's own clamp (0.98) */
+    PT_OPT_TRACE_REFILL_LANES = 12 /* a trace wave takes new rays from the queue once this many
+                                    of its 64 lanes are idle (1..64; 64: whole batches of 64
+                                    neighbouring rays); 0 (default) = per launch: camera rays
+                                    56, later closest-hit queues 44, shadow rays 40 */
 } pt_option;
 int pt_renderer_set_option(pt_renderer *r, int option, int64_t value);
 int pt_renderer_get_option(pt_renderer *r, int option, int64_t *value);

@@ -214,6 +214,39 @@ def test_bvh_closest_hit_matches_brute_force():
     r.close()
 
 
+def test_trace_schedule_does_not_change_results():
+    """The persistent trace waves take new rays when a threshold of their lanes is idle; each
+    ray still visits the tree in the same order, so hits and images are bit-identical for any
+    threshold (1: refill at every idle lane, 64: whole batches, 0: per-launch defaults)."""
+    say("case: trace refill thresholds")
+    rng = np.random.default_rng(41)
+    s = P.Scene.preset(1, seed=3, param=5)  # displaced icosphere, 20,480 triangles
+    r = P.Renderer(96, 80)
+    r.set_scene(s)
+    r.set_max_depth(6)
+    n = 30000
+    rays = np.concatenate([rng.uniform(-2, 2, (n, 3)), np.zeros((n, 1)), rng.normal(size=(n, 3)), np.full((n, 1), 3e38)], 1)
+    rays = rays.astype(np.float32)
+    imgs, hits, occl = [], [], []
+    for v in (0, 64, 1, 17):
+        r.set_option(P.OPT_TRACE_REFILL_LANES, v)
+        assert r.get_option(P.OPT_TRACE_REFILL_LANES) == v
+        r.reset()
+        r.render(8)
+        imgs.append(r.radiance().copy())
+        hits.append(r.intersect(rays))
+        occl.append(r.intersect(rays, any_hit=True)[3])
+    for i in range(1, 4):
+        assert np.array_equal(imgs[i], imgs[0]), i
+        assert all(np.array_equal(a, b) for a, b in zip(hits[i], hits[0])), i
+        assert np.array_equal(occl[i], occl[0]), i
+    assert (hits[0][3] != P.MISS).mean() > 0.05 and imgs[0].mean() > 0
+    with pytest.raises(P.PtError):
+        r.set_option(P.OPT_TRACE_REFILL_LANES, 65)
+    assert r.stats().stack_overflows == 0
+    r.close()
+
+
 # --------------------------------------------------------------------------- rendering
 def _cornell(W=64, H=64):
     s = P.Scene.preset(0)
