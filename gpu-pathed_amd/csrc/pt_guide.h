@@ -64,8 +64,10 @@ __device__ __forceinline__ void guideFeatures(const GuideSceneNorm &nm, f3 p, f3
 // host-side entry points (pt_guide.hip)
 void guideInitParams(float *params, uint64_t seed);
 void guideUploadBf16(const float *dParams, __bf16 *dW, __bf16 *dWt, float *dB, hipStream_t s);
-void guideInfer(const PtSceneView &sc, GuideSceneNorm norm, GuideNet net, const int *count, const float4 *rayO, const float4 *rayD,
-                const float4 *hits, int maxRows, float4 *out, int cus, hipStream_t s);
+// Mixture per slot of a segmented ray queue (count of segment s at segCount[s * segStride],
+// PT_QUEUE_SEGS segments of segLen slots).
+void guideInfer(const PtSceneView &sc, GuideSceneNorm norm, GuideNet net, const int *segCount, int segStride, int segLen,
+                const float4 *rayO, const float4 *rayD, const float4 *hits, float4 *out, int cus, hipStream_t s);
 void guideInferRaw(GuideNet net, const float *feats, int n, float *rawOut, hipStream_t s);
 // Gradients of the mean loss over `batch` records (rows (base + i) mod *count of the record
 // buffer; weights scaled to mean one by *wsum when wsum is given): per-block FP32 partials

@@ -103,18 +103,28 @@ __device__ __forceinline__ float vmfPdf(f3 mu, float kappa, f3 w) {
 // --------------------------------------------------------------------------------------------
 constexpr int INFER_WAVES = 2;
 
+// Rows are the slots [0, maxRows) of a segmented queue (count of segment s at
+// count[s * segStride], segments segLen slots long; a tile of 32 rows lies in one segment), or
+// with count = nullptr all maxRows rows of rawFeats.
 __global__ __launch_bounds__(64 * INFER_WAVES) void k_guide_infer(PtSceneView sc, GuideSceneNorm nm, GuideNet net, const int *count,
-                                                                  const float4 *rayO, const float4 *rayD, const float4 *hits, int maxRows,
-                                                                  float4 *out, const float *rawFeats, float *rawOut) {
+                                                                  int segStride, int segLen, const float4 *rayO, const float4 *rayD,
+                                                                  const float4 *hits, int maxRows, float4 *out, const float *rawFeats,
+                                                                  float *rawOut) {
     __shared__ float sRaw[INFER_WAVES][32][GUIDE_RAW];
     __shared__ int sElig[INFER_WAVES][32];
     __shared__ __bf16 sAct[INFER_WAVES][2][32 * ACT_STRIDE];
     __shared__ float sOut[INFER_WAVES][32][GUIDE_OUT + 1];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int n = count ? min(*count, maxRows) : maxRows;
     const int waveStride = gridDim.x * INFER_WAVES;
-    for (int tile = blockIdx.x * INFER_WAVES + wave; tile * 32 < n; tile += waveStride) {
+    for (int tile = blockIdx.x * INFER_WAVES + wave; tile * 32 < maxRows; tile += waveStride) {
         const int rowBase = tile * 32;
+        // rows [rowBase, n) of this tile hold records: the filled part of its queue segment
+        int n = maxRows;
+        if (count) {
+            const int seg = rowBase / segLen;
+            n = seg * segLen + min(count[seg * segStride], segLen);
+            if (rowBase >= n) continue;
+        }
         if (lane < 32) {
             const int i = rowBase + lane;
             float f[GUIDE_RAW];
@@ -518,10 +528,12 @@ void guideUploadBf16(const float *dParams, __bf16 *dW, __bf16 *dWt, float *dB, h
     PT_HIP_TRY(hipGetLastError());
 }
 
-void guideInfer(const PtSceneView &sc, GuideSceneNorm norm, GuideNet net, const int *count, const float4 *rayO, const float4 *rayD,
-                const float4 *hits, int maxRows, float4 *out, int cus, hipStream_t s) {
+void guideInfer(const PtSceneView &sc, GuideSceneNorm norm, GuideNet net, const int *segCount, int segStride, int segLen,
+                const float4 *rayO, const float4 *rayD, const float4 *hits, float4 *out, int cus, hipStream_t s) {
+    const int maxRows = PT_QUEUE_SEGS * segLen;
     const int grid = std::max(1, std::min((maxRows + 32 * INFER_WAVES - 1) / (32 * INFER_WAVES), cus * 4));
-    k_guide_infer<<<grid, 64 * INFER_WAVES, 0, s>>>(sc, norm, net, count, rayO, rayD, hits, maxRows, out, nullptr, nullptr);
+    k_guide_infer<<<grid, 64 * INFER_WAVES, 0, s>>>(sc, norm, net, segCount, segStride, segLen, rayO, rayD, hits, maxRows, out, nullptr,
+                                                    nullptr);
     PT_HIP_TRY(hipGetLastError());
 }
 
@@ -529,7 +541,7 @@ void guideInferRaw(GuideNet net, const float *feats, int n, float *rawOut, hipSt
     PtSceneView sc{};
     GuideSceneNorm nm{};
     const int grid = std::max(1, (n + 32 * INFER_WAVES - 1) / (32 * INFER_WAVES));
-    k_guide_infer<<<grid, 64 * INFER_WAVES, 0, s>>>(sc, nm, net, nullptr, nullptr, nullptr, nullptr, n, nullptr, feats, rawOut);
+    k_guide_infer<<<grid, 64 * INFER_WAVES, 0, s>>>(sc, nm, net, nullptr, 0, 64, nullptr, nullptr, nullptr, n, nullptr, feats, rawOut);
     PT_HIP_TRY(hipGetLastError());
 }
 

@@ -39,15 +39,15 @@
 
 namespace pt {
 
-static constexpr int kMaxBounces = 64;
-// counters per bounce: ray count, shadow count, extend fetch, shadow fetch, radiance adds
-// from shade, radiance adds from shadow
-enum { C_RAYS = 0, C_SHADOW = 1, C_FETCH_EXT = 2, C_FETCH_SHADOW = 3, C_LADD_SHADE = 4, C_LADD_SHADOW = 5, C_PER_BOUNCE = 6 };
-static constexpr int kNumCounters = C_PER_BOUNCE * kMaxBounces + 8;
+static constexpr int kMaxBounces = PT_MAX_BOUNCES;
+// Queue counters: PT_QUEUE_SEGS blocks of PT_SEG_STRIDE ints (ptQueueField), then a few
+// chunk-wide counters.
+static constexpr int kSegCounters = PT_QUEUE_SEGS * PT_SEG_STRIDE;
+static constexpr int kNumCounters = kSegCounters + 8;
 // guided mode: training records appended, records kept (capped), sum of their weights (float)
-static constexpr int kRecCountSlot = C_PER_BOUNCE * kMaxBounces + 1;
-static constexpr int kRecValidSlot = C_PER_BOUNCE * kMaxBounces + 2;
-static constexpr int kWsumSlot = C_PER_BOUNCE * kMaxBounces + 3;
+static constexpr int kRecCountSlot = kSegCounters + 1;
+static constexpr int kRecValidSlot = kSegCounters + 2;
+static constexpr int kWsumSlot = kSegCounters + 3;
 // device statistics (uint64): per bounce rays, shadows, shade adds, shadow adds; totals
 enum { S_RAYS = 0, S_SHADOW = kMaxBounces, S_LADD_SHADE = 2 * kMaxBounces, S_LADD_SHADOW = 3 * kMaxBounces, S_PATHS = 4 * kMaxBounces, S_COUNT };
 
@@ -75,6 +75,7 @@ struct ChunkParams {
     int misSample;    // 1: light-sample MIS weight uses the BSDF sampling density (consistent MIS)
     int W, H;
     uint64_t seed;
+    int segLen;       // queue segment length, ptSegLen(nPaths)
 };
 
 // ------------------------------------------------------------------------------------------
@@ -93,6 +94,11 @@ __device__ __forceinline__ int waveAppend(bool want, int *counter) {
     if (laneId() == leader) base = atomicAdd(counter, __popcll(m));
     base = __shfl(base, leader);
     return want ? base + (int)lanesBelow(m) : -1;
+}
+// Appends to segment `seg` of a segmented queue: the record's slot, or -1 for lanes without one.
+__device__ __forceinline__ int segAppend(bool want, int *counters, int seg, int bounce, int field, int segLen) {
+    const int k = waveAppend(want, counters + ptQueueField(seg, bounce, field));
+    return k < 0 ? -1 : seg * segLen + k;
 }
 // Adds a per-lane count once per wave (statistics counted over a persistent wave's lifetime
 // cost one atomic instead of one per iteration).
@@ -115,12 +121,25 @@ __device__ __forceinline__ void cameraRay(const PtCamera &c, int row, int col, f
             c.m[2][0] * dl.x + c.m[2][1] * dl.y + c.m[2][2] * dl.z);
 }
 
-__global__ __launch_bounds__(256) void k_raygen(ChunkParams cp, PtCamera cam, float4 *rayO, float4 *rayD, uint4 *state, float4 *beta,
-                                                float4 *Lpath, int *counters) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blockIdx.x == 0) {
-        for (int k = threadIdx.x; k < kNumCounters; k += blockDim.x) counters[k] = (k == C_RAYS) ? cp.nPaths : 0;
+// Clears every queue counter and makes field `field` of bounce 0 count a queue of n records
+// with segment length segLen: the camera layout (ptCameraSlot) or, with camera = 0, the linear
+// one (segment s holds min(segLen, n - s segLen) records).  One block per segment plus one for
+// the chunk-wide counters.
+__global__ __launch_bounds__(256) void k_queue_reset(int *counters, int field, int n, int segLen, int camera) {
+    if (blockIdx.x < PT_QUEUE_SEGS) {
+        const int seg = (int)blockIdx.x;
+        int *q = counters + seg * PT_SEG_STRIDE;
+        const int cnt = camera ? ptCameraCount(seg, n) : max(0, min(segLen, n - seg * segLen));
+        const int at = ptQueueField(0, 0, field);
+        for (int k = threadIdx.x; k < PT_SEG_STRIDE; k += blockDim.x) q[k] = k == at ? cnt : 0;
+    } else {
+        for (int k = kSegCounters + threadIdx.x; k < kNumCounters; k += blockDim.x) counters[k] = 0;
     }
+}
+
+__global__ __launch_bounds__(256) void k_raygen(ChunkParams cp, PtCamera cam, float4 *rayO, float4 *rayD, uint4 *state, float4 *beta,
+                                                float4 *Lpath) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= cp.nPaths) return;
     const int s = i / cp.npix;
     const int pixel = cp.pixBegin + (i - s * cp.npix);
@@ -130,10 +149,11 @@ __global__ __launch_bounds__(256) void k_raygen(ChunkParams cp, PtCamera cam, fl
     const float sx = rng.uniform(), sy = rng.uniform();
     f3 o, d;
     cameraRay(cam, row, col, sx, sy, &o, &d);
-    rayO[i] = f4(o, 0.f);
-    rayD[i] = f4(d, FLT_MAX);
-    state[i] = make_uint4((uint32_t)rng.state, (uint32_t)(rng.state >> 32), (uint32_t)i, 0u);
-    beta[i] = make_float4(1.f, 1.f, 1.f, -1.f);
+    const int q = ptCameraSlot(i, cp.segLen);
+    rayO[q] = f4(o, 0.f);
+    rayD[q] = f4(d, FLT_MAX);
+    state[q] = make_uint4((uint32_t)rng.state, (uint32_t)(rng.state >> 32), (uint32_t)i, 0u);
+    beta[q] = make_float4(1.f, 1.f, 1.f, -1.f);
     Lpath[i] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
@@ -344,62 +364,80 @@ __device__ __forceinline__ TraceStack traceStack(int *sstack, int *spill, int sp
     return ts;
 }
 
-// Work distribution of the persistent trace kernels: wave w first takes rays [64 w, 64 w + 64)
-// without any atomic; later rays come from the shared cursor (offset by the statically
-// assigned part) once `refill` or more lanes are idle, as contiguous runs of the queue so that
-// the rays in flight on a CU stay neighbours.  The cursor is never read without an atomic
-// (plain reads of it queue behind the atomics in its L2 channel).
+// Work distribution over a segmented queue (the trace kernels and shade).  Wave w's home
+// segment is w % PT_QUEUE_SEGS; it claims records there with one atomic on that segment's
+// cursor and, once its home is exhausted, helps up to kQueueSteal following segments before
+// it leaves.  Every segment has home waves (each kernel launches at least PT_QUEUE_SEGS
+// waves), so every record is claimed; stealing only evens out the segments' ends.
+static constexpr int kQueueSteal = 8;
 
-__device__ __forceinline__ int firstRays() { return (int)((blockIdx.x * blockDim.x + threadIdx.x) & ~63u); }
-__device__ __forceinline__ int nextRays(int *fetch) {
-    int base = 0;
-    if (laneId() == 0) base = atomicAdd(fetch, 64);
-    return __shfl(base, 0) + (int)(gridDim.x * blockDim.x);
+struct SegCursor {
+    int seg;    // segment being drained (wave-uniform)
+    int left;   // segments this wave may still visit
+};
+
+__device__ __forceinline__ int homeSegment() { return (int)(((blockIdx.x * blockDim.x + threadIdx.x) >> 6) % PT_QUEUE_SEGS); }
+__device__ __forceinline__ SegCursor segStart() { return SegCursor{homeSegment(), 1 + kQueueSteal}; }
+
+// Claims up to `want` records of the bounce's queue (count field `countField`, cursor field
+// `fetchField`): returns how many were granted, the first at slot *slot, or 0 when this wave
+// has no segment left.  Wave-uniform.
+__device__ __forceinline__ int segClaim(int *counters, int bounce, int countField, int fetchField, int segLen, int want, SegCursor &c,
+                                        int *slot) {
+    while (c.left > 0) {
+        int *seg = counters + c.seg * PT_SEG_STRIDE;
+        const int n = seg[ptQueueField(0, bounce, countField)];
+        if (n > 0) {
+            int b = 0;
+            if (laneId() == 0) b = atomicAdd(&seg[ptQueueField(0, bounce, fetchField)], want);
+            b = __shfl(b, 0);
+            if (b < n) {
+                *slot = c.seg * segLen + b;
+                return min(want, n - b);
+            }
+        }
+        c.seg = c.seg + 1 == PT_QUEUE_SEGS ? 0 : c.seg + 1;
+        c.left--;
+    }
+    return 0;
 }
 
-// Traces the queue [0, n): load(i, o, d, tmin, tmax) reads ray i, finish(i, hit, th) stores
-// its result.  Every wave leaves the loop once the cursor is past n and its lanes are idle.
+// Traces the bounce's queue (count field countField, cursor field fetchField): load(i, o, d,
+// tmin, tmax) reads the ray in slot i, finish(i, hit, th) stores its result.  A wave takes new
+// rays once `refill` or more of its lanes are idle, as a contiguous run of one segment so that
+// the rays in flight on a CU stay neighbours, and leaves once its segments are exhausted and
+// its lanes idle.
 template <bool ANY, int STACK, class Load, class Finish>
-__device__ __forceinline__ void traceQueue(const PtSceneView &sc, int n, int *fetch, int refill, const TraceStack &ts, int *overflow,
-                                           Load load, Finish finish) {
+__device__ __forceinline__ void traceQueue(const PtSceneView &sc, int *counters, int bounce, int countField, int fetchField, int segLen,
+                                           int refill, const TraceStack &ts, int *overflow, Load load, Finish finish) {
+    SegCursor cur = segStart();
     if (sc.numNodes == 0) {
         TraceHit th;
         th.t = FLT_MAX; th.u = 0.f; th.v = 0.f; th.prim = INT32_MIN;
-        for (int base = firstRays(); base < n; base = nextRays(fetch)) {
-            const int i = base + laneId();
-            if (i < n) finish(i, false, th);
-        }
+        int slot = 0, got;
+        while ((got = segClaim(counters, bounce, countField, fetchField, segLen, 64, cur, &slot)) > 0)
+            if (laneId() < got) finish(slot + laneId(), false, th);
         return;
     }
-    const int total = (int)(gridDim.x * blockDim.x);
-    bool drained = total >= n;   // wave-uniform: no ray is left for the cursor
     LaneRay r;
-    auto begin = [&](int i) {
-        f3 o, d;
-        float tmin, tmax;
-        load(i, o, d, tmin, tmax);
-        laneRayStart(r, o, d, tmin, tmax);
-    };
-    int ray = firstRays() + laneId();
-    if (ray < n) begin(ray);
-    else ray = -1;
+    int ray = -1;
     while (true) {
         const uint64_t idle = __ballot(ray < 0);
-        if (!drained && __popcll(idle) >= refill) {
-            const int k = __popcll(idle);
-            int b = 0;
-            if (laneId() == 0) b = atomicAdd(fetch, k);
-            b = __shfl(b, 0) + total;
-            drained = b + k >= n;
+        if (cur.left > 0 && __popcll(idle) >= refill) {
+            int slot = 0;
+            const int got = segClaim(counters, bounce, countField, fetchField, segLen, __popcll(idle), cur, &slot);
             if (ray < 0) {
-                const int i = b + (int)lanesBelow(idle);
-                if (i < n) {
-                    ray = i;
-                    begin(i);
+                const int k = (int)lanesBelow(idle);
+                if (k < got) {
+                    ray = slot + k;
+                    f3 o, d;
+                    float tmin, tmax;
+                    load(ray, o, d, tmin, tmax);
+                    laneRayStart(r, o, d, tmin, tmax);
                 }
             }
         }
-        // all lanes idle after a refill attempt means the cursor has passed n
+        // all lanes idle after a refill attempt means this wave's segments are exhausted
         if (__ballot(ray >= 0) == 0) break;
         if (ray >= 0 && traceRound<ANY, STACK>(sc, r, ts, overflow)) {
             finish(ray, r.prim >= 0, laneRayHit(r));
@@ -408,14 +446,15 @@ __device__ __forceinline__ void traceQueue(const PtSceneView &sc, int n, int *fe
     }
 }
 
-// Persistent closest-hit kernel.
+// Persistent closest-hit kernel over the rays of `bounce`.
 template <int STACK>
-__global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 7 : 5) void k_extend(PtSceneView sc, const int *count, int *fetch, const float4 *rayO, const float4 *rayD,
-                                                         float4 *hits, int refill, int *spill, int spillCap, int *overflow) {
+__global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 7 : 5) void k_extend(PtSceneView sc, int *counters, int bounce, int segLen,
+                                                                              const float4 *rayO, const float4 *rayD, float4 *hits, int refill,
+                                                                              int *spill, int spillCap, int *overflow) {
     __shared__ int sstack[STACK * kTraceThreads];
     const TraceStack ts = traceStack(sstack, spill, spillCap);
     traceQueue<false, STACK>(
-        sc, *count, fetch, refill, ts, overflow,
+        sc, counters, bounce, PT_Q_RAYS, PT_Q_FETCH_EXT, segLen, refill, ts, overflow,
         [&](int i, f3 &o, f3 &d, float &tmin, float &tmax) {
             const float4 ro = rayO[i], rd = rayD[i];
             o = xyz(ro);
@@ -426,17 +465,17 @@ __global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 7 : 5) void k_extend(P
         [&](int i, bool, const TraceHit &th) { hits[i] = make_float4(th.t, th.u, th.v, asFloat(th.prim)); });
 }
 
-// Persistent any-hit kernel for shadow rays (interval [1e-4, tmax]); an unoccluded ray adds
-// its light contribution to its path.
+// Persistent any-hit kernel for the shadow rays of `bounce` (interval [1e-4, tmax]); an
+// unoccluded ray adds its light contribution to its path.
 template <int STACK>
-__global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 7 : 5) void k_shadow(PtSceneView sc, const int *count, int *fetch, const float4 *sO, const float4 *sD,
-                                                         const float4 *sL, float4 *Lpath, int *laddCounter, int refill, int *spill,
-                                                         int spillCap, int *overflow) {
+__global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 7 : 5) void k_shadow(PtSceneView sc, int *counters, int bounce, int segLen,
+                                                                              const float4 *sO, const float4 *sD, const float4 *sL, float4 *Lpath,
+                                                                              int refill, int *spill, int spillCap, int *overflow) {
     __shared__ int sstack[STACK * kTraceThreads];
     const TraceStack ts = traceStack(sstack, spill, spillCap);
     int adds = 0;
     traceQueue<true, STACK>(
-        sc, *count, fetch, refill, ts, overflow,
+        sc, counters, bounce, PT_Q_SHADOW, PT_Q_FETCH_SHADOW, segLen, refill, ts, overflow,
         [&](int i, f3 &o, f3 &d, float &tmin, float &tmax) {
             const float4 so = sO[i], sd = sD[i];
             o = xyz(so);
@@ -454,7 +493,7 @@ __global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 7 : 5) void k_shadow(P
                 adds++;
             }
         });
-    waveSum(adds, laddCounter);
+    waveSum(adds, &counters[ptQueueField(homeSegment(), bounce, PT_Q_LADD_SHADOW)]);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -600,18 +639,18 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 5) void k_shade(PtSceneView sc, C
                                                const uint4 *state, const float4 *beta, const float4 *hits, float4 *nRayO, float4 *nRayD,
                                                uint4 *nState, float4 *nBeta, float4 *sO, float4 *sD, float4 *sL, float4 *Lpath,
                                                GuideShadeArgs ga) {
-    int *c = counters + C_PER_BOUNCE * bounce;
-    int *cNext = counters + C_PER_BOUNCE * (bounce + 1);
-    const int n = c[C_RAYS];
-    const int waveStride = gridDim.x * blockDim.x;
-    // Queue appends happen where each record is complete (one atomic per wave and branch:
-    // the lanes taking a branch append together), so no output record stays live across the
-    // rest of the vertex.
+    // Waves claim 64 rays at a time from one queue segment and append the continuation and
+    // shadow rays to the same segment of the next queues.  Appends happen where each record is
+    // complete (one atomic per wave and branch: the lanes taking a branch append together), so
+    // no output record stays live across the rest of the vertex.
+    SegCursor cur = segStart();
     int adds = 0;
-    for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n; base += waveStride) {
+    int base = 0, got;
+    while ((got = segClaim(counters, bounce, PT_Q_RAYS, PT_Q_FETCH_SHADE, cp.segLen, 64, cur, &base)) > 0) {
+        const int seg = cur.seg;
         const int i = base + laneId();
         bool addL = false;
-        if (i < n) {
+        if (laneId() < got) {
             const float4 ro = rayO[i], rd = rayD[i], h = hits[i], bp = beta[i];
             const uint4 st = state[i];
             const int pathId = (int)st.z;
@@ -709,7 +748,7 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 5) void k_shade(PtSceneView sc, C
                             contrib = thr * (ls.emitted * w * bsdfF(m, wo, wiL) * fabsf(dot(sp.n, ls.wi)) / ls.pdf);
                             pushShadow = !isZero(contrib) && !(contrib.x != contrib.x);
                         }
-                        const int ss = waveAppend(pushShadow, &c[C_SHADOW]);
+                        const int ss = segAppend(pushShadow, counters, seg, bounce, PT_Q_SHADOW, cp.segLen);
                         if (pushShadow) {
                             sO[ss] = f4(sp.p, ls.dist - 2e-4f);
                             sD[ss] = f4(ls.wi, asFloat(pathId));
@@ -722,7 +761,7 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 5) void k_shade(PtSceneView sc, C
                         nb = thr * (bs.f * fabsf(bs.wi.z) / bs.pdf);
                         pushRay = !isZero(nb) && nb.x == nb.x;
                     }
-                    const int rs = waveAppend(pushRay, &cNext[C_RAYS]);
+                    const int rs = segAppend(pushRay, counters, seg, bounce + 1, PT_Q_RAYS, cp.segLen);
                     const f3 wworld = toWorld(fr, bs.wi);
                     if (pushRay) {
                         nRayO[rs] = f4(sp.p, kBounceEps);
@@ -762,7 +801,7 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 5) void k_shade(PtSceneView sc, C
         }
         adds += addL ? 1 : 0;
     }
-    waveSum(adds, &c[C_LADD_SHADE]);
+    waveSum(adds, &counters[ptQueueField(homeSegment(), bounce, PT_Q_LADD_SHADE)]);
 }
 
 // Training targets: weight = (radiance after the vertex's sampled direction) / luminance of the
@@ -799,11 +838,17 @@ __global__ __launch_bounds__(256) void k_accumulate(ChunkParams cp, const float4
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (blockIdx.x == 0) {
         for (int b = threadIdx.x; b < kMaxBounces; b += blockDim.x) {
-            const int *c = counters + C_PER_BOUNCE * b;
-            atomicAdd(&stats[S_RAYS + b], (unsigned long long)c[C_RAYS]);
-            atomicAdd(&stats[S_SHADOW + b], (unsigned long long)c[C_SHADOW]);
-            atomicAdd(&stats[S_LADD_SHADE + b], (unsigned long long)c[C_LADD_SHADE]);
-            atomicAdd(&stats[S_LADD_SHADOW + b], (unsigned long long)c[C_LADD_SHADOW]);
+            unsigned long long v[4] = {0, 0, 0, 0};
+            for (int seg = 0; seg < PT_QUEUE_SEGS; seg++) {
+                v[0] += (unsigned long long)counters[ptQueueField(seg, b, PT_Q_RAYS)];
+                v[1] += (unsigned long long)counters[ptQueueField(seg, b, PT_Q_SHADOW)];
+                v[2] += (unsigned long long)counters[ptQueueField(seg, b, PT_Q_LADD_SHADE)];
+                v[3] += (unsigned long long)counters[ptQueueField(seg, b, PT_Q_LADD_SHADOW)];
+            }
+            atomicAdd(&stats[S_RAYS + b], v[0]);
+            atomicAdd(&stats[S_SHADOW + b], v[1]);
+            atomicAdd(&stats[S_LADD_SHADE + b], v[2]);
+            atomicAdd(&stats[S_LADD_SHADOW + b], v[3]);
         }
         if (threadIdx.x == 0) atomicAdd(&stats[S_PATHS], (unsigned long long)cp.nPaths);
     }
@@ -855,22 +900,13 @@ __global__ __launch_bounds__(256) void k_normals_shade(PtSceneView sc, int npix,
     a[2] += c.z;
 }
 
-__global__ void k_centre_rays(PtCamera cam, int npix, int W, float4 *rayO, float4 *rayD, int *counters) {
+__global__ void k_centre_rays(PtCamera cam, int npix, int W, float4 *rayO, float4 *rayD) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blockIdx.x == 0)
-        for (int k = threadIdx.x; k < kNumCounters; k += blockDim.x) counters[k] = (k == C_RAYS) ? npix : 0;
     if (p >= npix) return;
     f3 o, d;
     cameraRay(cam, p / W, p % W, 0.5f, 0.5f, &o, &d);
     rayO[p] = f4(o, 0.f);
     rayD[p] = f4(d, FLT_MAX);
-}
-
-__global__ void k_set_counter(int *counters, int idx, int value) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        for (int k = 0; k < kNumCounters; k++) counters[k] = 0;
-        counters[idx] = value;
-    }
 }
 
 __global__ void k_map_prims(const PtSceneView sc, const float4 *hits, int n, float4 *out) {
@@ -1116,26 +1152,40 @@ int refillLanes(const pt_renderer *r, int kind) {
     return kind == TRACE_CAMERA ? 56 : kind == TRACE_INCOHERENT ? 44 : 40;
 }
 
-void launchExtend(pt_renderer *r, const int *count, int *fetch, const float4 *ro, const float4 *rd, float4 *hits, int kind) {
+// Trace grids hold at least PT_QUEUE_SEGS waves: every queue segment needs home waves.
+int traceGrid(const pt_renderer *r) {
+    return std::max(r->cus * r->traceBlocksPerCU, (PT_QUEUE_SEGS * 64 + kTraceThreads - 1) / kTraceThreads);
+}
+
+void launchExtend(pt_renderer *r, int bounce, int segLen, const float4 *ro, const float4 *rd, float4 *hits, int kind) {
     const int refill = refillLanes(r, kind);
-    const int grid = r->cus * r->traceBlocksPerCU;
+    const int grid = traceGrid(r);
+    int *ctr = r->pp->counters.as<int>();
     int *ov = r->pp->overflow.as<int>();
     int cap = 0;
     int *sp = ensureSpill(r, grid, &cap);
-    if (r->ldsStack == 32) k_extend<32><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, ro, rd, hits, refill, sp, cap, ov);
-    else k_extend<16><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, ro, rd, hits, refill, sp, cap, ov);
+    if (r->ldsStack == 32) k_extend<32><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, ro, rd, hits, refill, sp, cap, ov);
+    else k_extend<16><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, ro, rd, hits, refill, sp, cap, ov);
     PT_HIP_TRY(hipGetLastError());
 }
-void launchShadow(pt_renderer *r, const int *count, int *fetch, int *ladd) {
+void launchShadow(pt_renderer *r, int bounce, int segLen) {
     const int refill = refillLanes(r, TRACE_SHADOW);
-    const int grid = r->cus * r->traceBlocksPerCU;
+    const int grid = traceGrid(r);
+    int *ctr = r->pp->counters.as<int>();
     int *ov = r->pp->overflow.as<int>();
     const float4 *o = r->pp->sO.as<float4>(), *d = r->pp->sD.as<float4>(), *l = r->pp->sL.as<float4>();
     float4 *L = r->pp->Lpath.as<float4>();
     int cap = 0;
     int *sp = ensureSpill(r, grid, &cap);
-    if (r->ldsStack == 32) k_shadow<32><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, o, d, l, L, ladd, refill, sp, cap, ov);
-    else k_shadow<16><<<grid, kTraceThreads, 0, r->stream>>>(r->view, count, fetch, o, d, l, L, ladd, refill, sp, cap, ov);
+    if (r->ldsStack == 32) k_shadow<32><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, o, d, l, L, refill, sp, cap, ov);
+    else k_shadow<16><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, o, d, l, L, refill, sp, cap, ov);
+    PT_HIP_TRY(hipGetLastError());
+}
+
+// Clears the queue counters and describes a linear queue of n records (slots 0 .. n - 1) in
+// field `field` of bounce 0; its segment length is ptSegLen(n).
+void queueLinear(pt_renderer *r, int field, int n) {
+    k_queue_reset<<<PT_QUEUE_SEGS + 1, 256, 0, r->stream>>>(r->pp->counters.as<int>(), field, n, ptSegLen(n), 0);
     PT_HIP_TRY(hipGetLastError());
 }
 
@@ -1194,29 +1244,31 @@ void runChunk(pt_renderer *r, const ChunkParams &cp) {
     const bool guided = r->type == PT_RENDERER_GUIDED;
     {
         KTimer t(r, K_RAYGEN);
+        k_queue_reset<<<PT_QUEUE_SEGS + 1, 256, 0, r->stream>>>(ctr, PT_Q_RAYS, cp.nPaths, cp.segLen, 1);
         k_raygen<<<(cp.nPaths + 255) / 256, 256, 0, r->stream>>>(cp, r->cam, r->pp->rayO[0].as<float4>(), r->pp->rayD[0].as<float4>(), r->pp->st[0].as<uint4>(),
-                                                                r->pp->beta[0].as<float4>(), r->pp->Lpath.as<float4>(), ctr);
+                                                                r->pp->beta[0].as<float4>(), r->pp->Lpath.as<float4>());
         PT_HIP_TRY(hipGetLastError());
         if (guided) {
             // camera rays were not spawned by a guided vertex
-            k_fill_i32<<<(cp.nPaths + 255) / 256, 256, 0, r->stream>>>(r->gRecIdx[0].as<int>(), cp.nPaths, -1);
+            const int slots = PT_QUEUE_SEGS * cp.segLen;
+            k_fill_i32<<<(slots + 255) / 256, 256, 0, r->stream>>>(r->gRecIdx[0].as<int>(), slots, -1);
             PT_HIP_TRY(hipGetLastError());
         }
     }
-    const int shadeGrid = (int)std::min<int64_t>((cp.nPaths + 255) / 256, (int64_t)r->cus * r->shadeBlocksPerCU);
+    // shade needs PT_QUEUE_SEGS waves or more (home waves for every segment)
+    const int shadeGrid = (int)std::max<int64_t>(PT_QUEUE_SEGS / 4, std::min<int64_t>((cp.nPaths + 255) / 256, (int64_t)r->cus * r->shadeBlocksPerCU));
     int cur = 0;
     for (int b = 0; b < cp.maxDepth; b++) {
-        int *c = ctr + C_PER_BOUNCE * b;
         {
             KTimer t(r, K_EXTEND);
-            launchExtend(r, &c[C_RAYS], &c[C_FETCH_EXT], r->pp->rayO[cur].as<float4>(), r->pp->rayD[cur].as<float4>(), r->pp->hits.as<float4>(),
+            launchExtend(r, b, cp.segLen, r->pp->rayO[cur].as<float4>(), r->pp->rayD[cur].as<float4>(), r->pp->hits.as<float4>(),
                          b == 0 ? TRACE_CAMERA : TRACE_INCOHERENT);
         }
         GuideShadeArgs ga{};
         if (guided && b + 1 < cp.maxDepth) {
             KTimer t(r, K_GUIDE_INFER);
-            guideInfer(r->view, guideNorm(r), guideNet(r), &c[C_RAYS], r->pp->rayO[cur].as<float4>(), r->pp->rayD[cur].as<float4>(), r->pp->hits.as<float4>(),
-                       cp.nPaths, r->gMix.as<float4>(), r->cus, r->stream);
+            guideInfer(r->view, guideNorm(r), guideNet(r), ctr + ptQueueField(0, b, PT_Q_RAYS), PT_SEG_STRIDE, cp.segLen, r->pp->rayO[cur].as<float4>(),
+                       r->pp->rayD[cur].as<float4>(), r->pp->hits.as<float4>(), r->gMix.as<float4>(), r->cus, r->stream);
         }
         if (guided) {
             ga.mix = r->gMix.as<float4>();
@@ -1243,7 +1295,7 @@ void runChunk(pt_renderer *r, const ChunkParams &cp) {
         }
         if (cp.nee) {
             KTimer t(r, K_SHADOW);
-            launchShadow(r, &c[C_SHADOW], &c[C_FETCH_SHADOW], &c[C_LADD_SHADOW]);
+            launchShadow(r, b, cp.segLen);
         }
         cur = 1 - cur;
     }
@@ -1283,6 +1335,7 @@ void planChunks(const pt_renderer *r, int spp, std::vector<ChunkParams> &out) {
         cp.samples = samples;
         cp.sampleBase = sampleBase;
         cp.nPaths = n * samples;
+        cp.segLen = ptSegLen(cp.nPaths);
         cp.maxDepth = r->maxDepth;
         cp.nee = r->nee;
         cp.misSample = r->misSample;
@@ -1664,10 +1717,9 @@ int pt_renderer_render_async(pt_renderer *r, int spp) {
         // spp = 1 each pass, g_buffer.cu:108-117)
         if (npix > r->poolWanted) r->poolWanted = npix;
         ensurePool(r, npix);
-        int *c = r->pp->counters.as<int>();
-        k_centre_rays<<<(npix + 255) / 256, 256, 0, r->stream>>>(r->cam, npix, r->W, r->pp->rayO[0].as<float4>(), r->pp->rayD[0].as<float4>(), c);
-        launchExtend(r, &c[C_RAYS], &c[C_FETCH_EXT], r->pp->rayO[0].as<float4>(), r->pp->rayD[0].as<float4>(), r->pp->hits.as<float4>(),
-                     TRACE_CAMERA);
+        k_centre_rays<<<(npix + 255) / 256, 256, 0, r->stream>>>(r->cam, npix, r->W, r->pp->rayO[0].as<float4>(), r->pp->rayD[0].as<float4>());
+        queueLinear(r, PT_Q_RAYS, npix);
+        launchExtend(r, 0, ptSegLen(npix), r->pp->rayO[0].as<float4>(), r->pp->rayD[0].as<float4>(), r->pp->hits.as<float4>(), TRACE_CAMERA);
         k_normals_shade<<<(npix + 255) / 256, 256, 0, r->stream>>>(r->view, npix, r->pp->rayO[0].as<float4>(), r->pp->rayD[0].as<float4>(),
                                                                   r->pp->hits.as<float4>(), r->accum.as<double>());
         PT_HIP_TRY(hipGetLastError());
@@ -1803,10 +1855,8 @@ int pt_renderer_hit_test(pt_renderer *r, int x, int y, int *material_id, int *is
     }
     PT_HIP_TRY(hipMemcpyAsync(r->pp->rayO[0].p, &ro, 16, hipMemcpyHostToDevice, r->stream));
     PT_HIP_TRY(hipMemcpyAsync(r->pp->rayD[0].p, &rd, 16, hipMemcpyHostToDevice, r->stream));
-    int *c = r->pp->counters.as<int>();
-    k_set_counter<<<1, 64, 0, r->stream>>>(c, C_RAYS, 1);
-    launchExtend(r, &c[C_RAYS], &c[C_FETCH_EXT], r->pp->rayO[0].as<float4>(), r->pp->rayD[0].as<float4>(), r->pp->hits.as<float4>(),
-                 TRACE_CAMERA);
+    queueLinear(r, PT_Q_RAYS, 1);
+    launchExtend(r, 0, ptSegLen(1), r->pp->rayO[0].as<float4>(), r->pp->rayD[0].as<float4>(), r->pp->hits.as<float4>(), TRACE_CAMERA);
     float4 h;
     PT_HIP_TRY(hipMemcpyAsync(&h, r->pp->hits.p, 16, hipMemcpyDeviceToHost, r->stream));
     PT_HIP_TRY(hipStreamSynchronize(r->stream));
@@ -1951,7 +2001,6 @@ int pt_renderer_intersect(pt_renderer *r, const float *rays, int64_t n, int any_
     if (r->inFlight) return fail(PT_ERR_STATE, "pt_renderer_intersect: a pass is still running");
     PT_API_BEGIN
     ensurePool(r);
-    int *c = r->pp->counters.as<int>();
     for (int64_t done = 0; done < n;) {
         const int cnt = (int)std::min<int64_t>(n - done, r->pp->pool);
         std::vector<float4> o(cnt), d(cnt);
@@ -1963,9 +2012,8 @@ int pt_renderer_intersect(pt_renderer *r, const float *rays, int64_t n, int any_
         if (!any_hit) {
             PT_HIP_TRY(hipMemcpyAsync(r->pp->rayO[0].p, o.data(), cnt * 16, hipMemcpyHostToDevice, r->stream));
             PT_HIP_TRY(hipMemcpyAsync(r->pp->rayD[0].p, d.data(), cnt * 16, hipMemcpyHostToDevice, r->stream));
-            k_set_counter<<<1, 64, 0, r->stream>>>(c, C_RAYS, cnt);
-            launchExtend(r, &c[C_RAYS], &c[C_FETCH_EXT], r->pp->rayO[0].as<float4>(), r->pp->rayD[0].as<float4>(), r->pp->hits.as<float4>(),
-                         TRACE_INCOHERENT);
+            queueLinear(r, PT_Q_RAYS, cnt);
+            launchExtend(r, 0, ptSegLen(cnt), r->pp->rayO[0].as<float4>(), r->pp->rayD[0].as<float4>(), r->pp->hits.as<float4>(), TRACE_INCOHERENT);
             k_map_prims<<<(cnt + 255) / 256, 256, 0, r->stream>>>(r->view, r->pp->hits.as<float4>(), cnt, r->pp->sL.as<float4>());
             std::vector<float4> h(cnt);
             PT_HIP_TRY(hipMemcpyAsync(h.data(), r->pp->sL.p, cnt * 16, hipMemcpyDeviceToHost, r->stream));
@@ -1983,8 +2031,8 @@ int pt_renderer_intersect(pt_renderer *r, const float *rays, int64_t n, int any_
             PT_HIP_TRY(hipMemcpyAsync(r->pp->sD.p, sd.data(), cnt * 16, hipMemcpyHostToDevice, r->stream));
             PT_HIP_TRY(hipMemcpyAsync(r->pp->sL.p, sl.data(), cnt * 16, hipMemcpyHostToDevice, r->stream));
             PT_HIP_TRY(hipMemcpyAsync(r->pp->Lpath.p, z.data(), cnt * 16, hipMemcpyHostToDevice, r->stream));
-            k_set_counter<<<1, 64, 0, r->stream>>>(c, C_SHADOW, cnt);
-            launchShadow(r, &c[C_SHADOW], &c[C_FETCH_SHADOW], &c[C_LADD_SHADOW]);
+            queueLinear(r, PT_Q_SHADOW, cnt);
+            launchShadow(r, 0, ptSegLen(cnt));
             std::vector<float4> h(cnt);
             PT_HIP_TRY(hipMemcpyAsync(h.data(), r->pp->Lpath.p, cnt * 16, hipMemcpyDeviceToHost, r->stream));
             PT_HIP_TRY(hipStreamSynchronize(r->stream));

@@ -47,6 +47,52 @@ static constexpr int PT_LEAF_MAX = 4;
 static constexpr int PT_NODE_F4 = 8;
 static constexpr int PT_EMPTY_CHILD = 0x7fffffff;
 
+// Segmented path queues.  Every queue of a chunk (the rays of a bounce, the shadow rays of a
+// bounce) is cut into PT_QUEUE_SEGS segments of segLen slots, segLen a multiple of 64: segment
+// s holds slots [s * segLen, s * segLen + count_s).  A path stays in its segment from bounce to
+// bounce, so a segment never receives more records than it has slots.  Each segment has its
+// own counters, PT_SEG_STRIDE ints (2 KiB) apart, and within a segment each field has its own
+// 256-byte row (one int per bounce): device-scope atomics serialise per 64-byte block (about
+// 88 per microsecond on one block, scaling with the number of blocks), so the work cursors and
+// append counters that one kernel hits spread over PT_QUEUE_SEGS x fields blocks.
+static constexpr int PT_MAX_BOUNCES = 64;
+static constexpr int PT_QUEUE_SEGS = 64;
+static constexpr int PT_QF = 8;  // counter fields per bounce
+static constexpr int PT_SEG_STRIDE = PT_QF * PT_MAX_BOUNCES;
+enum {
+    PT_Q_RAYS = 0,          // rays of the bounce in this segment
+    PT_Q_SHADOW = 1,        // shadow rays of the bounce
+    PT_Q_FETCH_EXT = 2,     // work cursors of extend, shade and shadow
+    PT_Q_FETCH_SHADE = 3,
+    PT_Q_FETCH_SHADOW = 4,
+    PT_Q_LADD_SHADE = 5,    // radiance additions (statistics)
+    PT_Q_LADD_SHADOW = 6
+};
+__host__ __device__ __forceinline__ int ptQueueField(int seg, int bounce, int field) {
+    return seg * PT_SEG_STRIDE + field * PT_MAX_BOUNCES + bounce;
+}
+// Segment length for a queue of n records: the smallest multiple of 64 with
+// PT_QUEUE_SEGS * segLen >= n.  A linear queue of n records fills the segments in order; the
+// camera queue deals runs of 64 paths to the segments in turn (ptCameraSlot).
+__host__ __device__ __forceinline__ int ptSegLen(int64_t n) {
+    const int64_t per = (n + PT_QUEUE_SEGS - 1) / PT_QUEUE_SEGS;
+    return (int)(per < 64 ? 64 : (per + 63) / 64 * 64);
+}
+// Camera queue: path p belongs to run p / 64, dealt to segment (p / 64) % PT_QUEUE_SEGS, so
+// every segment holds runs from the whole chunk (similar work per segment) and a run of 64
+// neighbouring pixels stays together.
+__host__ __device__ __forceinline__ int ptCameraSlot(int p, int segLen) {
+    const int run = p >> 6;
+    return (run % PT_QUEUE_SEGS) * segLen + (run / PT_QUEUE_SEGS) * 64 + (p & 63);
+}
+__host__ __device__ __forceinline__ int ptCameraCount(int seg, int n) {
+    const int full = n / (64 * PT_QUEUE_SEGS);
+    const int rest = n - full * 64 * PT_QUEUE_SEGS - seg * 64;
+    return full * 64 + (rest < 0 ? 0 : rest > 64 ? 64 : rest);
+}
+// Slots a pool of `paths` records needs for any queue laid out in segments.
+__host__ __device__ __forceinline__ int64_t ptQueueSlots(int64_t paths) { return (int64_t)PT_QUEUE_SEGS * ptSegLen(paths); }
+
 struct PtSceneView {
     const float4 *nodes;      // PT_NODE_F4 per 4-wide node
     const float4 *prims;      // 3 per primitive, BVH order
