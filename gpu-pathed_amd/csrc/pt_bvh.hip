@@ -319,7 +319,11 @@ __global__ void k_write4(const int2 *children, const int2 *ranges, const float *
         }
     }
     for (; cnt < 4; cnt++) {
-        for (int k = 0; k < 3; k++) lo[k][cnt] = hi[k][cnt] = 1e30f;
+        // an inverted box: every slab test misses it
+        for (int k = 0; k < 3; k++) {
+            lo[k][cnt] = 1e30f;
+            hi[k][cnt] = -1e30f;
+        }
         ref[cnt] = PT_EMPTY_CHILD;
     }
     float4 *o = nodes + (size_t)PT_NODE_F4 * index[i];
@@ -342,7 +346,7 @@ __global__ void k_write_single(const float *leafBoxes, int n, float4 *nodes) {
         }
     for (int k = 0; k < 3; k++) {
         nodes[2 * k] = make_float4(lo[k], 1e30f, 1e30f, 1e30f);
-        nodes[2 * k + 1] = make_float4(hi[k], 1e30f, 1e30f, 1e30f);
+        nodes[2 * k + 1] = make_float4(hi[k], -1e30f, -1e30f, -1e30f);
     }
     nodes[6] = make_float4(asFloat(~(((n - 1) << PT_LEAF_COUNT_SHIFT) | 0)), asFloat(PT_EMPTY_CHILD), asFloat(PT_EMPTY_CHILD),
                            asFloat(PT_EMPTY_CHILD));

@@ -266,14 +266,16 @@ __device__ __forceinline__ bool traceRound(const PtSceneView &sc, LaneRay &r, co
         }
     };
     const char *nodes = (const char *)sc.nodes;
-    const uint32_t fX = 16u - r.nX, fY = 80u - r.nY, fZ = 144u - r.nZ;
     bool missed = false;
     while (r.node >= 0) {
+        // 32-bit offsets from the array base (a node array stays below 4 GiB); a node is 128
+        // bytes, so the near bound's offset is an OR and the far bound of the same axis (the
+        // other float4 of the pair) is that offset XOR 16
         const uint32_t base = (uint32_t)r.node * (uint32_t)(PT_NODE_F4 * 16);
-        // 32-bit offsets from the array base (a node array stays below 4 GiB)
-        const float4 nx = *(const float4 *)(nodes + (base + r.nX)), fx = *(const float4 *)(nodes + (base + fX));
-        const float4 ny = *(const float4 *)(nodes + (base + r.nY)), fy = *(const float4 *)(nodes + (base + fY));
-        const float4 nz = *(const float4 *)(nodes + (base + r.nZ)), fz = *(const float4 *)(nodes + (base + fZ));
+        const uint32_t ax = base | r.nX, ay = base | r.nY, az = base | r.nZ;
+        const float4 nx = *(const float4 *)(nodes + ax), fx = *(const float4 *)(nodes + (ax ^ 16u));
+        const float4 ny = *(const float4 *)(nodes + ay), fy = *(const float4 *)(nodes + (ay ^ 16u));
+        const float4 nz = *(const float4 *)(nodes + az), fz = *(const float4 *)(nodes + (az ^ 16u));
         const float4 rf = *(const float4 *)(nodes + (base + 96u));
         float tn[4];
         int ref[4] = {asInt(rf.x), asInt(rf.y), asInt(rf.z), asInt(rf.w)};
@@ -282,9 +284,10 @@ __device__ __forceinline__ bool traceRound(const PtSceneView &sc, LaneRay &r, co
         const float bnz[4] = {nz.x, nz.y, nz.z, nz.w}, bfz[4] = {fz.x, fz.y, fz.z, fz.w};
 #pragma unroll
         for (int k = 0; k < 4; k++) {
+            // an unused slot holds an inverted box (lower bounds +1e30, upper -1e30): it misses
             const float lo = fmaxf(fmaxf(bnx[k] * r.ix - r.ox, bny[k] * r.iy - r.oy), fmaxf(bnz[k] * r.iz - r.oz, r.tmin));
             const float hi = fminf(fminf(bfx[k] * r.ix - r.ox, bfy[k] * r.iy - r.oy), fminf(bfz[k] * r.iz - r.oz, r.tmax));
-            tn[k] = (lo <= hi && ref[k] != PT_EMPTY_CHILD) ? lo : INFINITY;
+            tn[k] = lo <= hi ? lo : INFINITY;
         }
         if constexpr (!ANY) {
             cswap(tn[0], ref[0], tn[1], ref[1]);
@@ -1143,13 +1146,13 @@ int *ensureSpill(pt_renderer *r, int grid, int *cap) {
 // Idle lanes out of 64 at which a trace wave takes new rays.  Camera rays of neighbouring
 // pixels share most nodes, so their waves keep a batch together longer; the incoherent
 // queues refill earlier, which shortens the wait for a batch's longest ray.  Per-launch
-// optima on the 1.31M-triangle icosphere at 1024^2: camera rays 56, later closest-hit
-// queues 44, shadow rays 40 (64, whole batches only: 35.3 -> 27.9 ms of extend and 12.6 ->
-// 8.5 ms of shadow per 64-spp pass).
+// optima on the 1.31M-triangle icosphere at 1024^2, 64 spp: camera rays 56 (12.6 ms), the
+// first bounce's closest-hit queue 32 (10.7 ms; 19.6 ms with whole batches of 64), shadow
+// rays 32 (7.4 ms; 11.7 ms with whole batches).
 enum { TRACE_CAMERA = 0, TRACE_INCOHERENT = 1, TRACE_SHADOW = 2 };
 int refillLanes(const pt_renderer *r, int kind) {
     if (r->refillLanes > 0) return r->refillLanes;
-    return kind == TRACE_CAMERA ? 56 : kind == TRACE_INCOHERENT ? 44 : 40;
+    return kind == TRACE_CAMERA ? 56 : 32;
 }
 
 // Trace grids hold at least PT_QUEUE_SEGS waves: every queue segment needs home waves.

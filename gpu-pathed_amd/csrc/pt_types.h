@@ -43,7 +43,8 @@ static constexpr int PT_LEAF_MAX = 4;
 // 4-wide BVH node: 8 float4 = 128 bytes (one L2 line): lo.x, hi.x, lo.y, hi.y, lo.z, hi.z of
 // the four child boxes (one float4 per bound and axis), then the four child references as int
 // bits (>= 0 node index, < 0 leaf ~((count - 1) << PT_LEAF_COUNT_SHIFT | first), or
-// PT_EMPTY_CHILD for an unused slot), then one unused float4.
+// PT_EMPTY_CHILD for an unused slot, whose box is inverted: lower bounds +1e30, upper -1e30),
+// then one unused float4.
 static constexpr int PT_NODE_F4 = 8;
 static constexpr int PT_EMPTY_CHILD = 0x7fffffff;
 
