@@ -245,11 +245,17 @@ __device__ __forceinline__ TraceHit laneRayHit(const LaneRay &r) {
     return th;
 }
 
-// One traversal round of a lane's ray: descend through 4-wide nodes, nearest hit child first,
-// until a leaf or a node whose children all miss; test the leaf's primitives; pop the next
-// entry.  Returns true when the ray is finished: the stack is empty, or an any-hit query
-// found an occluder.  A wave runs the descent until all its lanes have reached a leaf or a
-// miss, then the leaf tests, so the node and leaf code never interleave within a round.
+// One traversal round of a lane's ray.  The wave descends through 4-wide nodes, nearest hit
+// child first, popping the stack after a node whose children all miss, until every lane has
+// found a leaf or exhausted its stack; a lane that finds a leaf while others are still
+// searching parks it and keeps descending (speculatively, with the interval it has), up to a
+// second leaf.  Then the parked leaves' primitives are tested together.  Leaves are tested in
+// the order a depth-first traversal reaches them, and a node visited only because its leaf
+// was parked holds no primitive at or below the final tmax, so the hit does not depend on the
+// schedule.  Returns true when the ray is finished: the stack is exhausted, or an any-hit query
+// found an occluder.
+static constexpr int kTraceDone = INT32_MIN;   // LaneRay::node once the stack is exhausted
+
 template <bool ANY, int STACK>
 __device__ __forceinline__ bool traceRound(const PtSceneView &sc, LaneRay &r, const TraceStack &ts, int *overflow) {
     const int stride = blockDim.x;
@@ -265,74 +271,85 @@ __device__ __forceinline__ bool traceRound(const PtSceneView &sc, LaneRay &r, co
             atomicAdd(overflow, 1);
         }
     };
-    const char *nodes = (const char *)sc.nodes;
-    bool missed = false;
-    while (r.node >= 0) {
-        // 32-bit offsets from the array base (a node array stays below 4 GiB); a node is 128
-        // bytes, so the near bound's offset is an OR and the far bound of the same axis (the
-        // other float4 of the pair) is that offset XOR 16
-        const uint32_t base = (uint32_t)r.node * (uint32_t)(PT_NODE_F4 * 16);
-        const uint32_t ax = base | r.nX, ay = base | r.nY, az = base | r.nZ;
-        const float4 nx = *(const float4 *)(nodes + ax), fx = *(const float4 *)(nodes + (ax ^ 16u));
-        const float4 ny = *(const float4 *)(nodes + ay), fy = *(const float4 *)(nodes + (ay ^ 16u));
-        const float4 nz = *(const float4 *)(nodes + az), fz = *(const float4 *)(nodes + (az ^ 16u));
-        const float4 rf = *(const float4 *)(nodes + (base + 96u));
-        float tn[4];
-        int ref[4] = {asInt(rf.x), asInt(rf.y), asInt(rf.z), asInt(rf.w)};
-        const float bnx[4] = {nx.x, nx.y, nx.z, nx.w}, bfx[4] = {fx.x, fx.y, fx.z, fx.w};
-        const float bny[4] = {ny.x, ny.y, ny.z, ny.w}, bfy[4] = {fy.x, fy.y, fy.z, fy.w};
-        const float bnz[4] = {nz.x, nz.y, nz.z, nz.w}, bfz[4] = {fz.x, fz.y, fz.z, fz.w};
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            // an unused slot holds an inverted box (lower bounds +1e30, upper -1e30): it misses
-            const float lo = fmaxf(fmaxf(bnx[k] * r.ix - r.ox, bny[k] * r.iy - r.oy), fmaxf(bnz[k] * r.iz - r.oz, r.tmin));
-            const float hi = fminf(fminf(bfx[k] * r.ix - r.ox, bfy[k] * r.iy - r.oy), fminf(bfz[k] * r.iz - r.oz, r.tmax));
-            tn[k] = lo <= hi ? lo : INFINITY;
-        }
-        if constexpr (!ANY) {
-            cswap(tn[0], ref[0], tn[1], ref[1]);
-            cswap(tn[2], ref[2], tn[3], ref[3]);
-            cswap(tn[0], ref[0], tn[2], ref[2]);
-            cswap(tn[1], ref[1], tn[3], ref[3]);
-            cswap(tn[1], ref[1], tn[2], ref[2]);
-            if (tn[0] == INFINITY) {
-                missed = true;
-                break;
-            }
-            // hit children are a prefix of the sorted slots: push all but the nearest, the
-            // second nearest on top
-            const int more = (tn[1] != INFINITY) + (tn[2] != INFINITY) + (tn[3] != INFINITY);
-            if (r.sp + 3 <= STACK) {
-                if (more >= 3) ts.lds[r.sp * stride] = ref[3];
-                if (more >= 2) ts.lds[(r.sp + more - 2) * stride] = ref[2];
-                if (more >= 1) ts.lds[(r.sp + more - 1) * stride] = ref[1];
-                r.sp += more;
-            } else {
-                if (more >= 3) push(ref[3]);
-                if (more >= 2) push(ref[2]);
-                if (more >= 1) push(ref[1]);
-            }
-            r.node = ref[0];
+    auto pop = [&]() {
+        if (r.sp == 0) {
+            r.node = kTraceDone;
         } else {
-            int next = PT_EMPTY_CHILD;
+            r.sp--;
+            r.node = r.sp < STACK ? ts.lds[r.sp * stride] : ts.spill[(size_t)(r.sp - STACK) * ts.spillStride];
+        }
+    };
+    const char *nodes = (const char *)sc.nodes;
+    int leaf = 0;   // parked leaf reference (< 0), or 0
+    while (__ballot(leaf == 0 && r.node != kTraceDone) != 0) {
+        if (r.node >= 0) {
+            // 32-bit offsets from the array base (a node array stays below 4 GiB); a node is
+            // 128 bytes, so the near bound's offset is an OR and the far bound of the same axis
+            // (the other float4 of the pair) is that offset XOR 16
+            const uint32_t base = (uint32_t)r.node * (uint32_t)(PT_NODE_F4 * 16);
+            const uint32_t ax = base | r.nX, ay = base | r.nY, az = base | r.nZ;
+            const float4 nx = *(const float4 *)(nodes + ax), fx = *(const float4 *)(nodes + (ax ^ 16u));
+            const float4 ny = *(const float4 *)(nodes + ay), fy = *(const float4 *)(nodes + (ay ^ 16u));
+            const float4 nz = *(const float4 *)(nodes + az), fz = *(const float4 *)(nodes + (az ^ 16u));
+            const float4 rf = *(const float4 *)(nodes + (base + 96u));
+            float tn[4];
+            int ref[4] = {asInt(rf.x), asInt(rf.y), asInt(rf.z), asInt(rf.w)};
+            const float bnx[4] = {nx.x, nx.y, nx.z, nx.w}, bfx[4] = {fx.x, fx.y, fx.z, fx.w};
+            const float bny[4] = {ny.x, ny.y, ny.z, ny.w}, bfy[4] = {fy.x, fy.y, fy.z, fy.w};
+            const float bnz[4] = {nz.x, nz.y, nz.z, nz.w}, bfz[4] = {fz.x, fz.y, fz.z, fz.w};
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                if (tn[k] != INFINITY) {
-                    if (next == PT_EMPTY_CHILD) next = ref[k];
-                    else push(ref[k]);
+                // an unused slot holds an inverted box (lower bounds +1e30, upper -1e30): it misses
+                const float lo = fmaxf(fmaxf(bnx[k] * r.ix - r.ox, bny[k] * r.iy - r.oy), fmaxf(bnz[k] * r.iz - r.oz, r.tmin));
+                const float hi = fminf(fminf(bfx[k] * r.ix - r.ox, bfy[k] * r.iy - r.oy), fminf(bfz[k] * r.iz - r.oz, r.tmax));
+                tn[k] = lo <= hi ? lo : INFINITY;
+            }
+            if constexpr (!ANY) {
+                cswap(tn[0], ref[0], tn[1], ref[1]);
+                cswap(tn[2], ref[2], tn[3], ref[3]);
+                cswap(tn[0], ref[0], tn[2], ref[2]);
+                cswap(tn[1], ref[1], tn[3], ref[3]);
+                cswap(tn[1], ref[1], tn[2], ref[2]);
+                if (tn[0] == INFINITY) {
+                    pop();
+                } else {
+                    // hit children are a prefix of the sorted slots: push all but the nearest,
+                    // the second nearest on top
+                    const int more = (tn[1] != INFINITY) + (tn[2] != INFINITY) + (tn[3] != INFINITY);
+                    if (r.sp + 3 <= STACK) {
+                        if (more >= 3) ts.lds[r.sp * stride] = ref[3];
+                        if (more >= 2) ts.lds[(r.sp + more - 2) * stride] = ref[2];
+                        if (more >= 1) ts.lds[(r.sp + more - 1) * stride] = ref[1];
+                        r.sp += more;
+                    } else {
+                        if (more >= 3) push(ref[3]);
+                        if (more >= 2) push(ref[2]);
+                        if (more >= 1) push(ref[1]);
+                    }
+                    r.node = ref[0];
                 }
+            } else {
+                int next = PT_EMPTY_CHILD;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    if (tn[k] != INFINITY) {
+                        if (next == PT_EMPTY_CHILD) next = ref[k];
+                        else push(ref[k]);
+                    }
+                }
+                if (next == PT_EMPTY_CHILD) pop();
+                else r.node = next;
             }
-            if (next == PT_EMPTY_CHILD) {
-                missed = true;
-                break;
-            }
-            r.node = next;
+        } else if (leaf == 0 && r.node != kTraceDone) {
+            leaf = r.node;
+            pop();
         }
+        // lanes holding a parked leaf and a second one, or done, wait for the others
     }
-    if (!missed) {
-        const int leaf = ~r.node;
-        const int first = leaf & ((1 << PT_LEAF_COUNT_SHIFT) - 1);
-        const int cnt = (leaf >> PT_LEAF_COUNT_SHIFT) + 1;
+    if (leaf != 0) {
+        const int e = ~leaf;
+        const int first = e & ((1 << PT_LEAF_COUNT_SHIFT) - 1);
+        const int cnt = (e >> PT_LEAF_COUNT_SHIFT) + 1;
         for (int k = first; k < first + cnt; k++) {
             const float4 p0 = sc.prims[3 * (size_t)k], p1 = sc.prims[3 * (size_t)k + 1];
             float t, u = 0.f, v = 0.f;
@@ -352,10 +369,7 @@ __device__ __forceinline__ bool traceRound(const PtSceneView &sc, LaneRay &r, co
             }
         }
     }
-    if (r.sp == 0) return true;
-    r.sp--;
-    r.node = r.sp < STACK ? ts.lds[r.sp * stride] : ts.spill[(size_t)(r.sp - STACK) * ts.spillStride];
-    return false;
+    return r.node == kTraceDone;
 }
 
 __device__ __forceinline__ TraceStack traceStack(int *sstack, int *spill, int spillCap) {
@@ -1146,13 +1160,13 @@ int *ensureSpill(pt_renderer *r, int grid, int *cap) {
 // Idle lanes out of 64 at which a trace wave takes new rays.  Camera rays of neighbouring
 // pixels share most nodes, so their waves keep a batch together longer; the incoherent
 // queues refill earlier, which shortens the wait for a batch's longest ray.  Per-launch
-// optima on the 1.31M-triangle icosphere at 1024^2, 64 spp: camera rays 56 (12.6 ms), the
-// first bounce's closest-hit queue 32 (10.7 ms; 19.6 ms with whole batches of 64), shadow
-// rays 32 (7.4 ms; 11.7 ms with whole batches).
+// optima on the 1.31M-triangle icosphere at 1024^2, 64 spp: camera rays 48 (11.5 ms), the
+// first bounce's closest-hit queue 32 (10.3 ms; 19.6 ms with whole batches of 64), shadow
+// rays 32 (7.3 ms; 11.7 ms with whole batches).
 enum { TRACE_CAMERA = 0, TRACE_INCOHERENT = 1, TRACE_SHADOW = 2 };
 int refillLanes(const pt_renderer *r, int kind) {
     if (r->refillLanes > 0) return r->refillLanes;
-    return kind == TRACE_CAMERA ? 56 : 32;
+    return kind == TRACE_CAMERA ? 48 : 32;
 }
 
 // Trace grids hold at least PT_QUEUE_SEGS waves: every queue segment needs home waves.

@@ -178,7 +178,7 @@ typedef enum {
     PT_OPT_TRACE_REFILL_LANES = 12 /* a trace wave takes new rays from the queue once this many
                                     of its 64 lanes are idle (1..64; 64: whole batches of 64
                                     neighbouring rays); 0 (default) = per launch: camera rays
-                                    56, later closest-hit queues and shadow rays 32 */
+                                    48, later closest-hit queues and shadow rays 32 */
 } pt_option;
 int pt_renderer_set_option(pt_renderer *r, int option, int64_t value);
 int pt_renderer_get_option(pt_renderer *r, int option, int64_t *value);
