@@ -69,12 +69,25 @@ void guideUploadBf16(const float *dParams, __bf16 *dW, __bf16 *dWt, float *dB, h
 void guideInfer(const PtSceneView &sc, GuideSceneNorm norm, GuideNet net, const int *segCount, int segStride, int segLen,
                 const float4 *rayO, const float4 *rayD, const float4 *hits, float4 *out, int cus, hipStream_t s);
 void guideInferRaw(GuideNet net, const float *feats, int n, float *rawOut, hipStream_t s);
+// Scratch of one training step over `batch` records: transposed BF16 layer inputs and output
+// deltas (GUIDE_LAYERS x GUIDE_W x guideRows(batch) each), FP32 gradient partials
+// (guideSlices(batch) x GUIDE_PARAMS), per-block loss sums of the forward/backward kernel
+// (fwdBlocks of them).
+constexpr int GUIDE_WG_ROWS = 2048;   // rows per slice of the weight-gradient product
+inline int guideRows(int batch) { return (batch + 31) / 32 * 32; }
+inline int guideSlices(int batch) { return (guideRows(batch) + GUIDE_WG_ROWS - 1) / GUIDE_WG_ROWS; }
+inline size_t guideActBytes(int batch) { return (size_t)GUIDE_LAYERS * GUIDE_W * guideRows(batch) * 2; }
+struct GuideTrainBufs {
+    __bf16 *act, *delta;
+    float *partials;
+    float *lossPart;
+    int fwdBlocks;
+};
 // Gradients of the mean loss over `batch` records (rows (base + i) mod *count of the record
-// buffer; weights scaled to mean one by *wsum when wsum is given): per-block FP32 partials
-// (`blocks` x GUIDE_PARAMS floats), per-block loss sums (`lossPart`), then the block-ordered
-// sum into `grad` and the mean loss into lossOut[0].
-void guideGradients(GuideNet net, const float4 *records, const int *count, int base, int batch, const float *wsum, float *partials, int blocks,
-                    float *lossPart, float *grad, float *lossOut, hipStream_t s);
+// buffer; weights scaled to mean one by *wsum when wsum is given) into `grad`, the mean loss
+// into lossOut[0].
+void guideGradients(GuideNet net, const float4 *records, const int *count, int base, int batch, const float *wsum, const GuideTrainBufs &tb,
+                    float *grad, float *lossOut, hipStream_t s);
 void guideAdam(float *params, float *m, float *v, const float *grad, int step, float lr, __bf16 *dW, __bf16 *dWt, float *dB, hipStream_t s);
 // C (32 x 32) = A (32 x 16) x B (16 x 32) with one v_mfma_f32_32x32x16_bf16 (lane-layout check)
 void mfmaTile(const float *A, const float *B, float *C, hipStream_t s);

@@ -15,10 +15,12 @@
 // Inference: blocks of 2 waves, each wave loops over 32-row tiles of the ray queue, computes
 // the raw features of its rows, encodes them one-blob into LDS, runs the 5 layers and writes the
 // mixture (8 vMF lobes + selection probability) per row.
-// Training: one wave per block, persistent over 32-row tiles of the record batch; forward with
-// the layer inputs kept transposed in LDS, KL loss gradient at the output, backward with
-// weight gradients accumulated in a per-block FP32 partial (no atomics); a second kernel sums
-// the partials in block order (deterministic) and applies Adam, refreshing the BF16 copies.
+// Training step: (1) one wave per block, persistent over 32-row tiles of the record batch:
+// forward, KL loss gradient at the output, backward through the layers; every layer's input
+// activations and output deltas are written transposed (BF16) to HBM; (2) the weight and bias
+// gradients as one MFMA product per layer with K = the batch rows, split into slices with FP32
+// partials; (3) the partials summed in slice order (deterministic) and Adam on the FP32
+// master weights, refreshing the BF16 copies.
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -268,29 +270,42 @@ __device__ float lossGradient(const float *o, f3 w, float pb, float wt, float *g
     return loss;
 }
 
-__global__ __launch_bounds__(64) void k_guide_train(GuideNet net, const float4 *recs, const int *countPtr, const float *wsumPtr, int base,
-                                                    int batch, float *partials, float *lossOut) {
-    __shared__ __bf16 sA[2][32 * ACT_STRIDE];
-    __shared__ __bf16 sAT[GUIDE_LAYERS][GUIDE_W * T_STRIDE];
+// Training, phase 1: forward and backward of 32-row tiles of the record batch, one wave per
+// block, persistent over tiles.  Each layer's input activations and its output deltas leave
+// the kernel transposed ([layer][feature][row], BF16) for the weight-gradient product of phase
+// 2; the layer inputs stay in LDS (row-major) as the next forward operand and as the ReLU
+// masks of the backward pass.  Rows past the batch get weight 0 and so zero deltas.
+__global__ __launch_bounds__(64) void k_guide_fwdbwd(GuideNet net, const float4 *recs, const int *countPtr, const float *wsumPtr, int base,
+                                                     int batch, int rows, __bf16 *actT, __bf16 *deltaT, float *lossOut) {
+    __shared__ __bf16 sAct[GUIDE_LAYERS][32 * ACT_STRIDE];
     __shared__ __bf16 sD[2][32 * ACT_STRIDE];
-    __shared__ __bf16 sDT[GUIDE_W * T_STRIDE];
     __shared__ float sOut[32][GUIDE_OUT + 1];
     __shared__ float sRec[32][8];
     const int lane = threadIdx.x;
-    float *P = partials + (size_t)blockIdx.x * GUIDE_PARAMS;
-    for (int i = lane; i < GUIDE_PARAMS; i += 64) P[i] = 0.f;
     const int count = *countPtr;
     const float wsum = wsumPtr ? *wsumPtr : 0.f;
     // normalise the weights to mean one over the record buffer
     const float wscale = (wsumPtr == nullptr) ? 1.f : (wsum > 0.f ? (float)count / wsum : 0.f);
+    // C-fragment rows of this lane: groups g = 0..3 of 4 consecutive rows from 8 g + 4 (lane >> 5)
+    const int half4 = 4 * (lane >> 5);
+    auto storeT = [&](__bf16 *dst, const float *v) {
+        // 16 values of one column (C-fragment order) as four 8-byte runs of 4 consecutive rows
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+            typedef __bf16 bf16x4 __attribute__((__vector_size__(4 * sizeof(__bf16))));
+            bf16x4 q = {(__bf16)v[4 * g], (__bf16)v[4 * g + 1], (__bf16)v[4 * g + 2], (__bf16)v[4 * g + 3]};
+            *(bf16x4 *)(dst + 8 * g + half4) = q;
+        }
+    };
     float lossAcc = 0.f;
-    const int ntiles = (batch + 31) / 32;
-    for (int tile = blockIdx.x; tile < ntiles && count > 0; tile += gridDim.x) {
+    const int ntiles = rows / 32;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int rowBase = tile * 32;
         if (lane < 32) {
-            const int r = tile * 32 + lane;
+            const int r = rowBase + lane;
             float f[GUIDE_RAW], om[3] = {0.f, 0.f, 0.f}, pb = 0.f, wt = 0.f;
             for (int k = 0; k < GUIDE_RAW; k++) f[k] = 0.f;
-            if (r < batch) {
+            if (r < batch && count > 0) {
                 const size_t idx = (size_t)(((int64_t)base + r) % count);
                 const float4 a = recs[idx * GUIDE_TREC_F4], b = recs[idx * GUIDE_TREC_F4 + 1];
                 const float4 c = recs[idx * GUIDE_TREC_F4 + 2], d = recs[idx * GUIDE_TREC_F4 + 3];
@@ -307,105 +322,122 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideNet net, const float4 *
         }
         WAVE_SYNC();
         {
-            const int row = lane & 31;
-            for (int kk = 0; kk < 4; kk++) {
-                const int k = 4 * (lane >> 5) + kk;
-                const float s = sOut[row][k];
-                for (int j = 0; j < GUIDE_BINS; j++) {
-                    const __bf16 v = (__bf16)oneBlob(s, j);
-                    sA[0][row * ACT_STRIDE + k * GUIDE_BINS + j] = v;
-                    sAT[0][(k * GUIDE_BINS + j) * T_STRIDE + row] = v;
+            // one-blob input: lane = input column (raw feature lane >> 3, bin lane & 7); all 32
+            // rows row-major into LDS and transposed to layer 0's activations
+            const int k = lane / GUIDE_BINS, j = lane % GUIDE_BINS;
+            __bf16 *dst = actT + (size_t)lane * rows + rowBase;
+            for (int q = 0; q < 4; q++) {
+                bf16x8 v;
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    const int row = 8 * q + i;
+                    v[i] = (__bf16)oneBlob(sOut[row][k], j);
+                    sAct[0][row * ACT_STRIDE + lane] = v[i];
                 }
+                *(bf16x8 *)(dst + 8 * q) = v;
             }
         }
         WAVE_SYNC();
-        int cur = 0;
         for (int L = 0; L < GUIDE_LAYERS; L++) {
             for (int ct = 0; ct < 2; ct++) {
-                const f32x16 acc = layerTile(sA[cur], net.w, L, ct, lane);
+                const f32x16 acc = layerTile(sAct[L], net.w, L, ct, lane);
                 const int col = ct * 32 + (lane & 31);
                 const float bias = net.b[L * GUIDE_W + col];
+                float h[16];
 #pragma unroll
                 for (int r = 0; r < 16; r++) {
                     const int row = cRow(lane, r);
                     const float v = acc[r] + bias;
-                    if (L < GUIDE_LAYERS - 1) {
-                        const __bf16 h = (__bf16)fmaxf(v, 0.f);
-                        sA[1 - cur][row * ACT_STRIDE + col] = h;
-                        sAT[L + 1][col * T_STRIDE + row] = h;
-                    } else if (col < GUIDE_OUT) {
-                        sOut[row][col] = v;
-                    }
+                    h[r] = fmaxf(v, 0.f);
+                    if (L < GUIDE_LAYERS - 1) sAct[L + 1][row * ACT_STRIDE + col] = (__bf16)h[r];
+                    else if (col < GUIDE_OUT) sOut[row][col] = v;
                 }
+                if (L < GUIDE_LAYERS - 1) storeT(actT + ((size_t)(L + 1) * GUIDE_W + col) * rows + rowBase, h);
             }
             WAVE_SYNC();
-            cur = 1 - cur;
         }
         if (lane < 32) {
             float g[GUIDE_W];
             const f3 w = mk(sRec[lane][0], sRec[lane][1], sRec[lane][2]);
             lossAcc += lossGradient(sOut[lane], w, sRec[lane][3], sRec[lane][4], g);
-            for (int k = 0; k < GUIDE_W; k++) {
-                const __bf16 v = (__bf16)g[k];
-                sD[0][lane * ACT_STRIDE + k] = v;
-                sDT[k * T_STRIDE + lane] = v;
-            }
+            for (int k = 0; k < GUIDE_W; k++) sD[0][lane * ACT_STRIDE + k] = (__bf16)g[k];
         }
         WAVE_SYNC();
-        int dcur = 0;
-        for (int L = GUIDE_LAYERS - 1; L >= 0; L--) {
-            float *GL = P + (size_t)L * GUIDE_LAYER_PARAMS;
-            // dW[out][in] += sum_rows delta[row][out] * a_L[row][in]  (K = 32 rows)
-            for (int mt = 0; mt < 2; mt++)
-                for (int nt = 0; nt < 2; nt++) {
-                    f32x16 acc = {};
-                    const __bf16 *arow = sDT + (mt * 32 + (lane & 31)) * T_STRIDE + (lane >> 5) * 8;
-                    const __bf16 *brow = sAT[L] + (nt * 32 + (lane & 31)) * T_STRIDE + (lane >> 5) * 8;
-                    acc = mfma(ld8(arow), ld8(brow), acc);
-                    acc = mfma(ld8(arow + 16), ld8(brow + 16), acc);
-                    const int in = nt * 32 + (lane & 31);
+        {
+            // output deltas transposed: lane = output column
+            __bf16 *dst = deltaT + ((size_t)(GUIDE_LAYERS - 1) * GUIDE_W + lane) * rows + rowBase;
+            for (int q = 0; q < 4; q++) {
+                bf16x8 v;
 #pragma unroll
-                    for (int r = 0; r < 16; r++) GL[(mt * 32 + cRow(lane, r)) * GUIDE_W + in] += acc[r];
+                for (int i = 0; i < 8; i++) v[i] = sD[0][(8 * q + i) * ACT_STRIDE + lane];
+                *(bf16x8 *)(dst + 8 * q) = v;
+            }
+        }
+        int dcur = 0;
+        for (int L = GUIDE_LAYERS - 1; L > 0; L--) {
+            // delta_{L-1}[row][in] = sum_out delta_L[row][out] W_L[out][in], masked by relu'
+            for (int nt = 0; nt < 2; nt++) {
+                f32x16 acc = {};
+                const __bf16 *arow = sD[dcur] + (lane & 31) * ACT_STRIDE + (lane >> 5) * 8;
+                const __bf16 *brow = net.wt + (size_t)L * GUIDE_W * GUIDE_W + (nt * 32 + (lane & 31)) * GUIDE_W + (lane >> 5) * 8;
+#pragma unroll
+                for (int ks = 0; ks < GUIDE_W / 16; ks++) acc = mfma(ld8(arow + ks * 16), ld8(brow + ks * 16), acc);
+                const int in = nt * 32 + (lane & 31);
+                float dv[16];
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    const int row = cRow(lane, r);
+                    const bool on = (float)sAct[L][row * ACT_STRIDE + in] > 0.f;
+                    dv[r] = on ? acc[r] : 0.f;
+                    sD[1 - dcur][row * ACT_STRIDE + in] = (__bf16)dv[r];
                 }
-            {
-                // db[out] = sum_rows delta[row][out]
-                float s = 0.f;
-                for (int row = 0; row < 32; row++) s += (float)sD[dcur][row * ACT_STRIDE + lane];
-                GL[GUIDE_W * GUIDE_W + lane] += s;
+                storeT(deltaT + ((size_t)(L - 1) * GUIDE_W + in) * rows + rowBase, dv);
             }
             WAVE_SYNC();
-            if (L > 0) {
-                // delta_{L-1}[row][in] = sum_out delta[row][out] W_L[out][in], masked by relu'
-                for (int nt = 0; nt < 2; nt++) {
-                    f32x16 acc = {};
-                    const __bf16 *arow = sD[dcur] + (lane & 31) * ACT_STRIDE + (lane >> 5) * 8;
-                    const __bf16 *brow = net.wt + (size_t)L * GUIDE_W * GUIDE_W + (nt * 32 + (lane & 31)) * GUIDE_W + (lane >> 5) * 8;
-#pragma unroll
-                    for (int ks = 0; ks < GUIDE_W / 16; ks++) acc = mfma(ld8(arow + ks * 16), ld8(brow + ks * 16), acc);
-                    const int in = nt * 32 + (lane & 31);
-#pragma unroll
-                    for (int r = 0; r < 16; r++) {
-                        const int row = cRow(lane, r);
-                        const bool on = (float)sAT[L][in * T_STRIDE + row] > 0.f;
-                        const __bf16 v = (__bf16)(on ? acc[r] : 0.f);
-                        sD[1 - dcur][row * ACT_STRIDE + in] = v;
-                    }
-                }
-                WAVE_SYNC();
-                // transposed copy of the new delta for the next weight-gradient product
-                for (int e = lane; e < 32 * GUIDE_W; e += 64) {
-                    const int row = e >> 6, in = e & 63;
-                    sDT[in * T_STRIDE + row] = sD[1 - dcur][row * ACT_STRIDE + in];
-                }
-                WAVE_SYNC();
-                dcur = 1 - dcur;
-            }
+            dcur = 1 - dcur;
         }
-        WAVE_SYNC();
     }
     // loss of this block: lanes 0..31 hold partial sums
     for (int off = 32; off >= 1; off >>= 1) lossAcc += __shfl_down(lossAcc, off);
     if (lane == 0 && lossOut) lossOut[blockIdx.x] = lossAcc;
+}
+
+// Training, phase 2: weight and bias gradients of one layer over one slice of
+// GUIDE_WG_ROWS rows, dW[out][in] = sum_rows delta[row][out] a[row][in] with K = rows on the
+// MFMA (four waves, one 32 x 32 output tile each) and db[out] = sum_rows delta[row][out]; the
+// slice's partials are summed in slice order afterwards.
+__global__ __launch_bounds__(256) void k_guide_wgrad(const __bf16 *actT, const __bf16 *deltaT, int rows, float *partials) {
+    const int slice = blockIdx.x, L = blockIdx.y;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int mt = wave >> 1, nt = wave & 1;
+    const int r0 = slice * GUIDE_WG_ROWS, r1 = min(r0 + GUIDE_WG_ROWS, rows);
+    const __bf16 *dp = deltaT + ((size_t)L * GUIDE_W + mt * 32 + (lane & 31)) * rows + (lane >> 5) * 8;
+    const __bf16 *ap = actT + ((size_t)L * GUIDE_W + nt * 32 + (lane & 31)) * rows + (lane >> 5) * 8;
+    // two accumulators over alternate K steps (rows is a multiple of 32) keep two MFMA chains
+    // and their loads in flight
+    f32x16 acc = {}, acc2 = {};
+    float bsum = 0.f;
+#pragma unroll 2
+    for (int k = r0; k < r1; k += 32) {
+        const bf16x8 a = ld8(dp + k), b = ld8(ap + k);
+        const bf16x8 a2 = ld8(dp + k + 16), b2 = ld8(ap + k + 16);
+        acc = mfma(a, b, acc);
+        acc2 = mfma(a2, b2, acc2);
+        if (nt == 0) {
+#pragma unroll
+            for (int i = 0; i < 8; i++) bsum += (float)a[i] + (float)a2[i];
+        }
+    }
+    acc += acc2;
+    float *P = partials + (size_t)slice * GUIDE_PARAMS + (size_t)L * GUIDE_LAYER_PARAMS;
+    const int in = nt * 32 + (lane & 31);
+#pragma unroll
+    for (int r = 0; r < 16; r++) P[(mt * 32 + cRow(lane, r)) * GUIDE_W + in] = acc[r];
+    if (nt == 0) {
+        // lanes l and l + 32 summed complementary rows of output column mt * 32 + (l & 31)
+        bsum += __shfl_down(bsum, 32);
+        if (lane < 32) P[GUIDE_W * GUIDE_W + mt * 32 + lane] = bsum;
+    }
 }
 
 // --------------------------------------------------------------------------------------------
@@ -436,12 +468,27 @@ __global__ void k_guide_pack(const float *params, __bf16 *w, __bf16 *wt, float *
     storeParam(p, liveParam(p) ? params[p] : 0.f, w, wt, b);
 }
 
-// Sum of the per-block partial gradients: 64 parameters per block, four slices of the
+// Sum of the per-slice partial gradients: 64 parameters per block, four slices of the
 // partials per parameter (k = slice, slice + 4, ...), the slices added in a fixed order, so
-// the result does not depend on scheduling.
-__global__ __launch_bounds__(256) void k_guide_reduce(const float *partials, int nblocks, const float *lossPart, float invBatch, float *grad,
-                                                      float *lossOut) {
+// the result does not depend on scheduling.  The last block sums the per-block losses the
+// same way (256 fixed strided partial sums, then a fixed tree).
+__global__ __launch_bounds__(256) void k_guide_reduce(const float *partials, int nblocks, const float *lossPart, int nloss, float invBatch,
+                                                      float *grad, float *lossOut) {
     __shared__ float part[4][64];
+    __shared__ float lsum[256];
+    if (blockIdx.x == gridDim.x - 1) {
+        if (!lossOut) return;
+        float l = 0.f;
+        for (int k = threadIdx.x; k < nloss; k += 256) l += lossPart[k];
+        lsum[threadIdx.x] = l;
+        __syncthreads();
+        for (int off = 128; off >= 1; off >>= 1) {
+            if ((int)threadIdx.x < off) lsum[threadIdx.x] += lsum[threadIdx.x + off];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) lossOut[0] = lsum[0] * invBatch;
+        return;
+    }
     const int p = blockIdx.x * 64 + (threadIdx.x & 63);
     const int slice = threadIdx.x >> 6;
     float g = 0.f;
@@ -453,11 +500,6 @@ __global__ __launch_bounds__(256) void k_guide_reduce(const float *partials, int
         const int j = threadIdx.x;
         const float t = (part[0][j] + part[1][j]) + (part[2][j] + part[3][j]);
         grad[p] = liveParam(p) ? t * invBatch : 0.f;
-    }
-    if (p == 0 && slice == 0 && lossOut) {
-        float l = 0.f;
-        for (int k = 0; k < nblocks; k++) l += lossPart[k];
-        lossOut[0] = l * invBatch;
     }
 }
 
@@ -545,10 +587,13 @@ void guideInferRaw(GuideNet net, const float *feats, int n, float *rawOut, hipSt
     PT_HIP_TRY(hipGetLastError());
 }
 
-void guideGradients(GuideNet net, const float4 *records, const int *count, int base, int batch, const float *wsum, float *partials, int blocks,
-                    float *lossPart, float *grad, float *lossOut, hipStream_t s) {
-    k_guide_train<<<blocks, 64, 0, s>>>(net, records, count, wsum, base, batch, partials, lossPart);
-    k_guide_reduce<<<(GUIDE_PARAMS + 63) / 64, 256, 0, s>>>(partials, blocks, lossPart, 1.f / (float)std::max(batch, 1), grad, lossOut);
+void guideGradients(GuideNet net, const float4 *records, const int *count, int base, int batch, const float *wsum, const GuideTrainBufs &tb,
+                    float *grad, float *lossOut, hipStream_t s) {
+    const int rows = guideRows(batch), slices = guideSlices(batch);
+    k_guide_fwdbwd<<<tb.fwdBlocks, 64, 0, s>>>(net, records, count, wsum, base, batch, rows, tb.act, tb.delta, tb.lossPart);
+    k_guide_wgrad<<<dim3(slices, GUIDE_LAYERS), 256, 0, s>>>(tb.act, tb.delta, rows, tb.partials);
+    k_guide_reduce<<<(GUIDE_PARAMS + 63) / 64 + 1, 256, 0, s>>>(tb.partials, slices, tb.lossPart, tb.fwdBlocks, 1.f / (float)std::max(batch, 1),
+                                                               grad, lossOut);
     PT_HIP_TRY(hipGetLastError());
 }
 

@@ -58,9 +58,9 @@ static constexpr int64_t kDefaultPool = 1ll << 26;
 // Trace blocks of 128 threads per CU: 14 fill the 7 waves per SIMD that the trace kernels'
 // 71 VGPRs and 8 KiB of LDS stack per block allow.
 static constexpr int kDefaultTraceBlocks = 14;
-// Guide-training blocks per CU: two one-wave blocks of 56 KiB LDS each fit a CU; the second
-// hides the first one's global-memory latency (1.3x faster steps than one block per CU)
-static constexpr int kTrainBlocksPerCU = 2;
+// Blocks per CU of the guide training's forward/backward kernel: one-wave blocks of 38.6 KiB
+// LDS each, four to a CU.
+static constexpr int kTrainBlocksPerCU = 4;
 static constexpr float kShadowEps = 1e-4f;   // path_tracer.cu:104-110
 static constexpr float kBounceEps = 1e-3f;   // path_tracer.cu:177
 
@@ -1023,7 +1023,7 @@ struct pt_renderer {
     int misSample = 0;
     // guiding network and its training state
     bool guideReady = false;
-    DevBuf gParams, gM, gV, gGrad, gPartials, gLossPart, gLoss, gW, gWt, gB, gMix, gRecords, gRecIdx[2];
+    DevBuf gParams, gM, gV, gGrad, gPartials, gLossPart, gLoss, gW, gWt, gB, gMix, gRecords, gRecIdx[2], gAct, gDelta;
     int64_t gRecCap = 0;
     int gTrainSteps = 16;   // Adam steps per chunk (room workload sweep: 2 -> 16 halves the MSE)
     int gBatch = 1 << 18;
@@ -1214,8 +1214,6 @@ void ensureGuide(pt_renderer *r) {
         r->gM.alloc(np * 4);
         r->gV.alloc(np * 4);
         r->gGrad.alloc(np * 4);
-        r->gPartials.alloc((size_t)r->cus * kTrainBlocksPerCU * np * 4);
-        r->gLossPart.alloc((size_t)r->cus * kTrainBlocksPerCU * 4);
         r->gLoss.alloc(4);
         r->gW.alloc(GUIDE_LAYERS * GUIDE_W * GUIDE_W * 2);
         r->gWt.alloc(GUIDE_LAYERS * GUIDE_W * GUIDE_W * 2);
@@ -1328,10 +1326,21 @@ void runChunk(pt_renderer *r, const ChunkParams &cp) {
         const int fblocks = (int)std::min<int64_t>((nrec + 255) / 256, (int64_t)r->cus * 4);
         k_guide_finalize<<<fblocks, 256, 0, r->stream>>>(r->gRecords.as<float4>(), nrec, ctr, r->pp->Lpath.as<float4>());
         PT_HIP_TRY(hipGetLastError());
+        // training scratch for the current batch size (reallocated only when it changes)
+        GuideTrainBufs tb;
+        tb.fwdBlocks = r->cus * kTrainBlocksPerCU;
+        r->gAct.alloc(guideActBytes(r->gBatch));
+        r->gDelta.alloc(guideActBytes(r->gBatch));
+        r->gPartials.alloc((size_t)guideSlices(r->gBatch) * GUIDE_PARAMS * 4);
+        r->gLossPart.alloc((size_t)tb.fwdBlocks * 4);
+        tb.act = r->gAct.as<__bf16>();
+        tb.delta = r->gDelta.as<__bf16>();
+        tb.partials = r->gPartials.as<float>();
+        tb.lossPart = r->gLossPart.as<float>();
         for (int s = 0; s < r->gTrainSteps; s++) {
             const int base = (int)((r->gTrainTotal * (int64_t)r->gBatch) & 0x3fffffff);
-            guideGradients(guideNet(r), r->gRecords.as<float4>(), &ctr[kRecValidSlot], base, r->gBatch, (const float *)&ctr[kWsumSlot],
-                           r->gPartials.as<float>(), kTrainBlocksPerCU * r->cus, r->gLossPart.as<float>(), r->gGrad.as<float>(), r->gLoss.as<float>(), r->stream);
+            guideGradients(guideNet(r), r->gRecords.as<float4>(), &ctr[kRecValidSlot], base, r->gBatch, (const float *)&ctr[kWsumSlot], tb,
+                           r->gGrad.as<float>(), r->gLoss.as<float>(), r->stream);
             guideAdam(r->gParams.as<float>(), r->gM.as<float>(), r->gV.as<float>(), r->gGrad.as<float>(), ++r->gAdamStep, r->gLr,
                       r->gW.as<__bf16>(), r->gWt.as<__bf16>(), r->gB.as<float>(), r->stream);
             r->gTrainTotal++;

@@ -343,12 +343,15 @@ int pt_unit_guide_gradients(const float *params, const float *records, int64_t n
     const float4 *dr = (const float4 *)st.in(records, (size_t)n * 16);
     const int cnt = (int)n;
     const int *dcount = st.in(&cnt, 1);
-    const int blocks = std::min(deviceCUs(), (int)((n + 31) / 32));
-    float *partials = st.out<float>((size_t)blocks * GUIDE_PARAMS);
-    float *lossPart = st.out<float>((size_t)blocks);
+    GuideTrainBufs tb;
+    tb.fwdBlocks = std::min(deviceCUs(), (int)((n + 31) / 32));
+    tb.act = st.out<__bf16>(guideActBytes(cnt) / 2);
+    tb.delta = st.out<__bf16>(guideActBytes(cnt) / 2);
+    tb.partials = st.out<float>((size_t)guideSlices(cnt) * GUIDE_PARAMS);
+    tb.lossPart = st.out<float>((size_t)tb.fwdBlocks);
     float *dgrad = st.out<float>(GUIDE_PARAMS);
     float *dloss = st.out<float>(1);
-    guideGradients(ns.net, dr, dcount, 0, cnt, nullptr, partials, blocks, lossPart, dgrad, dloss, st.s);
+    guideGradients(ns.net, dr, dcount, 0, cnt, nullptr, tb, dgrad, dloss, st.s);
     st.back(grad, dgrad, GUIDE_PARAMS);
     if (loss) st.back(loss, dloss, 1);
     st.sync();
