@@ -84,10 +84,12 @@ struct GuideTrainBufs {
     int fwdBlocks;
 };
 // Gradients of the mean loss over `batch` records (rows (base + i) mod *count of the record
-// buffer; weights scaled to mean one by *wsum when wsum is given) into `grad`, the mean loss
-// into lossOut[0].
-void guideGradients(GuideNet net, const float4 *records, const int *count, int base, int batch, const float *wsum, const GuideTrainBufs &tb,
-                    float *grad, float *lossOut, hipStream_t s);
+// sequence; weights scaled to mean one by *wsum when wsum is given) into `grad`, the mean loss
+// into lossOut[0].  With prefix = nullptr the sequence is records[0 ..); otherwise record v of
+// the sequence lies in segment s (prefix[s] <= v < prefix[s + 1], PT_QUEUE_SEGS segments) at
+// records[s * recSeg + v - prefix[s]].
+void guideGradients(GuideNet net, const float4 *records, const int *count, const int *prefix, int recSeg, int base, int batch,
+                    const float *wsum, const GuideTrainBufs &tb, float *grad, float *lossOut, hipStream_t s);
 void guideAdam(float *params, float *m, float *v, const float *grad, int step, float lr, __bf16 *dW, __bf16 *dWt, float *dB, hipStream_t s);
 // C (32 x 32) = A (32 x 16) x B (16 x 32) with one v_mfma_f32_32x32x16_bf16 (lane-layout check)
 void mfmaTile(const float *A, const float *B, float *C, hipStream_t s);

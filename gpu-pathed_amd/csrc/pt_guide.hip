@@ -69,6 +69,16 @@ __device__ __forceinline__ f32x16 layerTile(const __bf16 *act, const __bf16 *W, 
     return acc;
 }
 
+// The same product with the weights staged in LDS ([layer][out][ACT_STRIDE]).
+__device__ __forceinline__ f32x16 layerTileLds(const __bf16 *act, const __bf16 *sW, int L, int ct, int lane) {
+    f32x16 acc = {};
+    const __bf16 *arow = act + (lane & 31) * ACT_STRIDE + (lane >> 5) * 8;
+    const __bf16 *wrow = sW + (L * GUIDE_W + ct * 32 + (lane & 31)) * ACT_STRIDE + (lane >> 5) * 8;
+#pragma unroll
+    for (int ks = 0; ks < GUIDE_W / 16; ks++) acc = mfma(ld8(arow + ks * 16), ld8(wrow + ks * 16), acc);
+    return acc;
+}
+
 // mixture parameters from the 41 used raw outputs
 struct Mixture {
     f3 mu[GUIDE_LOBES];
@@ -103,7 +113,10 @@ __device__ __forceinline__ float vmfPdf(f3 mu, float kappa, f3 w) {
 // --------------------------------------------------------------------------------------------
 // inference
 // --------------------------------------------------------------------------------------------
-constexpr int INFER_WAVES = 2;
+// Waves per inference block: the block stages the forward weights in LDS once (46 KiB,
+// rows padded to ACT_STRIDE), each wave adds 16.6 KiB of tile buffers, and six waves fill a
+// CU's 160 KiB.
+constexpr int INFER_WAVES = 6;
 
 // Rows are the slots [0, maxRows) of a segmented queue (count of segment s at
 // count[s * segStride], segments segLen slots long; a tile of 32 rows lies in one segment), or
@@ -116,7 +129,13 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void k_guide_infer(PtSceneView sc
     __shared__ int sElig[INFER_WAVES][32];
     __shared__ __bf16 sAct[INFER_WAVES][2][32 * ACT_STRIDE];
     __shared__ float sOut[INFER_WAVES][32][GUIDE_OUT + 1];
+    __shared__ __bf16 sW[GUIDE_LAYERS * GUIDE_W * ACT_STRIDE];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int c = threadIdx.x; c < GUIDE_LAYERS * GUIDE_W * (GUIDE_W / 8); c += blockDim.x) {
+        const int row = c / (GUIDE_W / 8), q = c % (GUIDE_W / 8);
+        *(bf16x8 *)(sW + row * ACT_STRIDE + 8 * q) = ld8(net.w + (size_t)row * GUIDE_W + 8 * q);
+    }
+    __syncthreads();
     const int waveStride = gridDim.x * INFER_WAVES;
     for (int tile = blockIdx.x * INFER_WAVES + wave; tile * 32 < maxRows; tile += waveStride) {
         const int rowBase = tile * 32;
@@ -188,7 +207,7 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void k_guide_infer(PtSceneView sc
         int cur = 0;
         for (int L = 0; L < GUIDE_LAYERS; L++) {
             for (int ct = 0; ct < 2; ct++) {
-                const f32x16 acc = layerTile(sAct[wave][cur], net.w, L, ct, lane);
+                const f32x16 acc = layerTileLds(sAct[wave][cur], sW, L, ct, lane);
                 const int col = ct * 32 + (lane & 31);
                 const float bias = net.b[L * GUIDE_W + col];
 #pragma unroll
@@ -275,8 +294,9 @@ __device__ float lossGradient(const float *o, f3 w, float pb, float wt, float *g
 // the kernel transposed ([layer][feature][row], BF16) for the weight-gradient product of phase
 // 2; the layer inputs stay in LDS (row-major) as the next forward operand and as the ReLU
 // masks of the backward pass.  Rows past the batch get weight 0 and so zero deltas.
-__global__ __launch_bounds__(64) void k_guide_fwdbwd(GuideNet net, const float4 *recs, const int *countPtr, const float *wsumPtr, int base,
-                                                     int batch, int rows, __bf16 *actT, __bf16 *deltaT, float *lossOut) {
+__global__ __launch_bounds__(64) void k_guide_fwdbwd(GuideNet net, const float4 *recs, const int *countPtr, const int *prefix, int recSeg,
+                                                     const float *wsumPtr, int base, int batch, int rows, __bf16 *actT, __bf16 *deltaT,
+                                                     float *lossOut) {
     __shared__ __bf16 sAct[GUIDE_LAYERS][32 * ACT_STRIDE];
     __shared__ __bf16 sD[2][32 * ACT_STRIDE];
     __shared__ float sOut[32][GUIDE_OUT + 1];
@@ -306,7 +326,18 @@ __global__ __launch_bounds__(64) void k_guide_fwdbwd(GuideNet net, const float4 
             float f[GUIDE_RAW], om[3] = {0.f, 0.f, 0.f}, pb = 0.f, wt = 0.f;
             for (int k = 0; k < GUIDE_RAW; k++) f[k] = 0.f;
             if (r < batch && count > 0) {
-                const size_t idx = (size_t)(((int64_t)base + r) % count);
+                const int v = (int)(((int64_t)base + r) % count);
+                size_t idx = (size_t)v;
+                if (prefix) {
+                    // the last segment whose prefix does not exceed v holds record v
+                    int lo = 0, hi = PT_QUEUE_SEGS;
+                    while (hi - lo > 1) {
+                        const int mid = (lo + hi) >> 1;
+                        if (prefix[mid] <= v) lo = mid;
+                        else hi = mid;
+                    }
+                    idx = (size_t)lo * recSeg + (v - prefix[lo]);
+                }
                 const float4 a = recs[idx * GUIDE_TREC_F4], b = recs[idx * GUIDE_TREC_F4 + 1];
                 const float4 c = recs[idx * GUIDE_TREC_F4 + 2], d = recs[idx * GUIDE_TREC_F4 + 3];
                 f[0] = a.x; f[1] = a.y; f[2] = a.z; f[3] = a.w;
@@ -573,7 +604,7 @@ void guideUploadBf16(const float *dParams, __bf16 *dW, __bf16 *dWt, float *dB, h
 void guideInfer(const PtSceneView &sc, GuideSceneNorm norm, GuideNet net, const int *segCount, int segStride, int segLen,
                 const float4 *rayO, const float4 *rayD, const float4 *hits, float4 *out, int cus, hipStream_t s) {
     const int maxRows = PT_QUEUE_SEGS * segLen;
-    const int grid = std::max(1, std::min((maxRows + 32 * INFER_WAVES - 1) / (32 * INFER_WAVES), cus * 4));
+    const int grid = std::max(1, std::min((maxRows + 32 * INFER_WAVES - 1) / (32 * INFER_WAVES), cus));
     k_guide_infer<<<grid, 64 * INFER_WAVES, 0, s>>>(sc, norm, net, segCount, segStride, segLen, rayO, rayD, hits, maxRows, out, nullptr,
                                                     nullptr);
     PT_HIP_TRY(hipGetLastError());
@@ -587,10 +618,10 @@ void guideInferRaw(GuideNet net, const float *feats, int n, float *rawOut, hipSt
     PT_HIP_TRY(hipGetLastError());
 }
 
-void guideGradients(GuideNet net, const float4 *records, const int *count, int base, int batch, const float *wsum, const GuideTrainBufs &tb,
-                    float *grad, float *lossOut, hipStream_t s) {
+void guideGradients(GuideNet net, const float4 *records, const int *count, const int *prefix, int recSeg, int base, int batch,
+                    const float *wsum, const GuideTrainBufs &tb, float *grad, float *lossOut, hipStream_t s) {
     const int rows = guideRows(batch), slices = guideSlices(batch);
-    k_guide_fwdbwd<<<tb.fwdBlocks, 64, 0, s>>>(net, records, count, wsum, base, batch, rows, tb.act, tb.delta, tb.lossPart);
+    k_guide_fwdbwd<<<tb.fwdBlocks, 64, 0, s>>>(net, records, count, prefix, recSeg, wsum, base, batch, rows, tb.act, tb.delta, tb.lossPart);
     k_guide_wgrad<<<dim3(slices, GUIDE_LAYERS), 256, 0, s>>>(tb.act, tb.delta, rows, tb.partials);
     k_guide_reduce<<<(GUIDE_PARAMS + 63) / 64 + 1, 256, 0, s>>>(tb.partials, slices, tb.lossPart, tb.fwdBlocks, 1.f / (float)std::max(batch, 1),
                                                                grad, lossOut);

@@ -43,11 +43,12 @@ static constexpr int kMaxBounces = PT_MAX_BOUNCES;
 // Queue counters: PT_QUEUE_SEGS blocks of PT_SEG_STRIDE ints (ptQueueField), then a few
 // chunk-wide counters.
 static constexpr int kSegCounters = PT_QUEUE_SEGS * PT_SEG_STRIDE;
-static constexpr int kNumCounters = kSegCounters + 8;
-// guided mode: training records appended, records kept (capped), sum of their weights (float)
-static constexpr int kRecCountSlot = kSegCounters + 1;
+// guided mode: records kept, sum of their weights (float), and the exclusive prefix sums of
+// the per-segment record counts (PT_QUEUE_SEGS + 1 entries)
 static constexpr int kRecValidSlot = kSegCounters + 2;
 static constexpr int kWsumSlot = kSegCounters + 3;
+static constexpr int kRecPrefixSlot = kSegCounters + 8;
+static constexpr int kNumCounters = kRecPrefixSlot + PT_QUEUE_SEGS + 8;
 // device statistics (uint64): per bounce rays, shadows, shade adds, shadow adds; totals
 enum { S_RAYS = 0, S_SHADOW = kMaxBounces, S_LADD_SHADE = 2 * kMaxBounces, S_LADD_SHADOW = 3 * kMaxBounces, S_PATHS = 4 * kMaxBounces, S_COUNT };
 
@@ -621,8 +622,8 @@ __device__ __forceinline__ float pdfEnvLight(const PtSceneView &sc, f3 w) {
 // Guided-vertex inputs of the shade kernel (ignored by the plain integrator)
 struct GuideShadeArgs {
     const float4 *mix;      // GUIDE_REC_F4 float4 per queue slot (inference output)
-    float4 *records;        // GUIDE_TREC_F4 float4 per training record
-    int recCap;
+    float4 *records;        // GUIDE_TREC_F4 float4 per training record, recSeg per queue segment
+    int recSeg;
     const int *recIn;       // training record of the vertex that spawned each queued ray (-1: none)
     int *recOut;
     GuideSceneNorm norm;
@@ -708,10 +709,12 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 5) void k_shade(PtSceneView sc, C
                     const float4 *G = nullptr;
                     float frac = 0.f;
                     if constexpr (GUIDED) {
-                        G = ga.mix + (size_t)i * GUIDE_REC_F4;
-                        const float4 g10 = G[10];
-                        if (g10.y > 0.f) frac = fminf(g10.x, ga.fracCap);
-                        else G = nullptr;
+                        if (ga.mix) {
+                            G = ga.mix + (size_t)i * GUIDE_REC_F4;
+                            const float4 g10 = G[10];
+                            if (g10.y > 0.f) frac = fminf(g10.x, ga.fracCap);
+                            else G = nullptr;
+                        }
                     }
                     BsdfSample bs;
                     float pdfDir;  // density of the direction sampling technique at bs.wi
@@ -790,15 +793,16 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 5) void k_shade(PtSceneView sc, C
                         // a training record per guided vertex whose path continues
                         const float lt = lum(thr);
                         const bool wantRec = pushRay && G != nullptr && lt > 0.f;
-                        const int slotRec = waveAppend(wantRec, &counters[kRecCountSlot]);
+                        // records go to the segment's own part of the record buffer
+                        const int slotRec = waveAppend(wantRec, &counters[ptQueueField(seg, 0, PT_Q_RECORDS)]);
                         if (pushRay) {
                             int ridx = -1;
-                            if (wantRec && slotRec < ga.recCap) {
-                                ridx = slotRec;
+                            if (wantRec && slotRec < ga.recSeg) {
+                                ridx = seg * ga.recSeg + slotRec;
                                 float f[GUIDE_RAW];
                                 const float rough = type == PT_MAT_LAMBERTIAN ? 1.f : fminf(m.b.w * (1.f / 0.6f), 1.f);
                                 guideFeatures(ga.norm, sp.p, sp.n, -d, rough, f);
-                                float4 *R = ga.records + (size_t)slotRec * GUIDE_TREC_F4;
+                                float4 *R = ga.records + (size_t)ridx * GUIDE_TREC_F4;
                                 R[0] = make_float4(f[0], f[1], f[2], f[3]);
                                 R[1] = make_float4(f[4], f[5], f[6], f[7]);
                                 R[2] = f4(wworld, pbRec);
@@ -823,13 +827,25 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 5) void k_shade(PtSceneView sc, C
 
 // Training targets: weight = (radiance after the vertex's sampled direction) / luminance of the
 // throughput at the vertex, i.e. the MC estimate of f cos L_i / p for the sampled direction.
-__global__ __launch_bounds__(256) void k_guide_finalize(float4 *records, int recCap, int *counters, const float4 *Lpath) {
-    const int n = min(counters[kRecCountSlot], recCap);
-    if (blockIdx.x == 0 && threadIdx.x == 0) counters[kRecValidSlot] = n;
-    // grid-stride over the records; the weight sum is reduced per block so that the shared
-    // accumulator takes one atomic per block
+// Records live in per-segment parts of recSeg entries; block 0 also writes the exclusive prefix
+// sums of the kept counts, through which training reads the records as one sequence.
+__global__ __launch_bounds__(256) void k_guide_finalize(float4 *records, int recSeg, int *counters, const float4 *Lpath) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        int acc = 0;
+        for (int s = 0; s < PT_QUEUE_SEGS; s++) {
+            counters[kRecPrefixSlot + s] = acc;
+            acc += min(counters[ptQueueField(s, 0, PT_Q_RECORDS)], recSeg);
+        }
+        counters[kRecPrefixSlot + PT_QUEUE_SEGS] = acc;
+        counters[kRecValidSlot] = acc;
+    }
+    // grid-stride over the record slots; the weight sum is reduced per block so that the
+    // shared accumulator takes one atomic per block
     float w = 0.f;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int total = PT_QUEUE_SEGS * recSeg;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const int seg = i / recSeg;
+        if (i - seg * recSeg >= min(counters[ptQueueField(seg, 0, PT_Q_RECORDS)], recSeg)) continue;
         float4 *r = records + (size_t)i * GUIDE_TREC_F4 + 3;
         const float4 v = *r;
         const int pid = asInt(v.y);
@@ -1029,6 +1045,7 @@ struct pt_renderer {
     int gBatch = 1 << 18;
     float gLr = 1e-2f;
     float gFracCap = 0.8f;  // at least 20% of guided-vertex directions come from the BSDF
+    int gBounces = 0;       // guided vertices only at bounces < gBounces (0: every bounce)
     int gAdamStep = 0;
     int64_t gTrainTotal = 0;
     uint64_t gSeed = 1;
@@ -1280,15 +1297,17 @@ void runChunk(pt_renderer *r, const ChunkParams &cp) {
                          b == 0 ? TRACE_CAMERA : TRACE_INCOHERENT);
         }
         GuideShadeArgs ga{};
-        if (guided && b + 1 < cp.maxDepth) {
+        // vertices of bounces >= gBounces (when set) sample the BSDF only
+        const bool guidedBounce = guided && b + 1 < cp.maxDepth && (r->gBounces <= 0 || b < r->gBounces);
+        if (guidedBounce) {
             KTimer t(r, K_GUIDE_INFER);
             guideInfer(r->view, guideNorm(r), guideNet(r), ctr + ptQueueField(0, b, PT_Q_RAYS), PT_SEG_STRIDE, cp.segLen, r->pp->rayO[cur].as<float4>(),
                        r->pp->rayD[cur].as<float4>(), r->pp->hits.as<float4>(), r->gMix.as<float4>(), r->cus, r->stream);
         }
         if (guided) {
-            ga.mix = r->gMix.as<float4>();
+            ga.mix = guidedBounce ? r->gMix.as<float4>() : nullptr;
             ga.records = r->gRecords.as<float4>();
-            ga.recCap = (int)r->gRecCap;
+            ga.recSeg = (int)(r->gRecCap / PT_QUEUE_SEGS);
             ga.recIn = r->gRecIdx[cur].as<int>();
             ga.recOut = r->gRecIdx[1 - cur].as<int>();
             ga.norm = guideNorm(r);
@@ -1324,7 +1343,8 @@ void runChunk(pt_renderer *r, const ChunkParams &cp) {
         KTimer t(r, K_GUIDE_TRAIN);
         const int nrec = (int)r->gRecCap;
         const int fblocks = (int)std::min<int64_t>((nrec + 255) / 256, (int64_t)r->cus * 4);
-        k_guide_finalize<<<fblocks, 256, 0, r->stream>>>(r->gRecords.as<float4>(), nrec, ctr, r->pp->Lpath.as<float4>());
+        k_guide_finalize<<<fblocks, 256, 0, r->stream>>>(r->gRecords.as<float4>(), (int)(r->gRecCap / PT_QUEUE_SEGS), ctr,
+                                                         r->pp->Lpath.as<float4>());
         PT_HIP_TRY(hipGetLastError());
         // training scratch for the current batch size (reallocated only when it changes)
         GuideTrainBufs tb;
@@ -1339,8 +1359,8 @@ void runChunk(pt_renderer *r, const ChunkParams &cp) {
         tb.lossPart = r->gLossPart.as<float>();
         for (int s = 0; s < r->gTrainSteps; s++) {
             const int base = (int)((r->gTrainTotal * (int64_t)r->gBatch) & 0x3fffffff);
-            guideGradients(guideNet(r), r->gRecords.as<float4>(), &ctr[kRecValidSlot], base, r->gBatch, (const float *)&ctr[kWsumSlot], tb,
-                           r->gGrad.as<float>(), r->gLoss.as<float>(), r->stream);
+            guideGradients(guideNet(r), r->gRecords.as<float4>(), &ctr[kRecValidSlot], &ctr[kRecPrefixSlot], (int)(r->gRecCap / PT_QUEUE_SEGS),
+                           base, r->gBatch, (const float *)&ctr[kWsumSlot], tb, r->gGrad.as<float>(), r->gLoss.as<float>(), r->stream);
             guideAdam(r->gParams.as<float>(), r->gM.as<float>(), r->gV.as<float>(), r->gGrad.as<float>(), ++r->gAdamStep, r->gLr,
                       r->gW.as<__bf16>(), r->gWt.as<__bf16>(), r->gB.as<float>(), r->stream);
             r->gTrainTotal++;
@@ -1682,6 +1702,7 @@ int pt_renderer_set_option(pt_renderer *r, int opt, int64_t v) {
     case PT_OPT_PROFILE_KERNELS: r->profile = v != 0; break;
     case PT_OPT_CONSISTENT_MIS: r->misSample = v != 0; r->resetPending = true; break;
     case PT_OPT_GUIDE_TRAIN_STEPS: r->gTrainSteps = v < 0 ? 16 : (int)std::min<int64_t>(v, 1024); break;
+    case PT_OPT_GUIDE_BOUNCES: r->gBounces = v <= 0 ? 0 : (int)std::min<int64_t>(v, kMaxBounces); break;
     case PT_OPT_GUIDE_BATCH:
         if (v < 32 || v > (1 << 22)) return fail(PT_ERR_INVALID, "pt_renderer_set_option: guide batch must be in [32, 2^22]");
         r->gBatch = (int)v;
@@ -1719,6 +1740,7 @@ int pt_renderer_get_option(pt_renderer *r, int opt, int64_t *v) {
     case PT_OPT_PROFILE_KERNELS: *v = r->profile ? 1 : 0; break;
     case PT_OPT_CONSISTENT_MIS: *v = r->misSample; break;
     case PT_OPT_GUIDE_TRAIN_STEPS: *v = r->gTrainSteps; break;
+    case PT_OPT_GUIDE_BOUNCES: *v = r->gBounces; break;
     case PT_OPT_GUIDE_BATCH: *v = r->gBatch; break;
     case PT_OPT_GUIDE_LR_MICRO: *v = (int64_t)llroundf(r->gLr * 1e6f); break;
     case PT_OPT_GUIDE_MAX_FRACTION_MILLI: *v = (int64_t)llroundf(r->gFracCap * 1e3f); break;

@@ -67,7 +67,8 @@ enum {
     PT_Q_FETCH_SHADE = 3,
     PT_Q_FETCH_SHADOW = 4,
     PT_Q_LADD_SHADE = 5,    // radiance additions (statistics)
-    PT_Q_LADD_SHADOW = 6
+    PT_Q_LADD_SHADOW = 6,
+    PT_Q_RECORDS = 7        // guided: training records of the chunk (bounce 0's entry)
 };
 __host__ __device__ __forceinline__ int ptQueueField(int seg, int bounce, int field) {
     return seg * PT_SEG_STRIDE + field * PT_MAX_BOUNCES + bounce;

@@ -351,7 +351,7 @@ int pt_unit_guide_gradients(const float *params, const float *records, int64_t n
     tb.lossPart = st.out<float>((size_t)tb.fwdBlocks);
     float *dgrad = st.out<float>(GUIDE_PARAMS);
     float *dloss = st.out<float>(1);
-    guideGradients(ns.net, dr, dcount, 0, cnt, nullptr, tb, dgrad, dloss, st.s);
+    guideGradients(ns.net, dr, dcount, nullptr, 0, 0, cnt, nullptr, tb, dgrad, dloss, st.s);
     st.back(grad, dgrad, GUIDE_PARAMS);
     if (loss) st.back(loss, dloss, 1);
     st.sync();

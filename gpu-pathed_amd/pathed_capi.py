@@ -16,7 +16,7 @@ LAMBERTIAN, MIRROR, GLASS, MICROFACET = 0, 1, 2, 3
 RENDERER_CUDA, RENDERER_OPTIX, RENDERER_NORMALS, RENDERER_GUIDED = 0, 1, 2, 3
 OPT_POOL_PATHS, OPT_TRACE_BLOCKS_PER_CU, OPT_SHADE_BLOCKS_PER_CU, OPT_PROFILE_KERNELS = 0, 1, 2, 3
 OPT_CONSISTENT_MIS, OPT_GUIDE_TRAIN_STEPS, OPT_GUIDE_BATCH, OPT_GUIDE_LR_MICRO, OPT_GUIDE_RESET = 4, 5, 6, 7, 8
-OPT_TRACE_LDS_STACK, OPT_BATCH_GRAPH, OPT_GUIDE_MAX_FRACTION_MILLI, OPT_TRACE_REFILL_LANES = 9, 10, 11, 12
+OPT_TRACE_LDS_STACK, OPT_BATCH_GRAPH, OPT_GUIDE_MAX_FRACTION_MILLI, OPT_TRACE_REFILL_LANES, OPT_GUIDE_BOUNCES = 9, 10, 11, 12, 13
 MISS = np.iinfo(np.int32).min
 
 # every symbol declared in include/pathed.h

@@ -175,10 +175,13 @@ typedef enum {
     PT_OPT_GUIDE_MAX_FRACTION_MILLI = 11, /* guided: cap, in thousandths, on the probability of
                                     sampling the learned lobes (the rest samples the BSDF);
                                     default 800; 1000 leaves the network's own clamp (0.98) */
-    PT_OPT_TRACE_REFILL_LANES = 12 /* a trace wave takes new rays from the queue once this many
+    PT_OPT_TRACE_REFILL_LANES = 12, /* a trace wave takes new rays from the queue once this many
                                     of its 64 lanes are idle (1..64; 64: whole batches of 64
                                     neighbouring rays); 0 (default) = per launch: camera rays
                                     48, later closest-hit queues and shadow rays 32 */
+    PT_OPT_GUIDE_BOUNCES = 13    /* guided: only vertices of the first N bounces draw from the
+                                    learned lobes (their network inference and training records);
+                                    later vertices sample the BSDF; 0 (default) = every bounce */
 } pt_option;
 int pt_renderer_set_option(pt_renderer *r, int option, int64_t value);
 int pt_renderer_get_option(pt_renderer *r, int option, int64_t *value);
