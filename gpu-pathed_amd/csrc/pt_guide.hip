@@ -118,13 +118,68 @@ __device__ __forceinline__ float vmfPdf(f3 mu, float kappa, f3 w) {
 // CU's 160 KiB.
 constexpr int INFER_WAVES = 6;
 
+// Raw features of the vertex at queue slot i (hit point normalised to the scene box, shading
+// normal, outgoing direction, roughness); false where the vertex is not guided (a miss, a
+// mirror or a glass surface).
+__device__ __forceinline__ bool vertexFeatures(const PtSceneView &sc, const GuideSceneNorm &nm, int i, const float4 *rayO, const float4 *rayD,
+                                               const float4 *hits, float *f) {
+    const float4 h = hits[i];
+    const int slot = asInt(h.w);
+    if (slot == INT32_MIN) return false;
+    const float4 ro = rayO[i], rd = rayD[i];
+    const f3 o = xyz(ro), d = xyz(rd);
+    const float4 p0 = sc.prims[3 * (size_t)slot], p1 = sc.prims[3 * (size_t)slot + 1];
+    const PtMaterial m = sc.materials[asInt(p0.w)];
+    const int type = matType(m);
+    if (type != PT_MAT_LAMBERTIAN && type != PT_MAT_MICROFACET) return false;
+    f3 p, nn;
+    const int tag = asInt(p1.w);
+    if (tag == PT_TAG_SPHERE) {
+        p = o + d * h.x;
+        nn = normalize((p - xyz(p0)) / p1.x);
+    } else {
+        const float4 p2 = sc.prims[3 * (size_t)slot + 2];
+        const f3 a = xyz(p0), b = xyz(p1), c = xyz(p2);
+        p = a * (1.f - h.y - h.z) + b * h.y + c * h.z;
+        if (tag >= 0) {
+            const f3 na = xyz(sc.normals[3 * (size_t)tag]), nb = xyz(sc.normals[3 * (size_t)tag + 1]), nc = xyz(sc.normals[3 * (size_t)tag + 2]);
+            nn = normalize(na * (1.f - h.y - h.z) + nb * h.y + nc * h.z);
+        } else {
+            nn = normalize(cross(b - a, c - a));
+        }
+    }
+    const float rough = type == PT_MAT_LAMBERTIAN ? 1.f : fminf(m.b.w * (1.f / 0.6f), 1.f);
+    guideFeatures(nm, p, nn, -d, rough, f);
+    return true;
+}
+
+// Features of every filled slot of a segmented ray queue (count of segment s at
+// count[s * segStride], segments segLen slots long): two float4 per slot, the eighth feature
+// (roughness, > 0) set to -1 where the vertex is not guided.  A kernel of its own, at full
+// occupancy, so that the dependent gathers (hit -> primitive -> material) overlap.
+__global__ __launch_bounds__(256) void k_guide_features(PtSceneView sc, GuideSceneNorm nm, const int *count, int segStride, int segLen,
+                                                        const float4 *rayO, const float4 *rayD, const float4 *hits, int maxRows,
+                                                        float4 *feat) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < maxRows; i += gridDim.x * blockDim.x) {
+        const int seg = i / segLen;
+        if (i - seg * segLen >= count[seg * segStride]) continue;
+        float f[GUIDE_RAW];
+        for (int k = 0; k < GUIDE_RAW; k++) f[k] = 0.f;
+        if (!vertexFeatures(sc, nm, i, rayO, rayD, hits, f)) {
+            for (int k = 0; k < GUIDE_RAW; k++) f[k] = 0.f;
+            f[GUIDE_RAW - 1] = -1.f;
+        }
+        feat[2 * (size_t)i] = make_float4(f[0], f[1], f[2], f[3]);
+        feat[2 * (size_t)i + 1] = make_float4(f[4], f[5], f[6], f[7]);
+    }
+}
+
 // Rows are the slots [0, maxRows) of a segmented queue (count of segment s at
-// count[s * segStride], segments segLen slots long; a tile of 32 rows lies in one segment), or
-// with count = nullptr all maxRows rows of rawFeats.
-__global__ __launch_bounds__(64 * INFER_WAVES) void k_guide_infer(PtSceneView sc, GuideSceneNorm nm, GuideNet net, const int *count,
-                                                                  int segStride, int segLen, const float4 *rayO, const float4 *rayD,
-                                                                  const float4 *hits, int maxRows, float4 *out, const float *rawFeats,
-                                                                  float *rawOut) {
+// count[s * segStride], segments segLen slots long; a tile of 32 rows lies in one segment)
+// whose features k_guide_features wrote to feat, or with count = nullptr all maxRows rows of
+// rawFeats.
+__global__ __launch_bounds__(64 * INFER_WAVES) void k_guide_infer(GuideNet net, const int *count, int segStride, int segLen, const float4 *feat,
+                                                                  int maxRows, float4 *out, const float *rawFeats, float *rawOut) {
     __shared__ float sRaw[INFER_WAVES][32][GUIDE_RAW];
     __shared__ int sElig[INFER_WAVES][32];
     __shared__ __bf16 sAct[INFER_WAVES][2][32 * ACT_STRIDE];
@@ -156,36 +211,11 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void k_guide_infer(PtSceneView sc
                     for (int k = 0; k < GUIDE_RAW; k++) f[k] = rawFeats[(size_t)i * GUIDE_RAW + k];
                     elig = 1;
                 } else {
-                    const float4 h = hits[i];
-                    const int slot = asInt(h.w);
-                    if (slot != INT32_MIN) {
-                        const float4 ro = rayO[i], rd = rayD[i];
-                        const f3 o = xyz(ro), d = xyz(rd);
-                        const float4 p0 = sc.prims[3 * (size_t)slot], p1 = sc.prims[3 * (size_t)slot + 1];
-                        const PtMaterial m = sc.materials[asInt(p0.w)];
-                        const int type = matType(m);
-                        if (type == PT_MAT_LAMBERTIAN || type == PT_MAT_MICROFACET) {
-                            f3 p, nn;
-                            const int tag = asInt(p1.w);
-                            if (tag == PT_TAG_SPHERE) {
-                                p = o + d * h.x;
-                                nn = normalize((p - xyz(p0)) / p1.x);
-                            } else {
-                                const float4 p2 = sc.prims[3 * (size_t)slot + 2];
-                                const f3 a = xyz(p0), b = xyz(p1), c = xyz(p2);
-                                p = a * (1.f - h.y - h.z) + b * h.y + c * h.z;
-                                if (tag >= 0) {
-                                    const f3 na = xyz(sc.normals[3 * (size_t)tag]), nb = xyz(sc.normals[3 * (size_t)tag + 1]),
-                                             nc = xyz(sc.normals[3 * (size_t)tag + 2]);
-                                    nn = normalize(na * (1.f - h.y - h.z) + nb * h.y + nc * h.z);
-                                } else {
-                                    nn = normalize(cross(b - a, c - a));
-                                }
-                            }
-                            const float rough = type == PT_MAT_LAMBERTIAN ? 1.f : fminf(m.b.w * (1.f / 0.6f), 1.f);
-                            guideFeatures(nm, p, nn, -d, rough, f);
-                            elig = 1;
-                        }
+                    const float4 a = feat[2 * (size_t)i], b = feat[2 * (size_t)i + 1];
+                    if (b.w >= 0.f) {
+                        f[0] = a.x; f[1] = a.y; f[2] = a.z; f[3] = a.w;
+                        f[4] = b.x; f[5] = b.y; f[6] = b.z; f[7] = b.w;
+                        elig = 1;
                     }
                 }
             }
@@ -602,19 +632,18 @@ void guideUploadBf16(const float *dParams, __bf16 *dW, __bf16 *dWt, float *dB, h
 }
 
 void guideInfer(const PtSceneView &sc, GuideSceneNorm norm, GuideNet net, const int *segCount, int segStride, int segLen,
-                const float4 *rayO, const float4 *rayD, const float4 *hits, float4 *out, int cus, hipStream_t s) {
+                const float4 *rayO, const float4 *rayD, const float4 *hits, float4 *feat, float4 *out, int cus, hipStream_t s) {
     const int maxRows = PT_QUEUE_SEGS * segLen;
+    const int fgrid = std::max(1, std::min((maxRows + 255) / 256, cus * 8));
+    k_guide_features<<<fgrid, 256, 0, s>>>(sc, norm, segCount, segStride, segLen, rayO, rayD, hits, maxRows, feat);
     const int grid = std::max(1, std::min((maxRows + 32 * INFER_WAVES - 1) / (32 * INFER_WAVES), cus));
-    k_guide_infer<<<grid, 64 * INFER_WAVES, 0, s>>>(sc, norm, net, segCount, segStride, segLen, rayO, rayD, hits, maxRows, out, nullptr,
-                                                    nullptr);
+    k_guide_infer<<<grid, 64 * INFER_WAVES, 0, s>>>(net, segCount, segStride, segLen, feat, maxRows, out, nullptr, nullptr);
     PT_HIP_TRY(hipGetLastError());
 }
 
 void guideInferRaw(GuideNet net, const float *feats, int n, float *rawOut, hipStream_t s) {
-    PtSceneView sc{};
-    GuideSceneNorm nm{};
     const int grid = std::max(1, (n + 32 * INFER_WAVES - 1) / (32 * INFER_WAVES));
-    k_guide_infer<<<grid, 64 * INFER_WAVES, 0, s>>>(sc, nm, net, nullptr, 0, 64, nullptr, nullptr, nullptr, n, nullptr, feats, rawOut);
+    k_guide_infer<<<grid, 64 * INFER_WAVES, 0, s>>>(net, nullptr, 0, 64, nullptr, n, nullptr, feats, rawOut);
     PT_HIP_TRY(hipGetLastError());
 }
 

@@ -1039,7 +1039,7 @@ struct pt_renderer {
     int misSample = 0;
     // guiding network and its training state
     bool guideReady = false;
-    DevBuf gParams, gM, gV, gGrad, gPartials, gLossPart, gLoss, gW, gWt, gB, gMix, gRecords, gRecIdx[2], gAct, gDelta;
+    DevBuf gParams, gM, gV, gGrad, gPartials, gLossPart, gLoss, gW, gWt, gB, gMix, gRecords, gRecIdx[2], gAct, gDelta, gFeat;
     int64_t gRecCap = 0;
     int gTrainSteps = 16;   // Adam steps per chunk (room workload sweep: 2 -> 16 halves the MSE)
     int gBatch = 1 << 18;
@@ -1246,6 +1246,7 @@ void ensureGuide(pt_renderer *r) {
         r->gAdamStep = 0;
     }
     r->gMix.alloc((size_t)r->pp->pool * GUIDE_REC_F4 * sizeof(float4));
+    r->gFeat.alloc((size_t)r->pp->pool * 2 * sizeof(float4));
     r->gRecCap = 2 * r->pp->pool;
     r->gRecords.alloc((size_t)r->gRecCap * GUIDE_TREC_F4 * sizeof(float4));
     r->gRecIdx[0].alloc((size_t)r->pp->pool * 4);
@@ -1302,7 +1303,7 @@ void runChunk(pt_renderer *r, const ChunkParams &cp) {
         if (guidedBounce) {
             KTimer t(r, K_GUIDE_INFER);
             guideInfer(r->view, guideNorm(r), guideNet(r), ctr + ptQueueField(0, b, PT_Q_RAYS), PT_SEG_STRIDE, cp.segLen, r->pp->rayO[cur].as<float4>(),
-                       r->pp->rayD[cur].as<float4>(), r->pp->hits.as<float4>(), r->gMix.as<float4>(), r->cus, r->stream);
+                       r->pp->rayD[cur].as<float4>(), r->pp->hits.as<float4>(), r->gFeat.as<float4>(), r->gMix.as<float4>(), r->cus, r->stream);
         }
         if (guided) {
             ga.mix = guidedBounce ? r->gMix.as<float4>() : nullptr;
