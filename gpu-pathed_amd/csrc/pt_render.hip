@@ -181,8 +181,10 @@ struct TraceHit {
     int prim;
 };
 
+typedef __attribute__((address_space(3))) int LdsInt;
+
 struct TraceStack {
-    int *lds;       // this lane's first LDS entry; entries blockDim.x apart
+    LdsInt *lds;    // this lane's first LDS entry; entries blockDim.x apart
     int *spill;     // this lane's first spill entry; entries spillStride apart
     int spillCap;   // spill entries per lane
     int spillStride;
@@ -277,7 +279,11 @@ __device__ __forceinline__ bool traceRound(const PtSceneView &sc, LaneRay &r, co
             r.node = kTraceDone;
         } else {
             r.sp--;
-            r.node = r.sp < STACK ? ts.lds[r.sp * stride] : ts.spill[(size_t)(r.sp - STACK) * ts.spillStride];
+            // an LDS read at a clamped level and, only below the LDS part, the spill read (the
+            // LDS pointer's address space keeps the two from becoming one flat load)
+            int v = ts.lds[min(r.sp, STACK - 1) * stride];
+            if (r.sp >= STACK) v = ts.spill[(size_t)(r.sp - STACK) * ts.spillStride];
+            r.node = v;
         }
     };
     const char *nodes = (const char *)sc.nodes;
@@ -375,7 +381,7 @@ __device__ __forceinline__ bool traceRound(const PtSceneView &sc, LaneRay &r, co
 
 __device__ __forceinline__ TraceStack traceStack(int *sstack, int *spill, int spillCap) {
     TraceStack ts;
-    ts.lds = sstack + threadIdx.x;
+    ts.lds = (LdsInt *)(sstack + threadIdx.x);
     ts.spillStride = gridDim.x * blockDim.x;
     ts.spill = spill + blockIdx.x * blockDim.x + threadIdx.x;
     ts.spillCap = spillCap;
