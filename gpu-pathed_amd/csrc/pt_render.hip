@@ -659,7 +659,7 @@ __device__ __forceinline__ float mixturePdf(const float4 *G, f3 w) {
 // density is the directional technique's pdf in both MIS weights.
 // Plain shading fits 96 VGPRs without spilling: 5 waves per SIMD.
 template <bool GUIDED>
-__global__ __launch_bounds__(256, GUIDED ? 3 : 5) void k_shade(PtSceneView sc, ChunkParams cp, int bounce, int *counters, const float4 *rayO, const float4 *rayD,
+__global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, ChunkParams cp, int bounce, int *counters, const float4 *rayO, const float4 *rayD,
                                                const uint4 *state, const float4 *beta, const float4 *hits, float4 *nRayO, float4 *nRayD,
                                                uint4 *nState, float4 *nBeta, float4 *sO, float4 *sD, float4 *sL, float4 *Lpath,
                                                GuideShadeArgs ga) {
