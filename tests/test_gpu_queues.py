@@ -1,0 +1,93 @@
+"""GPU tests of the segmented path queues at awkward sizes.
+
+Every queue of a chunk is cut into 64 segments whose lengths are multiples of 64 and whose
+paths stay in their segment from bounce to bounce; camera paths are dealt to the segments in
+runs of 64.  These cases leave most segments empty (one path), give segments partial runs
+(widths that are not multiples of 64) or split a pass into several chunks, and check each path
+against the oracle, and chunked against unchunked images.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pathed_capi as P
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if P.device_count() == 0:
+        pytest.fail("no HIP device visible: the GPU tests need an MI355X")
+
+
+@pytest.mark.parametrize("W,H,spp", [(1, 1, 1), (3, 5, 2), (65, 1, 3), (17, 9, 7), (130, 3, 33)])
+def test_per_path_parity_at_odd_sizes(W, H, spp):
+    say(f"case: per-path parity {W}x{H}x{spp}")
+    s = P.Scene.preset(0)
+    r = P.Renderer(W, H)
+    r.set_scene(s)
+    r.set_max_depth(5)
+    r.render(spp)
+    n = W * H * spp
+    gl = r.path_radiance(n)
+    assert gl.shape == (n, 3)
+    o = O.Scene(s.export(), W, H)
+    pix = np.tile(np.arange(W * H), spp)
+    smp = np.repeat(np.arange(spp), W * H)
+    cl = o.trace_paths(pix, smp, max_depth=5, nee=1, mode=1, tri_test=1, ref_env=0, precision=32)
+    err = np.abs(gl - cl).max(1) / (np.abs(cl).max(1) + 1e-3)
+    frac = (err < 1e-3).mean()
+    say(f"per-path agreement {frac:.5f} over {n} paths")
+    assert frac >= 0.99 if n >= 100 else frac == 1.0
+    st = r.stats()
+    assert st.stack_overflows == 0
+    assert st.rays_total >= n  # every camera ray traced
+    r.close()
+
+
+def test_chunked_pass_equals_one_chunk():
+    """A 64x64x8 pass in chunks of 4096 paths (one sample row each) against the same pass in
+    one chunk: the same paths, so the same image up to the FP64 summation order."""
+    say("case: chunked vs one chunk")
+    s = P.Scene.preset(0)
+    imgs = []
+    for pool in (0, 4096):
+        r = P.Renderer(64, 64)
+        r.set_scene(s)
+        r.set_max_depth(6)
+        if pool:
+            r.set_option(P.OPT_POOL_PATHS, pool)
+        r.render(8)
+        imgs.append(r.radiance().copy())
+        st = r.stats()
+        assert st.stack_overflows == 0
+        r.close()
+    assert np.isfinite(imgs[0]).all() and imgs[0].mean() > 0
+    assert np.allclose(imgs[0], imgs[1], rtol=1e-6, atol=1e-7)
+
+
+def test_queue_statistics_add_up():
+    """Per-bounce ray counts summed over the 64 segments: the camera bounce holds every path,
+    later bounces no more than the bounce before (a path continues at most once)."""
+    say("case: queue statistics")
+    s = P.Scene.preset(1, seed=3, param=4)  # displaced icosphere, 5120 triangles
+    r = P.Renderer(45, 37)
+    r.set_scene(s)
+    r.set_max_depth(6)
+    r.render(5)
+    st = r.stats()
+    rays = list(st.bounce_rays)[:6]
+    say(f"rays per bounce {rays}, shadow {list(st.bounce_shadow)[:6]}")
+    assert rays[0] == 45 * 37 * 5
+    assert all(rays[b + 1] <= rays[b] for b in range(5))
+    assert all(list(st.bounce_shadow)[b] <= rays[b] for b in range(6))
+    assert st.rays_total == sum(rays) + sum(list(st.bounce_shadow)[:6])
+    r.close()
