@@ -39,7 +39,7 @@ struct PtEnv {
 //   n3 = (child0, child1, 0, 0) as int bits; child >= 0: internal node index,
 //        child < 0: leaf ~((count-1) << 28 | first primitive)
 static constexpr int PT_LEAF_COUNT_SHIFT = 28;
-static constexpr int PT_LEAF_MAX = 4;
+static constexpr int PT_LEAF_MAX = 2;
 // 4-wide BVH node: 8 float4 = 128 bytes (one L2 line): lo.x, hi.x, lo.y, hi.y, lo.z, hi.z of
 // the four child boxes (one float4 per bound and axis), then the four child references as int
 // bits (>= 0 node index, < 0 leaf ~((count - 1) << PT_LEAF_COUNT_SHIFT | first), or
