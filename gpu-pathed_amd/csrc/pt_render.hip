@@ -56,9 +56,10 @@ static constexpr int kTraceThreads = 128;
 // Default path-pool capacity: 2^26 paths (about 13 GB of SoA path state), a whole 1024^2 x
 // 64 spp pass in one chunk, so that every bounce's queues stay long.
 static constexpr int64_t kDefaultPool = 1ll << 26;
-// Trace blocks of 128 threads per CU: 14 fill the 7 waves per SIMD that the trace kernels'
-// 71 VGPRs and 8 KiB of LDS stack per block allow.
-static constexpr int kDefaultTraceBlocks = 14;
+// Trace blocks of 128 threads per CU: 14 would fill the 7 waves per SIMD that the trace
+// kernels' 71 VGPRs and 8 KiB of LDS stack per block allow; 12 measured best on C1 (10: 3580,
+// 12: 3625, 14: 3574, 16: 3598 Mrays/s), fewer waves sharing each CU's caches.
+static constexpr int kDefaultTraceBlocks = 12;
 // Blocks per CU of the guide training's forward/backward kernel: one-wave blocks of 38.6 KiB
 // LDS each, four to a CU.
 static constexpr int kTrainBlocksPerCU = 4;
@@ -1193,13 +1194,13 @@ int *ensureSpill(pt_renderer *r, int grid, int *cap) {
 // Idle lanes out of 64 at which a trace wave takes new rays.  Camera rays of neighbouring
 // pixels share most nodes, so their waves keep a batch together longer; the incoherent
 // queues refill earlier, which shortens the wait for a batch's longest ray.  Per-launch
-// optima on the 1.31M-triangle icosphere at 1024^2, 64 spp: camera rays 48 (11.5 ms), the
-// first bounce's closest-hit queue 32 (10.3 ms; 19.6 ms with whole batches of 64), shadow
-// rays 32 (7.3 ms; 11.7 ms with whole batches).
+// optima on the 1.31M-triangle icosphere at 1024^2, 64 spp: camera rays 48, the later
+// closest-hit queues and the shadow rays 40 (whole batches of 64 take 1.9x as long on the
+// first bounce's queue; 16 idle lanes cost 6% of the pass).
 enum { TRACE_CAMERA = 0, TRACE_INCOHERENT = 1, TRACE_SHADOW = 2 };
 int refillLanes(const pt_renderer *r, int kind) {
     if (r->refillLanes > 0) return r->refillLanes;
-    return kind == TRACE_CAMERA ? 48 : 32;
+    return kind == TRACE_CAMERA ? 48 : 40;
 }
 
 // Trace grids hold at least PT_QUEUE_SEGS waves: every queue segment needs home waves.

@@ -178,7 +178,7 @@ typedef enum {
     PT_OPT_TRACE_REFILL_LANES = 12, /* a trace wave takes new rays from the queue once this many
                                     of its 64 lanes are idle (1..64; 64: whole batches of 64
                                     neighbouring rays); 0 (default) = per launch: camera rays
-                                    48, later closest-hit queues and shadow rays 32 */
+                                    48, later closest-hit queues and shadow rays 40 */
     PT_OPT_GUIDE_BOUNCES = 13    /* guided: only vertices of the first N bounces draw from the
                                     learned lobes (their network inference and training records);
                                     later vertices sample the BSDF; 0 (default) = every bounce */
