@@ -332,16 +332,24 @@ PT_INLINE f3 envEmit(const PtEnv &e, f3 w) {
     const float4 v = e.rgba[(size_t)y * e.width + x];
     return mk(v.x, v.y, v.z);
 }
-PT_INLINE float envPdf(const PtEnv &e, f3 w) {
+// Radiance of direction w and the sampling density of w (solid angle) from one texel read.
+// sin(theta) from the xz components keeps its precision next to the poles, where y rounds to
+// +-1; on a pole the (phi, theta) map degenerates: density 0 there, and a light sample that
+// lands on one is dropped (pbrt-v3 InfiniteAreaLight::Pdf_Li).
+PT_INLINE f3 envEmitPdf(const PtEnv &e, f3 w, float *pdf) {
     float phi, theta;
     envAngles(w, &phi, &theta);
     int x, y;
     envTexel(e, phi, theta, &x, &y);
-    // sin(theta) from the xz components keeps its precision next to the poles, where y rounds
-    // to +-1; on a pole the (phi, theta) map degenerates: density 0 there, and a light sample
-    // that lands on one is dropped (pbrt-v3 InfiniteAreaLight::Pdf_Li)
+    const float4 v = e.rgba[(size_t)y * e.width + x];
     const float st = sqrtf(w.x * w.x + w.z * w.z);
-    return st > 0.f ? e.texelPdf[(size_t)y * e.width + x] / st : 0.f;
+    *pdf = st > 0.f ? v.w / st : 0.f;
+    return mk(v.x, v.y, v.z);
+}
+PT_INLINE float envPdf(const PtEnv &e, f3 w) {
+    float pdf;
+    envEmitPdf(e, w, &pdf);
+    return pdf;
 }
 PT_INLINE int aliasDraw(const float2 *tab, int n, float xi, float *rem) {
     const float s = xi * (float)n;
@@ -357,7 +365,7 @@ PT_INLINE int aliasDraw(const float2 *tab, int n, float xi, float *rem) {
     *rem = (f - e.x) / (1.0f - e.x);
     return asInt(e.y);
 }
-PT_INLINE f3 envSample(const PtEnv &e, float xi1, float xi2, float *pdf) {
+PT_INLINE f3 envSample(const PtEnv &e, float xi1, float xi2, float *pdf, f3 *emit) {
     float ry, rx;
     const int y = aliasDraw(e.marginal, e.height, xi1, &ry);
     const int x = aliasDraw(e.rows + (size_t)y * e.width, e.width, xi2, &rx);
@@ -369,7 +377,7 @@ PT_INLINE f3 envSample(const PtEnv &e, float xi1, float xi2, float *pdf) {
     sincosf(theta, &st, &ct);
     sincosf(phi, &sp, &cp);
     const f3 w = mk(st * cp, ct, st * sp);
-    *pdf = envPdf(e, w);
+    *emit = envEmitPdf(e, w, pdf);
     return w;
 }
 

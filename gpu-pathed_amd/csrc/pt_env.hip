@@ -88,6 +88,17 @@ __global__ void k_env_pack(const double *wts, const double *rowProb, const int *
 
 }  // namespace
 
+__global__ void k_env_pack_pdf(float4 *rgba, const float *texelPdf, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) rgba[i].w = texelPdf[i];
+}
+
+void packEnvPdf(float4 *rgba, const float *texelPdf, int n, hipStream_t stream) {
+    if (n <= 0) return;
+    k_env_pack_pdf<<<(n + 255) / 256, 256, 0, stream>>>(rgba, texelPdf, n);
+    PT_HIP_TRY(hipGetLastError());
+}
+
 void buildEnvTables(const float4 *rgba, int w, int h, int sinWeight, float2 *rows, float2 *marginal, float *texelPdf, double *rowProbOut,
                     int *rowAliasOut, double *margProbOut, int *margAliasOut, hipStream_t stream) {
     const size_t n = (size_t)w * h;

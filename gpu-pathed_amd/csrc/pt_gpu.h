@@ -64,6 +64,9 @@ void buildEnvTables(const float4 *rgba, int w, int h, int sinWeight, float2 *row
                     float *texelPdf, double *rowProbOut, int *rowAliasOut, double *margProbOut,
                     int *margAliasOut, hipStream_t stream);
 
+// Copies the texel pdf into the w component of the environment texels (PtEnv::rgba).
+void packEnvPdf(float4 *rgba, const float *texelPdf, int n, hipStream_t stream);
+
 int deviceCUs();
 
 }  // namespace pt

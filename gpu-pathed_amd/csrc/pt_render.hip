@@ -611,12 +611,13 @@ __device__ __forceinline__ LightSample sampleLights(const PtSceneView &sc, f3 p,
         ls.emitted = mk(m.b.x, m.b.y, m.b.z);
     } else {
         float pdf;
-        const f3 w = envSample(sc.env, l2, l3, &pdf);
+        f3 em;
+        const f3 w = envSample(sc.env, l2, l3, &pdf, &em);
         ls.wi = normalize(w);
         ls.n = -ls.wi;
         ls.dist = 1e16f;
         ls.pdf = pdf * choicePdf;
-        ls.emitted = envEmit(sc.env, w);
+        ls.emitted = em;
     }
     return ls;
 }
@@ -629,11 +630,6 @@ __device__ __forceinline__ float pdfSceneLight(const PtSceneView &sc, f3 ref, co
     const float proj = fmaxf(0.f, dot(s.n, wo));
     const float pdfSA = (1.f / s.area) * (dist * dist) / proj;
     return pdfSA / (float)(sc.numEmitters + (sc.env.present ? 1 : 0));
-}
-// sampler.h:127-140
-__device__ __forceinline__ float pdfEnvLight(const PtSceneView &sc, f3 w) {
-    if (!sc.env.present) return 0.f;
-    return envPdf(sc.env, w) * (1.f / (float)(sc.numEmitters + 1));
 }
 
 // Guided-vertex inputs of the shade kernel (ignored by the plain integrator)
@@ -702,9 +698,14 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
             const int slot = asInt(h.w);
             if (slot == INT32_MIN) {
                 // miss: environment (camera miss: environment_light.h:68-83 as is)
-                const f3 e = envEmit(sc.env, d);
-                if (depth == 0 || !cp.nee || prevPdf < 0.f) L = thr * e;
-                else L = thr * e * balance(prevPdf, pdfEnvLight(sc, d));
+                if (depth == 0 || !cp.nee || prevPdf < 0.f || !sc.env.present) {
+                    L = thr * envEmit(sc.env, d);
+                } else {
+                    // sampler.h:127-140: light-selection share of the environment density
+                    float pe;
+                    const f3 e = envEmitPdf(sc.env, d, &pe);
+                    L = thr * e * balance(prevPdf, pe * (1.f / (float)(sc.numEmitters + 1)));
+                }
             } else {
                 const SurfacePoint sp = surfaceAt(sc, slot, o, d, h.x, h.y, h.z);
                 const Frame fr = makeFrame(sp.n);
@@ -1608,7 +1609,7 @@ int pt_renderer_set_scene(pt_renderer *r, const pt_scene *s) {
         r->view.env.rgba = r->envRgba.as<float4>();
         r->view.env.rows = r->envRows.as<float2>();
         r->view.env.marginal = r->envMarg.as<float2>();
-        r->view.env.texelPdf = r->envPdf.as<float>();
+        packEnvPdf(r->envRgba.as<float4>(), r->envPdf.as<float>(), (int)ne, r->stream);
         r->view.env.width = s->envW;
         r->view.env.height = s->envH;
         r->view.env.present = 1;

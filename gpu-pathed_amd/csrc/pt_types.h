@@ -24,10 +24,10 @@ struct PtMaterial {
 };
 
 struct PtEnv {
-    const float4 *rgba;       // texels, row-major, theta rows from the +y pole
+    const float4 *rgba;       // texels, row-major, theta rows from the +y pole: (r, g, b,
+                              // texel pdf = pmf * W*H / (2 pi^2)), one gather for both
     const float2 *rows;       // per-row alias tables: (prob, alias bits), width*height
     const float2 *marginal;   // alias table over rows, height
-    const float *texelPdf;    // pmf * W*H / (2 pi^2)
     int width, height;
     int present;
 };

@@ -98,10 +98,10 @@ __global__ void k_env_sample(PtEnv env, const float *xi, int64_t n, float *dir, 
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         float p;
-        const f3 w = envSample(env, xi[2 * i], xi[2 * i + 1], &p);
+        f3 e;
+        const f3 w = envSample(env, xi[2 * i], xi[2 * i + 1], &p, &e);
         dir[3 * i] = w.x; dir[3 * i + 1] = w.y; dir[3 * i + 2] = w.z;
         pdf[i] = p;
-        const f3 e = envEmit(env, w);
         emit[3 * i] = e.x; emit[3 * i + 1] = e.y; emit[3 * i + 2] = e.z;
     }
     if (i < m) pout[i] = envPdf(env, mk(pdirs[3 * i], pdirs[3 * i + 1], pdirs[3 * i + 2]));
@@ -250,11 +250,11 @@ int pt_unit_env_sample(const float *rgba, int w, int h, int sinw, const float *x
     float2 *rows = st.out<float2>(ne), *marg = st.out<float2>((size_t)h);
     float *tp = st.out<float>(ne);
     buildEnvTables(drgba, w, h, sinw, rows, marg, tp, nullptr, nullptr, nullptr, nullptr, st.s);
+    packEnvPdf((float4 *)drgba, tp, (int)ne, st.s);
     PtEnv env;
     env.rgba = drgba;
     env.rows = rows;
     env.marginal = marg;
-    env.texelPdf = tp;
     env.width = w;
     env.height = h;
     env.present = 1;
