@@ -162,8 +162,8 @@ __global__ __launch_bounds__(256) void k_raygen(ChunkParams cp, PtCamera cam, fl
 // ------------------------------------------------------------------------------------------
 // BVH traversal (closest hit / any hit) over the 4-wide BVH: one 128-byte node fetch tests
 // four child boxes; the hit children are ordered near to far (a 5-exchange sorting network),
-// the nearest is visited next and the others are pushed far-first on a per-lane stack.  Any-
-// hit traversal skips the ordering.
+// the nearest is visited next and the others are pushed far-first on a per-lane stack (any-hit
+// queries too: nearer occluders are found sooner).
 //
 // The stack has two levels: the top LDS entries of each lane (stack[level * blockDim + lane],
 // LDS entries per lane) and, below them, a spill area in global memory interleaved by lane
@@ -322,41 +322,30 @@ __device__ __forceinline__ bool traceRound(const PtSceneView &sc, LaneRay &r, co
                 const float hi = fminf(fminf(bfx[k] * r.ix - r.ox, bfy[k] * r.iy - r.oy), fminf(bfz[k] * r.iz - r.oz, r.tmax));
                 tn[k] = lo <= hi ? lo : INFINITY;
             }
-            if constexpr (!ANY) {
-                cswap(tn[0], ref[0], tn[1], ref[1]);
-                cswap(tn[2], ref[2], tn[3], ref[3]);
-                cswap(tn[0], ref[0], tn[2], ref[2]);
-                cswap(tn[1], ref[1], tn[3], ref[3]);
-                cswap(tn[1], ref[1], tn[2], ref[2]);
-                if (tn[0] == INFINITY) {
-                    pop();
-                } else {
-                    // hit children are a prefix of the sorted slots: push all but the nearest,
-                    // the second nearest on top
-                    const int more = (tn[1] != INFINITY) + (tn[2] != INFINITY) + (tn[3] != INFINITY);
-                    if (r.sp + 3 <= STACK) {
-                        if (more >= 3) ts.lds[r.sp * stride] = ref[3];
-                        if (more >= 2) ts.lds[(r.sp + more - 2) * stride] = ref[2];
-                        if (more >= 1) ts.lds[(r.sp + more - 1) * stride] = ref[1];
-                        r.sp += more;
-                    } else {
-                        if (more >= 3) push(ref[3]);
-                        if (more >= 2) push(ref[2]);
-                        if (more >= 1) push(ref[1]);
-                    }
-                    r.node = ref[0];
-                }
+            // hit children nearest first (any-hit queries too: occluders nearer the origin are
+            // found sooner)
+            cswap(tn[0], ref[0], tn[1], ref[1]);
+            cswap(tn[2], ref[2], tn[3], ref[3]);
+            cswap(tn[0], ref[0], tn[2], ref[2]);
+            cswap(tn[1], ref[1], tn[3], ref[3]);
+            cswap(tn[1], ref[1], tn[2], ref[2]);
+            if (tn[0] == INFINITY) {
+                pop();
             } else {
-                int next = PT_EMPTY_CHILD;
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    if (tn[k] != INFINITY) {
-                        if (next == PT_EMPTY_CHILD) next = ref[k];
-                        else push(ref[k]);
-                    }
+                // hit children are a prefix of the sorted slots: push all but the nearest,
+                // the second nearest on top
+                const int more = (tn[1] != INFINITY) + (tn[2] != INFINITY) + (tn[3] != INFINITY);
+                if (r.sp + 3 <= STACK) {
+                    if (more >= 3) ts.lds[r.sp * stride] = ref[3];
+                    if (more >= 2) ts.lds[(r.sp + more - 2) * stride] = ref[2];
+                    if (more >= 1) ts.lds[(r.sp + more - 1) * stride] = ref[1];
+                    r.sp += more;
+                } else {
+                    if (more >= 3) push(ref[3]);
+                    if (more >= 2) push(ref[2]);
+                    if (more >= 1) push(ref[1]);
                 }
-                if (next == PT_EMPTY_CHILD) pop();
-                else r.node = next;
+                r.node = ref[0];
             }
         } else if (leaf == 0 && r.node != kTraceDone) {
             leaf = r.node;
