@@ -114,9 +114,9 @@ __device__ __forceinline__ float vmfPdf(f3 mu, float kappa, f3 w) {
 // inference
 // --------------------------------------------------------------------------------------------
 // Waves per inference block: the block stages the forward weights in LDS once (46 KiB,
-// rows padded to ACT_STRIDE), each wave adds 16.6 KiB of tile buffers, and six waves fill a
-// CU's 160 KiB.
-constexpr int INFER_WAVES = 6;
+// rows padded to ACT_STRIDE), each wave adds 12 KiB of tile buffers (one activation buffer,
+// overwritten in place layer by layer), and eight waves fill a CU's 160 KiB.
+constexpr int INFER_WAVES = 8;
 
 // Raw features of the vertex at queue slot i (hit point normalised to the scene box, shading
 // normal, outgoing direction, roughness); false where the vertex is not guided (a miss, a
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void k_guide_infer(GuideNet net, 
                                                                   int maxRows, float4 *out, const float *rawFeats, float *rawOut) {
     __shared__ float sRaw[INFER_WAVES][32][GUIDE_RAW];
     __shared__ int sElig[INFER_WAVES][32];
-    __shared__ __bf16 sAct[INFER_WAVES][2][32 * ACT_STRIDE];
+    __shared__ __bf16 sAct[INFER_WAVES][32 * ACT_STRIDE];
     __shared__ float sOut[INFER_WAVES][32][GUIDE_OUT + 1];
     __shared__ __bf16 sW[GUIDE_LAYERS * GUIDE_W * ACT_STRIDE];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void k_guide_infer(GuideNet net, 
         {
             // one-blob encoding: lane -> row (lane & 31), raw features 4 * (lane >> 5) .. + 3
             const int row = lane & 31;
-            __bf16 *dst = sAct[wave][0] + row * ACT_STRIDE;
+            __bf16 *dst = sAct[wave] + row * ACT_STRIDE;
             for (int kk = 0; kk < 4; kk++) {
                 const int k = 4 * (lane >> 5) + kk;
                 const float s = sRaw[wave][row][k];
@@ -234,22 +234,25 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void k_guide_infer(GuideNet net, 
             }
         }
         WAVE_SYNC();
-        int cur = 0;
         for (int L = 0; L < GUIDE_LAYERS; L++) {
+            // both output column tiles are computed before either is stored: every lane's
+            // reads of the layer input feed the MFMAs, so the output may overwrite the input
+            const f32x16 acc0 = layerTileLds(sAct[wave], sW, L, 0, lane);
+            const f32x16 acc1 = layerTileLds(sAct[wave], sW, L, 1, lane);
+            WAVE_SYNC();
+#pragma unroll
             for (int ct = 0; ct < 2; ct++) {
-                const f32x16 acc = layerTileLds(sAct[wave][cur], sW, L, ct, lane);
                 const int col = ct * 32 + (lane & 31);
                 const float bias = net.b[L * GUIDE_W + col];
 #pragma unroll
                 for (int r = 0; r < 16; r++) {
                     const int row = cRow(lane, r);
-                    const float v = acc[r] + bias;
-                    if (L < GUIDE_LAYERS - 1) sAct[wave][1 - cur][row * ACT_STRIDE + col] = (__bf16)fmaxf(v, 0.f);
+                    const float v = (ct ? acc1[r] : acc0[r]) + bias;
+                    if (L < GUIDE_LAYERS - 1) sAct[wave][row * ACT_STRIDE + col] = (__bf16)fmaxf(v, 0.f);
                     else if (col < GUIDE_OUT) sOut[wave][row][col] = v;
                 }
             }
             WAVE_SYNC();
-            cur = 1 - cur;
         }
         if (lane < 32) {
             const int i = rowBase + lane;
