@@ -279,9 +279,15 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void k_guide_infer(GuideNet net, 
 // training: gradients
 // --------------------------------------------------------------------------------------------
 // d loss / d raw outputs for one record; loss = -w (log q + log(c q + (1 - c) p_bsdf))
-__device__ float lossGradient(const float *o, f3 w, float pb, float wt, float *g) {
-    for (int k = 0; k < GUIDE_W; k++) g[k] = 0.f;
+// Stores the clamped gradient as one BF16 row (GUIDE_W entries, zero where the loss does not
+// depend on the output) and returns the loss.
+__device__ float lossGradient(const float *o, f3 w, float pb, float wt, __bf16 *g) {
+    const bf16x8 zero = {};
+#pragma unroll
+    for (int k = 0; k < GUIDE_W; k += 8) *(bf16x8 *)(g + k) = zero;
     if (!(wt > 0.f)) return 0.f;
+    // keep single outliers from dominating a step
+    auto put = [&](int k, float v) { g[k] = (__bf16)fminf(fmaxf(v, -1e3f), 1e3f); };
     Mixture mx;
     mixtureFromRaw(o, mx);
     float fk[GUIDE_LOBES];
@@ -297,10 +303,10 @@ __device__ float lossGradient(const float *o, f3 w, float pb, float wt, float *g
     const float dq = -wt * (1.f / q + c / mix);
     // selection probability (sigmoid, clamped)
     const float craw = 1.f / (1.f + __expf(-o[GO_FRAC]));
-    if (craw > GUIDE_FRAC_MIN && craw < GUIDE_FRAC_MAX) g[GO_FRAC] = -wt * (q - pb) / mix * craw * (1.f - craw);
+    if (craw > GUIDE_FRAC_MIN && craw < GUIDE_FRAC_MAX) put(GO_FRAC, -wt * (q - pb) / mix * craw * (1.f - craw));
     for (int k = 0; k < GUIDE_LOBES; k++) {
         // softmax logits
-        g[GO_LOGIT + k] = dq * mx.pi[k] * (fk[k] - q);
+        put(GO_LOGIT + k, dq * mx.pi[k] * (fk[k] - q));
         const float df = dq * mx.pi[k] * fk[k];  // d loss / d log f_k
         const float kap = mx.kappa[k];
         const float cosw = dot(mx.mu[k], w);
@@ -308,30 +314,31 @@ __device__ float lossGradient(const float *o, f3 w, float pb, float wt, float *g
         const float e2 = __expf(-2.f * kap);
         const float dlk = 1.f / kap - 2.f * e2 / (1.f - e2) + (cosw - 1.f);
         const float lk = o[GO_KAPPA + k];
-        if (lk > GUIDE_LOGK_MIN && lk < GUIDE_LOGK_MAX) g[GO_KAPPA + k] = df * dlk * kap;
+        if (lk > GUIDE_LOGK_MIN && lk < GUIDE_LOGK_MAX) put(GO_KAPPA + k, df * dlk * kap);
         // d log f / d m = kappa (w - (mu.w) mu) / |m|
         if (mx.mlen[k] > 1e-12f) {
             const float s = df * kap / mx.mlen[k];
-            g[GO_MEAN + 3 * k] = s * (w.x - cosw * mx.mu[k].x);
-            g[GO_MEAN + 3 * k + 1] = s * (w.y - cosw * mx.mu[k].y);
-            g[GO_MEAN + 3 * k + 2] = s * (w.z - cosw * mx.mu[k].z);
+            put(GO_MEAN + 3 * k, s * (w.x - cosw * mx.mu[k].x));
+            put(GO_MEAN + 3 * k + 1, s * (w.y - cosw * mx.mu[k].y));
+            put(GO_MEAN + 3 * k + 2, s * (w.z - cosw * mx.mu[k].z));
         }
     }
-    // keep single outliers from dominating a step
-    for (int k = 0; k < GUIDE_OUT; k++) g[k] = fminf(fmaxf(g[k], -1e3f), 1e3f);
     return loss;
 }
 
 // Training, phase 1: forward and backward of 32-row tiles of the record batch, one wave per
 // block, persistent over tiles.  Each layer's input activations and its output deltas leave
 // the kernel transposed ([layer][feature][row], BF16) for the weight-gradient product of phase
-// 2; the layer inputs stay in LDS (row-major) as the next forward operand and as the ReLU
-// masks of the backward pass.  Rows past the batch get weight 0 and so zero deltas.
+// 2.  In LDS a wave keeps one activation tile and one delta tile, each overwritten in place
+// (both 32-column halves of a layer are computed before either is stored), and the ReLU
+// masks of the backward pass stay in registers: the lane that computed an output element in
+// the forward pass is the lane that masks its delta.  Rows past the batch get weight 0 and so
+// zero deltas.
 __global__ __launch_bounds__(64) void k_guide_fwdbwd(GuideNet net, const float4 *recs, const int *countPtr, const int *prefix, int recSeg,
                                                      const float *wsumPtr, int base, int batch, int rows, __bf16 *actT, __bf16 *deltaT,
                                                      float *lossOut) {
-    __shared__ __bf16 sAct[GUIDE_LAYERS][32 * ACT_STRIDE];
-    __shared__ __bf16 sD[2][32 * ACT_STRIDE];
+    __shared__ __bf16 sAct[32 * ACT_STRIDE];
+    __shared__ __bf16 sD[32 * ACT_STRIDE];
     __shared__ float sOut[32][GUIDE_OUT + 1];
     __shared__ float sRec[32][8];
     const int lane = threadIdx.x;
@@ -396,35 +403,52 @@ __global__ __launch_bounds__(64) void k_guide_fwdbwd(GuideNet net, const float4 
                 for (int i = 0; i < 8; i++) {
                     const int row = 8 * q + i;
                     v[i] = (__bf16)oneBlob(sOut[row][k], j);
-                    sAct[0][row * ACT_STRIDE + lane] = v[i];
+                    sAct[row * ACT_STRIDE + lane] = v[i];
                 }
                 *(bf16x8 *)(dst + 8 * q) = v;
             }
         }
         WAVE_SYNC();
+        // ReLU masks of the four hidden layers' outputs, 32 bits per layer (bit 16 ct + r for
+        // this lane's C-fragment element r of column half ct): layers 0-1 in maskLo, 2-3 in
+        // maskHi, so that rolled layer loops select them without indexing registers
+        uint64_t maskLo = 0, maskHi = 0;
+#pragma unroll 1
         for (int L = 0; L < GUIDE_LAYERS; L++) {
+            const f32x16 acc0 = layerTile(sAct, net.w, L, 0, lane);
+            const f32x16 acc1 = layerTile(sAct, net.w, L, 1, lane);
+            WAVE_SYNC();
+            uint32_t m = 0;
+#pragma unroll
             for (int ct = 0; ct < 2; ct++) {
-                const f32x16 acc = layerTile(sAct[L], net.w, L, ct, lane);
                 const int col = ct * 32 + (lane & 31);
                 const float bias = net.b[L * GUIDE_W + col];
                 float h[16];
 #pragma unroll
                 for (int r = 0; r < 16; r++) {
                     const int row = cRow(lane, r);
-                    const float v = acc[r] + bias;
+                    const float v = (ct ? acc1[r] : acc0[r]) + bias;
                     h[r] = fmaxf(v, 0.f);
-                    if (L < GUIDE_LAYERS - 1) sAct[L + 1][row * ACT_STRIDE + col] = (__bf16)h[r];
-                    else if (col < GUIDE_OUT) sOut[row][col] = v;
+                    if (L < GUIDE_LAYERS - 1) {
+                        const __bf16 hb = (__bf16)h[r];
+                        sAct[row * ACT_STRIDE + col] = hb;
+                        m |= ((float)hb > 0.f ? 1u : 0u) << (16 * ct + r);
+                    } else if (col < GUIDE_OUT) {
+                        sOut[row][col] = v;
+                    }
                 }
                 if (L < GUIDE_LAYERS - 1) storeT(actT + ((size_t)(L + 1) * GUIDE_W + col) * rows + rowBase, h);
+            }
+            if (L < GUIDE_LAYERS - 1) {
+                const uint64_t mm = (uint64_t)m << (32 * (L & 1));
+                if (L < 2) maskLo |= mm;
+                else maskHi |= mm;
             }
             WAVE_SYNC();
         }
         if (lane < 32) {
-            float g[GUIDE_W];
             const f3 w = mk(sRec[lane][0], sRec[lane][1], sRec[lane][2]);
-            lossAcc += lossGradient(sOut[lane], w, sRec[lane][3], sRec[lane][4], g);
-            for (int k = 0; k < GUIDE_W; k++) sD[0][lane * ACT_STRIDE + k] = (__bf16)g[k];
+            lossAcc += lossGradient(sOut[lane], w, sRec[lane][3], sRec[lane][4], sD + lane * ACT_STRIDE);
         }
         WAVE_SYNC();
         {
@@ -433,32 +457,38 @@ __global__ __launch_bounds__(64) void k_guide_fwdbwd(GuideNet net, const float4 
             for (int q = 0; q < 4; q++) {
                 bf16x8 v;
 #pragma unroll
-                for (int i = 0; i < 8; i++) v[i] = sD[0][(8 * q + i) * ACT_STRIDE + lane];
+                for (int i = 0; i < 8; i++) v[i] = sD[(8 * q + i) * ACT_STRIDE + lane];
                 *(bf16x8 *)(dst + 8 * q) = v;
             }
         }
-        int dcur = 0;
+#pragma unroll 1
         for (int L = GUIDE_LAYERS - 1; L > 0; L--) {
             // delta_{L-1}[row][in] = sum_out delta_L[row][out] W_L[out][in], masked by relu'
+            f32x16 acc[2];
+#pragma unroll
             for (int nt = 0; nt < 2; nt++) {
-                f32x16 acc = {};
-                const __bf16 *arow = sD[dcur] + (lane & 31) * ACT_STRIDE + (lane >> 5) * 8;
+                f32x16 a = {};
+                const __bf16 *arow = sD + (lane & 31) * ACT_STRIDE + (lane >> 5) * 8;
                 const __bf16 *brow = net.wt + (size_t)L * GUIDE_W * GUIDE_W + (nt * 32 + (lane & 31)) * GUIDE_W + (lane >> 5) * 8;
 #pragma unroll
-                for (int ks = 0; ks < GUIDE_W / 16; ks++) acc = mfma(ld8(arow + ks * 16), ld8(brow + ks * 16), acc);
+                for (int ks = 0; ks < GUIDE_W / 16; ks++) a = mfma(ld8(arow + ks * 16), ld8(brow + ks * 16), a);
+                acc[nt] = a;
+            }
+            WAVE_SYNC();
+            const uint32_t m = (uint32_t)((L - 1 < 2 ? maskLo : maskHi) >> (32 * ((L - 1) & 1)));
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++) {
                 const int in = nt * 32 + (lane & 31);
                 float dv[16];
 #pragma unroll
                 for (int r = 0; r < 16; r++) {
                     const int row = cRow(lane, r);
-                    const bool on = (float)sAct[L][row * ACT_STRIDE + in] > 0.f;
-                    dv[r] = on ? acc[r] : 0.f;
-                    sD[1 - dcur][row * ACT_STRIDE + in] = (__bf16)dv[r];
+                    dv[r] = (m >> (16 * nt + r)) & 1u ? acc[nt][r] : 0.f;
+                    sD[row * ACT_STRIDE + in] = (__bf16)dv[r];
                 }
                 storeT(deltaT + ((size_t)(L - 1) * GUIDE_W + in) * rows + rowBase, dv);
             }
             WAVE_SYNC();
-            dcur = 1 - dcur;
         }
     }
     // loss of this block: lanes 0..31 hold partial sums

@@ -60,9 +60,9 @@ static constexpr int64_t kDefaultPool = 1ll << 26;
 // kernels' 71 VGPRs and 8 KiB of LDS stack per block allow; 12 measured best on C1 (10: 3580,
 // 12: 3625, 14: 3574, 16: 3598 Mrays/s), fewer waves sharing each CU's caches.
 static constexpr int kDefaultTraceBlocks = 12;
-// Blocks per CU of the guide training's forward/backward kernel: one-wave blocks of 38.6 KiB
-// LDS each, four to a CU.
-static constexpr int kTrainBlocksPerCU = 4;
+// Blocks per CU of the guide training's forward/backward kernel: one-wave blocks of 16.5 KiB
+// LDS and 197 registers per lane, two waves per SIMD.
+static constexpr int kTrainBlocksPerCU = 8;
 static constexpr float kShadowEps = 1e-4f;   // path_tracer.cu:104-110
 static constexpr float kBounceEps = 1e-3f;   // path_tracer.cu:177
 
