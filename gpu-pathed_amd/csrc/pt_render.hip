@@ -357,21 +357,23 @@ __device__ __forceinline__ bool traceRound(const PtSceneView &sc, LaneRay &r, co
         const int e = ~leaf;
         const int first = e & ((1 << PT_LEAF_COUNT_SHIFT) - 1);
         const int cnt = (e >> PT_LEAF_COUNT_SHIFT) + 1;
-        for (int k = first; k < first + cnt; k++) {
-            const float4 p0 = sc.prims[3 * (size_t)k], p1 = sc.prims[3 * (size_t)k + 1];
+        for (int j = 0; j < cnt; j++) {
+            // all three records of the slot (spheres have three too) are fetched before the
+            // type test, so the loads overlap
+            const size_t k = 3 * (size_t)(first + j);
+            const float4 p0 = sc.prims[k], p1 = sc.prims[k + 1], p2 = sc.prims[k + 2];
             float t, u = 0.f, v = 0.f;
             bool h;
             if (asInt(p1.w) == PT_TAG_SPHERE) {
                 h = sphereHit(xyz(p0), p1.x, r.wr.o, r.d, r.tmin, r.tmax, &t);
             } else {
-                const float4 p2 = sc.prims[3 * (size_t)k + 2];
                 h = wtHit(r.wr, xyz(p0), xyz(p1), xyz(p2), r.tmin, r.tmax, &t, &u, &v);
             }
             if (h) {
                 r.tmax = t;
                 r.u = u;
                 r.v = v;
-                r.prim = k;
+                r.prim = first + j;
                 if (ANY) return true;
             }
         }
