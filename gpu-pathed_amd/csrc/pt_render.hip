@@ -309,9 +309,9 @@ __device__ __forceinline__ bool traceRound(const PtSceneView &sc, LaneRay &r, co
             const float4 nx = *(const float4 *)(nodes + ax), fx = *(const float4 *)(nodes + (ax ^ 16u));
             const float4 ny = *(const float4 *)(nodes + ay), fy = *(const float4 *)(nodes + (ay ^ 16u));
             const float4 nz = *(const float4 *)(nodes + az), fz = *(const float4 *)(nodes + (az ^ 16u));
-            const float4 rf = *(const float4 *)(nodes + (base + 96u));
+            const int4 rf = *(const int4 *)(nodes + (base + 96u));
             float tn[4];
-            int ref[4] = {asInt(rf.x), asInt(rf.y), asInt(rf.z), asInt(rf.w)};
+            int ref[4] = {rf.x, rf.y, rf.z, rf.w};
             const float bnx[4] = {nx.x, nx.y, nx.z, nx.w}, bfx[4] = {fx.x, fx.y, fx.z, fx.w};
             const float bny[4] = {ny.x, ny.y, ny.z, ny.w}, bfy[4] = {fy.x, fy.y, fy.z, fy.w};
             const float bnz[4] = {nz.x, nz.y, nz.z, nz.w}, bfz[4] = {fz.x, fz.y, fz.z, fz.w};
@@ -322,6 +322,10 @@ __device__ __forceinline__ bool traceRound(const PtSceneView &sc, LaneRay &r, co
                 const float hi = fminf(fminf(bfx[k] * r.ix - r.ox, bfy[k] * r.iy - r.oy), fminf(bfz[k] * r.iz - r.oz, r.tmax));
                 tn[k] = lo <= hi ? lo : INFINITY;
             }
+            // the references are used only when a child is hit; this empty statement keeps
+            // their load beside the box loads instead of after the box test, a second round
+            // trip on the common path
+            asm volatile("" : "+v"(ref[0]), "+v"(ref[1]), "+v"(ref[2]), "+v"(ref[3]));
             // hit children nearest first (any-hit queries too: occluders nearer the origin are
             // found sooner)
             cswap(tn[0], ref[0], tn[1], ref[1]);
