@@ -365,7 +365,11 @@ __device__ __forceinline__ bool traceRound(const PtSceneView &sc, LaneRay &r, co
             // all three records of the slot (spheres have three too) are fetched before the
             // type test, so the loads overlap
             const size_t k = 3 * (size_t)(first + j);
-            const float4 p0 = sc.prims[k], p1 = sc.prims[k + 1], p2 = sc.prims[k + 2];
+            const float4 p0 = sc.prims[k];
+            float4 p1 = sc.prims[k + 1], p2 = sc.prims[k + 2];
+            // keeps the triangle-only components' loads ahead of the type test (the compiler
+            // would otherwise issue them inside the triangle branch, after the first ones return)
+            asm volatile("" : "+v"(p1.y), "+v"(p1.z), "+v"(p2.x), "+v"(p2.y), "+v"(p2.z));
             float t, u = 0.f, v = 0.f;
             bool h;
             if (asInt(p1.w) == PT_TAG_SPHERE) {
