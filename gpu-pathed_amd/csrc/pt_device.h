@@ -357,7 +357,9 @@ PT_INLINE int aliasDraw(const float2 *tab, int n, float xi, float *rem) {
     if (k > n - 1) k = n - 1;
     float f = s - (float)k;
     if (f >= 1.0f) f = 0x1.fffffep-1f;
-    const float2 e = tab[k];  // (prob, alias as int bits)
+    float2 e = tab[k];  // (prob, alias as int bits)
+    // one 8-byte load: the alias is otherwise fetched after the comparison, a second round trip
+    asm volatile("" : "+v"(e.y));
     if (f < e.x) {
         *rem = f / e.x;
         return k;

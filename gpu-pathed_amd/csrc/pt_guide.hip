@@ -129,6 +129,9 @@ __device__ __forceinline__ bool vertexFeatures(const PtSceneView &sc, const Guid
     const float4 ro = rayO[i], rd = rayD[i];
     const f3 o = xyz(ro), d = xyz(rd);
     const float4 p0 = sc.prims[3 * (size_t)slot], p1 = sc.prims[3 * (size_t)slot + 1];
+    float4 p2 = sc.prims[3 * (size_t)slot + 2];
+    // issues the third record's load with the first two instead of after the type tests
+    asm volatile("" : "+v"(p2.x), "+v"(p2.y), "+v"(p2.z));
     const PtMaterial m = sc.materials[asInt(p0.w)];
     const int type = matType(m);
     if (type != PT_MAT_LAMBERTIAN && type != PT_MAT_MICROFACET) return false;
@@ -138,7 +141,6 @@ __device__ __forceinline__ bool vertexFeatures(const PtSceneView &sc, const Guid
         p = o + d * h.x;
         nn = normalize((p - xyz(p0)) / p1.x);
     } else {
-        const float4 p2 = sc.prims[3 * (size_t)slot + 2];
         const f3 a = xyz(p0), b = xyz(p1), c = xyz(p2);
         p = a * (1.f - h.y - h.z) + b * h.y + c * h.z;
         if (tag >= 0) {

@@ -542,6 +542,9 @@ struct SurfacePoint {
 __device__ __forceinline__ SurfacePoint surfaceAt(const PtSceneView &sc, int slot, f3 o, f3 d, float t, float u, float v) {
     SurfacePoint s;
     const float4 p0 = sc.prims[3 * (size_t)slot], p1 = sc.prims[3 * (size_t)slot + 1];
+    float4 p2 = sc.prims[3 * (size_t)slot + 2];
+    // issues the third record's load with the first two instead of after the type test
+    asm volatile("" : "+v"(p2.x), "+v"(p2.y), "+v"(p2.z));
     s.mat = asInt(p0.w);
     const int tag = asInt(p1.w);
     if (tag == PT_TAG_SPHERE) {
@@ -551,7 +554,6 @@ __device__ __forceinline__ SurfacePoint surfaceAt(const PtSceneView &sc, int slo
         s.n = normalize((s.p - c) / r);
         s.area = sphereArea(r);
     } else {
-        const float4 p2 = sc.prims[3 * (size_t)slot + 2];
         const f3 a = xyz(p0), b = xyz(p1), c = xyz(p2);
         s.p = a * (1.f - u - v) + b * u + c * v;  // triangle.cu:36
         if (tag >= 0) {
@@ -579,7 +581,11 @@ __device__ __forceinline__ LightSample sampleLights(const PtSceneView &sc, f3 p,
     const int choice = (int)fminf((float)(choiceSize - 1), floorf(l1 * (float)choiceSize));
     const float choicePdf = 1.f / (float)choiceSize;
     if (choice < nl) {
-        const float4 e0 = sc.emitters[3 * choice], e1 = sc.emitters[3 * choice + 1];
+        const float4 e0 = sc.emitters[3 * choice];
+        float4 e1 = sc.emitters[3 * choice + 1], e2 = sc.emitters[3 * choice + 2];
+        // all three records load before the type test (the triangle-only components would
+        // otherwise wait for the tag)
+        asm volatile("" : "+v"(e1.x), "+v"(e1.y), "+v"(e1.z), "+v"(e2.x), "+v"(e2.y), "+v"(e2.z));
         f3 pt, n;
         float pdfA;
         if (asInt(e1.w) == PT_TAG_SPHERE) {
@@ -588,7 +594,6 @@ __device__ __forceinline__ LightSample sampleLights(const PtSceneView &sc, f3 p,
             n = pl;
             pdfA = 1.f / sphereArea(e1.x);
         } else {
-            const float4 e2 = sc.emitters[3 * choice + 2];
             const f3 a = xyz(e0), b = xyz(e1), c = xyz(e2);
             const float sr = sqrtf(l2);
             const float wa = 1.f - sr, wb = sr * (1.f - l3);
