@@ -65,9 +65,10 @@ __device__ __forceinline__ void guideFeatures(const GuideSceneNorm &nm, f3 p, f3
 void guideInitParams(float *params, uint64_t seed);
 void guideUploadBf16(const float *dParams, __bf16 *dW, __bf16 *dWt, float *dB, hipStream_t s);
 // Mixture per slot of a segmented ray queue (count of segment s at segCount[s * segStride],
-// PT_QUEUE_SEGS segments of segLen slots); feat holds two float4 of features per slot.
+// PT_QUEUE_SEGS segments of segLen slots); feat holds two float4 of features per slot.  With
+// camO.w != 0 the queue holds camera rays starting at camO and rayO is not read.
 void guideInfer(const PtSceneView &sc, GuideSceneNorm norm, GuideNet net, const int *segCount, int segStride, int segLen,
-                const float4 *rayO, const float4 *rayD, const float4 *hits, float4 *feat, float4 *out, int cus, hipStream_t s);
+                const float4 *rayO, const float4 *rayD, float4 camO, const float4 *hits, float4 *feat, float4 *out, int cus, hipStream_t s);
 void guideInferRaw(GuideNet net, const float *feats, int n, float *rawOut, hipStream_t s);
 // Scratch of one training step over `batch` records: transposed BF16 layer inputs and output
 // deltas (GUIDE_LAYERS x GUIDE_W x guideRows(batch) each), FP32 gradient partials

@@ -122,11 +122,12 @@ constexpr int INFER_WAVES = 8;
 // normal, outgoing direction, roughness); false where the vertex is not guided (a miss, a
 // mirror or a glass surface).
 __device__ __forceinline__ bool vertexFeatures(const PtSceneView &sc, const GuideSceneNorm &nm, int i, const float4 *rayO, const float4 *rayD,
-                                               const float4 *hits, float *f) {
+                                               float4 camO, const float4 *hits, float *f) {
     const float4 h = hits[i];
     const int slot = asInt(h.w);
     if (slot == INT32_MIN) return false;
-    const float4 ro = rayO[i], rd = rayD[i];
+    // camera rays (camO.w != 0) start at the camera and have no stored origin
+    const float4 ro = camO.w != 0.f ? camO : rayO[i], rd = rayD[i];
     const f3 o = xyz(ro), d = xyz(rd);
     const float4 p0 = sc.prims[3 * (size_t)slot], p1 = sc.prims[3 * (size_t)slot + 1];
     float4 p2 = sc.prims[3 * (size_t)slot + 2];
@@ -160,14 +161,14 @@ __device__ __forceinline__ bool vertexFeatures(const PtSceneView &sc, const Guid
 // (roughness, > 0) set to -1 where the vertex is not guided.  A kernel of its own, at full
 // occupancy, so that the dependent gathers (hit -> primitive -> material) overlap.
 __global__ __launch_bounds__(256) void k_guide_features(PtSceneView sc, GuideSceneNorm nm, const int *count, int segStride, int segLen,
-                                                        const float4 *rayO, const float4 *rayD, const float4 *hits, int maxRows,
+                                                        const float4 *rayO, const float4 *rayD, float4 camO, const float4 *hits, int maxRows,
                                                         float4 *feat) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < maxRows; i += gridDim.x * blockDim.x) {
         const int seg = i / segLen;
         if (i - seg * segLen >= count[seg * segStride]) continue;
         float f[GUIDE_RAW];
         for (int k = 0; k < GUIDE_RAW; k++) f[k] = 0.f;
-        if (!vertexFeatures(sc, nm, i, rayO, rayD, hits, f)) {
+        if (!vertexFeatures(sc, nm, i, rayO, rayD, camO, hits, f)) {
             for (int k = 0; k < GUIDE_RAW; k++) f[k] = 0.f;
             f[GUIDE_RAW - 1] = -1.f;
         }
@@ -667,10 +668,10 @@ void guideUploadBf16(const float *dParams, __bf16 *dW, __bf16 *dWt, float *dB, h
 }
 
 void guideInfer(const PtSceneView &sc, GuideSceneNorm norm, GuideNet net, const int *segCount, int segStride, int segLen,
-                const float4 *rayO, const float4 *rayD, const float4 *hits, float4 *feat, float4 *out, int cus, hipStream_t s) {
+                const float4 *rayO, const float4 *rayD, float4 camO, const float4 *hits, float4 *feat, float4 *out, int cus, hipStream_t s) {
     const int maxRows = PT_QUEUE_SEGS * segLen;
     const int fgrid = std::max(1, std::min((maxRows + 255) / 256, cus * 8));
-    k_guide_features<<<fgrid, 256, 0, s>>>(sc, norm, segCount, segStride, segLen, rayO, rayD, hits, maxRows, feat);
+    k_guide_features<<<fgrid, 256, 0, s>>>(sc, norm, segCount, segStride, segLen, rayO, rayD, camO, hits, maxRows, feat);
     const int grid = std::max(1, std::min((maxRows + 32 * INFER_WAVES - 1) / (32 * INFER_WAVES), cus));
     k_guide_infer<<<grid, 64 * INFER_WAVES, 0, s>>>(net, segCount, segStride, segLen, feat, maxRows, out, nullptr, nullptr);
     PT_HIP_TRY(hipGetLastError());

@@ -139,8 +139,10 @@ __global__ __launch_bounds__(256) void k_queue_reset(int *counters, int field, i
     }
 }
 
-__global__ __launch_bounds__(256) void k_raygen(ChunkParams cp, PtCamera cam, float4 *rayO, float4 *rayD, uint4 *state, float4 *beta,
-                                                float4 *Lpath) {
+// Writes only the camera rays' directions: their origin (the camera's), throughput (1),
+// previous pdf (-1, none), path index, depth (0) and RNG state (the stream after the two jitter
+// draws) are recomputed by the bounce-0 kernels, and shade writes the paths' radiance at bounce 0.
+__global__ __launch_bounds__(256) void k_raygen(ChunkParams cp, PtCamera cam, float4 *rayD) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= cp.nPaths) return;
     const int s = i / cp.npix;
@@ -151,12 +153,17 @@ __global__ __launch_bounds__(256) void k_raygen(ChunkParams cp, PtCamera cam, fl
     const float sx = rng.uniform(), sy = rng.uniform();
     f3 o, d;
     cameraRay(cam, row, col, sx, sy, &o, &d);
-    const int q = ptCameraSlot(i, cp.segLen);
-    rayO[q] = f4(o, 0.f);
-    rayD[q] = f4(d, FLT_MAX);
-    state[q] = make_uint4((uint32_t)rng.state, (uint32_t)(rng.state >> 32), (uint32_t)i, 0u);
-    beta[q] = make_float4(1.f, 1.f, 1.f, -1.f);
-    Lpath[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    rayD[ptCameraSlot(i, cp.segLen)] = f4(d, FLT_MAX);
+}
+
+// The camera-ray state k_raygen does not store: the RNG after the pixel's two jitter draws.
+__device__ __forceinline__ Rng cameraRng(const ChunkParams &cp, int path) {
+    const int s = path / cp.npix;
+    const int pixel = cp.pixBegin + (path - s * cp.npix);
+    Rng rng = seedRng(cp.seed, (uint32_t)pixel, (uint32_t)(cp.sampleBase + s));
+    rng.next();
+    rng.next();
+    return rng;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -480,20 +487,27 @@ __device__ __forceinline__ void traceQueue(const PtSceneView &sc, int *counters,
     }
 }
 
-// Persistent closest-hit kernel over the rays of `bounce`.
+// Persistent closest-hit kernel over the rays of `bounce`.  camO.w != 0: the rays start at
+// camO (camera rays, tmin 0) and rayO is not read.
 template <int STACK>
 __global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 7 : 5) void k_extend(PtSceneView sc, int *counters, int bounce, int segLen,
-                                                                              const float4 *rayO, const float4 *rayD, float4 *hits, int refill,
-                                                                              int *spill, int spillCap, int *overflow) {
+                                                                              const float4 *rayO, const float4 *rayD, float4 camO, float4 *hits,
+                                                                              int refill, int *spill, int spillCap, int *overflow) {
     __shared__ int sstack[STACK * kTraceThreads];
     const TraceStack ts = traceStack(sstack, spill, spillCap);
     traceQueue<false, STACK>(
         sc, counters, bounce, PT_Q_RAYS, PT_Q_FETCH_EXT, segLen, refill, ts, overflow,
         [&](int i, f3 &o, f3 &d, float &tmin, float &tmax) {
-            const float4 ro = rayO[i], rd = rayD[i];
-            o = xyz(ro);
+            const float4 rd = rayD[i];
+            if (camO.w != 0.f) {
+                o = xyz(camO);
+                tmin = 0.f;
+            } else {
+                const float4 ro = rayO[i];
+                o = xyz(ro);
+                tmin = ro.w;
+            }
             d = xyz(rd);
-            tmin = ro.w;
             tmax = rd.w;
         },
         [&](int i, bool, const TraceHit &th) { hits[i] = make_float4(th.t, th.u, th.v, asFloat(th.prim)); });
@@ -673,11 +687,16 @@ template <bool GUIDED>
 __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, ChunkParams cp, int bounce, int *counters, const float4 *rayO, const float4 *rayD,
                                                const uint4 *state, const float4 *beta, const float4 *hits, float4 *nRayO, float4 *nRayD,
                                                uint4 *nState, float4 *nBeta, float4 *sO, float4 *sD, float4 *sL, float4 *Lpath,
-                                               GuideShadeArgs ga) {
+                                               GuideShadeArgs ga, float4 camO) {
     // Waves claim 64 rays at a time from one queue segment and append the continuation and
     // shadow rays to the same segment of the next queues.  Appends happen where each record is
     // complete (one atomic per wave and branch: the lanes taking a branch append together), so
     // no output record stays live across the rest of the vertex.
+    //
+    // camO.w != 0 (bounce 0): the camera rays' origin, throughput, previous pdf, path index and
+    // RNG state are recomputed here instead of read (k_raygen stores only the directions), and
+    // the path's radiance is written instead of added to.
+    const bool camera = camO.w != 0.f;
     SegCursor cur = segStart();
     int adds = 0;
     int base = 0, got;
@@ -686,8 +705,20 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
         const int i = base + laneId();
         bool addL = false;
         if (laneId() < got) {
-            const float4 ro = rayO[i], rd = rayD[i], h = hits[i], bp = beta[i];
-            const uint4 st = state[i];
+            const float4 rd = rayD[i], h = hits[i];
+            float4 ro, bp;
+            uint4 st;
+            if (camera) {
+                const int p = ptCameraPath(seg, i - seg * cp.segLen);
+                const Rng g = cameraRng(cp, p);
+                ro = camO;
+                bp = make_float4(1.f, 1.f, 1.f, -1.f);
+                st = make_uint4((uint32_t)g.state, (uint32_t)(g.state >> 32), (uint32_t)p, 0u);
+            } else {
+                ro = rayO[i];
+                bp = beta[i];
+                st = state[i];
+            }
             const int pathId = (int)st.z;
             const int depth = (int)st.w;
             const f3 o = xyz(ro), d = xyz(rd), thr = xyz(bp);
@@ -836,7 +867,10 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
                 }
             }
             addL = !isZero(L);
-            if (addL) {
+            if (camera) {
+                // the first contribution to a zero radiance (0 + L, as an addition would give)
+                Lpath[pathId] = make_float4(0.f + L.x, 0.f + L.y, 0.f + L.z, 0.f);
+            } else if (addL) {
                 float4 acc = Lpath[pathId];
                 acc.x += L.x; acc.y += L.y; acc.z += L.z;
                 Lpath[pathId] = acc;
@@ -1213,15 +1247,16 @@ int traceGrid(const pt_renderer *r) {
     return std::max(r->cus * r->traceBlocksPerCU, (PT_QUEUE_SEGS * 64 + kTraceThreads - 1) / kTraceThreads);
 }
 
-void launchExtend(pt_renderer *r, int bounce, int segLen, const float4 *ro, const float4 *rd, float4 *hits, int kind) {
+void launchExtend(pt_renderer *r, int bounce, int segLen, const float4 *ro, const float4 *rd, float4 *hits, int kind,
+                  float4 camO = make_float4(0.f, 0.f, 0.f, 0.f)) {
     const int refill = refillLanes(r, kind);
     const int grid = traceGrid(r);
     int *ctr = r->pp->counters.as<int>();
     int *ov = r->pp->overflow.as<int>();
     int cap = 0;
     int *sp = ensureSpill(r, grid, &cap);
-    if (r->ldsStack == 32) k_extend<32><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, ro, rd, hits, refill, sp, cap, ov);
-    else k_extend<16><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, ro, rd, hits, refill, sp, cap, ov);
+    if (r->ldsStack == 32) k_extend<32><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, ro, rd, camO, hits, refill, sp, cap, ov);
+    else k_extend<16><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, ro, rd, camO, hits, refill, sp, cap, ov);
     PT_HIP_TRY(hipGetLastError());
 }
 void launchShadow(pt_renderer *r, int bounce, int segLen) {
@@ -1300,8 +1335,7 @@ void runChunk(pt_renderer *r, const ChunkParams &cp) {
     {
         KTimer t(r, K_RAYGEN);
         k_queue_reset<<<PT_QUEUE_SEGS + 1, 256, 0, r->stream>>>(ctr, PT_Q_RAYS, cp.nPaths, cp.segLen, 1);
-        k_raygen<<<(cp.nPaths + 255) / 256, 256, 0, r->stream>>>(cp, r->cam, r->pp->rayO[0].as<float4>(), r->pp->rayD[0].as<float4>(), r->pp->st[0].as<uint4>(),
-                                                                r->pp->beta[0].as<float4>(), r->pp->Lpath.as<float4>());
+        k_raygen<<<(cp.nPaths + 255) / 256, 256, 0, r->stream>>>(cp, r->cam, r->pp->rayD[0].as<float4>());
         PT_HIP_TRY(hipGetLastError());
         if (guided) {
             // camera rays were not spawned by a guided vertex
@@ -1312,12 +1346,16 @@ void runChunk(pt_renderer *r, const ChunkParams &cp) {
     }
     // shade needs PT_QUEUE_SEGS waves or more (home waves for every segment)
     const int shadeGrid = (int)std::max<int64_t>(PT_QUEUE_SEGS / 4, std::min<int64_t>((cp.nPaths + 255) / 256, (int64_t)r->cus * r->shadeBlocksPerCU));
+    // bounce 0 traces and shades the camera rays from the camera's origin (w = 1: in use)
+    const float4 camO = make_float4(r->cam.m[0][3], r->cam.m[1][3], r->cam.m[2][3], 1.f);
+    const float4 noCam = make_float4(0.f, 0.f, 0.f, 0.f);
     int cur = 0;
     for (int b = 0; b < cp.maxDepth; b++) {
+        const float4 bo = b == 0 ? camO : noCam;
         {
             KTimer t(r, K_EXTEND);
             launchExtend(r, b, cp.segLen, r->pp->rayO[cur].as<float4>(), r->pp->rayD[cur].as<float4>(), r->pp->hits.as<float4>(),
-                         b == 0 ? TRACE_CAMERA : TRACE_INCOHERENT);
+                         b == 0 ? TRACE_CAMERA : TRACE_INCOHERENT, bo);
         }
         GuideShadeArgs ga{};
         // vertices of bounces >= gBounces (when set) sample the BSDF only
@@ -1325,7 +1363,7 @@ void runChunk(pt_renderer *r, const ChunkParams &cp) {
         if (guidedBounce) {
             KTimer t(r, K_GUIDE_INFER);
             guideInfer(r->view, guideNorm(r), guideNet(r), ctr + ptQueueField(0, b, PT_Q_RAYS), PT_SEG_STRIDE, cp.segLen, r->pp->rayO[cur].as<float4>(),
-                       r->pp->rayD[cur].as<float4>(), r->pp->hits.as<float4>(), r->gFeat.as<float4>(), r->gMix.as<float4>(), r->cus, r->stream);
+                       r->pp->rayD[cur].as<float4>(), bo, r->pp->hits.as<float4>(), r->gFeat.as<float4>(), r->gMix.as<float4>(), r->cus, r->stream);
         }
         if (guided) {
             ga.mix = guidedBounce ? r->gMix.as<float4>() : nullptr;
@@ -1342,12 +1380,12 @@ void runChunk(pt_renderer *r, const ChunkParams &cp) {
                 k_shade<true><<<shadeGrid, 256, 0, r->stream>>>(
                     r->view, cp, b, ctr, r->pp->rayO[cur].as<float4>(), r->pp->rayD[cur].as<float4>(), r->pp->st[cur].as<uint4>(), r->pp->beta[cur].as<float4>(),
                     r->pp->hits.as<float4>(), r->pp->rayO[1 - cur].as<float4>(), r->pp->rayD[1 - cur].as<float4>(), r->pp->st[1 - cur].as<uint4>(),
-                    r->pp->beta[1 - cur].as<float4>(), r->pp->sO.as<float4>(), r->pp->sD.as<float4>(), r->pp->sL.as<float4>(), r->pp->Lpath.as<float4>(), ga);
+                    r->pp->beta[1 - cur].as<float4>(), r->pp->sO.as<float4>(), r->pp->sD.as<float4>(), r->pp->sL.as<float4>(), r->pp->Lpath.as<float4>(), ga, bo);
             else
                 k_shade<false><<<shadeGrid, 256, 0, r->stream>>>(
                     r->view, cp, b, ctr, r->pp->rayO[cur].as<float4>(), r->pp->rayD[cur].as<float4>(), r->pp->st[cur].as<uint4>(), r->pp->beta[cur].as<float4>(),
                     r->pp->hits.as<float4>(), r->pp->rayO[1 - cur].as<float4>(), r->pp->rayD[1 - cur].as<float4>(), r->pp->st[1 - cur].as<uint4>(),
-                    r->pp->beta[1 - cur].as<float4>(), r->pp->sO.as<float4>(), r->pp->sD.as<float4>(), r->pp->sL.as<float4>(), r->pp->Lpath.as<float4>(), ga);
+                    r->pp->beta[1 - cur].as<float4>(), r->pp->sO.as<float4>(), r->pp->sD.as<float4>(), r->pp->sL.as<float4>(), r->pp->Lpath.as<float4>(), ga, bo);
             PT_HIP_TRY(hipGetLastError());
         }
         if (cp.nee) {
@@ -2024,12 +2062,15 @@ int pt_renderer_get_kernel_stats(pt_renderer *r, pt_kernel_stat *out, int cap, i
     const double npix = (double)r->W * r->H * (double)r->kstat[K_TONEMAP].launches;
     // algorithmic bytes per unit (record sizes above)
     pt_kernel_stat *k = r->kstat;
+    // camera rays (bounce 0) carry only a direction: no origin, state or throughput reads, and
+    // shade writes their paths' radiance once instead of a read-modify-write
+    const double camRays = (double)s[S_RAYS], camAdds = (double)s[S_LADD_SHADE];
     k[K_RAYGEN].units = (int64_t)paths;
-    k[K_RAYGEN].bytes = 80.0 * paths;                    // writes rayO, rayD, state, beta, Lpath
+    k[K_RAYGEN].bytes = 16.0 * paths;                    // writes rayD
     k[K_EXTEND].units = (int64_t)rays;
-    k[K_EXTEND].bytes = 48.0 * rays;                     // reads rayO, rayD; writes hit
+    k[K_EXTEND].bytes = 48.0 * rays - 16.0 * camRays;    // reads rayO (not for camera rays), rayD; writes hit
     k[K_SHADE].units = (int64_t)rays;
-    k[K_SHADE].bytes = 80.0 * rays + 64.0 * nextRays + 48.0 * shadows + 32.0 * addShade;
+    k[K_SHADE].bytes = 80.0 * rays - 48.0 * camRays + 64.0 * nextRays + 48.0 * shadows + 32.0 * (addShade - camAdds) + 16.0 * camRays;
     k[K_SHADOW].units = (int64_t)shadows;
     k[K_SHADOW].bytes = 48.0 * shadows + 32.0 * addShadow;
     k[K_ACCUM].units = (int64_t)pixelsAcc;

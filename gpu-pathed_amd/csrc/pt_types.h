@@ -87,6 +87,8 @@ __host__ __device__ __forceinline__ int ptCameraSlot(int p, int segLen) {
     const int run = p >> 6;
     return (run % PT_QUEUE_SEGS) * segLen + (run / PT_QUEUE_SEGS) * 64 + (p & 63);
 }
+// Path of camera-queue slot `off` of segment `seg` (the inverse of ptCameraSlot).
+__host__ __device__ __forceinline__ int ptCameraPath(int seg, int off) { return ((off >> 6) * PT_QUEUE_SEGS + seg) * 64 + (off & 63); }
 __host__ __device__ __forceinline__ int ptCameraCount(int seg, int n) {
     const int full = n / (64 * PT_QUEUE_SEGS);
     const int rest = n - full * 64 * PT_QUEUE_SEGS - seg * 64;
