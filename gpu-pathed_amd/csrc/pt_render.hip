@@ -57,9 +57,9 @@ static constexpr int kTraceThreads = 128;
 // 64 spp pass in one chunk, so that every bounce's queues stay long.
 static constexpr int64_t kDefaultPool = 1ll << 26;
 // Trace blocks of 128 threads per CU: 14 would fill the 7 waves per SIMD that the trace
-// kernels' 71 VGPRs and 8 KiB of LDS stack per block allow; 12 measured best on C1 (10: 3580,
-// 12: 3625, 14: 3574, 16: 3598 Mrays/s), fewer waves sharing each CU's caches.
-static constexpr int kDefaultTraceBlocks = 12;
+// kernels' 72 VGPRs and 8 KiB of LDS stack per block allow; 11 measured best on C1 (10: 4156,
+// 11: 4185, 12: 4123, 14: 4095 Mrays/s), fewer waves sharing each CU's caches.
+static constexpr int kDefaultTraceBlocks = 11;
 // Blocks per CU of the guide training's forward/backward kernel: one-wave blocks of 16.5 KiB
 // LDS and 197 registers per lane, two waves per SIMD.
 static constexpr int kTrainBlocksPerCU = 8;
@@ -2179,7 +2179,7 @@ template <class Probe>
 TuneCfg coordinateDescent(TuneCfg cur, const std::vector<int64_t> &pools, Probe probe, int *probes, double *bestRate) {
     double best = probe(cur);
     int n = 1;
-    const std::vector<int> traces = {8, 12, 14, 16}, shades = {4, 8, 16}, ldss = {16, 32};
+    const std::vector<int> traces = {8, 11, 12, 14}, shades = {4, 8, 16}, ldss = {16, 32};
     for (int dim = 0; dim < 4; dim++) {
         const size_t count = dim == 0 ? pools.size() : dim == 1 ? traces.size() : dim == 2 ? ldss.size() : shades.size();
         for (size_t k = 0; k < count; k++) {

@@ -114,7 +114,7 @@ def test_autotune_batch_and_renderer():
     t = b.autotune(2)
     say(f"batch tune {t}")
     assert t["probes"] >= 4 and t["mrays_per_s"] > 0
-    assert t["trace_blocks_per_cu"] in (8, 12, 14, 16) and t["lds_stack"] in (16, 32)
+    assert t["trace_blocks_per_cu"] in (8, 11, 12, 14) and t["lds_stack"] in (16, 32)
     assert b.get_option(P.OPT_TRACE_BLOCKS_PER_CU) == t["trace_blocks_per_cu"]
     b.run()
     ref = _single(scenes[1], *JOBS[1][3:])[0]
