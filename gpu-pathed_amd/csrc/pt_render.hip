@@ -102,6 +102,21 @@ __device__ __forceinline__ int segAppend(bool want, int *counters, int seg, int 
     const int k = waveAppend(want, counters + ptQueueField(seg, bounce, field));
     return k < 0 ? -1 : seg * segLen + k;
 }
+// Appends to each lane's own segment `seg` (the lanes of a wave may hold records of different
+// segments): one atomic per distinct segment among the appending lanes; the slot
+// seg * segLen + index, or -1 for lanes without one.
+__device__ __forceinline__ int segAppendLanes(bool want, int *counters, int seg, int bounce, int field, int segLen) {
+    int k = -1;
+    uint64_t todo = __ballot(want);
+    while (todo != 0) {
+        const int s = __shfl(seg, __ffsll((unsigned long long)todo) - 1);
+        const bool mine = want && seg == s;
+        const int r = waveAppend(mine, counters + ptQueueField(s, bounce, field));
+        if (mine) k = s * segLen + r;
+        todo &= ~__ballot(mine);
+    }
+    return k;
+}
 // Adds a per-lane count once per wave (statistics counted over a persistent wave's lifetime
 // cost one atomic instead of one per iteration).
 __device__ __forceinline__ void waveSum(int v, int *counter) {
@@ -696,15 +711,56 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
     // camO.w != 0 (bounce 0): the camera rays' origin, throughput, previous pdf, path index and
     // RNG state are recomputed here instead of read (k_raygen stores only the directions), and
     // the path's radiance is written instead of added to.
+    //
+    // Misses (a fifth to most of a queue) take a few instructions, hits the whole BSDF and light
+    // code.  A wave shades the misses of each claimed run at once and parks the hits' slots in
+    // LDS, and shades hits 64 at a time, so the long hit code runs on full waves.  Every vertex
+    // is shaded exactly once (the last partial batch when the queue is exhausted); results do
+    // not depend on the order.
     const bool camera = camO.w != 0.f;
+    __shared__ int sPendSlot[4][128], sPendSeg[4][128];   // per wave: parked hits (< 128)
+    int *pSlot = sPendSlot[threadIdx.x >> 6], *pSeg = sPendSeg[threadIdx.x >> 6];
     SegCursor cur = segStart();
     int adds = 0;
-    int base = 0, got;
-    while ((got = segClaim(counters, bounce, PT_Q_RAYS, PT_Q_FETCH_SHADE, cp.segLen, 64, cur, &base)) > 0) {
-        const int seg = cur.seg;
-        const int i = base + laneId();
+    int npend = 0;          // parked hits (wave-uniform)
+    bool claiming = true;   // the wave's segments still have rays
+    while (true) {
+        int i = 0, seg = 0;
+        bool active;
+        if (npend >= 64 || (!claiming && npend > 0)) {
+            const int nb = min(npend, 64);
+            npend -= nb;
+            active = laneId() < nb;
+            if (active) {
+                i = pSlot[npend + laneId()];
+                seg = pSeg[npend + laneId()];
+            }
+        } else if (claiming) {
+            int base = 0;
+            const int got = segClaim(counters, bounce, PT_Q_RAYS, PT_Q_FETCH_SHADE, cp.segLen, 64, cur, &base);
+            if (got == 0) {
+                claiming = false;
+                continue;
+            }
+            seg = cur.seg;
+            i = base + laneId();
+            const bool valid = laneId() < got;
+            const bool hit = valid && asInt(hits[i].w) != INT32_MIN;
+            const uint64_t hm = __ballot(hit);
+            if (hit) {
+                const int at = npend + (int)lanesBelow(hm);
+                pSlot[at] = i;
+                pSeg[at] = seg;
+            }
+            npend += __popcll(hm);
+            active = valid && !hit;
+        } else {
+            break;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
         bool addL = false;
-        if (laneId() < got) {
+        if (active) {
             const float4 rd = rayD[i], h = hits[i];
             float4 ro, bp;
             uint4 st;
@@ -821,7 +877,7 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
                             contrib = thr * (ls.emitted * w * bsdfF(m, wo, wiL) * fabsf(dot(sp.n, ls.wi)) / ls.pdf);
                             pushShadow = !isZero(contrib) && !(contrib.x != contrib.x);
                         }
-                        const int ss = segAppend(pushShadow, counters, seg, bounce, PT_Q_SHADOW, cp.segLen);
+                        const int ss = segAppendLanes(pushShadow, counters, seg, bounce, PT_Q_SHADOW, cp.segLen);
                         if (pushShadow) {
                             sO[ss] = f4(sp.p, ls.dist - 2e-4f);
                             sD[ss] = f4(ls.wi, asFloat(pathId));
@@ -834,7 +890,7 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
                         nb = thr * (bs.f * fabsf(bs.wi.z) / bs.pdf);
                         pushRay = !isZero(nb) && nb.x == nb.x;
                     }
-                    const int rs = segAppend(pushRay, counters, seg, bounce + 1, PT_Q_RAYS, cp.segLen);
+                    const int rs = segAppendLanes(pushRay, counters, seg, bounce + 1, PT_Q_RAYS, cp.segLen);
                     const f3 wworld = toWorld(fr, bs.wi);
                     if (pushRay) {
                         nRayO[rs] = f4(sp.p, kBounceEps);
@@ -847,7 +903,7 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
                         const float lt = lum(thr);
                         const bool wantRec = pushRay && G != nullptr && lt > 0.f;
                         // records go to the segment's own part of the record buffer
-                        const int slotRec = waveAppend(wantRec, &counters[ptQueueField(seg, 0, PT_Q_RECORDS)]);
+                        const int slotRec = segAppendLanes(wantRec, counters, seg, 0, PT_Q_RECORDS, 0);
                         if (pushRay) {
                             int ridx = -1;
                             if (wantRec && slotRec < ga.recSeg) {
