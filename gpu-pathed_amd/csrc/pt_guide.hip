@@ -204,11 +204,11 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void k_guide_infer(GuideNet net, 
             n = seg * segLen + min(count[seg * segStride], segLen);
             if (rowBase >= n) continue;
         }
+        int elig = 0;
         if (lane < 32) {
             const int i = rowBase + lane;
             float f[GUIDE_RAW];
             for (int k = 0; k < GUIDE_RAW; k++) f[k] = 0.f;
-            int elig = 0;
             if (i < n) {
                 if (rawFeats) {
                     for (int k = 0; k < GUIDE_RAW; k++) f[k] = rawFeats[(size_t)i * GUIDE_RAW + k];
@@ -226,6 +226,14 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void k_guide_infer(GuideNet net, 
             sElig[wave][lane] = elig;
         }
         WAVE_SYNC();
+        if (__ballot(elig != 0) == 0) {
+            // no guided vertex in the tile (misses, mirror and glass hits): only the flag that
+            // makes shade sample the BSDF alone, no network evaluation
+            const int i = rowBase + lane;
+            if (lane < 32 && i < n && !rawOut) out[(size_t)i * GUIDE_REC_F4 + 10] = make_float4(0.f, 0.f, 0.f, 0.f);
+            WAVE_SYNC();
+            continue;
+        }
         {
             // one-blob encoding: lane -> row (lane & 31), raw features 4 * (lane >> 5) .. + 3
             const int row = lane & 31;
@@ -262,6 +270,9 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void k_guide_infer(GuideNet net, 
             if (i < n) {
                 if (rawOut) {
                     for (int k = 0; k < GUIDE_OUT; k++) rawOut[(size_t)i * GUIDE_OUT + k] = sOut[wave][lane][k];
+                } else if (!sElig[wave][lane]) {
+                    // shade reads only the flag of a vertex that is not guided
+                    out[(size_t)i * GUIDE_REC_F4 + 10] = make_float4(0.f, 0.f, 0.f, 0.f);
                 } else {
                     Mixture mx;
                     mixtureFromRaw(sOut[wave][lane], mx);
@@ -269,7 +280,7 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void k_guide_infer(GuideNet net, 
                     for (int k = 0; k < GUIDE_LOBES; k++) o[k] = f4(mx.mu[k], mx.kappa[k]);
                     o[8] = make_float4(mx.pi[0], mx.pi[1], mx.pi[2], mx.pi[3]);
                     o[9] = make_float4(mx.pi[4], mx.pi[5], mx.pi[6], mx.pi[7]);
-                    o[10] = make_float4(mx.frac, (float)sElig[wave][lane], 0.f, 0.f);
+                    o[10] = make_float4(mx.frac, 1.f, 0.f, 0.f);
                     o[11] = make_float4(0.f, 0.f, 0.f, 0.f);
                 }
             }
