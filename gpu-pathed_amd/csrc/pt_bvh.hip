@@ -7,10 +7,10 @@
 //    index)
 // 4. bottom-up AABB refit (atomic arrival counter per node; the second child to arrive
 //    merges) and depth
-// 5. collapse into a 4-wide BVH of 128-byte nodes: subtrees of <= PT_LEAF_MAX primitives
-//    become leaf ranges of the sorted primitive array, and every other level of the
-//    remaining internal nodes is skipped (a node at even depth takes its grandchildren as
-//    children), which halves the number of dependent node fetches per ray
+// 5. collapse into a 4-wide BVH of 128-byte nodes (on the host, over the refitted tree):
+//    subtrees of <= PT_LEAF_MAX primitives become leaf ranges of the sorted primitive array,
+//    and each 4-wide node opens the largest-area internal children of its binary node until
+//    it has four, about halving the number of dependent node fetches per ray
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -264,77 +264,6 @@ __global__ void k_depth(const int *parentNode, const int *parentLeaf, int n, int
     atomicMax(maxDepth, d);
 }
 
-__device__ __forceinline__ bool liveNode(const int2 *ranges, int c) { return c >= 0 && ranges[c].y - ranges[c].x + 1 > PT_LEAF_MAX; }
-
-// An internal node whose subtree holds more than PT_LEAF_MAX primitives and that sits at an
-// even depth (root = 0) becomes a 4-wide node.  flag[i] = 1 for those; maxLevel receives the
-// deepest 4-wide level.
-__global__ void k_flag4(const int2 *ranges, const int *parentNode, int ni, uint32_t *flag, int *maxLevel) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ni) return;
-    int d = 0;
-    for (int p = parentNode[i]; p >= 0 && d < ni; p = parentNode[p]) d++;
-    const bool f = liveNode(ranges, i) && (d & 1) == 0;
-    flag[i] = f ? 1u : 0u;
-    if (f) atomicMax(maxLevel, d / 2 + 1);
-}
-
-__device__ __forceinline__ int encodeChild(int c, const int2 *ranges) {
-    if (c < 0) return ~((0 << PT_LEAF_COUNT_SHIFT) | (~c));
-    const int2 r = ranges[c];
-    const int cnt = r.y - r.x + 1;
-    if (cnt <= PT_LEAF_MAX) return ~(((cnt - 1) << PT_LEAF_COUNT_SHIFT) | r.x);
-    return c;
-}
-
-// One thread per flagged node: gathers up to four child slots (a child that is itself a live
-// internal node contributes its two children instead) and writes the 128-byte node at its
-// compacted index.
-__global__ void k_write4(const int2 *children, const int2 *ranges, const float *leafBoxes, const float *nodeBoxes, const uint32_t *flag,
-                         const uint32_t *index, int ni, float4 *nodes) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ni || !flag[i]) return;
-    float lo[3][4], hi[3][4];
-    int ref[4];
-    int cnt = 0;
-    auto slot = [&](int c) {
-        const float *b = c >= 0 ? &nodeBoxes[6 * c] : &leafBoxes[6 * (~c)];
-        for (int k = 0; k < 3; k++) {
-            lo[k][cnt] = b[k];
-            hi[k][cnt] = b[3 + k];
-        }
-        const int e = encodeChild(c, ranges);
-        ref[cnt] = e >= 0 ? (int)index[e] : e;
-        cnt++;
-    };
-    const int2 c = children[i];
-    const int cc[2] = {c.x, c.y};
-    for (int k = 0; k < 2; k++) {
-        if (liveNode(ranges, cc[k])) {
-            const int2 g = children[cc[k]];
-            slot(g.x);
-            slot(g.y);
-        } else {
-            slot(cc[k]);
-        }
-    }
-    for (; cnt < 4; cnt++) {
-        // an inverted box: every slab test misses it
-        for (int k = 0; k < 3; k++) {
-            lo[k][cnt] = 1e30f;
-            hi[k][cnt] = -1e30f;
-        }
-        ref[cnt] = PT_EMPTY_CHILD;
-    }
-    float4 *o = nodes + (size_t)PT_NODE_F4 * index[i];
-    for (int k = 0; k < 3; k++) {
-        o[2 * k] = make_float4(lo[k][0], lo[k][1], lo[k][2], lo[k][3]);
-        o[2 * k + 1] = make_float4(hi[k][0], hi[k][1], hi[k][2], hi[k][3]);
-    }
-    o[6] = make_float4(asFloat(ref[0]), asFloat(ref[1]), asFloat(ref[2]), asFloat(ref[3]));
-    o[7] = make_float4(0.f, 0.f, 0.f, 0.f);
-}
-
 // All primitives in one leaf below a single 4-wide root (n <= PT_LEAF_MAX).
 __global__ void k_write_single(const float *leafBoxes, int n, float4 *nodes) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -359,6 +288,90 @@ __global__ void k_fill_int(int *p, int n, int v) {
 }
 
 inline int blocks(int64_t n, int t) { return (int)((n + t - 1) / t); }
+
+// Collapses the binary tree into 4-wide nodes on the host, top down: a 4-wide node starts
+// from its binary node's two children and repeatedly opens the child of largest surface area
+// that is still a live internal node (more than PT_LEAF_MAX primitives), until it has four
+// children or none is left to open.  Subtrees of <= PT_LEAF_MAX primitives become leaf
+// ranges.  The root is node 0, and the live children of a node take consecutive indices
+// (depth-first order).  Returns the node count; *depth receives the deepest 4-wide level.
+int collapseWide(const std::vector<int2> &children, const std::vector<int2> &ranges, const std::vector<float> &nodeBoxes,
+                 const std::vector<float> &leafBoxes, std::vector<float4> &out, int *depth) {
+    auto live = [&](int c) { return c >= 0 && ranges[c].y - ranges[c].x + 1 > PT_LEAF_MAX; };
+    auto box = [&](int c) { return c >= 0 ? &nodeBoxes[6 * (size_t)c] : &leafBoxes[6 * (size_t)(~c)]; };
+    auto area = [&](int c) {
+        const float *b = box(c);
+        const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+        return dx * dy + dy * dz + dz * dx;
+    };
+    auto leafRef = [&](int c) {
+        if (c < 0) return ~(~c);
+        const int2 r = ranges[c];
+        return ~(((r.y - r.x) << PT_LEAF_COUNT_SHIFT) | r.x);
+    };
+    out.clear();
+    out.resize(PT_NODE_F4);
+    int count = 1;
+    *depth = 1;
+    struct Item {
+        int bin, idx, level;
+    };
+    std::vector<Item> stack;
+    stack.push_back({0, 0, 1});
+    while (!stack.empty()) {
+        const Item it = stack.back();
+        stack.pop_back();
+        int kids[4] = {children[it.bin].x, children[it.bin].y, 0, 0};
+        int nk = 2;
+        while (nk < 4) {
+            int best = -1;
+            float bestA = -1.f;
+            for (int k = 0; k < nk; k++)
+                if (live(kids[k]) && area(kids[k]) > bestA) {
+                    bestA = area(kids[k]);
+                    best = k;
+                }
+            if (best < 0) break;
+            const int2 g = children[kids[best]];
+            kids[best] = g.x;
+            kids[nk++] = g.y;
+        }
+        float lo[3][4], hi[3][4];
+        int ref[4];
+        for (int k = 0; k < 4; k++) {
+            if (k < nk) {
+                const float *b = box(kids[k]);
+                for (int a = 0; a < 3; a++) {
+                    lo[a][k] = b[a];
+                    hi[a][k] = b[3 + a];
+                }
+                if (live(kids[k])) {
+                    ref[k] = count++;
+                    out.resize((size_t)PT_NODE_F4 * count);
+                    stack.push_back({kids[k], ref[k], it.level + 1});
+                    *depth = std::max(*depth, it.level + 1);
+                } else {
+                    ref[k] = leafRef(kids[k]);
+                }
+            } else {
+                // an inverted box: every slab test misses it
+                for (int a = 0; a < 3; a++) {
+                    lo[a][k] = 1e30f;
+                    hi[a][k] = -1e30f;
+                }
+                ref[k] = PT_EMPTY_CHILD;
+            }
+        }
+        float4 *o = &out[(size_t)PT_NODE_F4 * it.idx];
+        for (int a = 0; a < 3; a++) {
+            o[2 * a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
+            o[2 * a + 1] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
+        }
+        o[6] = make_float4(asFloat(ref[0]), asFloat(ref[1]), asFloat(ref[2]), asFloat(ref[3]));
+        o[7] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    return count;
+}
 
 }  // namespace
 
@@ -414,7 +427,7 @@ void buildBvh(const float4 *primsIn, int n, const float lo[3], const float hi[3]
         return;
     }
     const int ni = n - 1;
-    DevBuf children, ranges, parentNode, parentLeaf, nodeBoxes, arrivals, counters, flag, index;
+    DevBuf children, ranges, parentNode, parentLeaf, nodeBoxes, arrivals, counters;
     children.alloc(sizeof(int2) * ni);
     ranges.alloc(sizeof(int2) * ni);
     parentNode.alloc(sizeof(int) * ni);
@@ -422,8 +435,6 @@ void buildBvh(const float4 *primsIn, int n, const float lo[3], const float hi[3]
     nodeBoxes.alloc(sizeof(float) * 6 * ni);
     arrivals.alloc(sizeof(int) * ni);
     counters.alloc(sizeof(int) * 2);
-    flag.alloc(sizeof(uint32_t) * ni);
-    index.alloc(sizeof(uint32_t) * ni);
     k_fill_int<<<blocks(ni, 256), 256, 0, stream>>>(parentNode.as<int>(), ni, -1);
     k_fill_int<<<blocks(ni, 256), 256, 0, stream>>>(arrivals.as<int>(), ni, 0);
     k_fill_int<<<1, 64, 0, stream>>>(counters.as<int>(), 2, 0);
@@ -431,24 +442,25 @@ void buildBvh(const float4 *primsIn, int n, const float lo[3], const float hi[3]
     k_refit<<<blocks(n, 256), 256, 0, stream>>>(children.as<int2>(), parentNode.as<int>(), parentLeaf.as<int>(), boxes2.as<float>(),
                                                  nodeBoxes.as<float>(), arrivals.as<int>(), n);
     k_depth<<<blocks(n, 256), 256, 0, stream>>>(parentNode.as<int>(), parentLeaf.as<int>(), n, counters.as<int>());
-    k_flag4<<<blocks(ni, 256), 256, 0, stream>>>(ranges.as<int2>(), parentNode.as<int>(), ni, flag.as<uint32_t>(), counters.as<int>() + 1);
-    PT_HIP_TRY(hipMemcpyAsync(index.p, flag.p, sizeof(uint32_t) * ni, hipMemcpyDeviceToDevice, stream));
-    k_sort_scan<<<1, 1024, 0, stream>>>(index.as<uint32_t>(), ni);
     PT_HIP_TRY(hipGetLastError());
-    uint32_t tail[2] = {0, 0};
+    // the collapse runs on the host over the refitted binary tree
+    std::vector<int2> hChildren(ni), hRanges(ni);
+    std::vector<float> hNodeBoxes((size_t)6 * ni), hLeafBoxes((size_t)6 * n);
     int hc[2] = {0, 0};
-    PT_HIP_TRY(hipMemcpyAsync(&tail[0], index.as<uint32_t>() + ni - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-    PT_HIP_TRY(hipMemcpyAsync(&tail[1], flag.as<uint32_t>() + ni - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    PT_HIP_TRY(hipMemcpyAsync(hChildren.data(), children.p, sizeof(int2) * ni, hipMemcpyDeviceToHost, stream));
+    PT_HIP_TRY(hipMemcpyAsync(hRanges.data(), ranges.p, sizeof(int2) * ni, hipMemcpyDeviceToHost, stream));
+    PT_HIP_TRY(hipMemcpyAsync(hNodeBoxes.data(), nodeBoxes.p, sizeof(float) * 6 * ni, hipMemcpyDeviceToHost, stream));
+    PT_HIP_TRY(hipMemcpyAsync(hLeafBoxes.data(), boxes2.p, sizeof(float) * 6 * n, hipMemcpyDeviceToHost, stream));
     PT_HIP_TRY(hipMemcpyAsync(hc, counters.p, sizeof hc, hipMemcpyDeviceToHost, stream));
     PT_HIP_TRY(hipStreamSynchronize(stream));
-    const int count = (int)(tail[0] + tail[1]);
-    nodes.alloc(sizeof(float4) * PT_NODE_F4 * (size_t)std::max(count, 1));
-    k_write4<<<blocks(ni, 256), 256, 0, stream>>>(children.as<int2>(), ranges.as<int2>(), boxes2.as<float>(), nodeBoxes.as<float>(),
-                                                  flag.as<uint32_t>(), index.as<uint32_t>(), ni, nodes.as<float4>());
-    PT_HIP_TRY(hipGetLastError());
+    std::vector<float4> hNodes;
+    int wideDepth = 1;
+    const int count = collapseWide(hChildren, hRanges, hNodeBoxes, hLeafBoxes, hNodes, &wideDepth);
+    nodes.alloc(sizeof(float4) * PT_NODE_F4 * (size_t)count);
+    PT_HIP_TRY(hipMemcpyAsync(nodes.p, hNodes.data(), sizeof(float4) * hNodes.size(), hipMemcpyHostToDevice, stream));
     PT_HIP_TRY(hipStreamSynchronize(stream));
     info->numNodes = count;
-    info->depth = hc[1];
+    info->depth = wideDepth;
     info->binaryDepth = hc[0];
 }
 
