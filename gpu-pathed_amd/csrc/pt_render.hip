@@ -535,6 +535,9 @@ __global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 7 : 5) void k_shadow(P
                                                                               const float4 *sO, const float4 *sD, const float4 *sL, float4 *Lpath,
                                                                               int refill, int *spill, int spillCap, int *overflow) {
     __shared__ int sstack[STACK * kTraceThreads];
+    // each lane's path index, kept from the ray's load (a re-read after the traversal would be a
+    // dependent round trip ahead of the radiance update)
+    __shared__ int sPath[kTraceThreads];
     const TraceStack ts = traceStack(sstack, spill, spillCap);
     int adds = 0;
     traceQueue<true, STACK>(
@@ -545,10 +548,11 @@ __global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 7 : 5) void k_shadow(P
             d = xyz(sd);
             tmin = kShadowEps;
             tmax = so.w;
+            sPath[threadIdx.x] = asInt(sd.w);
         },
         [&](int i, bool occluded, const TraceHit &) {
             if (!occluded) {
-                const int pid = asInt(sD[i].w);
+                const int pid = sPath[threadIdx.x];
                 const float4 l = sL[i];
                 float4 acc = Lpath[pid];
                 acc.x += l.x; acc.y += l.y; acc.z += l.z;
