@@ -283,7 +283,8 @@ __device__ __forceinline__ TraceHit laneRayHit(const LaneRay &r) {
 static constexpr int kTraceDone = INT32_MIN;   // LaneRay::node once the stack is exhausted
 // Searching lanes a wave leaves behind to test its parked leaves: waiting for the last few
 // lanes of a round costs more than another round (C1: 4 -> 3398, 12 -> 3413, 20 -> 3347,
-// 32 -> 3250 Mrays/s; 0, wait for every lane: 3207)
+// 32 -> 3250 Mrays/s; 0, wait for every lane: 3207; on the current tree 8 -> 4395,
+// 12 -> 4420, 16 -> 4408)
 static constexpr int kDescentTail = 12;
 
 template <bool ANY, int STACK>
