@@ -289,12 +289,16 @@ __global__ void k_fill_int(int *p, int n, int v) {
 
 inline int blocks(int64_t n, int t) { return (int)((n + t - 1) / t); }
 
-// Collapses the binary tree into 4-wide nodes on the host, top down: a 4-wide node starts
-// from its binary node's two children and repeatedly opens the child of largest surface area
-// that is still a live internal node (more than PT_LEAF_MAX primitives), until it has four
-// children or none is left to open.  Subtrees of <= PT_LEAF_MAX primitives become leaf
-// ranges.  The root is node 0, and the live children of a node take consecutive indices
-// (depth-first order).  Returns the node count; *depth receives the deepest 4-wide level.
+// Collapses the binary tree into 4-wide nodes on the host.  Subtrees of <= PT_LEAF_MAX
+// primitives become leaf ranges; which internal nodes become 4-wide nodes is chosen to
+// minimise a surface-area cost (a 4-wide node costs kNodeCost x its area, a leaf kPrimCost x
+// its area x its primitives): bottom up, every internal node gets the cheapest way to spend
+// 1..4 child slots of its parent on its subtree (one slot: the node itself as a 4-wide node;
+// more: its two children share them), then the nodes are written top down from the root.
+// The root is node 0, and the live children of a node take consecutive indices (depth-first
+// order).  Returns the node count; *depth receives the deepest 4-wide level.
+constexpr float kNodeCost = 1.0f;
+constexpr float kPrimCost = 0.6f;
 int collapseWide(const std::vector<int2> &children, const std::vector<int2> &ranges, const std::vector<float> &nodeBoxes,
                  const std::vector<float> &leafBoxes, std::vector<float4> &out, int *depth) {
     auto live = [&](int c) { return c >= 0 && ranges[c].y - ranges[c].x + 1 > PT_LEAF_MAX; };
@@ -309,6 +313,60 @@ int collapseWide(const std::vector<int2> &children, const std::vector<int2> &ran
         const int2 r = ranges[c];
         return ~(((r.y - r.x) << PT_LEAF_COUNT_SHIFT) | r.x);
     };
+    // bottom-up costs over the live internal nodes (children before parents)
+    const size_t ni = children.size();
+    std::vector<float> W(ni), S2(ni), S3(ni);   // as a 4-wide node; spread over 2 / 3 slots
+    std::vector<uint8_t> split(3 * ni);         // slots of the left child when the children share 2..4
+    std::vector<uint8_t> opened(ni);            // bit k: spreading over k + 2 slots opens the node
+    auto cost = [&](int x, int slots) -> float {
+        if (!live(x)) {
+            const int cnt = x < 0 ? 1 : ranges[x].y - ranges[x].x + 1;
+            return kPrimCost * area(x) * (float)cnt;
+        }
+        return slots == 1 ? W[x] : slots == 2 ? S2[x] : S3[x];
+    };
+    std::vector<int> order, todo;
+    todo.push_back(0);
+    while (!todo.empty()) {
+        const int x = todo.back();
+        todo.pop_back();
+        order.push_back(x);
+        if (live(children[x].x)) todo.push_back(children[x].x);
+        if (live(children[x].y)) todo.push_back(children[x].y);
+    }
+    for (size_t k = order.size(); k-- > 0;) {
+        const int x = order[k];
+        const int l = children[x].x, r = children[x].y;
+        float share[5] = {0.f, 0.f, 0.f, 0.f, 0.f};   // both children in j slots
+        for (int j = 2; j <= 4; j++) {
+            float best = INFINITY;
+            int bi = 1;
+            for (int i = 1; i < j; i++) {
+                const float c = cost(l, std::min(i, 3)) + cost(r, std::min(j - i, 3));
+                if (c < best) {
+                    best = c;
+                    bi = i;
+                }
+            }
+            share[j] = best;
+            split[3 * x + (j - 2)] = (uint8_t)bi;
+        }
+        W[x] = kNodeCost * area(x) + share[4];
+        S2[x] = std::min(W[x], share[2]);
+        S3[x] = std::min(W[x], share[3]);
+        opened[x] = (uint8_t)((share[2] < W[x] ? 1 : 0) | (share[3] < W[x] ? 2 : 0));
+    }
+    // the children of a 4-wide node: its binary node's two children sharing 4 slots, opened
+    // where the cost tables say so
+    auto take = [&](auto &&self, int x, int slots, int *kids, int &nk) -> void {
+        if (!live(x) || slots == 1 || !(opened[x] & (1 << (slots - 2)))) {
+            kids[nk++] = x;
+            return;
+        }
+        const int i = split[3 * x + (slots - 2)];
+        self(self, children[x].x, i, kids, nk);
+        self(self, children[x].y, slots - i, kids, nk);
+    };
     out.clear();
     out.resize(PT_NODE_F4);
     int count = 1;
@@ -321,21 +379,11 @@ int collapseWide(const std::vector<int2> &children, const std::vector<int2> &ran
     while (!stack.empty()) {
         const Item it = stack.back();
         stack.pop_back();
-        int kids[4] = {children[it.bin].x, children[it.bin].y, 0, 0};
-        int nk = 2;
-        while (nk < 4) {
-            int best = -1;
-            float bestA = -1.f;
-            for (int k = 0; k < nk; k++)
-                if (live(kids[k]) && area(kids[k]) > bestA) {
-                    bestA = area(kids[k]);
-                    best = k;
-                }
-            if (best < 0) break;
-            const int2 g = children[kids[best]];
-            kids[best] = g.x;
-            kids[nk++] = g.y;
-        }
+        int kids[4] = {0, 0, 0, 0};
+        int nk = 0;
+        const int i4 = split[3 * it.bin + 2];
+        take(take, children[it.bin].x, i4, kids, nk);
+        take(take, children[it.bin].y, 4 - i4, kids, nk);
         float lo[3][4], hi[3][4];
         int ref[4];
         for (int k = 0; k < 4; k++) {
