@@ -396,13 +396,17 @@ struct Ray {
 };
 
 // Watertight ray/triangle intersection (Woop, Benthin, Wald 2013), two-sided, edges
-// inclusive, FP64 fallback on exactly-zero edge functions.  Precomputed per ray.
+// inclusive, FP64 fallback on exactly-zero edge functions.  Precomputed per ray.  Neither
+// function lets the compiler fuse a product into an add: the edge functions of two triangles
+// sharing an edge must be exact negatives of each other, which rounding each product on its
+// own guarantees and a fused multiply-add does not.
 struct WtRay {
     f3 o;
     int kx, ky, kz;
     float Sx, Sy, Sz;
 };
 PT_INLINE WtRay wtSetup(f3 o, f3 d) {
+#pragma clang fp contract(off)
     WtRay r;
     r.o = o;
     const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
@@ -421,6 +425,7 @@ PT_INLINE WtRay wtSetup(f3 o, f3 d) {
     return r;
 }
 PT_INLINE bool wtHit(const WtRay &r, f3 p0, f3 p1, f3 p2, float tMin, float tMax, float *tt, float *uu, float *vv) {
+#pragma clang fp contract(off)
     const f3 A = p0 - r.o, B = p1 - r.o, C = p2 - r.o;
     const float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
     const float Ax = comp(A, r.kx) - r.Sx * Akz, Ay = comp(A, r.ky) - r.Sy * Akz;
@@ -464,5 +469,15 @@ PT_INLINE bool sphereHit(f3 c, float rad, f3 o, f3 d, float tMin, float tMax, fl
 }
 
 PT_INLINE float balance(float p1, float p2) { return p1 / (p1 + p2); }  // mis.h:5-8
+
+// color.h:7-28 sRGB transfer and framebuffer.cu:35-40's byte conversion (clamp to [0, 1],
+// times 255, truncated).  The power is evaluated in FP64 and rounded once to float (a
+// correctly rounded powf), and no product is fused into an add, so the byte is a function of
+// the radiance value alone, the same on every device and in the oracle.
+PT_INLINE unsigned char srgbByte(float x) {
+#pragma clang fp contract(off)
+    const float s = x <= 0.0031308f ? x * 12.92f : 1.055f * (float)pow((double)x, (double)(1.f / 2.2f)) - 0.055f;
+    return (unsigned char)(fmaxf(0.f, fminf(1.f, s)) * 255.f);
+}
 
 }  // namespace pt

@@ -1020,14 +1020,11 @@ __global__ __launch_bounds__(256) void k_accumulate(ChunkParams cp, const float4
 __global__ __launch_bounds__(256) void k_tonemap(const double *accum, int npix, double invSpp, float *radiance, uchar4 *rgba) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npix) return;
-    float v[3];
     unsigned char o[3];
     for (int k = 0; k < 3; k++) {
-        v[k] = (float)(accum[3 * (size_t)p + k] * invSpp);
-        radiance[3 * (size_t)p + k] = v[k];
-        const float x = v[k];
-        const float s = x <= 0.0031308f ? x * 12.92f : 1.055f * powf(x, 1.f / 2.2f) - 0.055f;
-        o[k] = (unsigned char)(fmaxf(0.f, fminf(1.f, s)) * 255.f);
+        const float v = (float)(accum[3 * (size_t)p + k] * invSpp);
+        radiance[3 * (size_t)p + k] = v;
+        o[k] = srgbByte(v);
     }
     rgba[p] = make_uchar4(o[0], o[1], o[2], 255);
 }
@@ -1254,11 +1251,12 @@ void uploadEmitters(pt_renderer *r) {
 }
 
 // Path pool sized to the work at hand: `need` paths rounded up to a power of two, at least
-// 4096, at most the configured capacity; it grows, it does not shrink.
+// 4096, at most the configured capacity, then rounded up to a multiple of PT_QUEUE_SEGS * 64;
+// it grows, it does not shrink.  Every per-slot buffer holds `pool` entries, and a chunk of n
+// <= pool paths addresses queue slots below ptQueueSlots(n) <= pool only because pool is such a
+// multiple (a segment is a multiple of 64 slots long).
 void ensurePool(pt_renderer *r, int64_t need = 4096) {
-    int64_t want = 4096;
-    while (want < need) want <<= 1;
-    want = std::min<int64_t>(want, std::max<int64_t>(4096, r->poolWanted));
+    const int64_t want = ptPoolCapacity(need, r->poolWanted);
     if (r->pp->pool >= want) return;
     const size_t n = (size_t)want;
     for (int k = 0; k < 2; k++) {
@@ -2166,6 +2164,38 @@ int pt_renderer_get_path_radiance(pt_renderer *r, float *out, int64_t cap, int64
     *count = n;
     return PT_OK;
     PT_API_END("pt_renderer_get_path_radiance")
+}
+
+int pt_pool_layout(int width, int height, int spp, int64_t pool_option, int64_t *capacity, int64_t *chunks, int64_t *max_slot) {
+    if (width <= 0 || height <= 0 || spp <= 0 || pool_option > (1ll << 27) || !capacity || !max_slot)
+        return fail(PT_ERR_INVALID, "pt_pool_layout: bad arguments");
+    if ((int64_t)width * height > (1ll << 28)) return fail(PT_ERR_INVALID, "pt_pool_layout: image too large");
+    PT_API_BEGIN
+    // a renderer object without device state: the pass plan is the one render_async makes
+    pt_renderer r;
+    r.W = width;
+    r.H = height;
+    r.poolWanted = pool_option > 0 ? pool_option : kDefaultPool;
+    r.pp->pool = ptPoolCapacity((int64_t)width * height * spp, r.poolWanted);
+    std::vector<ChunkParams> plan;
+    planChunks(&r, spp, plan);
+    int64_t hi = -1;
+    int seen = -1;
+    for (const ChunkParams &cp : plan) {
+        if (cp.nPaths == seen) continue;  // chunks of one size share their layout
+        seen = cp.nPaths;
+        // camera queue: each segment's last run lies among the chunk's last PT_QUEUE_SEGS runs
+        for (int p = std::max(0, cp.nPaths - 64 * PT_QUEUE_SEGS); p < cp.nPaths; p++) hi = std::max<int64_t>(hi, ptCameraSlot(p, cp.segLen));
+        // linear queues fill the segments in order; appended queues (continuation, shadow rays,
+        // guided records) stay inside the segment of the vertex that spawned them
+        hi = std::max<int64_t>(hi, ptQueueSlots(cp.nPaths) - 1);
+    }
+    *capacity = r.pp->pool;
+    if (chunks) *chunks = (int64_t)plan.size();
+    *max_slot = hi;
+    r.pp->pool = 0;
+    return PT_OK;
+    PT_API_END("pt_pool_layout")
 }
 
 int pt_renderer_intersect(pt_renderer *r, const float *rays, int64_t n, int any_hit, float *out) {

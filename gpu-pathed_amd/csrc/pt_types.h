@@ -96,6 +96,18 @@ __host__ __device__ __forceinline__ int ptCameraCount(int seg, int n) {
 }
 // Slots a pool of `paths` records needs for any queue laid out in segments.
 __host__ __device__ __forceinline__ int64_t ptQueueSlots(int64_t paths) { return (int64_t)PT_QUEUE_SEGS * ptSegLen(paths); }
+// Path capacity allocated for `need` paths under a configured capacity of `wanted` paths:
+// `need` rounded up to a power of two (at least 4096), capped at `wanted`, and rounded up to a
+// multiple of PT_QUEUE_SEGS * 64, so that for every chunk of n <= capacity paths
+// ptQueueSlots(n) <= capacity: per-slot buffers of `capacity` entries hold every queue.
+__host__ __device__ __forceinline__ int64_t ptPoolCapacity(int64_t need, int64_t wanted) {
+    int64_t want = 4096;
+    while (want < need) want <<= 1;
+    const int64_t cap = wanted > 4096 ? wanted : 4096;
+    if (want > cap) want = cap;
+    const int64_t unit = (int64_t)PT_QUEUE_SEGS * 64;
+    return (want + unit - 1) / unit * unit;
+}
 
 struct PtSceneView {
     const float4 *nodes;      // PT_NODE_F4 per 4-wide node

@@ -125,6 +125,13 @@ __global__ void k_vmf_sample(const float *mu, const float *kappa, const float *x
     pdf[i] = vmfDensity(m, kappa[i], w);
 }
 
+// framebuffer.cu:35-40 bytes of a radiance buffer, through the renderer's device function
+__global__ void k_tonemap_bytes(const float *radiance, int64_t n, uchar4 *rgba) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    rgba[i] = make_uchar4(srgbByte(radiance[3 * i]), srgbByte(radiance[3 * i + 1]), srgbByte(radiance[3 * i + 2]), 255);
+}
+
 // Small RAII helper for the unit calls: device copies of host arrays on a private stream.
 struct Stage {
     hipStream_t s = nullptr;
@@ -283,6 +290,20 @@ int pt_unit_path_uniforms(uint64_t seed, const uint32_t *pixel, const uint32_t *
     st.sync();
     return PT_OK;
     PT_UNIT_END("pt_unit_path_uniforms")
+}
+
+int pt_unit_tonemap(const float *radiance, int64_t n, uint8_t *rgba) {
+    if (n < 0 || (n > 0 && (!radiance || !rgba))) return fail(PT_ERR_INVALID, "pt_unit_tonemap: bad arguments");
+    PT_UNIT_BEGIN("pt_unit_tonemap")
+    Stage st;
+    const float *dr = st.in(radiance, (size_t)n * 3);
+    uchar4 *dout = st.out<uchar4>((size_t)n);
+    k_tonemap_bytes<<<nblk(n), 256, 0, st.s>>>(dr, n, dout);
+    PT_HIP_TRY(hipGetLastError());
+    st.back((uchar4 *)rgba, dout, (size_t)n);
+    st.sync();
+    return PT_OK;
+    PT_UNIT_END("pt_unit_tonemap")
 }
 
 int pt_unit_vmf_sample(const float *mu, const float *kappa, const float *xi, int64_t n, float *dir, float *pdf) {

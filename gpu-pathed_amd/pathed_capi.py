@@ -32,7 +32,7 @@ EXPORTS = [
     "pt_renderer_get_path_radiance", "pt_renderer_intersect",
     "pt_unit_triangle_test", "pt_unit_bsdf_eval", "pt_unit_bsdf_sample", "pt_unit_env_tables", "pt_unit_env_sample",
     "pt_unit_path_uniforms", "pt_unit_guide_forward", "pt_unit_guide_gradients", "pt_unit_guide_adam", "pt_unit_mfma_tile",
-    "pt_unit_vmf_sample",
+    "pt_unit_vmf_sample", "pt_unit_tonemap", "pt_pool_layout",
     "pt_scene_load_obj", "pt_scene_load_ply", "pt_scene_load_exr_environment", "pt_scene_reference", "pt_exr_info",
     "pt_exr_read_rgba", "pt_exr_write_rgb",
     "pt_renderer_autotune", "pt_batch_create", "pt_batch_destroy", "pt_batch_add", "pt_batch_set_option", "pt_batch_get_option",
@@ -166,6 +166,9 @@ def lib():
         L.pt_unit_guide_adam.argtypes = [vp, vp, vp, vp, C.c_int, C.c_float]
         L.pt_unit_mfma_tile.argtypes = [vp, vp, vp]
         L.pt_unit_vmf_sample.argtypes = [vp, vp, vp, C.c_int64, vp, vp]
+        L.pt_unit_tonemap.argtypes = [vp, C.c_int64, vp]
+        L.pt_pool_layout.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                     C.POINTER(C.c_int64)]
     return _lib
 
 
@@ -572,6 +575,21 @@ def vmf_sample(mu, kappa, xi):
     p = np.zeros(k.size, np.float32)
     check(lib().pt_unit_vmf_sample(_p(m), _p(k), _p(x), k.size, _p(d), _p(p)))
     return d, p
+
+
+def tonemap(radiance):
+    """framebuffer bytes (n, 4) of radiance triples (n, 3) through the renderer's GPU tonemap."""
+    r = _f32(radiance).reshape(-1, 3)
+    out = np.zeros((r.shape[0], 4), np.uint8)
+    check(lib().pt_unit_tonemap(_p(r), r.shape[0], _p(out)))
+    return out
+
+
+def pool_layout(width, height, spp, pool_option=0):
+    """(capacity, chunks, max_slot) of a pass's path pool; host only."""
+    cap, chunks, hi = C.c_int64(), C.c_int64(), C.c_int64()
+    check(lib().pt_pool_layout(width, height, spp, pool_option, C.byref(cap), C.byref(chunks), C.byref(hi)))
+    return cap.value, chunks.value, hi.value
 
 
 def exr_read(path):

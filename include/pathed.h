@@ -247,10 +247,24 @@ int pt_renderer_get_path_radiance(pt_renderer *r, float *out, int64_t capacity, 
 /* Unit-level entry points (parity harness).  Host pointers in and out.                  */
 /* ------------------------------------------------------------------------------------ */
 /* Closest hit (any_hit = 0) or occlusion (any_hit = 1) for n rays (8 floats each:
- * o.xyz tmin d.xyz tmax) through the renderer's BVH (watertight triangle test).
- * out[4n]: t, u, v, primitive (int bits: >= 0 triangle index, < 0 ~sphere index,
- * INT32_MIN miss) -- the indexing of World::hit's PrimitiveIndex (world.cu:5-41). */
+ * o.xyz tmin d.xyz tmax) through the renderer's BVH (watertight triangle test), traced by
+ * the renderer's own extend and shadow kernels.
+ * any_hit = 0: over [tmin, tmax]; out[4n]: t, u, v, primitive (int bits: >= 0 triangle index,
+ * < 0 ~sphere index, INT32_MIN miss) -- the indexing of World::hit's PrimitiveIndex
+ * (world.cu:5-41); t = -1 on a miss.
+ * any_hit = 1: the shadow-ray query of directSampleLights (path_tracer.cu:104-110), over
+ * [1e-4, tmax] whatever tmin says; out[4n]: occluded rays 0, 0, 0, primitive 0; unoccluded
+ * rays -1, 0, 0, INT32_MIN (which occluder stopped the ray is not reported). */
 int pt_renderer_intersect(pt_renderer *r, const float *rays, int64_t n, int any_hit, float *out);
+
+/* Path-pool layout of a pass (host only, no device needed): for a width x height image at `spp`
+ * samples per pixel under PT_OPT_POOL_PATHS = pool_option (<= 0: the default), *capacity = the
+ * paths the renderer allocates per SoA buffer, *chunks = the pass's chunk count and *max_slot =
+ * the highest queue slot any chunk addresses (camera, linear and appended queues).
+ * max_slot < capacity for every input.  No reference counterpart (the reference allocates one
+ * record per pixel, path_tracer.cu:266-283). */
+int pt_pool_layout(int width, int height, int spp, int64_t pool_option, int64_t *capacity, int64_t *chunks,
+                   int64_t *max_slot);
 
 /* Single ray/triangle tests on the GPU (triangle.cu:7-48): tris[9n], rays[8n] -> out[4n]
  * (hit flag, t, u, v).  mode 0 = Moeller-Trumbore in the reference's operation order,
@@ -276,6 +290,10 @@ int pt_unit_env_tables(const float *rgba, int width, int height, int sin_theta_w
 int pt_unit_env_sample(const float *rgba, int width, int height, int sin_theta_weight,
                        const float *xi, int64_t n, float *dir, float *pdf, float *emit,
                        const float *pdf_dirs, int64_t m, float *pdf_out);
+
+/* The framebuffer bytes of framebuffer.cu:35-40 / color.h:7-28 for n radiance triples
+ * (radiance[3n]) through the renderer's tonemap device function: rgba[4n] (alpha 255). */
+int pt_unit_tonemap(const float *radiance, int64_t n, uint8_t *rgba);
 
 /* PCG32 path streams on the GPU: first k uniforms of (seed, pixel, sample) for n queries. */
 int pt_unit_path_uniforms(uint64_t seed, const uint32_t *pixel, const uint32_t *sample, int64_t n,

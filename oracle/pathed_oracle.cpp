@@ -1615,7 +1615,9 @@ int or_framebuffer_update(const float *pass, float *mean, int64_t npix, int pass
                 v = p;
             }
             mean[i * 3 + c] = v;
-            const float s = v <= 0.0031308f ? v * 12.92f : 1.055f * std::pow(v, 1.f / 2.2f) - 0.055f;
+            // color.h:7-28 with the float power correctly rounded (evaluated in FP64, rounded
+            // once), so the bytes are a function of the radiance alone
+            const float s = v <= 0.0031308f ? v * 12.92f : 1.055f * (float)std::pow((double)v, (double)(1.f / 2.2f)) - 0.055f;
             rgba[i * 4 + c] = (uint8_t)(std::max(0.f, std::min(1.f, s)) * 255);
         }
         rgba[i * 4 + 3] = 255;

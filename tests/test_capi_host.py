@@ -29,6 +29,32 @@ def test_every_declared_symbol_is_exported():
     assert sorted(P.EXPORTS) == names
 
 
+@pytest.mark.parametrize("pool", [0, 4096, 5000, 1_000_000, 1_003_455, (1 << 20) + 1, 3 * 4096 + 64, 123_457])
+def test_pool_holds_every_queue_slot(pool):
+    """Every queue slot a pass's chunks address (camera runs dealt to 64 segments, linear
+    queues, appended continuation / shadow / guided-record queues) lies inside the per-slot
+    buffers the renderer allocates, for any pool setting: image sizes that fill a chunk
+    exactly, leave a ragged last chunk, need one chunk per pixel run, or one pixel."""
+    rng = np.random.default_rng(pool)
+    shapes = [(1, 1, 1), (1024, 1024, 1), (1024, 1024, 64), (1000, 1000, 3), (777, 333, 5), (64, 64, 4), (4096, 4096, 1)]
+    shapes += [(int(rng.integers(1, 3000)), int(rng.integers(1, 3000)), int(rng.integers(1, 300))) for _ in range(20)]
+    for w, h, spp in shapes:
+        cap, chunks, hi = P.pool_layout(w, h, spp, pool)
+        assert hi < cap, (w, h, spp, pool, cap, hi)
+        assert cap % (64 * 64) == 0 and chunks >= 1
+        assert cap >= min(w * h * spp, 4096)
+        if pool:
+            # the configured capacity is honoured up to the segment rounding
+            assert cap < max(pool, 4096) + 64 * 64
+
+
+def test_pool_layout_rejects_bad_sizes():
+    with pytest.raises(P.PtError):
+        P.pool_layout(0, 10, 1)
+    with pytest.raises(P.PtError):
+        P.pool_layout(10, 10, 1, (1 << 27) + 1)
+
+
 def test_abi_version_and_errors():
     L = P.lib()
     assert L.pt_abi_version() == 1

@@ -9,6 +9,7 @@ import pytest
 
 import oracle as O
 import pathed_capi as P
+from helpers import srgb_sweep
 
 pytestmark = pytest.mark.gpu
 
@@ -201,16 +202,74 @@ def test_bvh_closest_hit_matches_brute_force():
     gt, gu, gv, gprim = r.intersect(rays)
     ct, cu, cv, cprim = o.intersect(rays, tri_test=1)
     agree = gprim == cprim
-    assert agree.mean() > 0.9995, agree.mean()
+    say(f"closest hit: {agree.mean():.6f} of rays name the same primitive")
+    # hit or miss is the same for every ray, and where the primitive differs the two are a
+    # tie: both hit at the same distance (primitives sharing an edge or overlapping)
+    assert np.array_equal(gprim == P.MISS, cprim == P.MISS)
+    tie = ~agree
+    assert np.all(np.abs(gt[tie] - ct[tie]) <= 1e-6 * np.abs(ct[tie])), (gt[tie], ct[tie])
+    assert tie.sum() <= 2
     hit = agree & (gprim != P.MISS)
     assert np.allclose(gt[hit], ct[hit], rtol=1e-5, atol=2e-5)
-    # occlusion queries agree with closest-hit existence
+    # occlusion over [1e-4, tmax] agrees with closest-hit existence (no ray here has its only
+    # hit closer than 1e-4)
     _, _, _, occ = r.intersect(rays, any_hit=True)
     vis_ref = cprim == P.MISS
-    assert ((occ == P.MISS) == vis_ref).mean() > 0.999
+    assert np.array_equal(occ == P.MISS, vis_ref)
     st = r.stats()
     # a real 4-wide tree: 20k primitives in leaves of <= 4 need > 1250 leaf slots
     assert st.stack_overflows == 0 and st.bvh_depth > 5 and st.bvh_nodes > 400
+    r.close()
+
+
+def _vertex_and_edge_rays(tris, origin):
+    """Rays from `origin` through every vertex and every edge midpoint of a mesh."""
+    t = tris.reshape(-1, 3, 3)
+    verts = np.unique(t.reshape(-1, 3), axis=0)
+    mids = np.concatenate([(t[:, 0] + t[:, 1]) / 2, (t[:, 1] + t[:, 2]) / 2, (t[:, 2] + t[:, 0]) / 2]).astype(np.float32)
+    tgt = np.concatenate([verts, mids]).astype(np.float32)
+    o = np.broadcast_to(np.asarray(origin, np.float32), tgt.shape)
+    d = (tgt - o).astype(np.float32)
+    n = len(d)
+    return np.concatenate([o, np.zeros((n, 1)), d, np.full((n, 1), 3e38)], 1).astype(np.float32)
+
+
+@pytest.mark.parametrize("level", [4, 8])
+def test_watertight_no_leaks_at_shared_edges(level):
+    """The displaced icosphere is closed and star-shaped around the origin: every ray from an
+    inside point hits it, also a ray aimed exactly at a shared vertex or edge.  Moeller-
+    Trumbore lets about 11% of such rays through between the triangles (the oracle shows it
+    below); the product's traversal (watertight test, padded boxes) must let none through,
+    and from outside every ray aimed at the centre through a vertex or edge hits too."""
+    say(f"case: watertight shared edges, icosphere level {level}")
+    s = P.Scene.preset(1, seed=3, param=level)
+    r = P.Renderer(8, 8)
+    r.set_scene(s)
+    tris = s.export()["tri_pos"]
+    rng = np.random.default_rng(level)
+    if len(tris) > 20000:
+        tris = tris[rng.choice(len(tris), 20000, replace=False)]
+    for origin in ([0, 0, 0], [0.013, -0.021, 0.007]):
+        rays = _vertex_and_edge_rays(tris, origin)
+        t, _, _, prim = r.intersect(rays)
+        leaks = int((prim == P.MISS).sum())
+        say(f"origin {origin}: {leaks} of {len(rays)} vertex/edge rays escape")
+        assert leaks == 0
+        assert np.all(t > 0)
+        _, _, _, occ = r.intersect(rays, any_hit=True)
+        assert int((occ == P.MISS).sum()) == 0
+    # from outside, aimed at the centre through the same points
+    inward = _vertex_and_edge_rays(tris, [0, 0, 0])
+    inward[:, 0:3] = inward[:, 4:7] * 3.0
+    inward[:, 4:7] = -inward[:, 4:7]
+    _, _, _, prim = r.intersect(inward)
+    assert int((prim == P.MISS).sum()) == 0
+    if level == 4:
+        # the same rays through the oracle's Moeller-Trumbore test do leak: the test has teeth
+        o = O.Scene(s.export(), 8, 8, bvh=False)
+        _, _, _, mprim = o.intersect(_vertex_and_edge_rays(tris, [0, 0, 0]), tri_test=0)
+        assert (mprim == P.MISS).mean() > 0.01
+    assert r.stats().stack_overflows == 0
     r.close()
 
 
@@ -244,6 +303,37 @@ def test_trace_schedule_does_not_change_results():
     with pytest.raises(P.PtError):
         r.set_option(P.OPT_TRACE_REFILL_LANES, 65)
     assert r.stats().stack_overflows == 0
+    r.close()
+
+
+# --------------------------------------------------------------------------- framebuffer
+def test_tonemap_bytes_bit_exact():
+    """framebuffer.cu:35-40 / color.h:7-28 bytes: the GPU tonemap equals the oracle bit for bit
+    on a sweep of radiance values that covers every byte boundary from both sides, the segment
+    switch at 0.0031308 and the special values."""
+    say("case: tonemap bytes")
+    v = srgb_sweep()
+    rgb = np.stack([v, np.roll(v, 1), np.roll(v, 2)], 1)
+    g = P.tonemap(rgb)
+    _, c = O.framebuffer_update(rgb.reshape(-1), np.zeros(rgb.size, np.float32), 1, 0)
+    c = c.reshape(-1, 4)
+    bad = np.where(np.any(g != c, axis=1))[0]
+    assert bad.size == 0, (rgb[bad[:5]], g[bad[:5]], c[bad[:5]])
+
+
+def test_rendered_framebuffer_bytes_equal_oracle_bytes():
+    """The renderer's RGBA8 framebuffer after several passes is the oracle's byte conversion of
+    the renderer's own mean radiance, bit for bit."""
+    say("case: rendered framebuffer bytes")
+    s, r = _cornell(96, 64)
+    r.set_max_depth(5)
+    for spp in (1, 3, 4):
+        r.render(spp)
+        rad = r.radiance()
+        fb = r.rgba8()
+        _, c = O.framebuffer_update(rad.reshape(-1), np.zeros(rad.size, np.float32), 1, 0)
+        assert np.array_equal(fb.reshape(-1, 4), c.reshape(-1, 4))
+        assert len(np.unique(fb[..., :3])) > 50
     r.close()
 
 

@@ -1,6 +1,7 @@
-"""The reference's own scene files on the GPU: the Cornell box OBJ (scene 0) and the glossy
-Cornell box with a mirror box and a glass ball (scene 1), loaded from the committed copies of
-its scenes/ files, against the oracle on the same arrays; and the session's EXR dump at
+"""The reference's own scene files on the GPU: the Cornell box OBJ (scene 0), the glossy
+Cornell box with a mirror box and a glass ball (scene 1) and the Veach MIS plates with five
+sphere lights and the wells6 environment (scene 3), loaded from the committed copies of its
+scenes/ files, against the oracle on the same arrays; and the session's EXR dump at
 power-of-two sample counts."""
 import os
 import sys
@@ -27,7 +28,7 @@ def need_gpu():
         pytest.fail("no HIP device visible: the GPU tests need an MI355X")
 
 
-@pytest.mark.parametrize("index", [0, 1])
+@pytest.mark.parametrize("index", [0, 1, 3])
 def test_reference_scene_per_path_parity(index):
     say(f"case: reference scene {index} per-path parity (64x64x4, depth 6)")
     W = H = 64

@@ -1,7 +1,7 @@
 """Scene and image files (host code of libpathed; no GPU needed).
 
 Pins: the reference's PLY known answer (test/test_ply_parser.cpp:11-22), the reference
-scenes' own OBJ/MTL files (fixtures under tests/golden/scenes, copied data), OpenEXR
+scenes' own OBJ/MTL/PLY/EXR files (fixtures under tests/golden/scenes, copied data), OpenEXR
 round trips, IEEE half rounding against numpy's float16, and a ZIP/ZIPS-compressed file
 written by an independent encoder in this test (zlib over the byte-interleaved,
 delta-predicted block of the OpenEXR format description).
@@ -18,7 +18,6 @@ import pathed_capi as P
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = os.path.join(HERE, "golden")
 SCENES = os.path.join(GOLD, "scenes")
-REFERENCE_SCENES = "/root/reference/scenes"
 
 
 def test_ply_known_answer():
@@ -97,8 +96,27 @@ def test_reference_scenes_0_and_1():
     a = s1.export()
     assert a["mat_type"][2] == P.GLASS and a["mat_ior"][2] == pytest.approx(1.4)
     assert a["sph"][0, 3] == pytest.approx(0.45)
-    with pytest.raises(P.PtError):
-        P.Scene.reference(3, SCENES)  # the MIS plates are not among the fixtures
+
+
+def test_reference_scene_3_mis_plates():
+    """scene.cu:95-167: the floor and four Beckmann plates (PLY, alpha 0.005 .. 0.1), five
+    emissive spheres in the reference's order, the wells6 environment (1000 x 500) and the
+    tilted, handedness-flipped camera."""
+    s = P.Scene.reference(3, SCENES)
+    i = s.info()
+    assert (i.n_triangles, i.n_spheres, i.n_materials, i.n_emitters) == (12, 5, 10, 5)
+    assert (i.env_width, i.env_height) == (1000, 500) and i.camera_flip == 1
+    assert np.allclose(list(i.camera_origin), [0, 2, 15]) and i.camera_vfov == pytest.approx(np.radians(28.0000262073138))
+    a = s.export()
+    assert a["tri_mat"].tolist() == [0, 0, 0, 0, 1, 1, 2, 2, 3, 3, 4, 4]
+    assert a["mat_type"][:5].tolist() == [P.LAMBERTIAN] + [P.MICROFACET] * 4
+    assert np.allclose(a["mat_alpha"][1:5], [0.005, 0.02, 0.05, 0.1])
+    assert np.allclose(a["mat_emit"][5:, 0], [800, 100, 901.803, 11.1111, 1.234567])
+    assert np.allclose(a["sph"][:, 3], [0.5, 0.1, 0.03333, 0.3, 0.9])
+    assert np.allclose(a["sph"][:, :3], [[10, 10, 4], [-1.25, 0, 0], [-3.75, 0, 0], [1.25, 0, 0], [3.75, 0, 0]])
+    assert (a["sph_mat"] == np.arange(5, 10)).all()
+    env = a["env_rgba"]
+    assert np.isfinite(env).all() and env[..., :3].min() >= 0 and env[..., :3].max() > 1
 
 
 def test_exr_float_round_trip(tmp_path):
@@ -201,12 +219,11 @@ def test_exr_environment_and_errors(tmp_path):
         P.exr_read(str(bad))
 
 
-@pytest.mark.skipif(not os.path.isdir(REFERENCE_SCENES), reason="the reference's scene assets are not mounted here")
 def test_reference_environment_assets():
     """The reference's own environment maps: wells6 (ZIP, FLOAT A/B/G/R, 1000x500) loads;
     the 1-pixel test map uses PIZ, which this reader reports as unsupported."""
-    img = P.exr_read(os.path.join(REFERENCE_SCENES, "assets", "20060807_wells6_hd.exr"))
+    img = P.exr_read(os.path.join(SCENES, "assets", "20060807_wells6_hd.exr"))
     assert img.shape == (500, 1000, 4)
     assert np.isfinite(img).all() and (img[..., :3] >= 0).all() and img[..., :3].max() > 1
     with pytest.raises(P.PtError, match="compression 4"):
-        P.exr_read(os.path.join(REFERENCE_SCENES, "assets", "1_pixel_test.exr"))
+        P.exr_read(os.path.join(SCENES, "assets", "1_pixel_test.exr"))

@@ -8,6 +8,7 @@ import pytest
 from scipy import stats
 
 import oracle as O
+from helpers import sphere_light_scene, srgb_sweep
 
 LAMB, MIRROR, GLASS, MICRO = 0, 1, 2, 3
 
@@ -502,6 +503,59 @@ def test_framebuffer_running_mean():
     v = mean.reshape(-1, 3)
     srgb = np.where(v <= 0.0031308, v * 12.92, 1.055 * np.power(v, 1 / 2.2) - 0.055)
     assert np.array_equal(rgba.reshape(-1, 4)[:, :3], (np.clip(srgb, 0, 1) * 255).astype(np.uint8))
+
+
+def test_srgb_bytes_against_fp64_transfer():
+    """color.h:7-28 + framebuffer.cu:35-40: the oracle's bytes equal the FP64 transfer
+    function truncated, except where the FP64 value lies within float rounding of a byte
+    boundary (there the float computation decides; both neighbours are allowed)."""
+    v = srgb_sweep()
+    _, rgba = O.framebuffer_update(np.repeat(v, 3), np.zeros(3 * v.size, np.float32), 1, 0)
+    got = rgba.reshape(-1, 4)[:, 0].astype(np.int64)
+    x = v.astype(np.float64)
+    with np.errstate(invalid="ignore"):
+        s = np.where(x <= np.float64(np.float32(0.0031308)), x * 12.92, 1.055 * np.power(x, np.float64(np.float32(1 / 2.2))) - 0.055)
+    s = np.where(np.isnan(s), 1.0, s)
+    want = np.floor(np.clip(s, 0, 1) * 255).astype(np.int64)
+    frac = np.clip(s, 0, 1) * 255 - want
+    near = (frac < 1e-4) | (frac > 1 - 1e-4)
+    assert np.all((got == want) | (near & (np.abs(got - want) <= 1)))
+    assert (got[:200001] == want[:200001]).mean() > 0.9999  # the uniform grid
+    # fixed points of the transfer: black, white, saturation above 1, nan as saturated
+    # radiance 1 gives byte 254: in float 1.055f - 0.055f rounds to 0.99999994
+    assert set(got[v == 0]) == {0} and set(got[v == 1]) == {254} and set(got[v >= 1.001]) == {255}
+    assert set(got[np.isnan(v)]) == {255} and set(got[np.isinf(v)]) == {255}
+    # monotone on each segment; the hybrid (exponent 1/2.2 instead of 1/2.4) drops from byte
+    # 10 to byte 5 where the linear segment hands over to the power segment
+    t = np.float32(0.0031308)
+    for seg in (np.isfinite(x) & (v <= t), np.isfinite(x) & (v > t)):
+        order = np.argsort(x[seg], kind="stable")
+        assert np.all(np.diff(got[seg][order]) >= 0)
+    assert set(got[v == t]) == {10} and set(got[v == np.nextafter(t, np.float32(1))]) == {5}
+
+
+# ----------------------------------------------------------------------------- sphere lights
+@pytest.mark.parametrize("target", [(0.0, 0.0, 0.0), (0.6, 0.0, 0.5)])
+@pytest.mark.parametrize("nee,mode,precision", [(1, 0, 64), (0, 0, 64), (1, 1, 32)])
+def test_sphere_light_closed_form(target, nee, mode, precision):
+    """sphere.h:30-44 (area sampling over the whole sphere, pdf 1 / (4 pi r^2)),
+    sampler.h:20-61 / 89-125 (solid-angle conversion, pdfSceneLights of a BSDF-sampled sphere
+    hit) and the integrator at one bounce: the floor's radiance under a sphere source equals
+    rho le r^2 h / d^3.  NEE + MIS (light sample + BSDF sample weighted by the balance
+    heuristic, both sphere pdfs involved), the naive estimator (BSDF sampling alone) and the
+    product's merged schedule all converge to it.  Planted in a copy of the oracle, an area of
+    2 pi r^2 moves the NEE estimate by -330 standard errors, and sampling only the z >= 0 half
+    of the sphere (with the full-sphere pdf) by +260 at the off-axis target."""
+    s, exact = sphere_light_scene(target)
+    o = O.Scene(s.export(), 1, 1)
+    n = 1 << 22
+    tot, sq, _ = o.render_pixels([0], spp=n, max_depth=2, nee=nee, mode=mode, tri_test=mode, ref_env=1 - mode,
+                                 precision=precision, seed=3)
+    m = tot[0] / n
+    se = np.sqrt(np.maximum(sq[0] / n - m * m, 0) / n)
+    z = (m - exact) / se
+    assert np.all(np.abs(z) < 5), (m, exact, z)
+    assert np.all(se < 2e-3 * exact)
 
 
 # ----------------------------------------------------------------------------- integrator
