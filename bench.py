@@ -9,13 +9,16 @@ accumulation.  Synthetic data, seeded.
 
 Prints one JSON line: Mrays/s (primary + extension + shadow rays actually traced, divided by
 the wall time of the timed steps), with the roofline of the dominant kernel measured live
-from HIP events on the renderer's stream, and the CPU oracle timed on a bounded sample of the
-same workload on the host cores.
+from HIP events on the renderer's stream, the CPU oracle timed on a bounded sample of the
+same workload on the host cores, and (mse_ratio) the other half of the headline metric: the
+equal-time mean squared error of the plain and the guided renderer on the hard-indirect room
+against the FP64 oracle's 65,536-sample reference, as their ratio.
 """
 import argparse
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 
@@ -49,35 +52,71 @@ def measured_traffic(kernel):
     return None
 
 
+def host_cpu():
+    """Threads the CPU baseline uses and what the host reports: the cores this process may run
+    on (affinity), capped at the per-GPU CPU share the environment sets (OMP_NUM_THREADS), the
+    machine's logical CPU count and its model name (lscpu)."""
+    nproc = os.cpu_count() or 1
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = nproc
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = max(1, min(aff, share) if share > 0 else aff)
+    model = None
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in txt.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return threads, nproc, aff, model
+
+
 def cpu_baseline(scene_arrays, budget_s):
     """The oracle (scalar C++ restatement of the reference's shading and intersection, with
     a median-split BVH) on the host cores: a band of rows of the same image at as many
-    samples per pixel as fit in about `budget_s` seconds."""
+    samples per pixel as fit in about budget_s / 3 seconds, timed three times (median)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads, nproc, aff, model = host_cpu()
     t0 = time.time()
     o = O.Scene(scene_arrays, WIDTH, HEIGHT, bvh=True)
-    log(f"cpu baseline: oracle scene + BVH built in {time.time() - t0:.1f}s")
+    log(f"cpu baseline: oracle scene + BVH built in {time.time() - t0:.1f}s; {threads} threads "
+        f"(nproc {nproc}, affinity {aff}, {model})")
     kw = dict(max_depth=DEPTH, nee=1, mode=1, tri_test=1, ref_env=0, precision=32, threads=threads)
-    # probe 16 rows of the image centre at 1 spp, then size rows x spp to the budget
+    # probe 16 rows of the image centre at 1 spp (also the warm-up), then size rows x spp so
+    # that one timed run takes about a third of the budget
     t0 = time.time()
     o.render(row_begin=504, row_end=520, spp=1, **kw)
     per_row = max(time.time() - t0, 1e-3) / 16
-    rows = int(min(HEIGHT, max(16, budget_s / per_row)))
-    spp = int(max(1, min(SPP, budget_s / (per_row * rows))))
+    run_s = budget_s / 3
+    rows = int(min(HEIGHT, max(16, run_s / per_row)))
+    spp = int(max(1, min(SPP, run_s / (per_row * rows))))
     rb = max(0, HEIGHT // 2 - rows // 2)
-    t0 = time.time()
-    _, _, rays = o.render(row_begin=rb, row_end=rb + rows, spp=spp, **kw)
-    dt = time.time() - t0
+    rates, times = [], []
+    for k in range(3):
+        t0 = time.time()
+        _, _, rays = o.render(row_begin=rb, row_end=rb + rows, spp=spp, sample_start=k * spp, **kw)
+        dt = time.time() - t0
+        rates.append(rays / dt / 1e6)
+        times.append(round(dt, 2))
+        log(f"cpu baseline run {k}: {rays} rays in {dt:.2f}s")
+    med = sorted(rates)[1]
     return {
-        "value": rays / dt / 1e6,
+        "value": med,
         "unit": "Mrays/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{spp} spp over rows [{rb}, {rb + rows}) of the {WIDTH}x{HEIGHT} image ({rows * WIDTH * spp} paths, {rays} rays, "
-                  f"{dt:.1f} s), same scene, depth {DEPTH}, NEE+MIS, FP32, median-split BVH, {threads} threads",
+        "sample": f"{spp} spp over rows [{rb}, {rb + rows}) of the {WIDTH}x{HEIGHT} image ({rows * WIDTH * spp} paths per run, "
+                  f"{rays} rays), same scene, depth {DEPTH}, NEE+MIS, FP32, median-split BVH; median of 3 runs "
+                  f"({times} s) on {threads} threads",
+        "runs_mrays_s": [round(x, 2) for x in rates],
+        "per_core_mrays_s": round(med / threads, 3),
+        "host": {"nproc": nproc, "affinity_cpus": aff, "cpu_share": threads, "model": model},
     }
 
 
@@ -234,6 +273,10 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-oracle work for cpu_baseline (0: skip)")
+    ap.add_argument("--room-budget", type=float, default=20.0,
+                    help="default workload: seconds per renderer of the equal-time guided-vs-plain room measurement "
+                         "reported as mse_ratio (0: skip)")
+    ap.add_argument("--room-repeats", type=int, default=2, help="default workload: seeds per renderer for mse_ratio")
     ap.add_argument("--level", type=int, default=8, help="icosphere subdivision level")
     ap.add_argument("--pool", type=int, default=0, help="path pool size (0: library default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -332,7 +375,19 @@ def main():
     if args.cpu_budget > 0:
         out["cpu_baseline"] = cpu_baseline(scene.export(), args.cpu_budget)
     r.close()
+    if args.room_budget > 0:
+        out["mse_ratio"] = room_summary(room_equal_time(P, args.room_budget, 1, args.room_repeats))
     print(json.dumps(out), flush=True)
+
+
+def room_summary(res):
+    """The equal-time room measurement without its convergence curves, with RMSE per renderer."""
+    keep = ("mse", "rel_mse", "spp", "seconds", "mrays_s", "guide_steps", "mse_runs")
+    out = {"metric": res["metric"], "value": res["value"], "unit": res["unit"], "config": res["config"]}
+    for m in ("plain", "guided"):
+        out[m] = {k: res[m][k] for k in keep if k in res[m]}
+        out[m]["rmse"] = round(math.sqrt(res[m]["mse"]), 5)
+    return out
 
 
 if __name__ == "__main__":

@@ -38,3 +38,20 @@ def sphere_light_scene(target, rho=0.5, le=2.0, radius=1.0, height=1.5):
     s.set_camera([0, 1, 3], target, [0, 1, 0], 1e-4)
     d = np.linalg.norm(np.array([0, height, 0], np.float64) - np.asarray(target, np.float64))
     return s, rho * le * radius**2 * height / d**3
+
+
+def path_agreement(o, gpu, pixels, samples, tol=1e-3, **kw):
+    """Per-path parity of GPU path radiances with the oracle's product-mode FP32 integrator on
+    the same PCG32 streams: the fraction of paths within `tol` relative (denominator |L| + 1e-3),
+    and, for scale, the same fraction between the oracle's own FP32 and FP64 runs of those
+    paths.  A path through glass or a near-specular lobe can branch differently after a
+    rounding difference anywhere on it; the second number is how often that happens to the
+    oracle itself, the floor that FP32 arithmetic sets for any two implementations."""
+    kw = dict(dict(nee=1, mode=1, tri_test=1, ref_env=0), **kw)
+    c32 = o.trace_paths(pixels, samples, precision=32, **kw)
+    c64 = o.trace_paths(pixels, samples, precision=64, **kw)
+
+    def agree(a, b):
+        return float((np.abs(a - b).max(1) / (np.abs(b).max(1) + 1e-3) < tol).mean())
+
+    return agree(gpu, c32), agree(c32, c64), c32

@@ -74,7 +74,7 @@ def test_gradients_match_oracle():
         rel = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
         cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30)
         print(f"layer {L}: rel L2 {rel:.4f} cos {cos:.6f}", flush=True)
-        assert rel < 0.03 and cos > 0.999
+        assert rel < 1e-3 and cos > 0.9999  # measured about 1e-4 and 1.000000
     # and against the exact FP64 gradient
     _, g64 = G.gradients(p, rec[:, :8].astype(np.float64), rec[:, 8:11].astype(np.float64), rec[:, 11].astype(np.float64),
                          rec[:, 15].astype(np.float64))

@@ -4,10 +4,13 @@ Per-path parity: with consistent MIS the light-sample weight uses the density sa
 with; the wavefront's paths match the oracle's restatement of that estimator path by path.
 Guided rendering draws directions from a network that changes between passes, so it has no
 per-path counterpart: its image is checked against the FP64 oracle within a confidence band
-(the one-sample mixture with the mixture density in both MIS weights is unbiased), and its
-training loss must fall as it learns.
+(the one-sample mixture with the mixture density in both MIS weights is unbiased), its
+training loss must fall as it learns, and on the hard-indirect room its error against the
+FP64 oracle's 65,536-sample reference is lower than the plain renderer's at the same sample
+count: what the trained network gains over the method without it.
 """
 import math
+import os
 import sys
 
 import numpy as np
@@ -82,7 +85,7 @@ def test_consistent_mis_per_path_parity():
     cr = o.trace_paths(pix, smp, max_depth=5, nee=1, mode=1, tri_test=1, ref_env=0, precision=32, mis_sample=0)
     frac_ref = ((np.abs(gl - cr).max(1) / (np.abs(cr).max(1) + 1e-3)) < 1e-3).mean()
     say(f"per-path agreement {frac:.5f} (against the reference weighting: {frac_ref:.5f})")
-    assert frac > 0.99
+    assert frac >= 0.999  # measured 0.99989
     assert frac_ref < frac - 0.05
     assert np.isfinite(gl).all()
     r.close()
@@ -166,3 +169,58 @@ def test_guided_training_lowers_loss():
     r.render(16)
     assert r.stats().guide_steps == 4
     r.close()
+
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ROOM_REFERENCE = os.path.join(ROOT, "tests", "golden", "room_reference_1024px.npz")
+
+
+def _room_pixel_errors(scene, guided, spp, seed):
+    """Squared errors (per pixel, summed over RGB) at the reference fixture's 1024 pixels of a
+    2048x1024 render of the room (depth 16, NEE + consistent MIS) after `spp` one-sample
+    passes; the guided renderer starts from an untrained network and trains after each."""
+    ref = np.load(ROOM_REFERENCE)
+    pix = ref["pixels"]
+    ref_mean = ref["sum"] / ref["spp"]
+    r = P.Renderer(2048, 1024)
+    r.set_scene(scene)
+    r.set_max_depth(16)
+    r.set_option(P.OPT_CONSISTENT_MIS, 1)
+    r.set_seed(seed)
+    if guided:
+        r.set_type(P.RENDERER_GUIDED)
+        r.set_option(P.OPT_GUIDE_RESET, seed + 1)
+    for _ in range(spp):
+        r.render(1)
+    assert r.spp() == spp
+    img = r.radiance_sum().reshape(-1, 3)[pix] / spp
+    st = r.stats()
+    r.close()
+    return ((img - ref_mean) ** 2).sum(1), st
+
+
+def test_guided_beats_plain_at_equal_samples():
+    """The room of 4M triangles lit through a 0.3-wide slot (the equal-time benchmark's scene):
+    at 96 samples per pixel each, the guided renderer's mean squared error against the FP64
+    oracle's 65,536-sample estimate (the committed fixture, whose own noise is 0.0014) is at
+    most half the plain renderer's, and a bootstrap over the 1024 pixels puts the ratio's lower
+    1% quantile above 1.  Both use the same PCG32 seeds; the oracle used its own."""
+    say("case: guided vs plain on the room at equal spp")
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    from make_room_reference import scene_fingerprint
+
+    ref = np.load(ROOM_REFERENCE)
+    scene = P.Scene.preset(2, seed=0)
+    assert np.allclose(scene_fingerprint(scene.export()), ref["fingerprint"], rtol=1e-9)
+    spp = 96
+    ep, sp = _room_pixel_errors(scene, False, spp, seed=11)
+    eg, sg = _room_pixel_errors(scene, True, spp, seed=11)
+    ratio = ep.mean() / eg.mean()
+    rng = np.random.default_rng(0)
+    idx = rng.integers(0, ep.size, (4000, ep.size))
+    boot = ep[idx].mean(1) / eg[idx].mean(1)
+    lo = float(np.quantile(boot, 0.01))
+    say(f"MSE plain {ep.mean() / 3:.5f}, guided {eg.mean() / 3:.5f}: ratio {ratio:.2f} (bootstrap 1% quantile {lo:.2f}); "
+        f"guide steps {sg.guide_steps}, rays plain {sp.rays_total} guided {sg.rays_total}")
+    assert sg.guide_steps >= spp and np.isfinite(eg).all()
+    assert ratio > 2.0 and lo > 1.0

@@ -9,7 +9,7 @@ import pytest
 
 import oracle as O
 import pathed_capi as P
-from helpers import srgb_sweep
+from helpers import path_agreement, srgb_sweep
 
 pytestmark = pytest.mark.gpu
 
@@ -211,11 +211,14 @@ def test_bvh_closest_hit_matches_brute_force():
     assert tie.sum() <= 2
     hit = agree & (gprim != P.MISS)
     assert np.allclose(gt[hit], ct[hit], rtol=1e-5, atol=2e-5)
-    # occlusion over [1e-4, tmax] agrees with closest-hit existence (no ray here has its only
-    # hit closer than 1e-4)
+    # occlusion (the shadow query, over [1e-4, tmax]) is the existence of a closest hit over
+    # the same interval, ray for ray
     _, _, _, occ = r.intersect(rays, any_hit=True)
-    vis_ref = cprim == P.MISS
-    assert np.array_equal(occ == P.MISS, vis_ref)
+    shadow = rays.copy()
+    shadow[:, 3] = 1e-4
+    _, _, _, sprim = o.intersect(shadow, tri_test=1)
+    assert np.array_equal(occ == P.MISS, sprim == P.MISS)
+    assert 0.05 < (occ == P.MISS).mean() < 0.95
     st = r.stats()
     # a real 4-wide tree: 20k primitives in leaves of <= 4 need > 1250 leaf slots
     assert st.stack_overflows == 0 and st.bvh_depth > 5 and st.bvh_nodes > 400
@@ -356,11 +359,11 @@ def test_cornell_per_path_parity():
     o = O.Scene(s.export(), 64, 64)
     pix = np.tile(np.arange(64 * 64), 4)
     smp = np.repeat(np.arange(4), 64 * 64)
-    cl = o.trace_paths(pix, smp, max_depth=6, nee=1, mode=1, tri_test=1, ref_env=0, precision=32)
-    err = np.abs(gl - cl).max(1) / (np.abs(cl).max(1) + 1e-3)
-    frac = (err < 1e-3).mean()
-    say(f"per-path agreement {frac:.5f}")
-    assert frac > 0.99
+    frac, self_frac, _ = path_agreement(o, gl, pix, smp, max_depth=6)
+    say(f"per-path agreement {frac:.5f} (oracle FP32 vs FP64: {self_frac:.5f})")
+    # measured 0.99884 (oracle FP32 vs its FP64: 0.99854): the paths that differ went through
+    # the glass sphere, where a last-bit difference picks the other Fresnel branch
+    assert frac >= 0.998 and frac >= self_frac - 0.002
     assert np.isfinite(gl).all()
     st = r.stats()
     assert st.stack_overflows == 0

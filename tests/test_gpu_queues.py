@@ -46,7 +46,8 @@ def test_per_path_parity_at_odd_sizes(W, H, spp):
     err = np.abs(gl - cl).max(1) / (np.abs(cl).max(1) + 1e-3)
     frac = (err < 1e-3).mean()
     say(f"per-path agreement {frac:.5f} over {n} paths")
-    assert frac >= 0.99 if n >= 100 else frac == 1.0
+    # measured 1.0 except 17x9x7 (0.99907: one path of 1071 through the glass sphere)
+    assert frac >= 0.998 if n >= 1000 else frac == 1.0
     st = r.stats()
     assert st.stack_overflows == 0
     assert st.rays_total >= n  # every camera ray traced

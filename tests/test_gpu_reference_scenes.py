@@ -11,6 +11,7 @@ import pytest
 
 import oracle as O
 import pathed_capi as P
+from helpers import path_agreement
 
 pytestmark = pytest.mark.gpu
 
@@ -41,11 +42,14 @@ def test_reference_scene_per_path_parity(index):
     o = O.Scene(s.export(), W, H)
     pix = np.tile(np.arange(W * H), 4)
     smp = np.repeat(np.arange(4), W * H)
-    cl = o.trace_paths(pix, smp, max_depth=6, nee=1, mode=1, tri_test=1, ref_env=0, precision=32)
-    err = np.abs(gl - cl).max(1) / (np.abs(cl).max(1) + 1e-3)
-    frac = (err < 1e-3).mean()
-    say(f"per-path agreement {frac:.5f}, mean radiance {gl.mean():.4f}")
-    assert frac > 0.99
+    frac, self_frac, _ = path_agreement(o, gl, pix, smp, max_depth=6)
+    say(f"per-path agreement {frac:.5f} (oracle FP32 vs FP64: {self_frac:.5f}), mean radiance {gl.mean():.4f}")
+    # measured: scene 0 0.99982, scene 1 0.99957, scene 3 0.95166.  Scene 3's plates of
+    # alpha 0.005 evaluate the reference's Beckmann terms next to their FP32 cancellation
+    # (tan^2 = (1 - cos^2) / cos^2 for cos ~ 1): there the oracle's own FP32 and FP64 runs agree
+    # on only 0.94 of the paths at 1e-3, and on 0.984 at 1e-2
+    floor = {0: 0.9995, 1: 0.999, 3: 0.945}[index]
+    assert frac >= floor and frac >= self_frac - 0.002
     assert gl.mean() > 0.01 and np.isfinite(gl).all()
     assert r.stats().stack_overflows == 0
     r.close()

@@ -56,7 +56,7 @@ def test_cornell_full_size_c0():
     W = H = 256
     frac, g, cl, st = _per_path(s, W, H, 16, 6, rows=range(0, 256, 32))
     say(f"per-path agreement {frac:.5f} over {g.shape[0]} paths; rays per pass {st.rays_total}")
-    assert frac > 0.99 and st.stack_overflows == 0
+    assert frac >= 0.998 and st.stack_overflows == 0  # measured 0.99915
     # image: GPU 16 spp x 8 passes vs the FP64 oracle at 256 spp (reference semantics)
     r = P.Renderer(W, H)
     r.set_scene(s)
@@ -91,7 +91,7 @@ def test_icosphere_c1_scene_per_path():
     s = P.Scene.preset(1, seed=0, param=8)
     frac, g, cl, st = _per_path(s, 64, 64, 2, 8, rows=range(0, 64, 2))
     say(f"per-path agreement {frac:.5f} over {g.shape[0]} paths; mean radiance {g.mean():.4f}")
-    assert frac > 0.99 and st.stack_overflows == 0
+    assert frac >= 0.9995 and st.stack_overflows == 0  # measured 1.00000
     assert np.isfinite(g).all() and g.mean() > 0
 
 
@@ -102,5 +102,5 @@ def test_room_c4_scene_per_path():
     s = P.Scene.preset(2, seed=0)
     frac, g, cl, st = _per_path(s, 64, 32, 2, 16, rows=range(0, 32, 2), consistent=True)
     say(f"per-path agreement {frac:.5f} over {g.shape[0]} paths; mean radiance {g.mean():.4f}")
-    assert frac > 0.98 and st.stack_overflows == 0
+    assert frac >= 0.995 and st.stack_overflows == 0  # measured 0.99951
     assert np.isfinite(g).all() and g.mean() > 0
