@@ -504,6 +504,12 @@ bool parseExrHeader(const std::vector<unsigned char> &d, ExrHeader &h, std::stri
         err = "missing channels or data window";
         return false;
     }
+    // widths and heights in 64-bit: xmax - xmin + 1 overflows int32 for extreme windows
+    const int64_t w = (int64_t)h.xmax - h.xmin + 1, hh = (int64_t)h.ymax - h.ymin + 1;
+    if (w > (1 << 16) || hh > (1 << 16) || w * hh > (1ll << 28)) {
+        err = "data window too large";
+        return false;
+    }
     return true;
 }
 
@@ -571,16 +577,21 @@ bool readExr(const std::string &path, int *W, int *H, std::vector<float> *rgba, 
     for (int b = 0; b < blocks; b++) {
         uint64_t off;
         memcpy(&off, &d[h.end + 8 * (size_t)b], 8);
-        if (off + 8 > d.size()) {
+        if (off > d.size() || d.size() - off < 8) {
             err = path + ": bad block offset";
             return false;
         }
         int32_t y0, size;
         memcpy(&y0, &d[off], 4);
         memcpy(&size, &d[off + 4], 4);
-        const int first = y0 - h.ymin;
+        const int64_t first64 = (int64_t)y0 - h.ymin;
+        if (first64 < 0 || first64 >= hh || size < 0 || d.size() - off - 8 < (uint64_t)size) {
+            err = path + ": bad block";
+            return false;
+        }
+        const int first = (int)first64;
         const int lines = std::min(linesPerBlock, hh - first);
-        if (first < 0 || first >= hh || size < 0 || off + 8 + (uint64_t)size > d.size()) {
+        if (lines <= 0) {
             err = path + ": bad block";
             return false;
         }

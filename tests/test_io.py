@@ -227,3 +227,32 @@ def test_reference_environment_assets():
     assert np.isfinite(img).all() and (img[..., :3] >= 0).all() and img[..., :3].max() > 1
     with pytest.raises(P.PtError, match="compression 4"):
         P.exr_read(os.path.join(SCENES, "assets", "1_pixel_test.exr"))
+
+
+def test_exr_crafted_offsets_and_windows_are_errors(tmp_path):
+    """Block offsets near 2^64, block sizes past the end, block line numbers outside the data
+    window and data windows whose width overflows 32 bits are reported as errors, not read."""
+    rgba = np.ones((4, 3, 4), np.float32)
+    good = str(tmp_path / "g.exr")
+    _write_zip_exr(good, rgba, 16, False)
+    data = open(good, "rb").read()
+    # locate the offset table: the header ends with a zero byte after screenWindowWidth's float
+    marker = b"screenWindowWidth\0float\0" + struct.pack("<I", 4) + struct.pack("<f", 1) + b"\0"
+    tpos = data.index(marker) + len(marker)
+    (boff,) = struct.unpack("<Q", data[tpos:tpos + 8])
+    cases = {
+        "offset near 2^64": data[:tpos] + struct.pack("<Q", (1 << 64) - 4) + data[tpos + 8:],
+        "offset past the end": data[:tpos] + struct.pack("<Q", len(data) - 4) + data[tpos + 8:],
+        "size past the end": data[:boff + 4] + struct.pack("<i", 1 << 30) + data[boff + 8:],
+        "line outside the window": data[:boff] + struct.pack("<i", -(1 << 31)) + data[boff + 4:],
+    }
+    win = struct.pack("<4i", 0, 0, 2, 3)
+    wpos = data.index(b"dataWindow\0box2i\0") + len(b"dataWindow\0box2i\0") + 4
+    assert data[wpos:wpos + 16] == win
+    cases["window width overflows int32"] = data[:wpos] + struct.pack("<4i", -(1 << 31), 0, (1 << 31) - 1, 3) + data[wpos + 16:]
+    for name, blob in cases.items():
+        p = tmp_path / "bad.exr"
+        p.write_bytes(blob)
+        with pytest.raises(P.PtError):
+            P.exr_read(str(p))
+    assert np.array_equal(P.exr_read(good), rgba)
