@@ -1951,6 +1951,12 @@ int pt_renderer_synchronize(pt_renderer *r) {
     PT_API_END("pt_renderer_synchronize")
 }
 
+int pt_renderer_get_stream(pt_renderer *r, void **stream) {
+    if (guard("pt_renderer_get_stream", r) || !stream) return fail(PT_ERR_INVALID, "pt_renderer_get_stream: bad arguments");
+    *stream = (void *)r->stream;
+    return PT_OK;
+}
+
 int pt_renderer_reset(pt_renderer *r) {
     if (guard("pt_renderer_reset", r)) return PT_ERR_INVALID;
     r->resetPending = true;

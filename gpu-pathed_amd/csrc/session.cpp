@@ -26,24 +26,6 @@ int SppOptimizer::estimateSpp() {
 void SppOptimizer::reset() { m_reset = true; }
 
 // ---------------------------------------------------------------------------------------
-// Scene (scene.h:38-61)
-// ---------------------------------------------------------------------------------------
-void Scene::setMaterialType(int id, MaterialType t) {
-    MaterialParams &m = m_materials.at(id);
-    if (m.type == t) return;
-    MaterialParams n;
-    n.type = t;
-    switch (t) {
-    case MaterialType::Lambertian: n.albedo = Vec3(0.f); n.emit = Vec3(0.f); break;
-    case MaterialType::Mirror: break;
-    case MaterialType::Glass: n.ior = 1.4f; break;
-    case MaterialType::Microfacet: n.alpha = 0.1f; break;
-    default: return;
-    }
-    m = n;
-}
-
-// ---------------------------------------------------------------------------------------
 // SceneModel (scene_model.cu:6-347)
 // ---------------------------------------------------------------------------------------
 SceneModel::SceneModel(const Scene *scene, float lightPosition, RendererType rendererType)
@@ -136,12 +118,16 @@ void SceneModel::zoomCamera(float ticks) {
 // ---------------------------------------------------------------------------------------
 // RenderSession (render_session.cu:55-141, render_session.h:59-118)
 // ---------------------------------------------------------------------------------------
-void RenderSession::check(int status) const {
+namespace {
+void check(int status) {
     if (status != PT_OK) throw std::runtime_error(pt_last_error());
 }
+}  // namespace
 
 RenderSession::RenderSession(int width, int height) : m_width(width), m_height(height), m_rendererType(RendererType::Optix) {
     if (width <= 0 || height <= 0) throw std::invalid_argument("RenderSession: width and height must be positive");
+    m_pathTracer = makeRenderer(m_rendererType);
+    m_deviceGlobals = std::make_unique<DeviceGlobals>();
     check(pt_scene_create(&m_hostScene));
     // render_session.cu:69-71: scene preset 0 and its camera
     check(pt_scene_preset(m_hostScene, 0, 0, 0));
@@ -149,7 +135,8 @@ RenderSession::RenderSession(int width, int height) : m_width(width), m_height(h
 }
 
 RenderSession::~RenderSession() {
-    if (m_renderer) pt_renderer_destroy(m_renderer);
+    m_pathTracer.reset();
+    m_deviceGlobals.reset();
     if (m_hostScene) pt_scene_destroy(m_hostScene);
 }
 
@@ -178,21 +165,6 @@ void RenderSession::buildSceneObjects() {
     m_sceneModel->subscribe([this](const SceneModelAttributes &a) { applyAttributes(a); });
 }
 
-void RenderSession::pushMaterial(int id) {
-    const MaterialParams &m = m_scene->getMaterial(id);
-    const float albedo[3] = {m.albedo.x(), m.albedo.y(), m.albedo.z()};
-    const float emit[3] = {m.emit.x(), m.emit.y(), m.emit.z()};
-    check(pt_renderer_set_material(m_renderer, id, (int)m.type, albedo, emit, m.ior, m.alpha));
-}
-
-void RenderSession::pushCamera() {
-    const Camera &c = m_scene->getCamera();
-    const float o[3] = {c.getOrigin().x(), c.getOrigin().y(), c.getOrigin().z()};
-    const float t[3] = {c.getTarget().x(), c.getTarget().y(), c.getTarget().z()};
-    const float u[3] = {c.getUp().x(), c.getUp().y(), c.getUp().z()};
-    check(pt_renderer_set_camera(m_renderer, o, t, u, c.getVerticalFOV(), c.getFlipHandedness() ? 1 : 0));
-}
-
 // render_session.cu:82-116
 void RenderSession::applyAttributes(const SceneModelAttributes &a) {
     if (a.rendererType != m_rendererType) {
@@ -214,116 +186,108 @@ void RenderSession::applyAttributes(const SceneModelAttributes &a) {
         }
     }
     m_scene->setCamera(a.camera);
+    m_pathTracer->reset();
     m_sppOptimizer.reset();
-    if (!m_renderer) {
+    if (!m_deviceGlobals->allocated()) {
         if (id >= 0) m_dirtyMaterials.push_back(id);
         return;
     }
-    if (m_rendering) {
-        // the reference synchronises the device inside the callback (render_session.cu:114)
-        check(pt_renderer_synchronize(m_renderer));
-        float ms = 0.f;
-        int finished = 0;
-        check(pt_renderer_poll(m_renderer, &finished, &ms));
-    }
-    check(pt_renderer_set_max_depth(m_renderer, a.maxDepth));
-    check(pt_renderer_set_nee(m_renderer, a.nextEventEstimation ? 1 : 0));
-    if (id >= 0) pushMaterial(id);
-    pushCamera();
-    check(pt_renderer_reset(m_renderer));
+    // the reference synchronises the device inside the callback (render_session.cu:114); the
+    // edits below wait for the pass in flight the same way
+    m_deviceGlobals->synchronize();
+    m_deviceGlobals->copyCamera(m_scene->getCamera());
+    if (id >= 0) m_deviceGlobals->updateMaterial(id, m_scene->getMaterial(id));
 }
 
+// render_session.cu:119-136
 RenderState RenderSession::init(unsigned int pbo1, unsigned int pbo2) {
-    m_pbo1 = pbo1;
-    m_pbo2 = pbo2;
+    if (m_rendering) {
+        m_deviceGlobals->synchronize();
+        m_rendering = false;
+    }
+    m_pbo[0].id = pbo1;
+    m_pbo[1].id = pbo2;
     m_computeOnPrimary = false;
-    if (!m_renderer) check(pt_renderer_create(m_width, m_height, &m_renderer));
-    check(pt_renderer_set_scene(m_renderer, m_hostScene));
+    m_deviceGlobals->mallocWorld(m_hostScene, m_width, m_height);
     // the scene model's state may have been edited before init
-    check(pt_renderer_set_max_depth(m_renderer, m_scene->getMaxDepth()));
-    check(pt_renderer_set_nee(m_renderer, m_scene->getNextEventEstimation() ? 1 : 0));
-    for (int id : m_dirtyMaterials) pushMaterial(id);
+    for (int id : m_dirtyMaterials) m_deviceGlobals->updateMaterial(id, m_scene->getMaterial(id));
     m_dirtyMaterials.clear();
-    pushCamera();
-    check(pt_renderer_set_type(m_renderer, (int)m_rendererType));
-    check(pt_renderer_reset(m_renderer));
-    m_initialized = true;
+    m_deviceGlobals->copyCamera(m_scene->getCamera());
+    m_pathTracer->init(m_width, m_height, *m_scene);
     return RenderState{false, pbo1};
 }
 
 void RenderSession::hitTest(int x, int y) {
-    if (!m_renderer) throw std::runtime_error("hitTest: call init() first");
-    if (m_rendering) {
-        check(pt_renderer_synchronize(m_renderer));
-    }
-    int id = -1, hit = 0;
-    check(pt_renderer_hit_test(m_renderer, x, y, &id, &hit));
-    m_sceneModel->setMaterialID(hit ? id : -1);
+    if (!m_deviceGlobals->allocated()) throw std::runtime_error("hitTest: call init() first");
+    // hit_test.cu:18-62 sets the scene model's selected material
+    m_sceneModel->setMaterialID(m_deviceGlobals->hitTest(x, y));
 }
 
+// render_session.h:59-70
 RenderState RenderSession::renderAsync() {
-    if (!m_renderer) throw std::runtime_error("renderAsync: call init() first");
+    if (!m_deviceGlobals->allocated()) throw std::runtime_error("renderAsync: call init() first");
     if (m_rendering) throw std::runtime_error("renderAsync: previous pass not finished; call pollRender()");
     const int spp = m_sppOptimizer.estimateSpp();
-    check(pt_renderer_render_async(m_renderer, spp));
-    m_currentSpp = spp;
+    m_currentRecord = m_pathTracer->renderAsync(spp, renderPbo(), *m_scene, *m_deviceGlobals);
     m_rendering = true;
-    return RenderState{true, m_computeOnPrimary ? m_pbo2 : m_pbo1};
+    return RenderState{true, displayPbo()};
 }
 
+void RenderSession::installRenderer() {
+    m_pathTracer = makeRenderer(m_rendererType);
+    m_pathTracer->init(m_width, m_height, *m_scene);
+}
+
+// render_session.h:72-117
 RenderState RenderSession::pollRender() {
-    const unsigned int display = m_computeOnPrimary ? m_pbo2 : m_pbo1;
-    if (!m_renderer || !m_rendering) return RenderState{false, display};
-    int finished = 0;
-    float ms = 0.f;
-    check(pt_renderer_poll(m_renderer, &finished, &ms));
+    if (!m_deviceGlobals->allocated() || !m_rendering) return RenderState{false, displayPbo()};
+    const bool finished = m_pathTracer->pollRender(renderPbo(), m_currentRecord);
     if (finished) {
         m_rendering = false;
-        m_sppOptimizer.track(m_currentSpp, std::max(ms, 1e-3f));
-        m_computeOnPrimary = !m_computeOnPrimary;
-        int64_t spp = 0;
-        check(pt_renderer_get_spp(m_renderer, &spp));
-        m_sceneModel->updateSpp((int)spp);
+        float milliseconds = 0.f;
+        if (hipEventElapsedTime(&milliseconds, m_currentRecord.beginEvent, m_currentRecord.endEvent) != hipSuccess) milliseconds = 0.f;
+        m_sppOptimizer.track(m_currentRecord.spp, std::max(milliseconds, 1e-3f));
+        m_computeOnPrimary = !m_computeOnPrimary;  // swapPBOs
+        const int spp = m_pathTracer->getSpp();
+        m_sceneModel->updateSpp(spp);
         // render_session.h:87-101: dump the radiance when this pass crossed a power of two
         if (!m_exrDumpPath.empty()) {
             for (int64_t t = 2; t <= (1 << 20); t <<= 1) {
-                if (spp >= t && spp - m_currentSpp < t) {
-                    const std::vector<float> radiance = getRadianceBuffer();
+                if (spp >= t && spp - m_currentRecord.spp < t) {
+                    const std::vector<float> radiance = m_pathTracer->getRadianceBuffer();
                     check(pt_exr_write_rgb(m_exrDumpPath.c_str(), radiance.data(), m_width, m_height, 1));
                     break;
                 }
             }
         }
         if (m_resetRenderer) {
-            // render_session.h:103-114: switch the integrator and restart accumulation
-            check(pt_renderer_set_type(m_renderer, (int)m_rendererType));
-            check(pt_renderer_reset(m_renderer));
+            // render_session.h:103-114: a new renderer of the selected type, accumulation
+            // restarts
+            installRenderer();
             m_sppOptimizer.reset();
             m_resetRenderer = false;
         }
     }
-    return RenderState{!finished, m_computeOnPrimary ? m_pbo2 : m_pbo1};
+    return RenderState{!finished, displayPbo()};
 }
 
 std::vector<float> RenderSession::getRadianceBuffer() {
-    if (!m_renderer) throw std::runtime_error("getRadianceBuffer: call init() first");
-    std::vector<float> out(3 * (size_t)m_width * m_height);
-    check(pt_renderer_get_radiance(m_renderer, out.data()));
-    return out;
+    if (!m_deviceGlobals->allocated()) throw std::runtime_error("getRadianceBuffer: call init() first");
+    if (m_rendering) m_deviceGlobals->synchronize();
+    return m_pathTracer->getRadianceBuffer();
 }
 
 std::vector<uint8_t> RenderSession::getFramebuffer() {
-    if (!m_renderer) throw std::runtime_error("getFramebuffer: call init() first");
-    std::vector<uint8_t> out(4 * (size_t)m_width * m_height);
-    check(pt_renderer_get_rgba8(m_renderer, out.data()));
-    return out;
+    if (!m_deviceGlobals->allocated()) throw std::runtime_error("getFramebuffer: call init() first");
+    // the display pixel buffer holds the last finished pass (black before the first)
+    const HeadlessPbo &p = m_computeOnPrimary ? m_pbo[1] : m_pbo[0];
+    if (p.rgba.empty()) return std::vector<uint8_t>(4 * (size_t)m_width * m_height, 0);
+    return p.rgba;
 }
 
 int64_t RenderSession::getAccumulatedSpp() {
-    if (!m_renderer) return 0;
-    int64_t s = 0;
-    check(pt_renderer_get_spp(m_renderer, &s));
-    return s;
+    if (!m_deviceGlobals->allocated()) return 0;
+    return m_pathTracer->getSpp();
 }
 
 void RenderSession::loadReferenceScene(int index, const std::string &scenesDir) {
@@ -340,7 +304,7 @@ void RenderSession::loadReferenceScene(int index, const std::string &scenesDir) 
     m_hostScene = fresh;
     m_dirtyMaterials.clear();
     buildSceneObjects();
-    if (m_renderer) init(m_pbo1, m_pbo2);
+    if (m_deviceGlobals->allocated()) init(m_pbo[0].id, m_pbo[1].id);
 }
 
 void RenderSession::loadPreset(int preset, uint64_t seed, int64_t param) {
@@ -348,7 +312,7 @@ void RenderSession::loadPreset(int preset, uint64_t seed, int64_t param) {
     check(pt_scene_preset(m_hostScene, preset, seed, param));
     m_dirtyMaterials.clear();
     buildSceneObjects();
-    if (m_renderer) init(m_pbo1, m_pbo2);
+    if (m_deviceGlobals->allocated()) init(m_pbo[0].id, m_pbo[1].id);
 }
 
 }  // namespace rays

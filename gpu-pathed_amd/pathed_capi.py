@@ -37,6 +37,7 @@ EXPORTS = [
     "pt_exr_read_rgba", "pt_exr_write_rgb",
     "pt_renderer_autotune", "pt_batch_create", "pt_batch_destroy", "pt_batch_add", "pt_batch_set_option", "pt_batch_get_option",
     "pt_batch_autotune", "pt_batch_run", "pt_batch_get_radiance", "pt_batch_get_render_stats", "pt_batch_get_stats",
+    "pt_renderer_get_stream",
 ]
 GUIDE_PARAMS = 20800
 

@@ -193,6 +193,10 @@ int pt_renderer_render_async(pt_renderer *r, int spp);
  * *milliseconds = its GPU time (begin/end events). */
 int pt_renderer_poll(pt_renderer *r, int *finished, float *milliseconds);
 int pt_renderer_synchronize(pt_renderer *r);
+/* The HIP stream (a hipStream_t) that the renderer's passes run on, so that a caller can order
+ * its own work (events, copies) after a pass: the events of RenderRecord (render_record.h:7-11)
+ * are recorded on it.  Owned by the renderer; valid until pt_renderer_destroy. */
+int pt_renderer_get_stream(pt_renderer *r, void **stream);
 /* Renderer::reset: accumulation restarts at the next pass. */
 int pt_renderer_reset(pt_renderer *r);
 /* Renderer::getSpp */
