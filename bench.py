@@ -206,7 +206,7 @@ def room_run(P, scene, mode, budget_s, spp_per_pass, seed, pix, ref_mean):
     return out
 
 
-def batch_renders(P, steps, probe_divisor, seed=2024):
+def batch_renders(P, steps, probe_divisor, seed=2024, schedule="wavefront"):
     """48 independent renders in one captured graph: resolution uniform in {128, 256, 512,
     1024}^2, spp uniform in {4, 16, 64, 256}, each a displaced geodesic sphere with a
     triangle count log-uniform in [1e4, 2e6] and per-face materials and environment as the
@@ -217,6 +217,7 @@ def batch_renders(P, steps, probe_divisor, seed=2024):
 
     rng = np.random.default_rng(seed)
     b = P.Batch()
+    b.set_option(P.OPT_BATCH_WAVEFRONT, int(schedule == "wavefront"))
     specs = []
     t0 = time.time()
     for i in range(48):
@@ -249,6 +250,7 @@ def batch_renders(P, steps, probe_divisor, seed=2024):
         "steps": steps, "ms_per_step": round(wall / steps, 2), "higher_is_better": True, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "48 renders, res in {128,256,512,1024}^2, spp in {4,16,64,256}, geodesic spheres with "
                                "1e4-2e6 triangles (log-uniform), depth 8, NEE+MIS, one hipGraph launch per run",
+                   "schedule": schedule,
                    "paths_per_run": int(sum(x[0] * x[0] * x[1] for x in specs)), "triangles": int(sum(x[2] for x in specs)),
                    "chunks": st["chunks"], "graph_nodes": st["graph_nodes"], "seed": seed},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2), "p99": round(float(np.percentile(lat, 99)), 2),
@@ -266,6 +268,8 @@ def main():
                     help="icosphere: Mrays/s of the 1M-triangle render (default); room: equal-time MSE, guided vs plain; "
                          "batch: 48 renders in one captured graph")
     ap.add_argument("--probe-divisor", type=int, default=16, help="batch: autotuner probes run spp / this")
+    ap.add_argument("--batch-schedule", choices=["wavefront", "sequential"], default="wavefront",
+                    help="batch: one wavefront over all renders (default) or render after render")
     ap.add_argument("--budget", type=float, default=20.0, help="room: seconds per renderer")
     ap.add_argument("--spp-per-pass", type=int, default=1, help="room: samples per pixel per progressive pass")
     ap.add_argument("--repeats", type=int, default=3, help="room: seeds per renderer (MSE averaged)")
@@ -279,6 +283,8 @@ def main():
     ap.add_argument("--room-repeats", type=int, default=2, help="default workload: seeds per renderer for mse_ratio")
     ap.add_argument("--level", type=int, default=8, help="icosphere subdivision level")
     ap.add_argument("--pool", type=int, default=0, help="path pool size (0: library default)")
+    ap.add_argument("--bvh", choices=["default", "lbvh", "ploc"], default="default",
+                    help="binary tree under the 4-wide BVH (default: the library's)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (torch) to calibrate HBM byte counters, then exit")
     args = ap.parse_args()
@@ -306,7 +312,7 @@ def main():
         print(json.dumps(room_equal_time(P, args.budget, args.spp_per_pass, args.repeats)), flush=True)
         return
     if args.workload == "batch":
-        print(json.dumps(batch_renders(P, args.steps, args.probe_divisor)), flush=True)
+        print(json.dumps(batch_renders(P, args.steps, args.probe_divisor, schedule=args.batch_schedule)), flush=True)
         return
     t0 = time.time()
     scene = P.Scene.preset(1, seed=0, param=args.level)
@@ -315,9 +321,12 @@ def main():
     r = P.Renderer(WIDTH, HEIGHT)
     if args.pool:
         r.set_option(P.OPT_POOL_PATHS, args.pool)
+    if args.bvh != "default":
+        r.set_option(P.OPT_BVH_BUILDER, P.BVH_LBVH if args.bvh == "lbvh" else P.BVH_PLOC)
     t0 = time.time()
     r.set_scene(scene)
-    log(f"upload + GPU BVH + alias tables: {time.time() - t0:.2f}s")
+    build_s = time.time() - t0
+    log(f"upload + GPU BVH + alias tables: {build_s:.2f}s")
     r.set_max_depth(DEPTH)
     r.set_nee(True)
     for i in range(args.warmup):
@@ -367,7 +376,9 @@ def main():
         "config": {"workload": f"displaced icosphere level {args.level} ({info.n_triangles} tris) + {info.env_width}x{info.env_height} "
                                f"synthetic HDR env, {WIDTH}x{HEIGHT}, {SPP} spp, depth {DEPTH}, NEE+MIS, FP64 accumulation",
                    "rays_per_step": st.rays_total // args.steps, "samples_per_s": round(samples),
-                   "bvh_depth": st.bvh_depth, "parallelism": "1 GPU"},
+                   "bvh_depth": st.bvh_depth, "bvh_nodes": st.bvh_nodes,
+                   "bvh_builder": "lbvh" if r.get_option(P.OPT_BVH_BUILDER) == P.BVH_LBVH else "ploc",
+                   "scene_upload_s": round(build_s, 3), "parallelism": "1 GPU"},
         "roofline": roof,
         "queue_roofline": queue_roof,
         "kernels": kernels,

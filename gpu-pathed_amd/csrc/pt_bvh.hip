@@ -1,16 +1,22 @@
-// pt_bvh.hip -- GPU LBVH builder for gfx950.
+// pt_bvh.hip -- GPU BVH builder for gfx950.
 //
 // 1. 63-bit Morton codes of primitive centroids (21 bits per axis) + primitive AABBs
 // 2. stable LSD radix sort of (code, index), 8 passes of 8 bits; ranking inside a tile with
 //    wave64 ballots (one 64-bit mask per key bit)
-// 3. Karras 2012 binary radix tree (one thread per internal node; equal codes ordered by
-//    index)
-// 4. bottom-up AABB refit (atomic arrival counter per node; the second child to arrive
-//    merges) and depth
-// 5. collapse into a 4-wide BVH of 128-byte nodes (on the host, over the refitted tree):
-//    subtrees of <= PT_LEAF_MAX primitives become leaf ranges of the sorted primitive array,
-//    and each 4-wide node opens the largest-area internal children of its binary node until
-//    it has four, about halving the number of dependent node fetches per ray
+// 3. a binary tree over the sorted primitives, one of
+//    - LBVH: Karras 2012 binary radix tree (one thread per internal node; equal codes ordered
+//      by index), then a bottom-up AABB refit (atomic arrival counter per node; the second
+//      child to arrive merges);
+//    - PLOC (Meister & Bittner 2018, parallel locally-ordered clustering): every cluster of the
+//      Morton-ordered list finds the neighbour within PLOC_RADIUS positions whose merged box
+//      has the least surface area; mutual nearest neighbours merge; the list is compacted with
+//      a prefix sum; repeat until one cluster is left.  The primitives are then put in the
+//      tree's depth-first leaf order so every subtree covers a contiguous range.
+// 4. collapse into a 4-wide BVH of 128-byte nodes, on the GPU: subtrees of <= PT_LEAF_MAX
+//    primitives become leaf ranges of the primitive array, and which internal nodes become
+//    4-wide nodes is chosen to minimise a surface-area cost (bottom-up dynamic programme over
+//    the binary tree with arrival counters; then the nodes are written level by level from
+//    the root, the live children of a node at consecutive indices).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -251,19 +257,6 @@ __global__ void k_refit(const int2 *children, const int *parentNode, const int *
     }
 }
 
-// depth of each leaf = number of internal nodes on its root path
-__global__ void k_depth(const int *parentNode, const int *parentLeaf, int n, int *maxDepth) {
-    const int leaf = blockIdx.x * blockDim.x + threadIdx.x;
-    if (leaf >= n) return;
-    int p = parentLeaf[leaf];
-    int d = 0;
-    while (p >= 0 && d < n) {
-        d++;
-        p = parentNode[p];
-    }
-    atomicMax(maxDepth, d);
-}
-
 // All primitives in one leaf below a single 4-wide root (n <= PT_LEAF_MAX).
 __global__ void k_write_single(const float *leafBoxes, int n, float4 *nodes) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -289,136 +282,445 @@ __global__ void k_fill_int(int *p, int n, int v) {
 
 inline int blocks(int64_t n, int t) { return (int)((n + t - 1) / t); }
 
-// Collapses the binary tree into 4-wide nodes on the host.  Subtrees of <= PT_LEAF_MAX
-// primitives become leaf ranges; which internal nodes become 4-wide nodes is chosen to
-// minimise a surface-area cost (a 4-wide node costs kNodeCost x its area, a leaf kPrimCost x
-// its area x its primitives): bottom up, every internal node gets the cheapest way to spend
-// 1..4 child slots of its parent on its subtree (one slot: the node itself as a 4-wide node;
-// more: its two children share them), then the nodes are written top down from the root.
-// The root is node 0, and the live children of a node take consecutive indices (depth-first
-// order).  Returns the node count; *depth receives the deepest 4-wide level.
+// --- prefix sums ------------------------------------------------------------------------
+// Exclusive scan of m values in place, the total written to data[m]: tiles of kScanTile values
+// scanned in LDS (one per block), the tile totals scanned by one block, then added back.
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+template <class T> __device__ T blockExclusiveScan(T v, T *sh, T *total) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        const T x = t >= off ? sh[t - off] : (T)0;
+        __syncthreads();
+        sh[t] += x;
+        __syncthreads();
+    }
+    const T incl = sh[t];
+    *total = sh[kScanThreads - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+template <class T> __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(T *data, int m, T *tileSums) {
+    __shared__ T sh[kScanThreads];
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    T v[kScanItems];
+    T sum = 0;
+    for (int k = 0; k < kScanItems; k++) {
+        v[k] = base + k < m ? data[base + k] : (T)0;
+        sum += v[k];
+    }
+    T total;
+    T run = blockExclusiveScan<T>(sum, sh, &total);
+    for (int k = 0; k < kScanItems; k++) {
+        if (base + k < m) data[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 0) tileSums[blockIdx.x] = total;
+}
+
+// one block: exclusive scan of the tile totals (any count), total to tileSums[tiles]
+template <class T> __global__ __launch_bounds__(kScanThreads) void k_scan_totals(T *tileSums, int tiles) {
+    __shared__ T sh[kScanThreads];
+    T carry = 0;
+    for (int b0 = 0; b0 < tiles; b0 += kScanThreads) {
+        const int i = b0 + threadIdx.x;
+        const T v = i < tiles ? tileSums[i] : (T)0;
+        T total;
+        const T ex = blockExclusiveScan<T>(v, sh, &total);
+        if (i < tiles) tileSums[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) tileSums[tiles] = carry;
+}
+
+template <class T> __global__ void k_scan_add(T *data, int m, const T *tileSums, int tiles) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) data[i] += tileSums[i / kScanTile];
+    if (i == 0) data[m] = tileSums[tiles];
+}
+
+// data holds m + 1 entries; tmp at least tiles(m) + 1
+template <class T> void exclusiveScan(T *data, int m, T *tmp, hipStream_t stream) {
+    const int tiles = std::max(1, (m + kScanTile - 1) / kScanTile);
+    k_scan_tiles<T><<<tiles, kScanThreads, 0, stream>>>(data, m, tmp);
+    k_scan_totals<T><<<1, kScanThreads, 0, stream>>>(tmp, tiles);
+    k_scan_add<T><<<std::max(1, (m + 255) / 256), 256, 0, stream>>>(data, m, tmp, tiles);
+    PT_HIP_TRY(hipGetLastError());
+}
+inline int scanTmpEntries(int m) { return (m + kScanTile - 1) / kScanTile + 2; }
+
+// --- binary tree shared by both builders --------------------------------------------------
+// children: >= 0 internal node, < 0 ~leaf (leaf = position in the Morton order); cnt and
+// first: primitives below an internal node and the position of the first in the final
+// primitive order; leafPos: final position of a leaf (nullptr: the Morton position).
+struct BinTree {
+    int n = 0, root = 0;
+    const int2 *children = nullptr;
+    const float *nodeBoxes = nullptr, *leafBoxes = nullptr;
+    const int *cnt = nullptr, *first = nullptr, *leafPos = nullptr;
+    const int *parentNode = nullptr, *parentLeaf = nullptr;
+};
+
+__device__ __forceinline__ int childCount(const BinTree &t, int c) { return c < 0 ? 1 : t.cnt[c]; }
+
+__device__ __forceinline__ float boxArea6(const float *b) {
+    const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+// depth of each leaf = number of internal nodes on its root path
+__global__ void k_depth(const int *parentNode, const int *parentLeaf, int n, int *maxDepth) {
+    const int leaf = blockIdx.x * blockDim.x + threadIdx.x;
+    if (leaf >= n) return;
+    int p = parentLeaf[leaf];
+    int d = 0;
+    while (p >= 0 && d < n) {
+        d++;
+        p = parentNode[p];
+    }
+    atomicMax(maxDepth, d);
+}
+
+// LBVH: counts and first positions from the Karras ranges (leaves stay in Morton order)
+__global__ void k_ranges_to_counts(const int2 *ranges, int ni, int *cnt, int *first) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ni) return;
+    const int2 r = ranges[i];
+    cnt[i] = r.y - r.x + 1;
+    first[i] = r.x;
+}
+
+// --- PLOC --------------------------------------------------------------------------------
+constexpr int PLOC_RADIUS = 16;
+constexpr int kPlocThreads = 256;
+
+__global__ void k_ploc_init(int n, const float *leafBoxes, int *clusters, float *cbox) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    clusters[i] = ~i;
+    for (int k = 0; k < 6; k++) cbox[6 * i + k] = leafBoxes[6 * i + k];
+}
+
+// Nearest neighbour of every cluster among the PLOC_RADIUS positions on either side: least
+// surface area of the merged box, ties to the lower position.
+__global__ __launch_bounds__(kPlocThreads) void k_ploc_nn(const float *cbox, int m, int *nn) {
+    __shared__ float sb[(kPlocThreads + 2 * PLOC_RADIUS) * 6];
+    const int start = blockIdx.x * kPlocThreads;
+    for (int k = threadIdx.x; k < (kPlocThreads + 2 * PLOC_RADIUS) * 6; k += kPlocThreads) {
+        const int pos = start - PLOC_RADIUS + k / 6;
+        sb[k] = pos >= 0 && pos < m ? cbox[(int64_t)pos * 6 + k % 6] : 0.f;
+    }
+    __syncthreads();
+    const int i = start + threadIdx.x;
+    if (i >= m) return;
+    const float *bi = &sb[(threadIdx.x + PLOC_RADIUS) * 6];
+    float best = INFINITY;
+    int bj = i;
+    const int j0 = max(0, i - PLOC_RADIUS), j1 = min(m - 1, i + PLOC_RADIUS);
+    for (int j = j0; j <= j1; j++) {
+        if (j == i) continue;
+        const float *bj6 = &sb[(j - start + PLOC_RADIUS) * 6];
+        float u[6];
+        for (int k = 0; k < 3; k++) {
+            u[k] = fminf(bi[k], bj6[k]);
+            u[3 + k] = fmaxf(bi[3 + k], bj6[3 + k]);
+        }
+        const float d = boxArea6(u);
+        if (d < best) {
+            best = d;
+            bj = j;
+        }
+    }
+    nn[i] = bj;
+}
+
+// Merge pairs (2k, 2k + 1): the fallback for an iteration in which no two clusters chose each
+// other (never observed; it keeps the loop finite).
+__global__ void k_ploc_pairs(int m, int *nn) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int j = i ^ 1;
+    nn[i] = j < m ? j : i;
+}
+
+// keep flag (low 32 bits) and new-node flag (high 32 bits) per cluster position
+__global__ void k_ploc_flags(const int *nn, int m, unsigned long long *flags) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int j = nn[i];
+    const bool mutual = j != i && nn[j] == i;
+    const unsigned long long keep = mutual && i > j ? 0ull : 1ull;
+    const unsigned long long leader = mutual && i < j ? 1ull : 0ull;
+    flags[i] = keep | leader << 32;
+}
+
+__global__ void k_ploc_merge(const int *clusters, const float *cbox, const int *nn, int m, const unsigned long long *pre, int nodeBase,
+                             int *clustersOut, float *cboxOut, int2 *children, float *nodeBoxes, int *cnt, int *parentNode, int *parentLeaf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int j = nn[i];
+    const bool mutual = j != i && nn[j] == i;
+    if (mutual && i > j) return;
+    const int pos = (int)(pre[i] & 0xffffffffull);
+    const float *a = &cbox[6 * (size_t)i];
+    if (!mutual) {
+        clustersOut[pos] = clusters[i];
+        for (int k = 0; k < 6; k++) cboxOut[6 * (size_t)pos + k] = a[k];
+        return;
+    }
+    const int id = nodeBase + (int)(pre[i] >> 32);
+    const int l = clusters[i], r = clusters[j];
+    const float *b = &cbox[6 * (size_t)j];
+    float u[6];
+    for (int k = 0; k < 3; k++) {
+        u[k] = fminf(a[k], b[k]);
+        u[3 + k] = fmaxf(a[3 + k], b[3 + k]);
+    }
+    children[id] = make_int2(l, r);
+    for (int k = 0; k < 6; k++) {
+        nodeBoxes[6 * (size_t)id + k] = u[k];
+        cboxOut[6 * (size_t)pos + k] = u[k];
+    }
+    cnt[id] = (l < 0 ? 1 : cnt[l]) + (r < 0 ? 1 : cnt[r]);
+    if (l < 0) parentLeaf[~l] = id; else parentNode[l] = id;
+    if (r < 0) parentLeaf[~r] = id; else parentNode[r] = id;
+    clustersOut[pos] = id;
+}
+
+// depth-first leaf order: a leaf's position is the number of leaves in the left subtrees its
+// root path passes
+__global__ void k_leaf_positions(BinTree t, int *leafPos) {
+    const int leaf = blockIdx.x * blockDim.x + threadIdx.x;
+    if (leaf >= t.n) return;
+    int off = 0, cur = ~leaf, p = t.parentLeaf[leaf];
+    // a root path has fewer than n nodes; the step bound only guards against a malformed tree
+    for (int steps = 0; p >= 0 && steps < t.n; steps++) {
+        const int2 c = t.children[p];
+        if (c.y == cur) off += childCount(t, c.x);
+        cur = p;
+        p = t.parentNode[p];
+    }
+    leafPos[leaf] = off;
+}
+
+// first position of every internal node: written by its leftmost leaf
+__global__ void k_node_first(BinTree t, int *first) {
+    const int leaf = blockIdx.x * blockDim.x + threadIdx.x;
+    if (leaf >= t.n) return;
+    const int o = t.leafPos[leaf];
+    int cur = ~leaf, p = t.parentLeaf[leaf];
+    for (int steps = 0; p >= 0 && t.children[p].x == cur && steps < t.n; steps++) {
+        first[p] = o;
+        cur = p;
+        p = t.parentNode[p];
+    }
+}
+
+__global__ void k_scatter_prims(const float4 *in, const int *leafPos, int n, float4 *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int o = leafPos[i];
+    out[3 * (size_t)o] = in[3 * (size_t)i];
+    out[3 * (size_t)o + 1] = in[3 * (size_t)i + 1];
+    out[3 * (size_t)o + 2] = in[3 * (size_t)i + 2];
+}
+
+// --- 4-wide collapse -----------------------------------------------------------------------
+// A 4-wide node costs kNodeCost x its area, a leaf kPrimCost x its area x its primitives.
+// Bottom up, every live internal node (more than PT_LEAF_MAX primitives) gets the cheapest way
+// to spend 1..3 child slots of its parent on its subtree: one slot is the node itself as a
+// 4-wide node (W), two or three spread its subtree over its children (S2, S3), opening it.
 constexpr float kNodeCost = 1.0f;
 constexpr float kPrimCost = 0.6f;
-int collapseWide(const std::vector<int2> &children, const std::vector<int2> &ranges, const std::vector<float> &nodeBoxes,
-                 const std::vector<float> &leafBoxes, std::vector<float4> &out, int *depth) {
-    auto live = [&](int c) { return c >= 0 && ranges[c].y - ranges[c].x + 1 > PT_LEAF_MAX; };
-    auto box = [&](int c) { return c >= 0 ? &nodeBoxes[6 * (size_t)c] : &leafBoxes[6 * (size_t)(~c)]; };
-    auto area = [&](int c) {
-        const float *b = box(c);
-        const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
-        return dx * dy + dy * dz + dz * dx;
-    };
-    auto leafRef = [&](int c) {
-        if (c < 0) return ~(~c);
-        const int2 r = ranges[c];
-        return ~(((r.y - r.x) << PT_LEAF_COUNT_SHIFT) | r.x);
-    };
-    // bottom-up costs over the live internal nodes (children before parents)
-    const size_t ni = children.size();
-    std::vector<float> W(ni), S2(ni), S3(ni);   // as a 4-wide node; spread over 2 / 3 slots
-    std::vector<uint8_t> split(3 * ni);         // slots of the left child when the children share 2..4
-    std::vector<uint8_t> opened(ni);            // bit k: spreading over k + 2 slots opens the node
-    auto cost = [&](int x, int slots) -> float {
-        if (!live(x)) {
-            const int cnt = x < 0 ? 1 : ranges[x].y - ranges[x].x + 1;
-            return kPrimCost * area(x) * (float)cnt;
-        }
-        return slots == 1 ? W[x] : slots == 2 ? S2[x] : S3[x];
-    };
-    std::vector<int> order, todo;
-    todo.push_back(0);
-    while (!todo.empty()) {
-        const int x = todo.back();
-        todo.pop_back();
-        order.push_back(x);
-        if (live(children[x].x)) todo.push_back(children[x].x);
-        if (live(children[x].y)) todo.push_back(children[x].y);
-    }
-    for (size_t k = order.size(); k-- > 0;) {
-        const int x = order[k];
-        const int l = children[x].x, r = children[x].y;
-        float share[5] = {0.f, 0.f, 0.f, 0.f, 0.f};   // both children in j slots
-        for (int j = 2; j <= 4; j++) {
-            float best = INFINITY;
-            int bi = 1;
-            for (int i = 1; i < j; i++) {
-                const float c = cost(l, std::min(i, 3)) + cost(r, std::min(j - i, 3));
-                if (c < best) {
-                    best = c;
-                    bi = i;
+
+struct CollapseTables {
+    float *W, *S2, *S3;
+    unsigned char *split;   // 3 per node: slots of the left child when the children share 2..4
+    unsigned char *opened;  // bit k: spreading over k + 2 slots opens the node
+};
+
+__device__ __forceinline__ bool liveChild(const BinTree &t, int c) { return c >= 0 && t.cnt[c] > PT_LEAF_MAX; }
+__device__ __forceinline__ const float *boxOf(const BinTree &t, int c) {
+    return c >= 0 ? &t.nodeBoxes[6 * (size_t)c] : &t.leafBoxes[6 * (size_t)(~c)];
+}
+
+// cost of child c given `slots` of its parent (tables of live children are read volatile:
+// other workgroups wrote them during this kernel)
+__device__ float slotCost(const BinTree &t, const CollapseTables &tb, int c, int slots) {
+    if (!liveChild(t, c)) return kPrimCost * boxArea6(boxOf(t, c)) * (float)childCount(t, c);
+    const volatile float *src = slots == 1 ? tb.W : slots == 2 ? tb.S2 : tb.S3;
+    return src[c];
+}
+
+__global__ void k_collapse_costs(BinTree t, CollapseTables tb, int *arrivals) {
+    const int leaf = blockIdx.x * blockDim.x + threadIdx.x;
+    if (leaf >= t.n) return;
+    int p = t.parentLeaf[leaf];
+    for (int steps = 0; p >= 0 && steps < t.n; steps++) {
+        __threadfence();
+        if (atomicAdd(&arrivals[p], 1) == 0) break;  // the sibling subtree is not done yet
+        __threadfence();
+        if (t.cnt[p] > PT_LEAF_MAX) {
+            const int2 c = t.children[p];
+            float share[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+            for (int j = 2; j <= 4; j++) {
+                float best = INFINITY;
+                int bi = 1;
+                for (int i = 1; i < j; i++) {
+                    const float v = slotCost(t, tb, c.x, min(i, 3)) + slotCost(t, tb, c.y, min(j - i, 3));
+                    if (v < best) {
+                        best = v;
+                        bi = i;
+                    }
                 }
+                share[j] = best;
+                tb.split[3 * (size_t)p + (j - 2)] = (unsigned char)bi;
             }
-            share[j] = best;
-            split[3 * x + (j - 2)] = (uint8_t)bi;
+            const float w = kNodeCost * boxArea6(&t.nodeBoxes[6 * (size_t)p]) + share[4];
+            volatile float *W = tb.W, *S2 = tb.S2, *S3 = tb.S3;
+            W[p] = w;
+            S2[p] = fminf(w, share[2]);
+            S3[p] = fminf(w, share[3]);
+            tb.opened[p] = (unsigned char)((share[2] < w ? 1 : 0) | (share[3] < w ? 2 : 0));
         }
-        W[x] = kNodeCost * area(x) + share[4];
-        S2[x] = std::min(W[x], share[2]);
-        S3[x] = std::min(W[x], share[3]);
-        opened[x] = (uint8_t)((share[2] < W[x] ? 1 : 0) | (share[3] < W[x] ? 2 : 0));
+        p = t.parentNode[p];
     }
-    // the children of a 4-wide node: its binary node's two children sharing 4 slots, opened
-    // where the cost tables say so
-    auto take = [&](auto &&self, int x, int slots, int *kids, int &nk) -> void {
-        if (!live(x) || slots == 1 || !(opened[x] & (1 << (slots - 2)))) {
-            kids[nk++] = x;
-            return;
+}
+
+// The children of the 4-wide node made from binary node x: its two children sharing four
+// slots, opened where the cost tables say so, in left-to-right order.
+__device__ int wideChildren(const BinTree &t, const CollapseTables &tb, int x, int *kids) {
+    int stackNode[4], stackSlots[4], sp = 0, nk = 0;
+    const int2 c = t.children[x];
+    const int i4 = tb.split[3 * (size_t)x + 2];
+    stackNode[sp] = c.y;
+    stackSlots[sp++] = 4 - i4;
+    stackNode[sp] = c.x;
+    stackSlots[sp++] = i4;
+    while (sp > 0) {
+        sp--;
+        const int y = stackNode[sp], s = stackSlots[sp];
+        if (!liveChild(t, y) || s == 1 || !(tb.opened[y] & (1 << (s - 2)))) {
+            kids[nk++] = y;
+            continue;
         }
-        const int i = split[3 * x + (slots - 2)];
-        self(self, children[x].x, i, kids, nk);
-        self(self, children[x].y, slots - i, kids, nk);
-    };
-    out.clear();
-    out.resize(PT_NODE_F4);
-    int count = 1;
-    *depth = 1;
-    struct Item {
-        int bin, idx, level;
-    };
-    std::vector<Item> stack;
-    stack.push_back({0, 0, 1});
-    while (!stack.empty()) {
-        const Item it = stack.back();
-        stack.pop_back();
-        int kids[4] = {0, 0, 0, 0};
-        int nk = 0;
-        const int i4 = split[3 * it.bin + 2];
-        take(take, children[it.bin].x, i4, kids, nk);
-        take(take, children[it.bin].y, 4 - i4, kids, nk);
-        float lo[3][4], hi[3][4];
-        int ref[4];
-        for (int k = 0; k < 4; k++) {
-            if (k < nk) {
-                const float *b = box(kids[k]);
-                for (int a = 0; a < 3; a++) {
-                    lo[a][k] = b[a];
-                    hi[a][k] = b[3 + a];
-                }
-                if (live(kids[k])) {
-                    ref[k] = count++;
-                    out.resize((size_t)PT_NODE_F4 * count);
-                    stack.push_back({kids[k], ref[k], it.level + 1});
-                    *depth = std::max(*depth, it.level + 1);
-                } else {
-                    ref[k] = leafRef(kids[k]);
-                }
+        const int i = tb.split[3 * (size_t)y + (s - 2)];
+        const int2 cy = t.children[y];
+        stackNode[sp] = cy.y;
+        stackSlots[sp++] = s - i;
+        stackNode[sp] = cy.x;
+        stackSlots[sp++] = i;
+    }
+    return nk;
+}
+
+__global__ void k_level_count(BinTree t, CollapseTables tb, const int2 *frontier, int m, unsigned *live) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    int kids[4];
+    const int nk = wideChildren(t, tb, frontier[i].x, kids);
+    unsigned c = 0;
+    for (int k = 0; k < nk; k++) c += liveChild(t, kids[k]) ? 1u : 0u;
+    live[i] = c;
+}
+
+// Writes the level's nodes; live children go to the next level at nodeBase + their rank.
+__global__ void k_level_write(BinTree t, CollapseTables tb, const int2 *frontier, int m, const unsigned *rank, int nodeBase, int2 *next,
+                              float4 *nodes) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int2 f = frontier[i];
+    int kids[4];
+    const int nk = wideChildren(t, tb, f.x, kids);
+    float lo[3][4], hi[3][4];
+    int ref[4];
+    int r = (int)rank[i];
+    for (int k = 0; k < 4; k++) {
+        if (k < nk) {
+            const float *b = boxOf(t, kids[k]);
+            for (int a = 0; a < 3; a++) {
+                lo[a][k] = b[a];
+                hi[a][k] = b[3 + a];
+            }
+            const int y = kids[k];
+            if (liveChild(t, y)) {
+                ref[k] = nodeBase + r;
+                next[r] = make_int2(y, nodeBase + r);
+                r++;
             } else {
-                // an inverted box: every slab test misses it
-                for (int a = 0; a < 3; a++) {
-                    lo[a][k] = 1e30f;
-                    hi[a][k] = -1e30f;
-                }
-                ref[k] = PT_EMPTY_CHILD;
+                const int cnt = childCount(t, y);
+                const int first = y < 0 ? (t.leafPos ? t.leafPos[~y] : ~y) : t.first[y];
+                ref[k] = ~(((cnt - 1) << PT_LEAF_COUNT_SHIFT) | first);
             }
+        } else {
+            // an inverted box: every slab test misses it
+            for (int a = 0; a < 3; a++) {
+                lo[a][k] = 1e30f;
+                hi[a][k] = -1e30f;
+            }
+            ref[k] = PT_EMPTY_CHILD;
         }
-        float4 *o = &out[(size_t)PT_NODE_F4 * it.idx];
-        for (int a = 0; a < 3; a++) {
-            o[2 * a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
-            o[2 * a + 1] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
-        }
-        o[6] = make_float4(asFloat(ref[0]), asFloat(ref[1]), asFloat(ref[2]), asFloat(ref[3]));
-        o[7] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    return count;
+    float4 *o = &nodes[(size_t)PT_NODE_F4 * f.y];
+    for (int a = 0; a < 3; a++) {
+        o[2 * a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
+        o[2 * a + 1] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
+    }
+    o[6] = make_float4(asFloat(ref[0]), asFloat(ref[1]), asFloat(ref[2]), asFloat(ref[3]));
+    o[7] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Collapses the binary tree into 4-wide nodes (root = node 0); returns the node count and the
+// number of levels.
+int collapseWide(const BinTree &t, DevBuf &nodes, hipStream_t stream, int *levels) {
+    const int ni = t.n - 1;
+    DevBuf w, s2, s3, split, opened, arrivals, live, scanTmp, front[2], tmpNodes;
+    w.alloc(sizeof(float) * ni);
+    s2.alloc(sizeof(float) * ni);
+    s3.alloc(sizeof(float) * ni);
+    split.alloc((size_t)3 * ni);
+    opened.alloc((size_t)ni);
+    arrivals.alloc(sizeof(int) * ni);
+    CollapseTables tb{w.as<float>(), s2.as<float>(), s3.as<float>(), split.as<unsigned char>(), opened.as<unsigned char>()};
+    k_fill_int<<<blocks(ni, 256), 256, 0, stream>>>(arrivals.as<int>(), ni, 0);
+    k_collapse_costs<<<blocks(t.n, 256), 256, 0, stream>>>(t, tb, arrivals.as<int>());
+    PT_HIP_TRY(hipGetLastError());
+    // every 4-wide node stands for a distinct live internal node: at most ni of them
+    tmpNodes.alloc(sizeof(float4) * PT_NODE_F4 * (size_t)std::max(ni, 1));
+    live.alloc(sizeof(unsigned) * ((size_t)ni + 1));
+    scanTmp.alloc(sizeof(unsigned) * (size_t)scanTmpEntries(ni + 1));
+    front[0].alloc(sizeof(int2) * (size_t)ni);
+    front[1].alloc(sizeof(int2) * (size_t)ni);
+    const int2 root = make_int2(t.root, 0);
+    PT_HIP_TRY(hipMemcpyAsync(front[0].p, &root, sizeof root, hipMemcpyHostToDevice, stream));
+    int m = 1, total = 1, level = 1, cur = 0;
+    while (true) {
+        k_level_count<<<blocks(m, 256), 256, 0, stream>>>(t, tb, front[cur].as<int2>(), m, live.as<unsigned>());
+        exclusiveScan<unsigned>(live.as<unsigned>(), m, scanTmp.as<unsigned>(), stream);
+        k_level_write<<<blocks(m, 256), 256, 0, stream>>>(t, tb, front[cur].as<int2>(), m, live.as<unsigned>(), total,
+                                                          front[1 - cur].as<int2>(), tmpNodes.as<float4>());
+        PT_HIP_TRY(hipGetLastError());
+        unsigned nextM = 0;
+        PT_HIP_TRY(hipMemcpyAsync(&nextM, live.as<unsigned>() + m, sizeof nextM, hipMemcpyDeviceToHost, stream));
+        PT_HIP_TRY(hipStreamSynchronize(stream));
+        if (nextM == 0) break;
+        if (total + (int64_t)nextM > ni) throw HipFailure("BVH collapse: more 4-wide nodes than internal nodes");
+        total += (int)nextM;
+        m = (int)nextM;
+        level++;
+        cur = 1 - cur;
+    }
+    nodes.alloc(sizeof(float4) * PT_NODE_F4 * (size_t)total);
+    PT_HIP_TRY(hipMemcpyAsync(nodes.p, tmpNodes.p, sizeof(float4) * PT_NODE_F4 * (size_t)total, hipMemcpyDeviceToDevice, stream));
+    // temporaries are freed on return
+    PT_HIP_TRY(hipStreamSynchronize(stream));
+    *levels = level;
+    return total;
 }
 
 }  // namespace
@@ -432,7 +734,7 @@ int deviceCUs() {
 }
 
 void buildBvh(const float4 *primsIn, int n, const float lo[3], const float hi[3], DevBuf &nodes, float4 *primsSorted, hipStream_t stream,
-              BvhInfo *info) {
+              BvhInfo *info, int builder) {
     info->numNodes = 0;
     info->depth = 0;
     info->binaryDepth = 0;
@@ -475,40 +777,94 @@ void buildBvh(const float4 *primsIn, int n, const float lo[3], const float hi[3]
         return;
     }
     const int ni = n - 1;
-    DevBuf children, ranges, parentNode, parentLeaf, nodeBoxes, arrivals, counters;
+    DevBuf children, cnt, first, parentNode, parentLeaf, nodeBoxes, counters, leafPos;
     children.alloc(sizeof(int2) * ni);
-    ranges.alloc(sizeof(int2) * ni);
+    cnt.alloc(sizeof(int) * ni);
+    first.alloc(sizeof(int) * ni);
     parentNode.alloc(sizeof(int) * ni);
     parentLeaf.alloc(sizeof(int) * n);
     nodeBoxes.alloc(sizeof(float) * 6 * ni);
-    arrivals.alloc(sizeof(int) * ni);
     counters.alloc(sizeof(int) * 2);
     k_fill_int<<<blocks(ni, 256), 256, 0, stream>>>(parentNode.as<int>(), ni, -1);
-    k_fill_int<<<blocks(ni, 256), 256, 0, stream>>>(arrivals.as<int>(), ni, 0);
     k_fill_int<<<1, 64, 0, stream>>>(counters.as<int>(), 2, 0);
-    k_karras<<<blocks(ni, 256), 256, 0, stream>>>(kin, n, children.as<int2>(), ranges.as<int2>(), parentNode.as<int>(), parentLeaf.as<int>());
-    k_refit<<<blocks(n, 256), 256, 0, stream>>>(children.as<int2>(), parentNode.as<int>(), parentLeaf.as<int>(), boxes2.as<float>(),
-                                                 nodeBoxes.as<float>(), arrivals.as<int>(), n);
+    BinTree t;
+    t.n = n;
+    t.children = children.as<int2>();
+    t.nodeBoxes = nodeBoxes.as<float>();
+    t.leafBoxes = boxes2.as<float>();
+    t.cnt = cnt.as<int>();
+    t.first = first.as<int>();
+    t.parentNode = parentNode.as<int>();
+    t.parentLeaf = parentLeaf.as<int>();
+    if (builder == PT_BVH_PLOC) {
+        // clusters start as the Morton-ordered leaves; each iteration merges mutual nearest
+        // neighbours, whose new nodes take the next indices in position order
+        DevBuf clus[2], cbox[2], nn, flags, scanTmp, primsTmp;
+        for (int k = 0; k < 2; k++) {
+            clus[k].alloc(sizeof(int) * n);
+            cbox[k].alloc(sizeof(float) * 6 * n);
+        }
+        nn.alloc(sizeof(int) * n);
+        flags.alloc(sizeof(unsigned long long) * ((size_t)n + 1));
+        scanTmp.alloc(sizeof(unsigned long long) * (size_t)scanTmpEntries(n + 1));
+        k_ploc_init<<<blocks(n, 256), 256, 0, stream>>>(n, boxes2.as<float>(), clus[0].as<int>(), cbox[0].as<float>());
+        int m = n, nodeBase = 0, cur = 0;
+        while (m > 1) {
+            k_ploc_nn<<<blocks(m, kPlocThreads), kPlocThreads, 0, stream>>>(cbox[cur].as<float>(), m, nn.as<int>());
+            for (int attempt = 0; attempt < 2; attempt++) {
+                if (attempt == 1) k_ploc_pairs<<<blocks(m, 256), 256, 0, stream>>>(m, nn.as<int>());
+                k_ploc_flags<<<blocks(m, 256), 256, 0, stream>>>(nn.as<int>(), m, flags.as<unsigned long long>());
+                exclusiveScan<unsigned long long>(flags.as<unsigned long long>(), m, scanTmp.as<unsigned long long>(), stream);
+                unsigned long long tot = 0;
+                PT_HIP_TRY(hipMemcpyAsync(&tot, flags.as<unsigned long long>() + m, sizeof tot, hipMemcpyDeviceToHost, stream));
+                PT_HIP_TRY(hipStreamSynchronize(stream));
+                const int merged = (int)(tot >> 32);
+                if (merged == 0 && attempt == 0) continue;  // no mutual pair: merge neighbours pairwise
+                if (merged == 0 || nodeBase + merged > ni) throw HipFailure("PLOC: clustering made no progress");
+                k_ploc_merge<<<blocks(m, 256), 256, 0, stream>>>(clus[cur].as<int>(), cbox[cur].as<float>(), nn.as<int>(), m,
+                                                                 flags.as<unsigned long long>(), nodeBase, clus[1 - cur].as<int>(),
+                                                                 cbox[1 - cur].as<float>(), children.as<int2>(), nodeBoxes.as<float>(),
+                                                                 cnt.as<int>(), parentNode.as<int>(), parentLeaf.as<int>());
+                PT_HIP_TRY(hipGetLastError());
+                nodeBase += merged;
+                m = (int)(tot & 0xffffffffull);
+                cur = 1 - cur;
+                break;
+            }
+        }
+        if (nodeBase != ni) throw HipFailure("PLOC: the tree does not have n - 1 internal nodes");
+        t.root = ni - 1;  // the last merge
+        // depth-first leaf order, then every internal node's first position
+        leafPos.alloc(sizeof(int) * n);
+        k_leaf_positions<<<blocks(n, 256), 256, 0, stream>>>(t, leafPos.as<int>());
+        t.leafPos = leafPos.as<int>();
+        k_node_first<<<blocks(n, 256), 256, 0, stream>>>(t, first.as<int>());
+        primsTmp.alloc(sizeof(float4) * 3 * (size_t)n);
+        PT_HIP_TRY(hipMemcpyAsync(primsTmp.p, primsSorted, sizeof(float4) * 3 * (size_t)n, hipMemcpyDeviceToDevice, stream));
+        k_scatter_prims<<<blocks(n, 256), 256, 0, stream>>>(primsTmp.as<float4>(), leafPos.as<int>(), n, primsSorted);
+        PT_HIP_TRY(hipGetLastError());
+        PT_HIP_TRY(hipStreamSynchronize(stream));  // primsTmp is freed on leaving this block
+    } else {
+        DevBuf ranges, arrivals;
+        ranges.alloc(sizeof(int2) * ni);
+        arrivals.alloc(sizeof(int) * ni);
+        k_fill_int<<<blocks(ni, 256), 256, 0, stream>>>(arrivals.as<int>(), ni, 0);
+        k_karras<<<blocks(ni, 256), 256, 0, stream>>>(kin, n, children.as<int2>(), ranges.as<int2>(), parentNode.as<int>(), parentLeaf.as<int>());
+        k_refit<<<blocks(n, 256), 256, 0, stream>>>(children.as<int2>(), parentNode.as<int>(), parentLeaf.as<int>(), boxes2.as<float>(),
+                                                     nodeBoxes.as<float>(), arrivals.as<int>(), n);
+        k_ranges_to_counts<<<blocks(ni, 256), 256, 0, stream>>>(ranges.as<int2>(), ni, cnt.as<int>(), first.as<int>());
+        PT_HIP_TRY(hipGetLastError());
+        PT_HIP_TRY(hipStreamSynchronize(stream));
+        t.root = 0;
+    }
     k_depth<<<blocks(n, 256), 256, 0, stream>>>(parentNode.as<int>(), parentLeaf.as<int>(), n, counters.as<int>());
     PT_HIP_TRY(hipGetLastError());
-    // the collapse runs on the host over the refitted binary tree
-    std::vector<int2> hChildren(ni), hRanges(ni);
-    std::vector<float> hNodeBoxes((size_t)6 * ni), hLeafBoxes((size_t)6 * n);
+    int levels = 1;
+    const int count = collapseWide(t, nodes, stream, &levels);
     int hc[2] = {0, 0};
-    PT_HIP_TRY(hipMemcpyAsync(hChildren.data(), children.p, sizeof(int2) * ni, hipMemcpyDeviceToHost, stream));
-    PT_HIP_TRY(hipMemcpyAsync(hRanges.data(), ranges.p, sizeof(int2) * ni, hipMemcpyDeviceToHost, stream));
-    PT_HIP_TRY(hipMemcpyAsync(hNodeBoxes.data(), nodeBoxes.p, sizeof(float) * 6 * ni, hipMemcpyDeviceToHost, stream));
-    PT_HIP_TRY(hipMemcpyAsync(hLeafBoxes.data(), boxes2.p, sizeof(float) * 6 * n, hipMemcpyDeviceToHost, stream));
-    PT_HIP_TRY(hipMemcpyAsync(hc, counters.p, sizeof hc, hipMemcpyDeviceToHost, stream));
-    PT_HIP_TRY(hipStreamSynchronize(stream));
-    std::vector<float4> hNodes;
-    int wideDepth = 1;
-    const int count = collapseWide(hChildren, hRanges, hNodeBoxes, hLeafBoxes, hNodes, &wideDepth);
-    nodes.alloc(sizeof(float4) * PT_NODE_F4 * (size_t)count);
-    PT_HIP_TRY(hipMemcpyAsync(nodes.p, hNodes.data(), sizeof(float4) * hNodes.size(), hipMemcpyHostToDevice, stream));
-    PT_HIP_TRY(hipStreamSynchronize(stream));
+    PT_HIP_TRY(hipMemcpy(hc, counters.p, sizeof hc, hipMemcpyDeviceToHost));
     info->numNodes = count;
-    info->depth = wideDepth;
+    info->depth = levels;
     info->binaryDepth = hc[0];
 }
 

@@ -52,11 +52,15 @@ struct BvhInfo {
     int binaryDepth = 0;   // levels of internal nodes of the binary radix tree
 };
 
-// Builds the LBVH over `n` primitive records (3 float4 each, input order) on `stream` and
-// collapses it into a 4-wide BVH: allocates `nodes` (PT_NODE_F4 float4 per node) and writes
-// the records in BVH order to `primsSorted`.  lo/hi bound all primitive centroids.
+// Binary-tree builders of buildBvh.
+enum { PT_BVH_LBVH = 0, PT_BVH_PLOC = 1 };
+
+// Builds a binary BVH over `n` primitive records (3 float4 each, input order) on `stream`
+// (builder: PT_BVH_LBVH or PT_BVH_PLOC) and collapses it into a 4-wide BVH on the GPU:
+// allocates `nodes` (PT_NODE_F4 float4 per node) and writes the records in BVH order to
+// `primsSorted`.  lo/hi bound all primitive centroids.
 void buildBvh(const float4 *primsIn, int n, const float lo[3], const float hi[3], DevBuf &nodes,
-              float4 *primsSorted, hipStream_t stream, BvhInfo *info);
+              float4 *primsSorted, hipStream_t stream, BvhInfo *info, int builder);
 
 // Environment tables in FP64 on the GPU: per-row and marginal alias tables packed as
 // (prob as float, alias as int bits), texel pdf.  Optionally returns the FP64 tables.

@@ -529,6 +529,292 @@ bool unzipBlock(const unsigned char *src, size_t n, std::vector<unsigned char> &
     return true;
 }
 
+// ---------------------------------------------------------------------------------------------
+// PIZ blocks (OpenEXR's wavelet + Huffman codec, 32 scanlines per block).  A block holds, in this
+// order: the range [lo, hi] of a bitmap of the 16-bit values present, those bitmap bytes, the
+// length of the Huffman stream, and the stream.  Decoding reverses the encoder's three steps:
+// Huffman decoding to 16-bit words, the 2D Haar-like wavelet per channel plane, and the
+// value-compaction table built from the bitmap.  The bit layouts and arithmetic below are the
+// format's; this is an independent implementation of them.
+// ---------------------------------------------------------------------------------------------
+namespace piz {
+
+constexpr int kRange = 1 << 16;             // 16-bit values
+constexpr int kBitmapBytes = kRange / 8;
+constexpr int kMaxCodeLen = 58;
+constexpr int kShortZeroRun = 59;           // 6-bit length codes 59..62: 2..5 zero lengths
+constexpr int kLongZeroRun = 63;            // 63 + 8 bits: 6..261 zero lengths
+constexpr int kShortestLongRun = 2 + kLongZeroRun - kShortZeroRun;
+
+// MSB-first bit reader over [p, end) with at most `bits` bits in total.
+struct Bits {
+    const unsigned char *p = nullptr, *end = nullptr;
+    int64_t left = 0;   // bits still allowed
+    uint32_t cur = 0;
+    int have = 0;       // bits of cur not yet consumed
+    bool get(int n, uint64_t &v) {
+        v = 0;
+        for (int i = 0; i < n; i++) {
+            if (left <= 0) return false;
+            if (have == 0) {
+                if (p >= end) return false;
+                cur = *p++;
+                have = 8;
+            }
+            have--;
+            left--;
+            v = (v << 1) | ((cur >> have) & 1u);
+        }
+        return true;
+    }
+};
+
+uint32_t readU32(const unsigned char *q) { return (uint32_t)q[0] | (uint32_t)q[1] << 8 | (uint32_t)q[2] << 16 | (uint32_t)q[3] << 24; }
+
+// Huffman stream: 20-byte header (lowest symbol, highest symbol = the run-length pseudo-symbol,
+// table bytes, stream bits, reserved), the run-length-coded table of code lengths, then the
+// codes.  Codes are canonical: per length from the longest down, the first code of length l
+// is (first code of length l + 1 + count of length l + 1) / 2, and symbols of one length take
+// consecutive codes in increasing order.  The pseudo-symbol repeats the previous word
+// (8-bit count).
+bool hufDecode(const unsigned char *in, size_t n, uint16_t *out, size_t nOut, std::string &err) {
+    if (n == 0) {
+        if (nOut != 0) err = "PIZ: empty Huffman stream";
+        return nOut == 0;
+    }
+    if (n < 20) {
+        err = "PIZ: truncated Huffman header";
+        return false;
+    }
+    const uint32_t im = readU32(in), iM = readU32(in + 4), tableBytes = readU32(in + 8), nBits = readU32(in + 12);
+    if (im > iM || iM >= (uint32_t)kRange + 1 || tableBytes > n - 20) {
+        err = "PIZ: bad Huffman table header";
+        return false;
+    }
+    std::vector<uint8_t> len((size_t)kRange + 1, 0);
+    Bits tb;
+    tb.p = in + 20;
+    tb.end = in + 20 + tableBytes;
+    tb.left = (int64_t)tableBytes * 8;
+    for (uint32_t sym = im; sym <= iM; sym++) {
+        uint64_t l = 0;
+        if (!tb.get(6, l)) {
+            err = "PIZ: truncated code-length table";
+            return false;
+        }
+        if (l == kLongZeroRun || l >= kShortZeroRun) {
+            uint64_t run = 0;
+            if (l == kLongZeroRun) {
+                if (!tb.get(8, run)) {
+                    err = "PIZ: truncated code-length table";
+                    return false;
+                }
+                run += kShortestLongRun;
+            } else {
+                run = l - kShortZeroRun + 2;
+            }
+            if (sym + run - 1 > iM) {
+                err = "PIZ: code-length run past the last symbol";
+                return false;
+            }
+            sym += (uint32_t)run - 1;  // lengths stay 0
+            continue;
+        }
+        len[sym] = (uint8_t)l;
+    }
+    // canonical codes
+    uint64_t count[kMaxCodeLen + 2] = {0}, first[kMaxCodeLen + 2] = {0};
+    for (uint32_t sym = im; sym <= iM; sym++) count[len[sym]]++;
+    uint64_t c = 0;
+    for (int l = kMaxCodeLen; l > 0; l--) {
+        first[l] = c;
+        c = (c + count[l]) >> 1;
+    }
+    std::vector<std::vector<uint32_t>> byLen(kMaxCodeLen + 1);
+    for (uint32_t sym = im; sym <= iM; sym++)
+        if (len[sym] > 0) byLen[len[sym]].push_back(sym);
+    // decode
+    Bits bs;
+    bs.p = in + 20 + tableBytes;
+    bs.end = in + n;
+    bs.left = nBits;
+    if ((uint64_t)nBits > (uint64_t)(bs.end - bs.p) * 8) {
+        err = "PIZ: Huffman stream shorter than its bit count";
+        return false;
+    }
+    size_t o = 0;
+    while (bs.left > 0 && o < nOut) {
+        uint64_t code = 0;
+        int l = 0;
+        uint32_t sym = 0;
+        bool found = false;
+        while (!found) {
+            uint64_t bit = 0;
+            if (l == kMaxCodeLen || !bs.get(1, bit)) break;
+            code = (code << 1) | bit;
+            l++;
+            if (count[l] && code >= first[l] && code - first[l] < count[l]) {
+                sym = byLen[l][(size_t)(code - first[l])];
+                found = true;
+            }
+        }
+        if (!found) {
+            // the last byte's padding bits may not form a code
+            if (bs.left <= 0) break;
+            err = "PIZ: invalid Huffman code";
+            return false;
+        }
+        if (sym == iM) {
+            uint64_t rep = 0;
+            if (!bs.get(8, rep) || o == 0 || o + rep > nOut) {
+                err = "PIZ: bad run of repeated words";
+                return false;
+            }
+            const uint16_t prev = out[o - 1];
+            for (uint64_t k = 0; k < rep; k++) out[o++] = prev;
+        } else {
+            out[o++] = (uint16_t)sym;
+        }
+    }
+    if (o != nOut) {
+        err = "PIZ: Huffman stream holds " + std::to_string(o) + " words, the block needs " + std::to_string(nOut);
+        return false;
+    }
+    return true;
+}
+
+// One inverse lifting step of the wavelet: 14-bit variant (values < 2^14, signed arithmetic on
+// 16-bit words) and 16-bit variant (modular arithmetic).
+inline void unlift14(uint16_t l, uint16_t h, uint16_t &a, uint16_t &b) {
+    const int hs = (int16_t)h;
+    const int as = (int16_t)l + (hs & 1) + (hs >> 1);
+    a = (uint16_t)(int16_t)as;
+    b = (uint16_t)(int16_t)(as - hs);
+}
+inline void unlift16(uint16_t l, uint16_t h, uint16_t &a, uint16_t &b) {
+    const int m = l, d = h;
+    const int bb = (m - (d >> 1)) & 0xffff;
+    const int aa = (d + bb - 0x8000) & 0xffff;
+    a = (uint16_t)aa;
+    b = (uint16_t)bb;
+}
+
+// Inverse 2D wavelet of an nx x ny plane (element (x, y) at v[x * ox + y * oy]), coarsest level
+// first.  maxValue < 2^14 selects the 14-bit lifting.
+void unwavelet(uint16_t *v, int nx, int ox, int ny, int oy, uint16_t maxValue) {
+    auto step = maxValue < (1 << 14) ? unlift14 : unlift16;
+    const int n = std::min(nx, ny);
+    int p = 1;
+    while (p <= n) p <<= 1;
+    p >>= 1;
+    int p2 = p;
+    p >>= 1;
+    while (p >= 1) {
+        const int oy1 = oy * p, ox1 = ox * p;
+        int y = 0;
+        for (; y + p2 <= ny; y += p2) {
+            uint16_t *row = v + (size_t)y * oy;
+            int x = 0;
+            for (; x + p2 <= nx; x += p2) {
+                uint16_t *q00 = row + (size_t)x * ox, *q01 = q00 + ox1, *q10 = q00 + oy1, *q11 = q10 + ox1;
+                uint16_t i00, i01, i10, i11;
+                step(*q00, *q10, i00, i10);
+                step(*q01, *q11, i01, i11);
+                step(i00, i01, *q00, *q01);
+                step(i10, i11, *q10, *q11);
+            }
+            if (nx & p) {
+                uint16_t *q00 = row + (size_t)x * ox, *q10 = q00 + oy1;
+                uint16_t a;
+                step(*q00, *q10, a, *q10);
+                *q00 = a;
+            }
+        }
+        if (ny & p) {
+            uint16_t *row = v + (size_t)y * oy;
+            for (int x = 0; x + p2 <= nx; x += p2) {
+                uint16_t *q00 = row + (size_t)x * ox, *q01 = q00 + ox1;
+                uint16_t a;
+                step(*q00, *q01, a, *q01);
+                *q00 = a;
+            }
+        }
+        p2 = p;
+        p >>= 1;
+    }
+}
+
+// One PIZ block of `lines` scanlines, `w` pixels wide, channels of `words` 16-bit words per
+// pixel (1 half, 2 float/uint), decoded into `out` in scanline order (per line, each channel's
+// w pixels).
+bool decodeBlock(const unsigned char *src, size_t n, int w, int lines, const std::vector<int> &words, std::vector<unsigned char> &out,
+                 std::string &err) {
+    if (n < 4) {
+        err = "PIZ: truncated block";
+        return false;
+    }
+    const unsigned lo = src[0] | src[1] << 8, hi = src[2] | src[3] << 8;
+    size_t at = 4;
+    std::vector<uint8_t> bitmap(kBitmapBytes, 0);
+    if (hi >= (unsigned)kBitmapBytes) {
+        err = "PIZ: bad bitmap range";
+        return false;
+    }
+    if (lo <= hi) {
+        if (n - at < hi - lo + 1) {
+            err = "PIZ: truncated bitmap";
+            return false;
+        }
+        memcpy(&bitmap[lo], src + at, hi - lo + 1);
+        at += hi - lo + 1;
+    }
+    // the values present, in increasing order (0 always); the stored words index this list
+    std::vector<uint16_t> lut(kRange, 0);
+    int k = 0;
+    for (int v = 0; v < kRange; v++)
+        if (v == 0 || (bitmap[v >> 3] & (1 << (v & 7)))) lut[k++] = (uint16_t)v;
+    const uint16_t maxValue = (uint16_t)(k - 1);
+    if (n - at < 4) {
+        err = "PIZ: truncated block";
+        return false;
+    }
+    const uint32_t hufBytes = readU32(src + at);
+    at += 4;
+    if (hufBytes > n - at) {
+        err = "PIZ: Huffman stream past the block";
+        return false;
+    }
+    size_t total = 0;
+    for (int wd : words) total += (size_t)w * lines * wd;
+    std::vector<uint16_t> buf(total);
+    if (!hufDecode(src + at, hufBytes, buf.data(), total, err)) return false;
+    // each channel's plane(s): w x lines pixels of `wd` interleaved words
+    size_t start = 0;
+    for (int wd : words) {
+        for (int j = 0; j < wd; j++) unwavelet(buf.data() + start + j, w, wd, lines, w * wd, maxValue);
+        start += (size_t)w * lines * wd;
+    }
+    for (uint16_t &v : buf) v = lut[v];
+    // channel planes -> scanlines
+    out.resize(total * 2);
+    unsigned char *o = out.data();
+    std::vector<size_t> chanAt(words.size());
+    size_t acc = 0;
+    for (size_t c = 0; c < words.size(); c++) {
+        chanAt[c] = acc;
+        acc += (size_t)w * lines * words[c];
+    }
+    for (int y = 0; y < lines; y++)
+        for (size_t c = 0; c < words.size(); c++) {
+            const size_t cnt = (size_t)w * words[c];
+            memcpy(o, buf.data() + chanAt[c] + (size_t)y * cnt, cnt * 2);
+            o += cnt * 2;
+        }
+    return true;
+}
+
+}  // namespace piz
+
 bool readExr(const std::string &path, int *W, int *H, std::vector<float> *rgba, std::string &err) {
     std::vector<unsigned char> d;
     if (!readFile(path, d)) {
@@ -549,8 +835,9 @@ bool readExr(const std::string &path, int *W, int *H, std::vector<float> *rgba, 
     case EXR_NONE: linesPerBlock = 1; break;
     case EXR_ZIPS: linesPerBlock = 1; break;
     case EXR_ZIP: linesPerBlock = 16; break;
+    case EXR_PIZ: linesPerBlock = 32; break;
     default:
-        err = path + ": unsupported OpenEXR compression " + std::to_string(h.compression) + " (NONE, ZIPS and ZIP are read)";
+        err = path + ": unsupported OpenEXR compression " + std::to_string(h.compression) + " (NONE, ZIPS, ZIP and PIZ are read)";
         return false;
     }
     // channels are stored sorted by name; pick R, G, B, A
@@ -599,6 +886,13 @@ bool readExr(const std::string &path, int *W, int *H, std::vector<float> *rgba, 
         const unsigned char *src = &d[off + 8];
         if ((size_t)size == expect) {
             block.assign(src, src + expect);  // stored uncompressed (always for NONE)
+        } else if (h.compression == EXR_PIZ) {
+            std::vector<int> words;
+            for (const ExrChannel &c : h.channels) words.push_back(c.type == EXR_HALF ? 1 : 2);
+            if (!piz::decodeBlock(src, (size_t)size, w, lines, words, block, err)) {
+                err = path + ": " + err;
+                return false;
+            }
         } else if (!unzipBlock(src, (size_t)size, block, expect)) {
             err = path + ": corrupt ZIP block";
             return false;

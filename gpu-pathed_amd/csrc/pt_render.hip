@@ -60,11 +60,42 @@ static constexpr int64_t kDefaultPool = 1ll << 26;
 // kernels' 72 VGPRs and 8 KiB of LDS stack per block allow; 11 measured best on C1 (10: 4156,
 // 11: 4185, 12: 4123, 14: 4095 Mrays/s), fewer waves sharing each CU's caches.
 static constexpr int kDefaultTraceBlocks = 11;
+// Binary tree under the 4-wide BVH.
+// LBVH measured faster than PLOC on the displaced icosphere at 1024^2, 64 spp (4467 vs 4348
+// Mrays/s; extend 2053 vs 2160 us per launch, although PLOC's tree has 12% fewer 4-wide nodes).
+static constexpr int kDefaultBvhBuilder = PT_BVH_LBVH;
 // Blocks per CU of the guide training's forward/backward kernel: one-wave blocks of 16.5 KiB
 // LDS and 197 registers per lane, two waves per SIMD.
 static constexpr int kTrainBlocksPerCU = 8;
 static constexpr float kShadowEps = 1e-4f;   // path_tracer.cu:104-110
 static constexpr float kBounceEps = 1e-3f;   // path_tracer.cu:177
+
+// One render of a batch wavefront (renders of different scenes sharing one set of queues):
+// where its BVH starts in the merged node array, its lights, environment and camera, its pixel
+// grid and its accumulator.
+struct BatchRender {
+    int root;                 // 4-wide root node in the merged node array
+    int emitBegin;            // first of its emitters in the merged emitter array
+    int numEmitters;
+    int maxDepth;
+    int W, H, npix;
+    uint64_t seed;
+    PtEnv env;
+    PtCamera cam;
+    double *accum;            // 3 doubles per pixel
+};
+
+// A run of sample rows of one render inside a chunk of a batch wavefront: chunk paths
+// [pathBegin, pathBegin + count * npix), sample-major; accumulate threads [accBegin,
+// accBegin + npix).
+struct BatchGroup {
+    int pathBegin;
+    int accBegin;
+    int render;
+    int npix;
+    int sampleBegin;
+    int count;
+};
 
 struct ChunkParams {
     int pixBegin;     // first pixel of the chunk
@@ -78,7 +109,44 @@ struct ChunkParams {
     int W, H;
     uint64_t seed;
     int segLen;       // queue segment length, ptSegLen(nPaths)
+    // batch wavefront only (nullptr otherwise): the chunk's groups, the batch's renders, and
+    // per render the rays its paths spawned (continuation + shadow)
+    const BatchGroup *groups = nullptr;
+    int numGroups = 0;
+    const int *bucketGroup = nullptr;   // group of path b << kBatchBucketShift, per bucket b
+    const BatchRender *renders = nullptr;
+    unsigned long long *renderRays = nullptr;
 };
+
+// Paths per bucket of the batch wavefront's path-to-group table.
+static constexpr int kBatchBucketShift = 12;
+
+// The group of chunk path `path` (groups ordered by pathBegin, the first at 0): the group of
+// the path's bucket, then forward over the groups that start inside the bucket (none when
+// groups are longer than a bucket, as one sample row of 64 x 64 pixels already is).
+__device__ __forceinline__ int batchGroupOf(const ChunkParams &cp, int path) {
+    int g = cp.bucketGroup[path >> kBatchBucketShift];
+    while (g + 1 < cp.numGroups && cp.groups[g + 1].pathBegin <= path) g++;
+    return g;
+}
+
+// Render, pixel and sample row of a chunk path of a batch wavefront.
+struct PathSource {
+    int render, pixel, sample;
+};
+__device__ __forceinline__ PathSource batchPathSource(const ChunkParams &cp, int path) {
+    const BatchGroup g = cp.groups[batchGroupOf(cp, path)];
+    const int local = path - g.pathBegin;
+    const int s = local / g.npix;
+    PathSource ps;
+    ps.render = g.render;
+    ps.pixel = local - s * g.npix;
+    ps.sample = g.sampleBegin + s;
+    return ps;
+}
+
+// Batch wavefront path state: depth in the low 16 bits of state.w, render above.
+__device__ __forceinline__ uint32_t packDepth(int depth, int render) { return (uint32_t)depth | ((uint32_t)render << 16); }
 
 // ------------------------------------------------------------------------------------------
 // wave helpers
@@ -116,6 +184,18 @@ __device__ __forceinline__ int segAppendLanes(bool want, int *counters, int seg,
         todo &= ~__ballot(mine);
     }
     return k;
+}
+// Adds each lane's v (> 0) to counter[key]: one atomic per distinct key among those lanes.
+__device__ __forceinline__ void waveAddKeyed(int v, int key, unsigned long long *counter) {
+    uint64_t todo = __ballot(v > 0);
+    while (todo != 0) {
+        const int k = __shfl(key, __ffsll((unsigned long long)todo) - 1);
+        const bool mine = v > 0 && key == k;
+        int sum = mine ? v : 0;
+        for (int off = 32; off >= 1; off >>= 1) sum += __shfl_xor(sum, off);
+        if (laneId() == 0) atomicAdd(&counter[k], (unsigned long long)sum);
+        todo &= ~__ballot(mine);
+    }
 }
 // Adds a per-lane count once per wave (statistics counted over a persistent wave's lifetime
 // cost one atomic instead of one per iteration).
@@ -169,6 +249,23 @@ __global__ __launch_bounds__(256) void k_raygen(ChunkParams cp, PtCamera cam, fl
     f3 o, d;
     cameraRay(cam, row, col, sx, sy, &o, &d);
     rayD[ptCameraSlot(i, cp.segLen)] = f4(d, FLT_MAX);
+}
+
+// Batch wavefront: every path of the chunk from its render's camera; stores origin (with the
+// render in w) and direction.
+__global__ __launch_bounds__(256) void k_raygen_batch(ChunkParams cp, float4 *rayO, float4 *rayD) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cp.nPaths) return;
+    const PathSource ps = batchPathSource(cp, i);
+    const BatchRender &R = cp.renders[ps.render];
+    const int row = ps.pixel / R.W, col = ps.pixel - row * R.W;
+    Rng rng = seedRng(R.seed, (uint32_t)ps.pixel, (uint32_t)ps.sample);
+    const float sx = rng.uniform(), sy = rng.uniform();
+    f3 o, d;
+    cameraRay(R.cam, row, col, sx, sy, &o, &d);
+    const int slot = ptCameraSlot(i, cp.segLen);
+    rayO[slot] = f4(o, asFloat(ps.render));
+    rayD[slot] = f4(d, FLT_MAX);
 }
 
 // The camera-ray state k_raygen does not store: the RNG after the pixel's two jitter draws.
@@ -238,7 +335,7 @@ struct LaneRay {
     int node, sp;
 };
 
-__device__ __forceinline__ void laneRayStart(LaneRay &r, f3 o, f3 d, float tmin, float tmax) {
+__device__ __forceinline__ void laneRayStart(LaneRay &r, f3 o, f3 d, float tmin, float tmax, int root = 0) {
     r.wr = wtSetup(o, d);
     r.d = d;
     r.ix = 1.f / (fabsf(d.x) > 1e-20f ? d.x : copysignf(1e-20f, d.x));
@@ -258,7 +355,7 @@ __device__ __forceinline__ void laneRayStart(LaneRay &r, f3 o, f3 d, float tmin,
     r.u = 0.f;
     r.v = 0.f;
     r.prim = INT32_MIN;
-    r.node = 0;
+    r.node = root;
     r.sp = 0;
 }
 
@@ -460,7 +557,8 @@ __device__ __forceinline__ int segClaim(int *counters, int bounce, int countFiel
 }
 
 // Traces the bounce's queue (count field countField, cursor field fetchField): load(i, o, d,
-// tmin, tmax) reads the ray in slot i, finish(i, hit, th) stores its result.  A wave takes new
+// tmin, tmax, root) reads the ray in slot i (root: the BVH node its traversal starts at),
+// finish(i, hit, th) stores its result.  A wave takes new
 // rays once `refill` or more of its lanes are idle, as a contiguous run of one segment so that
 // the rays in flight on a CU stay neighbours, and leaves once its segments are exhausted and
 // its lanes idle.
@@ -489,8 +587,9 @@ __device__ __forceinline__ void traceQueue(const PtSceneView &sc, int *counters,
                     ray = slot + k;
                     f3 o, d;
                     float tmin, tmax;
-                    load(ray, o, d, tmin, tmax);
-                    laneRayStart(r, o, d, tmin, tmax);
+                    int root = 0;
+                    load(ray, o, d, tmin, tmax, root);
+                    laneRayStart(r, o, d, tmin, tmax, root);
                 }
             }
         }
@@ -504,18 +603,26 @@ __device__ __forceinline__ void traceQueue(const PtSceneView &sc, int *counters,
 }
 
 // Persistent closest-hit kernel over the rays of `bounce`.  camO.w != 0: the rays start at
-// camO (camera rays, tmin 0) and rayO is not read.
-template <int STACK>
+// camO (camera rays, tmin 0) and rayO is not read.  BATCH (a batch wavefront): rayO.w holds
+// the ray's render, whose root node starts the traversal; tmin is 0 at bounce 0, else the
+// continuation offset.
+template <int STACK, bool BATCH = false>
 __global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 7 : 5) void k_extend(PtSceneView sc, int *counters, int bounce, int segLen,
                                                                               const float4 *rayO, const float4 *rayD, float4 camO, float4 *hits,
-                                                                              int refill, int *spill, int spillCap, int *overflow) {
+                                                                              int refill, int *spill, int spillCap, int *overflow,
+                                                                              const BatchRender *renders = nullptr) {
     __shared__ int sstack[STACK * kTraceThreads];
     const TraceStack ts = traceStack(sstack, spill, spillCap);
     traceQueue<false, STACK>(
         sc, counters, bounce, PT_Q_RAYS, PT_Q_FETCH_EXT, segLen, refill, ts, overflow,
-        [&](int i, f3 &o, f3 &d, float &tmin, float &tmax) {
+        [&](int i, f3 &o, f3 &d, float &tmin, float &tmax, int &root) {
             const float4 rd = rayD[i];
-            if (camO.w != 0.f) {
+            if constexpr (BATCH) {
+                const float4 ro = rayO[i];
+                o = xyz(ro);
+                tmin = bounce == 0 ? 0.f : kBounceEps;
+                root = renders[asInt(ro.w)].root;
+            } else if (camO.w != 0.f) {
                 o = xyz(camO);
                 tmin = 0.f;
             } else {
@@ -531,10 +638,12 @@ __global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 7 : 5) void k_extend(P
 
 // Persistent any-hit kernel for the shadow rays of `bounce` (interval [1e-4, tmax]); an
 // unoccluded ray adds its light contribution to its path.
-template <int STACK>
+// BATCH: sL.w holds the ray's render, whose root node starts the traversal.
+template <int STACK, bool BATCH = false>
 __global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 7 : 5) void k_shadow(PtSceneView sc, int *counters, int bounce, int segLen,
                                                                               const float4 *sO, const float4 *sD, const float4 *sL, float4 *Lpath,
-                                                                              int refill, int *spill, int spillCap, int *overflow) {
+                                                                              int refill, int *spill, int spillCap, int *overflow,
+                                                                              const BatchRender *renders = nullptr) {
     __shared__ int sstack[STACK * kTraceThreads];
     // each lane's path index, kept from the ray's load (a re-read after the traversal would be a
     // dependent round trip ahead of the radiance update)
@@ -543,13 +652,14 @@ __global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 7 : 5) void k_shadow(P
     int adds = 0;
     traceQueue<true, STACK>(
         sc, counters, bounce, PT_Q_SHADOW, PT_Q_FETCH_SHADOW, segLen, refill, ts, overflow,
-        [&](int i, f3 &o, f3 &d, float &tmin, float &tmax) {
+        [&](int i, f3 &o, f3 &d, float &tmin, float &tmax, int &root) {
             const float4 so = sO[i], sd = sD[i];
             o = xyz(so);
             d = xyz(sd);
             tmin = kShadowEps;
             tmax = so.w;
             sPath[threadIdx.x] = asInt(sd.w);
+            if constexpr (BATCH) root = renders[asInt(((const float *)&sL[i])[3])].root;
         },
         [&](int i, bool occluded, const TraceHit &) {
             if (!occluded) {
@@ -703,7 +813,7 @@ __device__ __forceinline__ float mixturePdf(const float4 *G, f3 w) {
 // the BSDF, the continuation is weighted by f cos / (c q + (1 - c) p_bsdf), and the same mixture
 // density is the directional technique's pdf in both MIS weights.
 // Plain shading fits 96 VGPRs without spilling: 5 waves per SIMD.
-template <bool GUIDED>
+template <bool GUIDED, bool BATCH = false>
 __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, ChunkParams cp, int bounce, int *counters, const float4 *rayO, const float4 *rayD,
                                                const uint4 *state, const float4 *beta, const float4 *hits, float4 *nRayO, float4 *nRayD,
                                                uint4 *nState, float4 *nBeta, float4 *sO, float4 *sD, float4 *sL, float4 *Lpath,
@@ -716,6 +826,11 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
     // camO.w != 0 (bounce 0): the camera rays' origin, throughput, previous pdf, path index and
     // RNG state are recomputed here instead of read (k_raygen stores only the directions), and
     // the path's radiance is written instead of added to.
+    //
+    // BATCH (a batch wavefront): every path belongs to one render of the batch (state.w above
+    // the depth; rayO.w at bounce 0), whose lights, environment, depth limit and PCG32 streams
+    // it uses; continuation rays carry the render in rayO.w and shadow rays in sL.w, and each
+    // render counts the rays its paths spawn.
     //
     // Misses (a fifth to most of a queue) take a few instructions, hits the whole BSDF and light
     // code.  A wave shades the misses of each claimed run at once and parks the hits' slots in
@@ -765,23 +880,44 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
         bool addL = false;
+        int rid = 0, spawned = 0;   // batch wavefront: the path's render, rays its vertex spawned
         if (active) {
             const float4 rd = rayD[i], h = hits[i];
             float4 ro, bp;
             uint4 st;
             if (camera) {
                 const int p = ptCameraPath(seg, i - seg * cp.segLen);
-                const Rng g = cameraRng(cp, p);
-                ro = camO;
                 bp = make_float4(1.f, 1.f, 1.f, -1.f);
-                st = make_uint4((uint32_t)g.state, (uint32_t)(g.state >> 32), (uint32_t)p, 0u);
+                if constexpr (BATCH) {
+                    const PathSource ps = batchPathSource(cp, p);
+                    Rng g = seedRng(cp.renders[ps.render].seed, (uint32_t)ps.pixel, (uint32_t)ps.sample);
+                    g.next();
+                    g.next();
+                    ro = rayO[i];
+                    st = make_uint4((uint32_t)g.state, (uint32_t)(g.state >> 32), (uint32_t)p, packDepth(0, ps.render));
+                } else {
+                    const Rng g = cameraRng(cp, p);
+                    ro = camO;
+                    st = make_uint4((uint32_t)g.state, (uint32_t)(g.state >> 32), (uint32_t)p, 0u);
+                }
             } else {
                 ro = rayO[i];
                 bp = beta[i];
                 st = state[i];
             }
             const int pathId = (int)st.z;
-            const int depth = (int)st.w;
+            const int depth = BATCH ? (int)(st.w & 0xffffu) : (int)st.w;
+            rid = BATCH ? (int)(st.w >> 16) : 0;
+            // the scene as this path sees it: a batch render's lights, environment, depth limit
+            PtSceneView lv = sc;
+            int maxDepth = cp.maxDepth;
+            if constexpr (BATCH) {
+                const BatchRender &R = cp.renders[rid];
+                lv.emitters = sc.emitters + 3 * (size_t)R.emitBegin;
+                lv.numEmitters = R.numEmitters;
+                lv.env = R.env;
+                maxDepth = R.maxDepth;
+            }
             const f3 o = xyz(ro), d = xyz(rd), thr = xyz(bp);
             const float prevPdf = bp.w;
             f3 L = mk(0.f);
@@ -794,31 +930,36 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
             const int slot = asInt(h.w);
             if (slot == INT32_MIN) {
                 // miss: environment (camera miss: environment_light.h:68-83 as is)
-                if (depth == 0 || !cp.nee || prevPdf < 0.f || !sc.env.present) {
-                    L = thr * envEmit(sc.env, d);
+                if (depth == 0 || !cp.nee || prevPdf < 0.f || !lv.env.present) {
+                    L = thr * envEmit(lv.env, d);
                 } else {
                     // sampler.h:127-140: light-selection share of the environment density
                     float pe;
-                    const f3 e = envEmitPdf(sc.env, d, &pe);
-                    L = thr * e * balance(prevPdf, pe * (1.f / (float)(sc.numEmitters + 1)));
+                    const f3 e = envEmitPdf(lv.env, d, &pe);
+                    L = thr * e * balance(prevPdf, pe * (1.f / (float)(lv.numEmitters + 1)));
                 }
             } else {
-                const SurfacePoint sp = surfaceAt(sc, slot, o, d, h.x, h.y, h.z);
+                const SurfacePoint sp = surfaceAt(lv, slot, o, d, h.x, h.y, h.z);
                 const Frame fr = makeFrame(sp.n);
                 const f3 wo = normalize(toLocal(fr, -d));
-                const PtMaterial m = sc.materials[sp.mat];
+                const PtMaterial m = lv.materials[sp.mat];
                 const int type = matType(m);
                 // lambertian.cu:5-11: emission seen from the front only
                 if (type == PT_MAT_LAMBERTIAN && wo.z >= 0.f && (m.b.x != 0.f || m.b.y != 0.f || m.b.z != 0.f)) {
                     const f3 e = mk(m.b.x, m.b.y, m.b.z);
                     if (depth == 0 || !cp.nee || prevPdf < 0.f) L = thr * e;
-                    else L = thr * e * balance(prevPdf, pdfSceneLight(sc, o, sp));
+                    else L = thr * e * balance(prevPdf, pdfSceneLight(lv, o, sp));
                 }
-                if (depth + 1 < cp.maxDepth) {
+                if (depth + 1 < maxDepth) {
                     Rng rng;
-                    const int s = pathId / cp.npix;
-                    const int pixel = cp.pixBegin + (pathId - s * cp.npix);
-                    rng.inc = streamInc(cp.seed, (uint32_t)pixel, (uint32_t)(cp.sampleBase + s));
+                    if constexpr (BATCH) {
+                        const PathSource ps = batchPathSource(cp, pathId);
+                        rng.inc = streamInc(cp.renders[ps.render].seed, (uint32_t)ps.pixel, (uint32_t)ps.sample);
+                    } else {
+                        const int s = pathId / cp.npix;
+                        const int pixel = cp.pixBegin + (pathId - s * cp.npix);
+                        rng.inc = streamInc(cp.seed, (uint32_t)pixel, (uint32_t)(cp.sampleBase + s));
+                    }
                     rng.state = ((uint64_t)st.y << 32) | st.x;
                     const float4 *G = nullptr;
                     float frac = 0.f;
@@ -868,11 +1009,11 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
                         // directSampleLights (path_tracer.cu:83-126); a scene without lights
                         // draws the three uniforms and adds nothing
                         const float l1 = rng.uniform(), l2 = rng.uniform(), l3 = rng.uniform();
-                        const bool anyLight = sc.numEmitters > 0 || sc.env.present;
+                        const bool anyLight = lv.numEmitters > 0 || lv.env.present;
                         bool pushShadow = false;
                         f3 contrib = mk(0.f);
                         LightSample ls;
-                        if (anyLight) ls = sampleLights(sc, sp.p, l1, l2, l3);
+                        if (anyLight) ls = sampleLights(lv, sp.p, l1, l2, l3);
                         if (anyLight && dot(ls.n, ls.wi) < 0.f && ls.pdf > 0.f) {
                             const f3 wiL = toLocal(fr, ls.wi);
                             float pB;
@@ -882,11 +1023,12 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
                             contrib = thr * (ls.emitted * w * bsdfF(m, wo, wiL) * fabsf(dot(sp.n, ls.wi)) / ls.pdf);
                             pushShadow = !isZero(contrib) && !(contrib.x != contrib.x);
                         }
+                        spawned += pushShadow ? 1 : 0;
                         const int ss = segAppendLanes(pushShadow, counters, seg, bounce, PT_Q_SHADOW, cp.segLen);
                         if (pushShadow) {
                             sO[ss] = f4(sp.p, ls.dist - 2e-4f);
                             sD[ss] = f4(ls.wi, asFloat(pathId));
-                            sL[ss] = f4(contrib, 0.f);
+                            sL[ss] = f4(contrib, BATCH ? asFloat(rid) : 0.f);
                         }
                     }
                     bool pushRay = false;
@@ -895,12 +1037,14 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
                         nb = thr * (bs.f * fabsf(bs.wi.z) / bs.pdf);
                         pushRay = !isZero(nb) && nb.x == nb.x;
                     }
+                    spawned += pushRay ? 1 : 0;
                     const int rs = segAppendLanes(pushRay, counters, seg, bounce + 1, PT_Q_RAYS, cp.segLen);
                     const f3 wworld = toWorld(fr, bs.wi);
                     if (pushRay) {
-                        nRayO[rs] = f4(sp.p, kBounceEps);
+                        nRayO[rs] = f4(sp.p, BATCH ? asFloat(rid) : kBounceEps);
                         nRayD[rs] = f4(wworld, FLT_MAX);
-                        nState[rs] = make_uint4((uint32_t)rng.state, (uint32_t)(rng.state >> 32), (uint32_t)pathId, (uint32_t)(depth + 1));
+                        nState[rs] = make_uint4((uint32_t)rng.state, (uint32_t)(rng.state >> 32), (uint32_t)pathId,
+                                                BATCH ? packDepth(depth + 1, rid) : (uint32_t)(depth + 1));
                         nBeta[rs] = f4(nb, bs.delta ? -1.f : pdfDir);
                     }
                     if constexpr (GUIDED) {
@@ -938,6 +1082,7 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
             }
         }
         adds += addL ? 1 : 0;
+        if constexpr (BATCH) waveAddKeyed(spawned, rid, cp.renderRays);
     }
     waveSum(adds, &counters[ptQueueField(homeSegment(), bounce, PT_Q_LADD_SHADE)]);
 }
@@ -1016,6 +1161,75 @@ __global__ __launch_bounds__(256) void k_accumulate(ChunkParams cp, const float4
     a[2] += sz;
 }
 
+// Batch wavefront: per (group, pixel) of the chunk, the FP64 sum over the group's sample rows
+// into its render's accumulator; block 0 folds the chunk's queue counters into `stats`.
+__global__ __launch_bounds__(256) void k_accumulate_batch(ChunkParams cp, int accThreads, const float4 *Lpath, const int *counters,
+                                                          unsigned long long *stats) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0) {
+        for (int b = threadIdx.x; b < kMaxBounces; b += blockDim.x) {
+            unsigned long long v[2] = {0, 0};
+            for (int seg = 0; seg < PT_QUEUE_SEGS; seg++) {
+                v[0] += (unsigned long long)counters[ptQueueField(seg, b, PT_Q_RAYS)];
+                v[1] += (unsigned long long)counters[ptQueueField(seg, b, PT_Q_SHADOW)];
+            }
+            atomicAdd(&stats[S_RAYS + b], v[0]);
+            atomicAdd(&stats[S_SHADOW + b], v[1]);
+        }
+        if (threadIdx.x == 0) atomicAdd(&stats[S_PATHS], (unsigned long long)cp.nPaths);
+    }
+    if (t >= accThreads) return;
+    int lo = 0, hi = cp.numGroups - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cp.groups[mid].accBegin <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchGroup g = cp.groups[lo];
+    const int p = t - g.accBegin;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int s = 0; s < g.count; s++) {
+        const float4 l = Lpath[(size_t)g.pathBegin + (size_t)s * g.npix + p];
+        sx += (double)l.x;
+        sy += (double)l.y;
+        sz += (double)l.z;
+    }
+    double *a = cp.renders[g.render].accum + 3 * (size_t)p;
+    a[0] += sx;
+    a[1] += sy;
+    a[2] += sz;
+}
+
+// Merging one scene into a batch's arrays: node references move by the node and primitive
+// offsets (leaf ranges keep their counts), primitive and emitter records by the material and
+// normal offsets.
+__global__ void k_rebase_nodes(float4 *nodes, int count, int nodeOff, int primOff) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    float4 &refs = nodes[(size_t)PT_NODE_F4 * i + 6];
+    int r[4] = {asInt(refs.x), asInt(refs.y), asInt(refs.z), asInt(refs.w)};
+    for (int k = 0; k < 4; k++) {
+        if (r[k] == PT_EMPTY_CHILD) continue;
+        if (r[k] >= 0) {
+            r[k] += nodeOff;
+        } else {
+            const int v = ~r[k];
+            const int mask = (1 << PT_LEAF_COUNT_SHIFT) - 1;
+            r[k] = ~((v & ~mask) | ((v & mask) + primOff));
+        }
+    }
+    refs = make_float4(asFloat(r[0]), asFloat(r[1]), asFloat(r[2]), asFloat(r[3]));
+}
+
+__global__ void k_rebase_prims(float4 *recs, int count, int matOff, int normOff) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    float4 &p0 = recs[3 * (size_t)i], &p1 = recs[3 * (size_t)i + 1];
+    p0.w = asFloat(asInt(p0.w) + matOff);
+    const int tag = asInt(p1.w);
+    if (tag >= 0) p1.w = asFloat(tag + normOff);
+}
+
 // mean radiance + the uchar4 framebuffer of framebuffer.cu:35-40 (color.h:7-28 sRGB)
 __global__ __launch_bounds__(256) void k_tonemap(const double *accum, int npix, double invSpp, float *radiance, uchar4 *rgba) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1085,6 +1299,14 @@ __global__ void k_stamp(unsigned long long *slot) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *slot = (unsigned long long)wall_clock64();
 }
 
+// The same completion stamp into slots[0 .. n): renders that complete together.
+__global__ void k_stamp_range(unsigned long long *slots, int n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const unsigned long long t = wall_clock64();
+        for (int k = 0; k < n; k++) slots[k] = t;
+    }
+}
+
 }  // namespace pt
 
 // ============================================================================================
@@ -1112,6 +1334,7 @@ struct pt_renderer {
     DevBuf nodes, prims, primsIn, normals, materials, emitters, envRgba, envRows, envMarg, envPdf;
     PtSceneView view{};
     BvhInfo bvh;
+    int bvhBuilder = kDefaultBvhBuilder;
     int stackNeed = 1;    // deepest traversal stack the tree can need (3 per 4-wide level)
     int ldsStack = 16;    // stack entries per lane kept in LDS; the rest spill to the pool's `spill`
     int refillLanes = 0;  // idle lanes of a trace wave that trigger taking new rays (0: per launch)
@@ -1165,6 +1388,8 @@ struct pt_renderer {
     uint64_t gSeed = 1;
     float sceneLo[3] = {0.f, 0.f, 0.f}, sceneHi[3] = {1.f, 1.f, 1.f};
     double trainMs = 0.0;
+    // the merged wavefront of a batch: its per-render table (device); null for a renderer
+    const BatchRender *batchRenders = nullptr;
 };
 
 namespace {
@@ -1314,8 +1539,14 @@ void launchExtend(pt_renderer *r, int bounce, int segLen, const float4 *ro, cons
     int *ov = r->pp->overflow.as<int>();
     int cap = 0;
     int *sp = ensureSpill(r, grid, &cap);
-    if (r->ldsStack == 32) k_extend<32><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, ro, rd, camO, hits, refill, sp, cap, ov);
-    else k_extend<16><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, ro, rd, camO, hits, refill, sp, cap, ov);
+    const BatchRender *br = r->batchRenders;
+    if (br) {
+        if (r->ldsStack == 32) k_extend<32, true><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, ro, rd, camO, hits, refill, sp, cap, ov, br);
+        else k_extend<16, true><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, ro, rd, camO, hits, refill, sp, cap, ov, br);
+    } else {
+        if (r->ldsStack == 32) k_extend<32><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, ro, rd, camO, hits, refill, sp, cap, ov);
+        else k_extend<16><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, ro, rd, camO, hits, refill, sp, cap, ov);
+    }
     PT_HIP_TRY(hipGetLastError());
 }
 void launchShadow(pt_renderer *r, int bounce, int segLen) {
@@ -1327,8 +1558,14 @@ void launchShadow(pt_renderer *r, int bounce, int segLen) {
     float4 *L = r->pp->Lpath.as<float4>();
     int cap = 0;
     int *sp = ensureSpill(r, grid, &cap);
-    if (r->ldsStack == 32) k_shadow<32><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, o, d, l, L, refill, sp, cap, ov);
-    else k_shadow<16><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, o, d, l, L, refill, sp, cap, ov);
+    const BatchRender *br = r->batchRenders;
+    if (br) {
+        if (r->ldsStack == 32) k_shadow<32, true><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, o, d, l, L, refill, sp, cap, ov, br);
+        else k_shadow<16, true><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, o, d, l, L, refill, sp, cap, ov, br);
+    } else {
+        if (r->ldsStack == 32) k_shadow<32><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, o, d, l, L, refill, sp, cap, ov);
+        else k_shadow<16><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, o, d, l, L, refill, sp, cap, ov);
+    }
     PT_HIP_TRY(hipGetLastError());
 }
 
@@ -1685,7 +1922,7 @@ int pt_renderer_set_scene(pt_renderer *r, const pt_scene *s) {
         r->sceneLo[k] = std::isfinite(lo[k]) ? lo[k] : 0.f;
         r->sceneHi[k] = std::isfinite(hi[k]) ? hi[k] : 1.f;
     }
-    buildBvh(r->primsIn.as<float4>(), (int)n, lo, hi, r->nodes, r->prims.as<float4>(), r->stream, &r->bvh);
+    buildBvh(r->primsIn.as<float4>(), (int)n, lo, hi, r->nodes, r->prims.as<float4>(), r->stream, &r->bvh, r->bvhBuilder);
     r->primsIn.release();
     // ordered 4-wide traversal pushes at most three references per level
     r->stackNeed = 3 * r->bvh.depth + 1;
@@ -1823,6 +2060,10 @@ int pt_renderer_set_option(pt_renderer *r, int opt, int64_t v) {
     case PT_OPT_CONSISTENT_MIS: r->misSample = v != 0; r->resetPending = true; break;
     case PT_OPT_GUIDE_TRAIN_STEPS: r->gTrainSteps = v < 0 ? 16 : (int)std::min<int64_t>(v, 1024); break;
     case PT_OPT_GUIDE_BOUNCES: r->gBounces = v <= 0 ? 0 : (int)std::min<int64_t>(v, kMaxBounces); break;
+    case PT_OPT_BVH_BUILDER:
+        if (v < 0 || v > 2) return fail(PT_ERR_INVALID, "pt_renderer_set_option: BVH builder must be 0, 1 (LBVH) or 2 (PLOC)");
+        r->bvhBuilder = v == 0 ? kDefaultBvhBuilder : v == 1 ? PT_BVH_LBVH : PT_BVH_PLOC;
+        break;
     case PT_OPT_GUIDE_BATCH:
         if (v < 32 || v > (1 << 22)) return fail(PT_ERR_INVALID, "pt_renderer_set_option: guide batch must be in [32, 2^22]");
         r->gBatch = (int)v;
@@ -1861,6 +2102,7 @@ int pt_renderer_get_option(pt_renderer *r, int opt, int64_t *v) {
     case PT_OPT_CONSISTENT_MIS: *v = r->misSample; break;
     case PT_OPT_GUIDE_TRAIN_STEPS: *v = r->gTrainSteps; break;
     case PT_OPT_GUIDE_BOUNCES: *v = r->gBounces; break;
+    case PT_OPT_BVH_BUILDER: *v = r->bvhBuilder == PT_BVH_LBVH ? 1 : 2; break;
     case PT_OPT_GUIDE_BATCH: *v = r->gBatch; break;
     case PT_OPT_GUIDE_LR_MICRO: *v = (int64_t)llroundf(r->gLr * 1e6f); break;
     case PT_OPT_GUIDE_MAX_FRACTION_MILLI: *v = (int64_t)llroundf(r->gFracCap * 1e3f); break;
@@ -2324,6 +2566,15 @@ std::vector<int64_t> poolCandidates(int64_t maxPaths) {
 // ============================================================================================
 // Batch of independent renders in one captured graph
 // ============================================================================================
+// A chunk of the batch wavefront: its launch parameters (groups[firstGroup ..]), its
+// accumulate grid, and the renders (batch order positions) whose last rows it holds.
+struct WaveChunk {
+    ChunkParams cp{};
+    int firstGroup = 0;
+    int accThreads = 0;
+    std::vector<int> finishing;
+};
+
 struct pt_batch {
     hipStream_t stream = nullptr;
     std::shared_ptr<PathPool> pp = std::make_shared<PathPool>();
@@ -2331,6 +2582,15 @@ struct pt_batch {
     std::vector<int> spp;
     TuneCfg cfg{kDefaultPool, kDefaultTraceBlocks, 8, 16};
     bool useGraph = true;
+    // one wavefront over all renders (paths of different renders in the same queues), or each
+    // render's own chunks one render after the other
+    bool wavefront = true;
+    pt_renderer *wave = nullptr;   // merged scene + launch settings of the wavefront
+    size_t waveJobs = 0;           // renders merged into `wave`
+    DevBuf renderTable, groupTable, bucketTable, renderRays, waveStats;
+    std::vector<WaveChunk> waveChunks;
+    std::vector<BatchGroup> waveGroups;
+    std::vector<int> waveLead;     // renders with no rows before the first chunk (depth 0)
     DevBuf stamps;
     double clockKHz = 100000.0;
     // last run
@@ -2352,6 +2612,7 @@ struct pt_batch {
         graph = nullptr;
         graphScale = -1;
     }
+    ~pt_batch() { delete wave; }
 };
 
 namespace {
@@ -2363,14 +2624,27 @@ bool sameCfg(const TuneCfg &a, const TuneCfg &b) { return a.pool == b.pool && a.
 // stamp per render.  spp of each render is divided by `scale` (autotuner probes).
 // Device memory the batch's kernels need (shared pool, stack spill area, stamps) and the
 // render order; allocates, so it runs before any stream capture.
+void buildWave(pt_batch *b);
+void planWave(pt_batch *b, int scale);
+
 void prepareBatch(pt_batch *b, int scale) {
     for (pt_renderer *r : b->jobs) applyCfg(r, b->cfg);
-    int64_t need = 0;
+    int64_t need = 0, total = 0;
     int deepest = 0;
     for (size_t j = 0; j < b->jobs.size(); j++) {
         pt_renderer *r = b->jobs[j];
-        need = std::max<int64_t>(need, (int64_t)r->W * r->H * std::max(1, b->spp[j] / scale));
+        const int64_t paths = (int64_t)r->W * r->H * std::max(1, b->spp[j] / scale);
+        need = std::max<int64_t>(need, paths);
+        total += paths;
         if (r->stackNeed > b->jobs[deepest]->stackNeed) deepest = (int)j;
+    }
+    // the wavefront fills chunks with whole sample rows of several renders: size the pool by
+    // all of them, and at least one row of the largest image
+    if (b->wavefront) {
+        int64_t row = 0;
+        for (pt_renderer *r : b->jobs) row = std::max<int64_t>(row, (int64_t)r->W * r->H);
+        b->jobs[0]->poolWanted = std::max<int64_t>(b->cfg.pool, row);
+        need = std::max<int64_t>(std::min<int64_t>(total, b->cfg.pool), row);
     }
     ensurePool(b->jobs[0], need);
     int cap = 0;
@@ -2383,9 +2657,253 @@ void prepareBatch(pt_batch *b, int scale) {
         return (double)r->W * r->H * b->spp[j] * std::log2(2.0 + (double)r->triMat.size() + (double)r->sphMat.size());
     };
     std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) { return cost(x) < cost(y); });
+    if (b->wavefront) {
+        buildWave(b);
+        applyCfg(b->wave, b->cfg);
+        planWave(b, scale);
+    }
+}
+
+// The merged scene of the batch wavefront: every render's nodes, primitives, normals,
+// materials and emitters concatenated (references rebased), and the per-render table.
+void buildWave(pt_batch *b) {
+    if (b->wave && b->waveJobs == b->jobs.size()) return;
+    delete b->wave;
+    b->wave = nullptr;
+    pt_renderer *w = new pt_renderer();
+    b->wave = w;
+    w->stream = b->stream;
+    w->ownsStream = false;
+    w->pp = b->pp;
+    w->cus = b->jobs[0]->cus;
+    int64_t nN = 0, nP = 0, nNrm = 0, nM = 0, nE = 0;
+    for (pt_renderer *r : b->jobs) {
+        nN += r->bvh.numNodes;
+        nP += r->view.numPrims;
+        nNrm += (int64_t)(r->normals.bytes / sizeof(float4));
+        nM += (int64_t)r->mats.size();
+        nE += r->view.numEmitters;
+        w->stackNeed = std::max(w->stackNeed, r->stackNeed);
+    }
+    if (nP >= (1ll << PT_LEAF_COUNT_SHIFT) || nN >= INT32_MAX / 2) throw HipFailure("batch: too many primitives for one wavefront");
+    w->nodes.alloc(sizeof(float4) * PT_NODE_F4 * (size_t)std::max<int64_t>(nN, 1));
+    w->prims.alloc(sizeof(float4) * 3 * (size_t)std::max<int64_t>(nP, 1));
+    w->normals.alloc(sizeof(float4) * (size_t)std::max<int64_t>(nNrm, 1));
+    w->materials.alloc(sizeof(PtMaterial) * (size_t)std::max<int64_t>(nM, 1));
+    w->emitters.alloc(sizeof(float4) * 3 * (size_t)std::max<int64_t>(nE, 1));
+    std::vector<BatchRender> table(b->jobs.size());
+    int64_t oN = 0, oP = 0, oNrm = 0, oM = 0, oE = 0;
+    for (size_t j = 0; j < b->jobs.size(); j++) {
+        pt_renderer *r = b->jobs[j];
+        const int nn = r->bvh.numNodes, np = r->view.numPrims, ne = r->view.numEmitters;
+        const int64_t nrm = (int64_t)(r->normals.bytes / sizeof(float4)), nm = (int64_t)r->mats.size();
+        float4 *dn = w->nodes.as<float4>() + (size_t)PT_NODE_F4 * oN, *dp = w->prims.as<float4>() + 3 * (size_t)oP;
+        float4 *de = w->emitters.as<float4>() + 3 * (size_t)oE;
+        PT_HIP_TRY(hipMemcpyAsync(dn, r->nodes.p, sizeof(float4) * PT_NODE_F4 * (size_t)nn, hipMemcpyDeviceToDevice, b->stream));
+        PT_HIP_TRY(hipMemcpyAsync(dp, r->prims.p, sizeof(float4) * 3 * (size_t)np, hipMemcpyDeviceToDevice, b->stream));
+        if (nrm > 0)
+            PT_HIP_TRY(hipMemcpyAsync(w->normals.as<float4>() + oNrm, r->normals.p, sizeof(float4) * (size_t)nrm, hipMemcpyDeviceToDevice, b->stream));
+        PT_HIP_TRY(hipMemcpyAsync(w->materials.as<PtMaterial>() + oM, r->materials.p, sizeof(PtMaterial) * (size_t)nm, hipMemcpyDeviceToDevice,
+                                  b->stream));
+        if (ne > 0) PT_HIP_TRY(hipMemcpyAsync(de, r->emitters.p, sizeof(float4) * 3 * (size_t)ne, hipMemcpyDeviceToDevice, b->stream));
+        if (nn > 0) k_rebase_nodes<<<(nn + 255) / 256, 256, 0, b->stream>>>(dn, nn, (int)oN, (int)oP);
+        if (np > 0) k_rebase_prims<<<(np + 255) / 256, 256, 0, b->stream>>>(dp, np, (int)oM, (int)oNrm);
+        if (ne > 0) k_rebase_prims<<<(ne + 255) / 256, 256, 0, b->stream>>>(de, ne, (int)oM, (int)oNrm);
+        PT_HIP_TRY(hipGetLastError());
+        BatchRender &R = table[j];
+        R.root = (int)oN;
+        R.emitBegin = (int)oE;
+        R.numEmitters = ne;
+        R.maxDepth = r->maxDepth;
+        R.W = r->W;
+        R.H = r->H;
+        R.npix = r->W * r->H;
+        R.seed = r->seed;
+        R.env = r->view.env;
+        R.cam = r->cam;
+        R.accum = r->accum.as<double>();
+        oN += nn;
+        oP += np;
+        oNrm += nrm;
+        oM += nm;
+        oE += ne;
+    }
+    w->view.nodes = w->nodes.as<float4>();
+    w->view.prims = w->prims.as<float4>();
+    w->view.normals = w->normals.as<float4>();
+    w->view.materials = w->materials.as<PtMaterial>();
+    w->view.emitters = w->emitters.as<float4>();
+    w->view.numPrims = (int)nP;
+    w->view.numNodes = (int)nN;
+    w->view.numEmitters = 0;   // per render (BatchRender)
+    w->view.env = PtEnv{};
+    b->renderTable.alloc(sizeof(BatchRender) * table.size());
+    PT_HIP_TRY(hipMemcpyAsync(b->renderTable.p, table.data(), sizeof(BatchRender) * table.size(), hipMemcpyHostToDevice, b->stream));
+    b->renderRays.alloc(sizeof(unsigned long long) * b->jobs.size());
+    b->waveStats.alloc(S_COUNT * sizeof(unsigned long long));
+    w->batchRenders = b->renderTable.as<BatchRender>();
+    PT_HIP_TRY(hipStreamSynchronize(b->stream));
+    b->waveJobs = b->jobs.size();
+}
+
+// Cuts the batch's sample rows (renders in batch order, each render's rows in order) into
+// chunks of at most the pool's paths; a chunk's groups are runs of rows of one render.
+void planWave(pt_batch *b, int scale) {
+    // the configured capacity (at least one row of the largest image), within the allocated
+    // pool (which does not shrink)
+    int64_t row = 0;
+    for (pt_renderer *r : b->jobs) row = std::max<int64_t>(row, (int64_t)r->W * r->H);
+    const int64_t pool = std::min<int64_t>(b->pp->pool, std::max<int64_t>({4096, b->cfg.pool, row}));
+    b->waveChunks.clear();
+    b->waveGroups.clear();
+    b->waveLead.clear();
+    const pt_renderer *first = b->jobs[0];
+    WaveChunk cur;
+    int64_t paths = 0, acc = 0;
+    auto close = [&]() {
+        if (paths == 0) {
+            // renders that added no rows (depth 0) complete with the chunk before, or first
+            if (!b->waveChunks.empty()) {
+                for (int k : cur.finishing) b->waveChunks.back().finishing.push_back(k);
+            } else {
+                b->waveLead = cur.finishing;
+            }
+            cur.finishing.clear();
+            return;
+        }
+        ChunkParams &cp = cur.cp;
+        cp.nPaths = (int)paths;
+        cp.segLen = ptSegLen(paths);
+        cp.nee = first->nee;
+        cp.misSample = first->misSample;
+        cp.numGroups = (int)b->waveGroups.size() - cur.firstGroup;
+        cur.accThreads = (int)acc;
+        b->waveChunks.push_back(cur);
+        cur = WaveChunk();
+        cur.firstGroup = (int)b->waveGroups.size();
+        paths = 0;
+        acc = 0;
+    };
+    cur.firstGroup = 0;
+    for (size_t k = 0; k < b->order.size(); k++) {
+        const int j = b->order[k];
+        const pt_renderer *r = b->jobs[j];
+        const int npix = r->W * r->H;
+        // a render of depth 0 traces nothing (its image stays black, as on its own)
+        const int spp = r->maxDepth > 0 ? std::max(1, b->spp[j] / scale) : 0;
+        for (int s0 = 0; s0 < spp;) {
+            const int64_t room = (pool - paths) / npix;
+            if (room <= 0) {
+                if (paths == 0) throw HipFailure("batch: a sample row does not fit the path pool");
+                close();
+                continue;
+            }
+            const int c = (int)std::min<int64_t>(spp - s0, room);
+            BatchGroup g;
+            g.pathBegin = (int)paths;
+            g.accBegin = (int)acc;
+            g.render = j;
+            g.npix = npix;
+            g.sampleBegin = s0;
+            g.count = c;
+            b->waveGroups.push_back(g);
+            cur.cp.maxDepth = std::max(cur.cp.maxDepth, r->maxDepth);
+            paths += (int64_t)c * npix;
+            acc += npix;
+            s0 += c;
+        }
+        cur.finishing.push_back((int)k);
+    }
+    close();
+    auto consecutive = [](const std::vector<int> &v) {
+        for (size_t i = 1; i < v.size(); i++)
+            if (v[i] != v[i - 1] + 1) return false;
+        return true;
+    };
+    bool ok = consecutive(b->waveLead);
+    for (const WaveChunk &c : b->waveChunks) ok = ok && consecutive(c.finishing);
+    if (!ok) throw HipFailure("batch: renders completing with a chunk are not consecutive");
+    b->groupTable.alloc(sizeof(BatchGroup) * std::max<size_t>(b->waveGroups.size(), 1));
+    PT_HIP_TRY(hipMemcpy(b->groupTable.p, b->waveGroups.data(), sizeof(BatchGroup) * b->waveGroups.size(), hipMemcpyHostToDevice));
+    // per chunk: the group of each bucket's first path
+    std::vector<int> buckets;
+    std::vector<size_t> bucketAt;
+    for (const WaveChunk &c : b->waveChunks) {
+        bucketAt.push_back(buckets.size());
+        const int nb = ((c.cp.nPaths - 1) >> kBatchBucketShift) + 1;
+        int g = 0;
+        for (int k = 0; k < nb; k++) {
+            const int path = k << kBatchBucketShift;
+            while (g + 1 < c.cp.numGroups && b->waveGroups[c.firstGroup + g + 1].pathBegin <= path) g++;
+            buckets.push_back(g);
+        }
+    }
+    b->bucketTable.alloc(sizeof(int) * std::max<size_t>(buckets.size(), 1));
+    PT_HIP_TRY(hipMemcpy(b->bucketTable.p, buckets.data(), sizeof(int) * buckets.size(), hipMemcpyHostToDevice));
+    for (size_t k = 0; k < b->waveChunks.size(); k++) {
+        WaveChunk &c = b->waveChunks[k];
+        c.cp.groups = b->groupTable.as<BatchGroup>() + c.firstGroup;
+        c.cp.bucketGroup = b->bucketTable.as<int>() + bucketAt[k];
+        c.cp.renders = b->renderTable.as<BatchRender>();
+        c.cp.renderRays = b->renderRays.as<unsigned long long>();
+    }
+}
+
+// One chunk of the batch wavefront: raygen from every group's camera, the bounces with the
+// batch kernels, accumulation into each render's accumulator.
+void runWaveChunk(pt_batch *b, const WaveChunk &c) {
+    pt_renderer *w = b->wave;
+    const ChunkParams &cp = c.cp;
+    PathPool &pp = *b->pp;
+    int *ctr = pp.counters.as<int>();
+    k_queue_reset<<<PT_QUEUE_SEGS + 1, 256, 0, b->stream>>>(ctr, PT_Q_RAYS, cp.nPaths, cp.segLen, 1);
+    k_raygen_batch<<<(cp.nPaths + 255) / 256, 256, 0, b->stream>>>(cp, pp.rayO[0].as<float4>(), pp.rayD[0].as<float4>());
+    PT_HIP_TRY(hipGetLastError());
+    const int shadeGrid = (int)std::max<int64_t>(PT_QUEUE_SEGS / 4, std::min<int64_t>((cp.nPaths + 255) / 256, (int64_t)w->cus * w->shadeBlocksPerCU));
+    int cur = 0;
+    for (int bn = 0; bn < cp.maxDepth; bn++) {
+        launchExtend(w, bn, cp.segLen, pp.rayO[cur].as<float4>(), pp.rayD[cur].as<float4>(), pp.hits.as<float4>(),
+                     bn == 0 ? TRACE_CAMERA : TRACE_INCOHERENT);
+        // w = 1 marks bounce 0 (camera vertices); batch kernels take the origin from rayO
+        const float4 camFlag = make_float4(0.f, 0.f, 0.f, bn == 0 ? 1.f : 0.f);
+        k_shade<false, true><<<shadeGrid, 256, 0, b->stream>>>(
+            w->view, cp, bn, ctr, pp.rayO[cur].as<float4>(), pp.rayD[cur].as<float4>(), pp.st[cur].as<uint4>(), pp.beta[cur].as<float4>(),
+            pp.hits.as<float4>(), pp.rayO[1 - cur].as<float4>(), pp.rayD[1 - cur].as<float4>(), pp.st[1 - cur].as<uint4>(),
+            pp.beta[1 - cur].as<float4>(), pp.sO.as<float4>(), pp.sD.as<float4>(), pp.sL.as<float4>(), pp.Lpath.as<float4>(), GuideShadeArgs{},
+            camFlag);
+        PT_HIP_TRY(hipGetLastError());
+        if (cp.nee) launchShadow(w, bn, cp.segLen);
+        cur = 1 - cur;
+    }
+    k_accumulate_batch<<<(c.accThreads + 255) / 256, 256, 0, b->stream>>>(cp, c.accThreads, pp.Lpath.as<float4>(), ctr,
+                                                                          b->waveStats.as<unsigned long long>());
+    PT_HIP_TRY(hipGetLastError());
+}
+
+void enqueueWave(pt_batch *b, int scale) {
+    unsigned long long *st = b->stamps.as<unsigned long long>();
+    k_stamp<<<1, 64, 0, b->stream>>>(st);
+    for (pt_renderer *r : b->jobs) zeroAccum(r);
+    k_zero_u64<<<1, 512, 0, b->stream>>>(b->waveStats.as<unsigned long long>(), S_COUNT);
+    const int nj = (int)b->jobs.size();
+    k_zero_u64<<<(nj + 255) / 256, 256, 0, b->stream>>>(b->renderRays.as<unsigned long long>(), nj);
+    PT_HIP_TRY(hipGetLastError());
+    // the renders completing with a chunk are consecutive in the batch order
+    if (!b->waveLead.empty()) k_stamp_range<<<1, 64, 0, b->stream>>>(st + 1 + b->waveLead.front(), (int)b->waveLead.size());
+    for (const WaveChunk &c : b->waveChunks) {
+        runWaveChunk(b, c);
+        if (!c.finishing.empty()) k_stamp_range<<<1, 64, 0, b->stream>>>(st + 1 + c.finishing.front(), (int)c.finishing.size());
+    }
+    for (size_t j = 0; j < b->jobs.size(); j++) b->jobs[j]->spp = std::max(1, b->spp[j] / scale);
+    b->chunks = (int)b->waveChunks.size();
+    PT_HIP_TRY(hipGetLastError());
 }
 
 void enqueueBatch(pt_batch *b, int scale) {
+    if (b->wavefront) {
+        enqueueWave(b, scale);
+        return;
+    }
     unsigned long long *st = b->stamps.as<unsigned long long>();
     k_stamp<<<1, 64, 0, b->stream>>>(st);
     b->chunks = 0;
@@ -2450,6 +2968,30 @@ double runBatch(pt_batch *b, int scale) {
     for (size_t k = 0; k < b->order.size(); k++) b->latencyMs[b->order[k]] = (double)(st[1 + k] - st[0]) / b->clockKHz;
     b->wallMs = ms;
     return ms;
+}
+
+int64_t jobRays(pt_renderer *r);
+
+// Rays traced by render j of the batch in its last run: from the render's own statistics, or
+// in the wavefront its paths' spawned rays plus its camera rays.
+int64_t renderRays(pt_batch *b, int j) {
+    if (!b->wavefront) return jobRays(b->jobs[j]);
+    unsigned long long n = 0;
+    PT_HIP_TRY(hipMemcpy(&n, b->renderRays.as<unsigned long long>() + j, sizeof n, hipMemcpyDeviceToHost));
+    const pt_renderer *r = b->jobs[j];
+    return (int64_t)n + (int64_t)r->W * r->H * r->spp;
+}
+
+int64_t batchRays(pt_batch *b) {
+    int64_t n = 0;
+    if (b->wavefront) {
+        std::vector<unsigned long long> s(S_COUNT);
+        PT_HIP_TRY(hipMemcpy(s.data(), b->waveStats.p, S_COUNT * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        for (int k = 0; k < kMaxBounces; k++) n += (int64_t)(s[S_RAYS + k] + s[S_SHADOW + k]);
+    } else {
+        for (pt_renderer *r : b->jobs) n += jobRays(r);
+    }
+    return n;
 }
 
 int64_t jobRays(pt_renderer *r) {
@@ -2540,6 +3082,10 @@ int pt_batch_set_option(pt_batch *b, int option, int64_t v) {
         b->cfg.lds = v == 0 ? 16 : (int)v;
         break;
     case PT_OPT_BATCH_GRAPH: b->useGraph = v != 0; break;
+    case PT_OPT_BATCH_WAVEFRONT:
+        b->wavefront = v != 0;
+        b->dropGraph();
+        break;
     default: return fail(PT_ERR_INVALID, "pt_batch_set_option: unknown option");
     }
     return PT_OK;
@@ -2553,6 +3099,7 @@ int pt_batch_get_option(pt_batch *b, int option, int64_t *v) {
     case PT_OPT_SHADE_BLOCKS_PER_CU: *v = b->cfg.shade; break;
     case PT_OPT_TRACE_LDS_STACK: *v = b->cfg.lds; break;
     case PT_OPT_BATCH_GRAPH: *v = b->useGraph ? 1 : 0; break;
+    case PT_OPT_BATCH_WAVEFRONT: *v = b->wavefront ? 1 : 0; break;
     default: return fail(PT_ERR_INVALID, "pt_batch_get_option: unknown option");
     }
     return PT_OK;
@@ -2571,16 +3118,19 @@ int pt_batch_autotune(pt_batch *b, int probe_divisor, pt_tune_result *out) {
     if (!b || probe_divisor < 1) return fail(PT_ERR_INVALID, "pt_batch_autotune: bad arguments");
     if (b->jobs.empty()) return fail(PT_ERR_STATE, "pt_batch_autotune: no renders");
     PT_API_BEGIN
-    int64_t maxPaths = 0;
-    for (size_t j = 0; j < b->jobs.size(); j++)
-        maxPaths = std::max<int64_t>(maxPaths, (int64_t)b->jobs[j]->W * b->jobs[j]->H * std::max(1, b->spp[j] / probe_divisor));
+    int64_t maxPaths = 0, totalPaths = 0;
+    for (size_t j = 0; j < b->jobs.size(); j++) {
+        const int64_t paths = (int64_t)b->jobs[j]->W * b->jobs[j]->H * std::max(1, b->spp[j] / probe_divisor);
+        maxPaths = std::max<int64_t>(maxPaths, paths);
+        totalPaths += paths;
+    }
+    // the wavefront's chunks hold rows of several renders
+    if (b->wavefront) maxPaths = totalPaths;
     auto probe = [&](const TuneCfg &c) {
         b->cfg = c;
         runBatch(b, probe_divisor);  // capture (or first enqueue) and warm-up
         const double ms = std::min(runBatch(b, probe_divisor), runBatch(b, probe_divisor));  // the faster of two
-        int64_t rays = 0;
-        for (pt_renderer *r : b->jobs) rays += jobRays(r);
-        return (double)rays / (ms * 1e3);
+        return (double)batchRays(b) / (ms * 1e3);
     };
     int probes = 0;
     double rate = 0.0;
@@ -2622,7 +3172,7 @@ int pt_batch_get_render_stats(pt_batch *b, int index, pt_batch_render_stats *out
     out->height = r->H;
     out->spp = (int)r->spp;
     out->triangles = (int64_t)r->triMat.size();
-    out->rays = jobRays(r);
+    out->rays = renderRays(b, index);
     out->latency_ms = index < (int)b->latencyMs.size() ? b->latencyMs[index] : 0.0;
     return PT_OK;
     PT_API_END("pt_batch_get_render_stats")
@@ -2637,9 +3187,7 @@ int pt_batch_get_stats(pt_batch *b, pt_batch_stats *out) {
     out->chunks = b->chunks;
     out->graph_nodes = (int64_t)b->graphNodes;
     out->wall_ms = b->wallMs;
-    int64_t rays = 0;
-    for (pt_renderer *r : b->jobs) rays += jobRays(r);
-    out->rays = rays;
+    out->rays = batchRays(b);
     int ov = 0;
     PT_HIP_TRY(hipMemcpy(&ov, b->pp->overflow.p, sizeof(int), hipMemcpyDeviceToHost));
     out->stack_overflows = ov;

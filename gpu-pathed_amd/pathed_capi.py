@@ -12,11 +12,14 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("PATHED_LIB") or os.path.join(HERE, "libpathed.so")
 
 PT_OK = 0
+ERR_INVALID, ERR_DEVICE, ERR_MEMORY, ERR_STATE, ERR_NUMERIC, ERR_IO = -1, -2, -3, -4, -5, -6
 LAMBERTIAN, MIRROR, GLASS, MICROFACET = 0, 1, 2, 3
 RENDERER_CUDA, RENDERER_OPTIX, RENDERER_NORMALS, RENDERER_GUIDED = 0, 1, 2, 3
 OPT_POOL_PATHS, OPT_TRACE_BLOCKS_PER_CU, OPT_SHADE_BLOCKS_PER_CU, OPT_PROFILE_KERNELS = 0, 1, 2, 3
 OPT_CONSISTENT_MIS, OPT_GUIDE_TRAIN_STEPS, OPT_GUIDE_BATCH, OPT_GUIDE_LR_MICRO, OPT_GUIDE_RESET = 4, 5, 6, 7, 8
 OPT_TRACE_LDS_STACK, OPT_BATCH_GRAPH, OPT_GUIDE_MAX_FRACTION_MILLI, OPT_TRACE_REFILL_LANES, OPT_GUIDE_BOUNCES = 9, 10, 11, 12, 13
+OPT_BVH_BUILDER, OPT_BATCH_WAVEFRONT = 14, 15
+BVH_LBVH, BVH_PLOC = 1, 2
 MISS = np.iinfo(np.int32).min
 
 # every symbol declared in include/pathed.h

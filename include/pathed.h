@@ -179,9 +179,16 @@ typedef enum {
                                     of its 64 lanes are idle (1..64; 64: whole batches of 64
                                     neighbouring rays); 0 (default) = per launch: camera rays
                                     48, later closest-hit queues and shadow rays 40 */
-    PT_OPT_GUIDE_BOUNCES = 13    /* guided: only vertices of the first N bounces draw from the
+    PT_OPT_GUIDE_BOUNCES = 13,   /* guided: only vertices of the first N bounces draw from the
                                     learned lobes (their network inference and training records);
                                     later vertices sample the BSDF; 0 (default) = every bounce */
+    PT_OPT_BVH_BUILDER = 14,     /* binary tree under the 4-wide BVH, used from the next
+                                    pt_renderer_set_scene: 1 = LBVH (Karras radix tree),
+                                    2 = PLOC (locally-ordered clustering); 0 = the default */
+    PT_OPT_BATCH_WAVEFRONT = 15  /* batches only: 1 (default) = one wavefront over every render
+                                    (paths of different renders share the queues; chunks hold
+                                    sample rows of several renders); 0 = each render's own
+                                    chunks, one render after the other */
 } pt_option;
 int pt_renderer_set_option(pt_renderer *r, int option, int64_t value);
 int pt_renderer_get_option(pt_renderer *r, int option, int64_t *value);

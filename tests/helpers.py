@@ -55,3 +55,19 @@ def path_agreement(o, gpu, pixels, samples, tol=1e-3, **kw):
         return float((np.abs(a - b).max(1) / (np.abs(b).max(1) + 1e-3) < tol).mean())
 
     return agree(gpu, c32), agree(c32, c64), c32
+
+
+def one_texel_plane_scene(scenes_dir, vfov_deg=10.0):
+    """The reference's scene 2 (scene.cu:73-94: the plane of tests/plane.obj, albedo 1, lit
+    only by the 1000 x 500 map 1_pixel_test.exr, whose single non-black texel is row 239,
+    column 753 at radiance 1e4) seen from straight above through a narrow pinhole, so that every
+    pixel sees the plane.  The plane's normal is +y and the texel spans polar angles
+    [239, 240] pi / 500 from the +y pole and 2 pi / 1000 of azimuth; irradiance from a uniform
+    patch is L * dphi * integral of cos(theta) sin(theta) dtheta = L * dphi *
+    (sin^2(theta1) - sin^2(theta0)) / 2, and the Lambertian plane returns albedo / pi of it.
+    Returns (scene, closed-form radiance)."""
+    s = P.Scene.reference(2, scenes_dir)
+    s.set_camera([0, 1, 0], [0, 0, 0], [0, 0, -1], np.radians(vfov_deg))
+    th0, th1 = 239 * np.pi / 500, 240 * np.pi / 500
+    irradiance = 1e4 * (2 * np.pi / 1000) * (np.sin(th1) ** 2 - np.sin(th0) ** 2) / 2
+    return s, irradiance / np.pi

@@ -2,8 +2,14 @@
 
 A render inside a batch consumes the same PCG32 streams as the same render on its own
 renderer, so with one chunk each the images agree to the last FP32 rounding of the mean.
-The captured graph is replayed several times, with eager work of the same program between the
-replays, and every replay must reproduce the eager image.
+That holds for both schedules: the wavefront (paths of every render in the same queues, chunks
+of sample rows of several renders) and render after render.  The captured graph is replayed
+several times, with eager work of the same program between the replays, and every replay must
+reproduce the eager image.
+
+At the batch workload's extremes (a 1024^2 x 256 spp render of ~2M triangles next to a
+128^2 x 4 spp one) each render's pixel means on a seeded pixel subset match the oracle run
+on the same PCG32 streams.
 """
 import sys
 
@@ -48,19 +54,24 @@ def _single(scene, w, h, spp, depth):
     return img, rays
 
 
-def _batch(scenes, graph=True):
+def _batch(scenes, graph=True, wavefront=True, pool=0):
     b = P.Batch()
     for sc, (_, _, _, w, h, spp, depth) in zip(scenes, JOBS):
         b.add(sc, w, h, spp, depth)
     b.set_option(P.OPT_BATCH_GRAPH, int(graph))
+    b.set_option(P.OPT_BATCH_WAVEFRONT, int(wavefront))
+    if pool:
+        b.set_option(P.OPT_POOL_PATHS, pool)
     return b
 
 
-def test_batch_matches_single_renders_and_graph_replays():
-    say("case: batch of 3 renders (graph) vs single renderers; 3 graph replays")
+@pytest.mark.parametrize("wavefront", [1, 0])
+def test_batch_matches_single_renders_and_graph_replays(wavefront):
+    say(f"case: batch of 3 renders (graph, wavefront {wavefront}) vs single renderers; 3 graph replays")
     scenes = _scenes()
     ref = [_single(sc, w, h, spp, depth) for sc, (_, _, _, w, h, spp, depth) in zip(scenes, JOBS)]
-    b = _batch(scenes)
+    b = _batch(scenes, wavefront=bool(wavefront))
+    assert b.get_option(P.OPT_BATCH_WAVEFRONT) == wavefront
     b.run()
     st = b.stats()
     say(f"batch stats {st}")
@@ -79,12 +90,13 @@ def test_batch_matches_single_renders_and_graph_replays():
     b.close()
 
 
-def test_batch_direct_launches_equal_graph():
-    say("case: batch direct launches vs graph")
+@pytest.mark.parametrize("wavefront", [1, 0])
+def test_batch_direct_launches_equal_graph(wavefront):
+    say(f"case: batch direct launches vs graph, wavefront {wavefront}")
     scenes = _scenes()
-    g = _batch(scenes, graph=True)
+    g = _batch(scenes, graph=True, wavefront=bool(wavefront))
     g.run()
-    d = _batch(scenes, graph=False)
+    d = _batch(scenes, graph=False, wavefront=bool(wavefront))
     d.run()
     assert d.stats()["graph_nodes"] == 0
     for i in range(3):
@@ -93,17 +105,28 @@ def test_batch_direct_launches_equal_graph():
     d.close()
 
 
-def test_batch_latency_follows_order():
-    say("case: batch latencies")
-    b = _batch(_scenes())
+@pytest.mark.parametrize("wavefront,pool", [(0, 0), (1, 0), (1, 12288)])
+def test_batch_latency_follows_order(wavefront, pool):
+    """Renders complete in the batch's order (cheapest first by pixels x spp x log2 primitives).
+    Render after render, each completes after the previous one; in the wavefront a chunk's
+    renders complete together, so with the whole batch in one chunk the latencies are equal and
+    with a pool of one render's paths they follow the order."""
+    say(f"case: batch latencies, wavefront {wavefront}, pool {pool}")
+    scenes = _scenes()
+    b = _batch(scenes, wavefront=bool(wavefront), pool=pool)
     b.run()
-    lat = [b.render_stats(i)["latency_ms"] for i in range(3)]
-    say(f"latencies {lat}")
-    assert all(x > 0 for x in lat)
-    # renders complete shortest-first by the cost model pixels x spp x log2(primitives)
-    cost = [w * h * spp * np.log2(2 + P.Scene.preset(p, seed=s, param=q).info().n_triangles)
-            for p, s, q, w, h, spp, _ in JOBS]
-    assert list(np.argsort(lat)) == list(np.argsort(cost, kind="stable"))
+    lat = np.array([b.render_stats(i)["latency_ms"] for i in range(3)])
+    st = b.stats()
+    say(f"latencies {lat}, chunks {st['chunks']}")
+    assert (lat > 0).all()
+    cost = [w * h * spp * np.log2(2 + sc.info().n_triangles) for sc, (_, _, _, w, h, spp, _) in zip(scenes, JOBS)]
+    order = np.argsort(cost, kind="stable")
+    assert np.all(np.diff(lat[order]) >= 0)
+    if not wavefront or pool:
+        assert np.all(np.diff(lat[order]) > 0)
+    if wavefront and not pool:
+        # one chunk: one completion stamp for all three
+        assert st["chunks"] == 1 and np.ptp(lat) == 0
     b.close()
 
 
@@ -131,3 +154,44 @@ def test_autotune_batch_and_renderer():
     assert r.spp() == 4  # probe samples were discarded
     assert r.stats().stack_overflows == 0
     r.close()
+
+
+EXTREMES = [  # (seed, triangles, resolution, spp): the batch workload's largest and smallest renders
+    (3001, 2_000_000, 1024, 256),
+    (3002, 10_000, 128, 4),
+    (3003, 200_000, 512, 16),
+]
+
+
+def test_batch_extremes_match_the_oracle():
+    """One wavefront over a 1024^2 x 256 spp render of ~2M triangles (268M paths, several
+    chunks), a 128^2 x 4 spp render of 10k triangles and a 512^2 x 16 spp one: at 48 seeded pixels
+    of each render, the batch's pixel means equal the oracle's means of the same PCG32 streams
+    (each path of the pixel traced by the product-mode FP32 integrator) to 1e-3 relative."""
+    import oracle as O
+
+    say("case: batch at the workload's extremes vs the oracle")
+    scenes = [P.Scene.preset(4, seed=s, param=t) for s, t, _, _ in EXTREMES]
+    b = P.Batch()
+    for sc, (_, _, res, spp) in zip(scenes, EXTREMES):
+        b.add(sc, res, res, spp, 8)
+    b.run()
+    st = b.stats()
+    say(f"batch stats {st}")
+    assert st["stack_overflows"] == 0 and st["chunks"] >= 4
+    rng = np.random.default_rng(7)
+    for i, (sc, (seed, tris, res, spp)) in enumerate(zip(scenes, EXTREMES)):
+        img = b.radiance(i).reshape(-1, 3)
+        pix = np.sort(rng.choice(res * res, 48, replace=False))
+        o = O.Scene(sc.export(), res, res, bvh=True)
+        sums, _, rays = o.render_pixels(pix, spp=spp, max_depth=8, nee=1, mode=1, tri_test=1, ref_env=0, precision=32, threads=16)
+        want = sums / spp
+        got = img[pix].astype(np.float64)
+        rel = np.abs(got - want).max(1) / (np.abs(want).max(1) + 1e-3)
+        frac = float((rel < 1e-3).mean())
+        rs = b.render_stats(i)
+        say(f"render {i}: {res}^2 x {spp} spp, {sc.info().n_triangles} tris: {frac:.4f} of pixels within 1e-3 "
+            f"(max rel {rel.max():.2e}); {rs['rays']} rays, latency {rs['latency_ms']:.1f} ms")
+        assert frac >= 0.97 and np.isfinite(img).all(), i
+        assert rs["rays"] > res * res * spp
+    b.close()

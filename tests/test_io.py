@@ -98,6 +98,18 @@ def test_reference_scenes_0_and_1():
     assert a["sph"][0, 3] == pytest.approx(0.45)
 
 
+def test_reference_scene_2_one_texel_environment():
+    """scene.cu:73-94: the plane of tests/plane.obj (white Lambertian) lit only by the
+    PIZ-compressed 1_pixel_test.exr map."""
+    s = P.Scene.reference(2, SCENES)
+    i = s.info()
+    assert (i.n_triangles, i.n_spheres, i.n_materials, i.n_emitters) == (2, 0, 1, 0)
+    assert (i.env_width, i.env_height) == (1000, 500)
+    env = s.export()["env_rgba"]
+    # the environment's rows run from the +y pole; the file's row 239 is the lit one
+    assert np.argwhere(env[..., :3].max(2) > 0).tolist() == [[239, 753]]
+
+
 def test_reference_scene_3_mis_plates():
     """scene.cu:95-167: the floor and four Beckmann plates (PLY, alpha 0.005 .. 0.1), five
     emissive spheres in the reference's order, the wells6 environment (1000 x 500) and the
@@ -220,13 +232,270 @@ def test_exr_environment_and_errors(tmp_path):
 
 
 def test_reference_environment_assets():
-    """The reference's own environment maps: wells6 (ZIP, FLOAT A/B/G/R, 1000x500) loads;
-    the 1-pixel test map uses PIZ, which this reader reports as unsupported."""
+    """The reference's own environment maps: wells6 (ZIP, FLOAT A/B/G/R, 1000x500) and the
+    1-pixel test map (PIZ, FLOAT B/G/R, 1000x500, 16 PIZ blocks of 32 lines): the latter decodes
+    to black everywhere except one texel of radiance 1e4 (row 239, column 753), which a wrong
+    Huffman table, bit order or wavelet step would not produce."""
     img = P.exr_read(os.path.join(SCENES, "assets", "20060807_wells6_hd.exr"))
     assert img.shape == (500, 1000, 4)
     assert np.isfinite(img).all() and (img[..., :3] >= 0).all() and img[..., :3].max() > 1
-    with pytest.raises(P.PtError, match="compression 4"):
-        P.exr_read(os.path.join(SCENES, "assets", "1_pixel_test.exr"))
+    one = P.exr_read(os.path.join(SCENES, "assets", "1_pixel_test.exr"))
+    assert one.shape == (500, 1000, 4) and (one[..., 3] == 1).all()
+    lit = np.argwhere(one[..., :3].max(2) != 0)
+    assert lit.tolist() == [[239, 753]]
+    assert one[239, 753, :3].tolist() == [1e4, 1e4, 1e4]
+
+
+# ------------------------------------------------------------------ PIZ: independent encoder
+# Written from the OpenEXR PIZ description (bitmap of the values present and their compaction
+# table, 2D lifting wavelet per channel plane, canonical Huffman codes with a run-length
+# pseudo-symbol), to check the reader on data that the reference's one PIZ file does not
+# cover: HALF channels, several blocks, ragged block heights, both wavelet widths and runs.
+
+def _piz_wavelet(v, nx, ox, ny, oy, maxv):
+    """Forward 2D wavelet in place on a uint16 numpy array (element (x, y) at x*ox + y*oy)."""
+    w14 = maxv < (1 << 14)
+
+    def lift(a, b):
+        if w14:
+            a16, b16 = int(np.int16(np.uint16(a).view(np.int16))), int(np.int16(np.uint16(b).view(np.int16)))
+            m = (a16 + b16) >> 1
+            d = a16 - b16
+            return np.uint16(m & 0xffff), np.uint16(d & 0xffff)
+        ao = (int(a) + 0x8000) & 0xffff
+        m = (ao + int(b)) >> 1
+        d = ao - int(b)
+        if d < 0:
+            m = (m + 0x8000) & 0xffff
+        return np.uint16(m), np.uint16(d & 0xffff)
+
+    n = min(nx, ny)
+    p, p2 = 1, 2
+    while p2 <= n:
+        y = 0
+        while y + p2 <= ny:
+            x = 0
+            while x + p2 <= nx:
+                i00 = x * ox + y * oy
+                i01, i10 = i00 + p * ox, i00 + p * oy
+                i11 = i10 + p * ox
+                a00, a01 = lift(v[i00], v[i01])
+                a10, a11 = lift(v[i10], v[i11])
+                v[i00], v[i10] = lift(a00, a10)
+                v[i01], v[i11] = lift(a01, a11)
+                x += p2
+            if nx & p:
+                i00 = x * ox + y * oy
+                i10 = i00 + p * oy
+                v[i00], v[i10] = lift(v[i00], v[i10])
+            y += p2
+        if ny & p:
+            x = 0
+            while x + p2 <= nx:
+                i00 = x * ox + y * oy
+                i01 = i00 + p * ox
+                v[i00], v[i01] = lift(v[i00], v[i01])
+                x += p2
+        p, p2 = p2, p2 << 1
+
+
+class _BitWriter:
+    def __init__(self):
+        self.bits = []
+
+    def put(self, value, n):
+        self.bits += [(value >> (n - 1 - k)) & 1 for k in range(n)]
+
+    def bytes(self):
+        b = self.bits + [0] * (-len(self.bits) % 8)
+        return bytes(int("".join(map(str, b[i:i + 8])), 2) for i in range(0, len(b), 8))
+
+
+def _huffman_lengths(freq):
+    import heapq
+    heap = [(f, i, (s,)) for i, (s, f) in enumerate(sorted(freq.items()))]
+    heapq.heapify(heap)
+    lengths = {s: 0 for s in freq}
+    if len(heap) == 1:
+        lengths[heap[0][2][0]] = 1
+        return lengths
+    tie = len(heap)
+    while len(heap) > 1:
+        f1, _, a = heapq.heappop(heap)
+        f2, _, b = heapq.heappop(heap)
+        for s in a + b:
+            lengths[s] += 1
+        heapq.heappush(heap, (f1 + f2, tie, a + b))
+        tie += 1
+    return lengths
+
+
+def _huffman_encode(words):
+    """Huffman stream of uint16 words: header, run-length-coded code lengths, codes; runs of
+    one word longer than 3 repeats use the pseudo-symbol (max word + 1) and an 8-bit count."""
+    words = [int(x) for x in words]
+    rlc = max(words) + 1
+    tokens = []
+    i = 0
+    while i < len(words):
+        j = i + 1
+        while j < len(words) and words[j] == words[i] and j - i - 1 < 255:
+            j += 1
+        run = j - i - 1
+        tokens.append(words[i])
+        if run > 3:
+            tokens.append(("run", run))
+        else:
+            tokens += [words[i]] * run
+        i = j
+    freq = {}
+    for t in tokens:
+        s_ = rlc if isinstance(t, tuple) else t
+        freq[s_] = freq.get(s_, 0) + 1
+    lengths = _huffman_lengths(freq)
+    im, iM = min(freq), rlc
+    # canonical codes: first code per length from the longest down, symbols in order
+    count = [0] * 60
+    for s_ in range(im, iM + 1):
+        count[lengths.get(s_, 0)] += 1
+    count[0] = 0
+    first, c = [0] * 60, 0
+    for l in range(58, 0, -1):
+        first[l] = c
+        c = (c + count[l]) >> 1
+    nxt, codes = list(first), {}
+    for s_ in range(im, iM + 1):
+        l = lengths.get(s_, 0)
+        if l:
+            codes[s_] = (nxt[l], l)
+            nxt[l] += 1
+    table = _BitWriter()
+    s_ = im
+    while s_ <= iM:
+        l = lengths.get(s_, 0)
+        if l == 0:
+            run = 1
+            while s_ + run <= iM and lengths.get(s_ + run, 0) == 0 and run < 261:
+                run += 1
+            if run >= 6:
+                table.put(63, 6)
+                table.put(run - 6, 8)
+            elif run >= 2:
+                table.put(59 + run - 2, 6)
+            else:
+                table.put(0, 6)
+            s_ += run
+            continue
+        table.put(l, 6)
+        s_ += 1
+    data = _BitWriter()
+    for t in tokens:
+        if isinstance(t, tuple):
+            data.put(*codes[rlc])
+            data.put(t[1], 8)
+        else:
+            data.put(codes[t][0], codes[t][1])
+    tb = table.bytes()
+    return struct.pack("<5I", im, iM, len(tb), len(data.bits), 0) + tb + data.bytes()
+
+
+def _piz_block(planes):
+    """planes: list of (uint16 array of one channel's words for the block's lines, nx, words
+    per pixel, ny).  Returns the PIZ block bytes."""
+    allw = np.concatenate([p for p, *_ in planes])
+    present = np.zeros(1 << 16, bool)
+    present[allw] = True
+    present[0] = False
+    bitmap = np.packbits(present.reshape(-1, 8)[:, ::-1], axis=1).reshape(-1)
+    nz = np.nonzero(bitmap)[0]
+    lo, hi = (int(nz[0]), int(nz[-1])) if nz.size else (8191, 0)
+    values = np.concatenate([[0], np.nonzero(present)[0]])
+    fwd = np.zeros(1 << 16, np.uint16)
+    fwd[values] = np.arange(values.size, dtype=np.uint16)
+    maxv = values.size - 1
+    coded = []
+    for words, nx, wd, ny in planes:
+        v = fwd[words].copy()
+        for j in range(wd):  # each interleaved word of a pixel is its own plane
+            _piz_wavelet(v[j:], nx, wd, ny, nx * wd, maxv)
+        coded.append(v)
+    huf = _huffman_encode(np.concatenate(coded))
+    out = struct.pack("<HH", lo, hi)
+    if lo <= hi:
+        out += bitmap[lo:hi + 1].tobytes()
+    return out + struct.pack("<I", len(huf)) + huf, maxv
+
+
+def _write_piz_exr(path, rgba, half):
+    h, w, _ = rgba.shape
+    pt = 1 if half else 2
+    dt = np.float16 if half else np.float32
+
+    def attr(name, typ, data):
+        return name.encode() + b"\0" + typ.encode() + b"\0" + struct.pack("<I", len(data)) + data
+
+    names = "ABGR"
+    ch = b"".join(n.encode() + b"\0" + struct.pack("<iB3xii", pt, 0, 1, 1) for n in names) + b"\0"
+    hdr = struct.pack("<II", 20000630, 2)
+    hdr += attr("channels", "chlist", ch)
+    hdr += attr("compression", "compression", bytes([4]))
+    hdr += attr("dataWindow", "box2i", struct.pack("<4i", 0, 0, w - 1, h - 1))
+    hdr += attr("displayWindow", "box2i", struct.pack("<4i", 0, 0, w - 1, h - 1))
+    hdr += attr("lineOrder", "lineOrder", b"\0")
+    hdr += attr("pixelAspectRatio", "float", struct.pack("<f", 1))
+    hdr += attr("screenWindowCenter", "v2f", struct.pack("<ff", 0, 0))
+    hdr += attr("screenWindowWidth", "float", struct.pack("<f", 1)) + b"\0"
+    nblk = (h + 31) // 32
+    blocks, kinds = [], []
+    for b in range(nblk):
+        rows = rgba[b * 32:(b + 1) * 32]
+        ny = rows.shape[0]
+        planes = []
+        for c in (3, 2, 1, 0):
+            words = np.frombuffer(rows[:, :, c].astype(dt).tobytes(), np.uint16).copy()
+            planes.append((words, w, 1 if half else 2, ny))
+        blk, maxv = _piz_block(planes)
+        raw = b"".join(rows[r, :, c].astype(dt).tobytes() for r in range(ny) for c in (3, 2, 1, 0))
+        # a block exactly as long as its raw data is read as raw data, so it is never kept;
+        # PIZ blocks longer than raw are legal and are kept to exercise the codec
+        assert len(blk) != len(raw)
+        kinds.append(("piz", 14 if maxv < (1 << 14) else 16))
+        blocks.append(struct.pack("<ii", b * 32, len(blk)) + blk)
+    offs, pos = [], len(hdr) + 8 * nblk
+    for blk in blocks:
+        offs.append(pos)
+        pos += len(blk)
+    with open(path, "wb") as f:
+        f.write(hdr + b"".join(struct.pack("<Q", o) for o in offs) + b"".join(blocks))
+    return kinds
+
+
+@pytest.mark.parametrize("half,h,w,kind,bits", [(True, 37, 23, "smooth", 14), (False, 70, 19, "smooth", 14), (True, 33, 8, "noise", 14),
+                                                (False, 40, 256, "noise", 16), (True, 64, 16, "flat", 14)])
+def test_exr_piz_read(tmp_path, half, h, w, kind, bits):
+    """PIZ files from the independent encoder above decode to the exact pixels: HALF and FLOAT
+    channels, several 32-line blocks with a ragged last one, smooth data (few distinct values:
+    14-bit wavelet), noise (many values: 16-bit wavelet) and constant runs (run-length codes)."""
+    rng = np.random.default_rng(h * 100 + w)
+    y, x = np.mgrid[0:h, 0:w]
+    if kind == "smooth":
+        base = (np.sin(x / 5.0) + np.cos(y / 7.0) + 2.5).astype(np.float32)
+        rgba = np.stack([base, base * 0.5, base * 0.25, np.ones_like(base)], -1)
+        rgba = np.round(rgba * 8) / 8
+    elif kind == "noise":
+        rgba = rng.lognormal(0, 2, (h, w, 4)).astype(np.float32)
+    else:
+        rgba = np.full((h, w, 4), 0.75, np.float32)
+        rgba[10:20, 3:9, 0] = 3.0
+    rgba = rgba.astype(np.float32)
+    p = str(tmp_path / "p.exr")
+    kinds = _write_piz_exr(p, rgba, half)
+    # the first block went through the codec, with the wavelet width the case is meant for
+    assert kinds[0] == ("piz", bits), kinds
+    back = P.exr_read(p)
+    want = rgba.astype(np.float16).astype(np.float32) if half else rgba
+    assert back.shape == (h, w, 4)
+    assert np.array_equal(back, want)
 
 
 def test_exr_crafted_offsets_and_windows_are_errors(tmp_path):
