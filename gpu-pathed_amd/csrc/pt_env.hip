@@ -8,7 +8,12 @@
 // and fused multiply-adds are disabled here, so the tables are reproducible bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <string>
+
+#include "../../include/pathed.h"
 #include "pt_gpu.h"
+#include "pt_internal.h"
 
 #pragma clang fp contract(off)
 
@@ -99,6 +104,28 @@ void packEnvPdf(float4 *rgba, const float *texelPdf, int n, hipStream_t stream) 
     PT_HIP_TRY(hipGetLastError());
 }
 
+namespace {
+
+// The FP64 working arrays of one table build.
+struct EnvScratch {
+    double *wts, *rowProb, *rowSum, *margProb, *total;
+    int *rowAlias, *margAlias;
+};
+
+void launchEnvTables(const float4 *rgba, int w, int h, int sinWeight, const EnvScratch &s, float2 *rows, float2 *marginal,
+                     float *texelPdf, hipStream_t stream) {
+    const size_t n = (size_t)w * h;
+    const int nb = (int)((n + 255) / 256);
+    k_env_weights<<<nb, 256, 0, stream>>>(rgba, w, h, sinWeight, s.wts);
+    k_env_rows<<<(h + 63) / 64, 64, 0, stream>>>(s.wts, w, h, s.rowProb, s.rowAlias, s.rowSum);
+    k_env_marginal<<<1, 64, 0, stream>>>(s.rowSum, h, s.margProb, s.margAlias, s.total);
+    const int nb2 = (int)((std::max(n, (size_t)h) + 255) / 256);
+    k_env_pack<<<nb2, 256, 0, stream>>>(s.wts, s.rowProb, s.rowAlias, s.margProb, s.margAlias, s.total, w, h, rows, marginal, texelPdf);
+    PT_HIP_TRY(hipGetLastError());
+}
+
+}  // namespace
+
 void buildEnvTables(const float4 *rgba, int w, int h, int sinWeight, float2 *rows, float2 *marginal, float *texelPdf, double *rowProbOut,
                     int *rowAliasOut, double *margProbOut, int *margAliasOut, hipStream_t stream) {
     const size_t n = (size_t)w * h;
@@ -110,14 +137,8 @@ void buildEnvTables(const float4 *rgba, int w, int h, int sinWeight, float2 *row
     mp.alloc(h * sizeof(double));
     ma.alloc(h * sizeof(int));
     tot.alloc(sizeof(double));
-    const int nb = (int)((n + 255) / 256);
-    k_env_weights<<<nb, 256, 0, stream>>>(rgba, w, h, sinWeight, wts.as<double>());
-    k_env_rows<<<(h + 63) / 64, 64, 0, stream>>>(wts.as<double>(), w, h, rp.as<double>(), ra.as<int>(), rs.as<double>());
-    k_env_marginal<<<1, 64, 0, stream>>>(rs.as<double>(), h, mp.as<double>(), ma.as<int>(), tot.as<double>());
-    const int nb2 = (int)((std::max(n, (size_t)h) + 255) / 256);
-    k_env_pack<<<nb2, 256, 0, stream>>>(wts.as<double>(), rp.as<double>(), ra.as<int>(), mp.as<double>(), ma.as<int>(), tot.as<double>(), w, h,
-                                        rows, marginal, texelPdf);
-    PT_HIP_TRY(hipGetLastError());
+    const EnvScratch sc{wts.as<double>(), rp.as<double>(), rs.as<double>(), mp.as<double>(), tot.as<double>(), ra.as<int>(), ma.as<int>()};
+    launchEnvTables(rgba, w, h, sinWeight, sc, rows, marginal, texelPdf, stream);
     if (rowProbOut) PT_HIP_TRY(hipMemcpyAsync(rowProbOut, rp.p, n * sizeof(double), hipMemcpyDeviceToHost, stream));
     if (rowAliasOut) PT_HIP_TRY(hipMemcpyAsync(rowAliasOut, ra.p, n * sizeof(int), hipMemcpyDeviceToHost, stream));
     if (margProbOut) PT_HIP_TRY(hipMemcpyAsync(margProbOut, mp.p, h * sizeof(double), hipMemcpyDeviceToHost, stream));
@@ -126,4 +147,79 @@ void buildEnvTables(const float4 *rgba, int w, int h, int sinWeight, float2 *row
     PT_HIP_TRY(hipStreamSynchronize(stream));
 }
 
+// ---------------------------------------------------------------------------------------------
+// Table build without allocation: the working arrays live in one fixed device array per GPU.
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+constexpr size_t kAlign = 256;
+constexpr size_t alignUp(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
+constexpr size_t scratchBytes(size_t texels, size_t rows) {
+    return alignUp(texels * sizeof(double)) * 2 + alignUp(texels * sizeof(int)) + alignUp(rows * sizeof(double)) * 2 +
+           alignUp(rows * sizeof(int)) + alignUp(sizeof(double));
+}
+constexpr size_t kScratchBytes = scratchBytes(PT_ENV_NOALLOC_MAX_TEXELS, PT_ENV_NOALLOC_MAX_ROWS);
+
+// Device-side working memory of pt_env_tables_noalloc.  Host code never uses this name as an
+// address (on the host it is a stand-in symbol): it uses the per-device pointer that
+// hipGetSymbolAddress returns.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex gScratchMutex;
+void *gScratchPtr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per device.
+void *scratchOnDevice(int dev) {
+    std::lock_guard<std::mutex> lock(gScratchMutex);
+    if (!gScratchPtr[dev]) {
+        void *ptr = nullptr;
+        PT_HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) throw HipFailure("hipGetSymbolAddress(scratch) returned NULL");
+        gScratchPtr[dev] = ptr;
+    }
+    return gScratchPtr[dev];
+}
+
+}  // namespace
+
 }  // namespace pt
+
+extern "C" int pt_env_tables_noalloc(const float *rgba, int width, int height, int sin_theta_weight, float *rows, float *marginal,
+                                     float *texel_pdf, void *stream) {
+    using namespace pt;
+    if (!rgba || !rows || !marginal || !texel_pdf || width <= 0 || height <= 0)
+        return fail(PT_ERR_INVALID, "pt_env_tables_noalloc: bad arguments");
+    if ((int64_t)width * height > PT_ENV_NOALLOC_MAX_TEXELS || height > PT_ENV_NOALLOC_MAX_ROWS)
+        return fail(PT_ERR_INVALID, "pt_env_tables_noalloc: map of " + std::to_string(width) + "x" + std::to_string(height) +
+                                        " exceeds the fixed scratch (at most " + std::to_string(PT_ENV_NOALLOC_MAX_TEXELS) +
+                                        " texels and " + std::to_string(PT_ENV_NOALLOC_MAX_ROWS) + " rows)");
+    try {
+        int dev = 0;
+        PT_HIP_TRY(hipGetDevice(&dev));
+        if (dev < 0 || dev >= kMaxDevices) return fail(PT_ERR_DEVICE, "pt_env_tables_noalloc: device ordinal out of range");
+        unsigned char *base = (unsigned char *)scratchOnDevice(dev);
+        const size_t n = (size_t)width * height, hh = (size_t)height;
+        EnvScratch sc;
+        size_t at = 0;
+        auto carve = [&](size_t bytes) {
+            unsigned char *p = base + at;
+            at += alignUp(bytes);
+            return p;
+        };
+        sc.wts = (double *)carve(n * sizeof(double));
+        sc.rowProb = (double *)carve(n * sizeof(double));
+        sc.rowAlias = (int *)carve(n * sizeof(int));
+        sc.rowSum = (double *)carve(hh * sizeof(double));
+        sc.margProb = (double *)carve(hh * sizeof(double));
+        sc.margAlias = (int *)carve(hh * sizeof(int));
+        sc.total = (double *)carve(sizeof(double));
+        if (at > kScratchBytes) return fail(PT_ERR_INVALID, "pt_env_tables_noalloc: scratch layout exceeds its array");
+        // every working array is written in full before it is read, so nothing is cleared
+        launchEnvTables((const float4 *)rgba, width, height, sin_theta_weight, sc, (float2 *)rows, (float2 *)marginal, texel_pdf,
+                        (hipStream_t)stream);
+        return PT_OK;
+    } catch (const HipFailure &e) {
+        return fail(PT_ERR_DEVICE, std::string("pt_env_tables_noalloc: ") + e.msg);
+    }
+}

@@ -40,7 +40,7 @@ EXPORTS = [
     "pt_exr_read_rgba", "pt_exr_write_rgb",
     "pt_renderer_autotune", "pt_batch_create", "pt_batch_destroy", "pt_batch_add", "pt_batch_set_option", "pt_batch_get_option",
     "pt_batch_autotune", "pt_batch_run", "pt_batch_get_radiance", "pt_batch_get_render_stats", "pt_batch_get_stats",
-    "pt_renderer_get_stream",
+    "pt_env_tables_noalloc", "pt_renderer_get_stream",
 ]
 GUIDE_PARAMS = 20800
 
@@ -162,6 +162,7 @@ def lib():
         L.pt_unit_bsdf_eval.argtypes = [vp, vp, vp, C.c_int64, vp, vp]
         L.pt_unit_bsdf_sample.argtypes = [vp, vp, vp, C.c_int64, vp]
         L.pt_unit_env_tables.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp]
+        L.pt_env_tables_noalloc.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp]
         L.pt_unit_env_sample.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, C.c_int64, vp, vp, vp, vp, C.c_int64, vp]
         L.pt_unit_path_uniforms.argtypes = [C.c_uint64, vp, vp, C.c_int64, C.c_int, vp]
         L.pt_device_count.argtypes = [C.POINTER(C.c_int)]

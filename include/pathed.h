@@ -296,6 +296,21 @@ int pt_unit_bsdf_sample(const float *mat, const float *wo, const float *xi, int6
 int pt_unit_env_tables(const float *rgba, int width, int height, int sin_theta_weight,
                        double *row_prob, int *row_alias, double *marg_prob, int *marg_alias,
                        float *texel_pdf);
+/* The tables of pt_unit_env_tables without any device allocation, for callers that cannot
+ * allocate: code under HIP graph capture, serving loops, harnesses that forbid hipMalloc.  No
+ * counterpart in the reference (environment_light.cu:14-56 builds its CMF tables on the host).
+ * All pointers are device memory of the current HIP device: rgba[4*w*h] (RGBA float); outputs
+ * rows[2*w*h] (per texel: row alias probability as float, alias index as int bits),
+ * marginal[2*h] (the same over rows), texel_pdf[w*h] (pmf * w*h / (2 pi^2)).  The FP64
+ * working arrays live in one fixed array per device (about 42 MB, held from the library's
+ * first use on a device until the process ends), so w*h <= PT_ENV_NOALLOC_MAX_TEXELS and
+ * h <= PT_ENV_NOALLOC_MAX_ROWS; larger maps fail with PT_ERR_INVALID.  Enqueued on `stream`
+ * (a hipStream_t; NULL = the null stream) and returns without waiting.  Calls that overlap on
+ * one device share that array and race: serialise them (one stream, or events). */
+#define PT_ENV_NOALLOC_MAX_TEXELS (1 << 21)
+#define PT_ENV_NOALLOC_MAX_ROWS (1 << 12)
+int pt_env_tables_noalloc(const float *rgba, int width, int height, int sin_theta_weight,
+                          float *rows, float *marginal, float *texel_pdf, void *stream);
 /* Environment sampling and pdf on the GPU: xi[2n] -> dir[3n], pdf[n], emit[3n];
  * pdf_dirs[3m] -> pdf_out[m]. */
 int pt_unit_env_sample(const float *rgba, int width, int height, int sin_theta_weight,

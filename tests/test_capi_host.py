@@ -204,3 +204,15 @@ def test_material_edits_follow_reference_rules():
         s.renderAsync()
     st = s.pollRender()
     assert st.isRendering is False
+
+
+def test_env_tables_noalloc_rejects_bad_sizes_on_the_host():
+    """The allocation-free table build refuses maps beyond its fixed scratch, and NULL pointers,
+    before it touches a device (the pointers here are never dereferenced)."""
+    L = P.lib()
+    fake = ctypes.c_void_p(0x1000)
+    assert L.pt_env_tables_noalloc(fake, 4096, 1024, 0, fake, fake, fake, None) == P.ERR_INVALID
+    assert b"exceeds the fixed scratch" in L.pt_last_error()
+    assert L.pt_env_tables_noalloc(fake, 1, (1 << 12) + 1, 0, fake, fake, fake, None) == P.ERR_INVALID
+    assert L.pt_env_tables_noalloc(None, 4, 4, 0, fake, fake, fake, None) == P.ERR_INVALID
+    assert L.pt_env_tables_noalloc(fake, 0, 4, 0, fake, fake, fake, None) == P.ERR_INVALID
