@@ -37,8 +37,13 @@ constexpr int GUIDE_REC_F4 = 12;
 // continuation's contribution, weight)
 constexpr int GUIDE_TREC_F4 = 4;
 
+// The BF16 forward-weight buffer holds two copies of the weights: [5][64 out][64 in]
+// row-major, then the same values as MFMA A fragments of the inference kernel's transposed
+// layer product (GUIDE_W * GUIDE_W per layer).
+constexpr int GUIDE_WFRAG_OFF = GUIDE_LAYERS * GUIDE_W * GUIDE_W;
+constexpr int GUIDE_WBUF = 2 * GUIDE_WFRAG_OFF;   // bf16 elements of the forward-weight buffer
 struct GuideNet {
-    const __bf16 *w;    // [5][64 out][64 in]
+    const __bf16 *w;    // [5][64 out][64 in], then the fragment copy at GUIDE_WFRAG_OFF
     const __bf16 *wt;   // [5][64 in][64 out]
     const float *b;     // [5][64]
 };
@@ -94,5 +99,6 @@ void guideGradients(GuideNet net, const float4 *records, const int *count, const
 void guideAdam(float *params, float *m, float *v, const float *grad, int step, float lr, __bf16 *dW, __bf16 *dWt, float *dB, hipStream_t s);
 // C (32 x 32) = A (32 x 16) x B (16 x 32) with one v_mfma_f32_32x32x16_bf16 (lane-layout check)
 void mfmaTile(const float *A, const float *B, float *C, hipStream_t s);
+void mfmaTile16(const float *A, const float *B, float *C, hipStream_t s);
 
 }  // namespace pt

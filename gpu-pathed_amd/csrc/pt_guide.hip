@@ -12,9 +12,10 @@
 // k-values are contiguous: activations row-major for the forward A operand, transposed copies
 // for the weight-gradient products.
 //
-// Inference: blocks of 2 waves, each wave loops over 32-row tiles of the ray queue, computes
-// the raw features of its rows, encodes them one-blob into LDS, runs the 5 layers and writes the
-// mixture (8 vMF lobes + selection probability) per row.
+// Inference: blocks of 8 waves, each wave loops over 64-row tiles of the ray queue, encodes
+// the rows' raw features one-blob straight into MFMA operands, runs the 5 layers with the
+// activations in registers (v_mfma_f32_16x16x32_bf16 on the transposed product, weights
+// permuted to match) and writes the mixture (8 vMF lobes + selection probability) per row.
 // Training step: (1) one wave per block, persistent over 32-row tiles of the record batch:
 // forward, KL loss gradient at the output, backward through the layers; every layer's input
 // activations and output deltas are written transposed (BF16) to HBM; (2) the weight and bias
@@ -52,6 +53,14 @@ __device__ __forceinline__ bf16x8 ld8(const __bf16 *p) { return *(const bf16x8 *
 
 __device__ __forceinline__ f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0); }
 
+// v_mfma_f32_16x16x32_bf16 (inference): A: row (l & 15), 8 consecutive k starting at
+// 8 * (l >> 4) of the 32-wide K step; B: col (l & 15), the same 8 k; C: 4 floats per lane,
+// element r at row 4 * (l >> 4) + r, col (l & 15) (checked on the device by
+// pt_unit_mfma_tile16).  A 32 x 64 layer output is 2 x 4 such tiles of two K steps each:
+// eight independent accumulation chains per wave instead of two.
+typedef float f32x4 __attribute__((__vector_size__(4 * sizeof(float))));
+__device__ __forceinline__ f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+
 __device__ __forceinline__ float luminance(float4 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
 
 __device__ __forceinline__ float oneBlob(float s, int j) {
@@ -64,16 +73,6 @@ __device__ __forceinline__ f32x16 layerTile(const __bf16 *act, const __bf16 *W, 
     f32x16 acc = {};
     const __bf16 *arow = act + (lane & 31) * ACT_STRIDE + (lane >> 5) * 8;
     const __bf16 *wrow = W + (size_t)L * GUIDE_W * GUIDE_W + (ct * 32 + (lane & 31)) * GUIDE_W + (lane >> 5) * 8;
-#pragma unroll
-    for (int ks = 0; ks < GUIDE_W / 16; ks++) acc = mfma(ld8(arow + ks * 16), ld8(wrow + ks * 16), acc);
-    return acc;
-}
-
-// The same product with the weights staged in LDS ([layer][out][ACT_STRIDE]).
-__device__ __forceinline__ f32x16 layerTileLds(const __bf16 *act, const __bf16 *sW, int L, int ct, int lane) {
-    f32x16 acc = {};
-    const __bf16 *arow = act + (lane & 31) * ACT_STRIDE + (lane >> 5) * 8;
-    const __bf16 *wrow = sW + (L * GUIDE_W + ct * 32 + (lane & 31)) * ACT_STRIDE + (lane >> 5) * 8;
 #pragma unroll
     for (int ks = 0; ks < GUIDE_W / 16; ks++) acc = mfma(ld8(arow + ks * 16), ld8(wrow + ks * 16), acc);
     return acc;
@@ -113,10 +112,10 @@ __device__ __forceinline__ float vmfPdf(f3 mu, float kappa, f3 w) {
 // --------------------------------------------------------------------------------------------
 // inference
 // --------------------------------------------------------------------------------------------
-// Waves per inference block: the block stages the forward weights in LDS once (46 KiB,
-// rows padded to ACT_STRIDE), each wave adds 12 KiB of tile buffers (one activation buffer,
-// overwritten in place layer by layer), and eight waves fill a CU's 160 KiB.
+// Inference blocks: eight waves share the LDS-staged weight fragments (40 KiB) and biases;
+// each wave owns a 64-row tile at a time and 12.25 KiB of LDS for its raw outputs.
 constexpr int INFER_WAVES = 8;
+constexpr int INFER_ROWS = 64;
 
 // Raw features of the vertex at queue slot i (hit point normalised to the scene box, shading
 // normal, outgoing direction, roughness); false where the vertex is not guided (a miss, a
@@ -177,26 +176,41 @@ __global__ __launch_bounds__(256) void k_guide_features(PtSceneView sc, GuideSce
     }
 }
 
+// Inference computes every layer transposed, H^T = W A^T, on v_mfma_f32_16x16x32_bf16 with
+// the weights as the A operand (M = 16 outputs) and the activations as the B operand (N = 16
+// rows).  The C fragment of output block ob then leaves in lane l the outputs
+// 16 ob + 4 (l >> 4) + r (r = 0..3) of row l & 15, and the eight of output blocks 2 ks and
+// 2 ks + 1 are exactly the B fragment of the next layer's K step ks once the contraction index
+// is permuted: K position 8 g + p (lane group g = l >> 4) stands for input feature kFeature(g,
+// p, ks).  Activations therefore stay in registers from the input encoding to the last layer;
+// the weights are stored permuted the same way (wFragIndex), one 16-byte fragment per lane per
+// (layer, output block, K step).
+__host__ __device__ __forceinline__ int kFeature(int g, int p, int ks) { return 32 * ks + (p < 4 ? 4 * g + p : 16 + 4 * g + (p - 4)); }
+// position of W[L][out][in] in the fragment copy (the inverse of kFeature)
+__host__ __device__ __forceinline__ int wFragIndex(int L, int out, int in) {
+    const int ob = out >> 4, ks = in >> 5, q = in & 31;
+    const int g = (q & 15) >> 2, p = (q & 3) + (q >= 16 ? 4 : 0);
+    return (((L * 4 + ob) * 2 + ks) * 64 + 16 * g + (out & 15)) * 8 + p;
+}
+
 // Rows are the slots [0, maxRows) of a segmented queue (count of segment s at
-// count[s * segStride], segments segLen slots long; a tile of 32 rows lies in one segment)
-// whose features k_guide_features wrote to feat, or with count = nullptr all maxRows rows of
-// rawFeats.
+// count[s * segStride], segments segLen slots long, a multiple of 64, so that a tile of 64 rows
+// lies in one segment) whose features k_guide_features wrote to feat, or with count = nullptr
+// all maxRows rows of rawFeats.
 __global__ __launch_bounds__(64 * INFER_WAVES) void k_guide_infer(GuideNet net, const int *count, int segStride, int segLen, const float4 *feat,
                                                                   int maxRows, float4 *out, const float *rawFeats, float *rawOut) {
-    __shared__ float sRaw[INFER_WAVES][32][GUIDE_RAW];
-    __shared__ int sElig[INFER_WAVES][32];
-    __shared__ __bf16 sAct[INFER_WAVES][32 * ACT_STRIDE];
-    __shared__ float sOut[INFER_WAVES][32][GUIDE_OUT + 1];
-    __shared__ __bf16 sW[GUIDE_LAYERS * GUIDE_W * ACT_STRIDE];
+    __shared__ bf16x8 sWf[GUIDE_LAYERS * 8 * 64];
+    __shared__ float4 sB[GUIDE_LAYERS * GUIDE_W / 4];
+    __shared__ float sOut[INFER_WAVES][INFER_ROWS][GUIDE_OUT + 1];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int c = threadIdx.x; c < GUIDE_LAYERS * GUIDE_W * (GUIDE_W / 8); c += blockDim.x) {
-        const int row = c / (GUIDE_W / 8), q = c % (GUIDE_W / 8);
-        *(bf16x8 *)(sW + row * ACT_STRIDE + 8 * q) = ld8(net.w + (size_t)row * GUIDE_W + 8 * q);
-    }
+    const int g = lane >> 4, lr = lane & 15;
+    for (int c = threadIdx.x; c < GUIDE_LAYERS * 8 * 64; c += blockDim.x) sWf[c] = ld8(net.w + GUIDE_WFRAG_OFF + 8 * (size_t)c);
+    for (int c = threadIdx.x; c < GUIDE_LAYERS * GUIDE_W / 4; c += blockDim.x) sB[c] = ((const float4 *)net.b)[c];
     __syncthreads();
+    float(*so)[GUIDE_OUT + 1] = sOut[wave];
     const int waveStride = gridDim.x * INFER_WAVES;
-    for (int tile = blockIdx.x * INFER_WAVES + wave; tile * 32 < maxRows; tile += waveStride) {
-        const int rowBase = tile * 32;
+    for (int tile = blockIdx.x * INFER_WAVES + wave; tile * INFER_ROWS < maxRows; tile += waveStride) {
+        const int rowBase = tile * INFER_ROWS;
         // rows [rowBase, n) of this tile hold records: the filled part of its queue segment
         int n = maxRows;
         if (count) {
@@ -204,85 +218,93 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void k_guide_infer(GuideNet net, 
             n = seg * segLen + min(count[seg * segStride], segLen);
             if (rowBase >= n) continue;
         }
-        int elig = 0;
-        if (lane < 32) {
-            const int i = rowBase + lane;
-            float f[GUIDE_RAW];
-            for (int k = 0; k < GUIDE_RAW; k++) f[k] = 0.f;
-            if (i < n) {
+        // this lane's row in the output stage
+        const int mine = rowBase + lane;
+        const bool elig = mine < n && (rawFeats != nullptr || feat[2 * (size_t)mine + 1].w >= 0.f);
+        if (__ballot(elig) == 0) {
+            // no guided vertex in the tile (misses, mirror and glass hits): only the flag that
+            // makes shade sample the BSDF alone, no network evaluation
+            if (mine < n && !rawOut) out[(size_t)mine * GUIDE_REC_F4 + 10] = make_float4(0.f, 0.f, 0.f, 0.f);
+            continue;
+        }
+        // layer 0's B fragments: one-blob encoding.  Input feature 8 k + j is bin j of raw
+        // feature k; the lane's K positions need raw features 2 i + (g >> 1), i = 0..3, at bins
+        // 4 (g & 1) + 0..3.
+        bf16x8 act[4][2];
+#pragma unroll
+        for (int rb = 0; rb < 4; rb++) {
+            const int row = rowBase + rb * 16 + lr;
+            float f[4] = {0.f, 0.f, 0.f, 0.f};
+            if (row < n) {
                 if (rawFeats) {
-                    for (int k = 0; k < GUIDE_RAW; k++) f[k] = rawFeats[(size_t)i * GUIDE_RAW + k];
-                    elig = 1;
+#pragma unroll
+                    for (int i = 0; i < 4; i++) f[i] = rawFeats[(size_t)row * GUIDE_RAW + 2 * i + (g >> 1)];
                 } else {
-                    const float4 a = feat[2 * (size_t)i], b = feat[2 * (size_t)i + 1];
+                    const float4 a = feat[2 * (size_t)row], b = feat[2 * (size_t)row + 1];
                     if (b.w >= 0.f) {
-                        f[0] = a.x; f[1] = a.y; f[2] = a.z; f[3] = a.w;
-                        f[4] = b.x; f[5] = b.y; f[6] = b.z; f[7] = b.w;
-                        elig = 1;
+                        const bool hi = (g >> 1) != 0;
+                        f[0] = hi ? a.y : a.x;
+                        f[1] = hi ? a.w : a.z;
+                        f[2] = hi ? b.y : b.x;
+                        f[3] = hi ? b.w : b.z;
                     }
                 }
             }
-            for (int k = 0; k < GUIDE_RAW; k++) sRaw[wave][lane][k] = f[k];
-            sElig[wave][lane] = elig;
+#pragma unroll
+            for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+                for (int p = 0; p < 8; p++) {
+                    const int kf = kFeature(g, p, ks);
+                    act[rb][ks][p] = (__bf16)oneBlob(f[(kf >> 3) >> 1], kf & 7);
+                }
         }
-        WAVE_SYNC();
-        if (__ballot(elig != 0) == 0) {
-            // no guided vertex in the tile (misses, mirror and glass hits): only the flag that
-            // makes shade sample the BSDF alone, no network evaluation
-            const int i = rowBase + lane;
-            if (lane < 32 && i < n && !rawOut) out[(size_t)i * GUIDE_REC_F4 + 10] = make_float4(0.f, 0.f, 0.f, 0.f);
-            WAVE_SYNC();
-            continue;
-        }
-        {
-            // one-blob encoding: lane -> row (lane & 31), raw features 4 * (lane >> 5) .. + 3
-            const int row = lane & 31;
-            __bf16 *dst = sAct[wave] + row * ACT_STRIDE;
-            for (int kk = 0; kk < 4; kk++) {
-                const int k = 4 * (lane >> 5) + kk;
-                const float s = sRaw[wave][row][k];
-                for (int j = 0; j < GUIDE_BINS; j++) dst[k * GUIDE_BINS + j] = (__bf16)oneBlob(s, j);
-            }
-        }
-        WAVE_SYNC();
+#pragma unroll
         for (int L = 0; L < GUIDE_LAYERS; L++) {
-            // both output column tiles are computed before either is stored: every lane's
-            // reads of the layer input feed the MFMAs, so the output may overwrite the input
-            const f32x16 acc0 = layerTileLds(sAct[wave], sW, L, 0, lane);
-            const f32x16 acc1 = layerTileLds(sAct[wave], sW, L, 1, lane);
-            WAVE_SYNC();
+            const bool last = L == GUIDE_LAYERS - 1;
+            f32x4 acc[4][4];
 #pragma unroll
-            for (int ct = 0; ct < 2; ct++) {
-                const int col = ct * 32 + (lane & 31);
-                const float bias = net.b[L * GUIDE_W + col];
+            for (int ob = 0; ob < 4; ob++) {
+                if (last && ob * 16 >= GUIDE_OUT) continue;
+                const float4 bias = sB[(L * GUIDE_W + ob * 16) / 4 + g];
+                const bf16x8 w0 = sWf[((L * 4 + ob) * 2) * 64 + lane], w1 = sWf[((L * 4 + ob) * 2 + 1) * 64 + lane];
 #pragma unroll
-                for (int r = 0; r < 16; r++) {
-                    const int row = cRow(lane, r);
-                    const float v = (ct ? acc1[r] : acc0[r]) + bias;
-                    if (L < GUIDE_LAYERS - 1) sAct[wave][row * ACT_STRIDE + col] = (__bf16)fmaxf(v, 0.f);
-                    else if (col < GUIDE_OUT) sOut[wave][row][col] = v;
+                for (int rb = 0; rb < 4; rb++) {
+                    const f32x4 c = {bias.x, bias.y, bias.z, bias.w};
+                    acc[rb][ob] = mfma16(w1, act[rb][1], mfma16(w0, act[rb][0], c));
                 }
             }
-            WAVE_SYNC();
+            if (!last) {
+#pragma unroll
+                for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+                    for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+                        for (int p = 0; p < 8; p++) act[rb][ks][p] = (__bf16)fmaxf(acc[rb][2 * ks + (p >> 2)][p & 3], 0.f);
+            } else {
+#pragma unroll
+                for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+                    for (int ob = 0; ob * 16 < GUIDE_OUT; ob++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++) so[rb * 16 + lr][ob * 16 + 4 * g + r] = acc[rb][ob][r];
+            }
         }
-        if (lane < 32) {
-            const int i = rowBase + lane;
-            if (i < n) {
-                if (rawOut) {
-                    for (int k = 0; k < GUIDE_OUT; k++) rawOut[(size_t)i * GUIDE_OUT + k] = sOut[wave][lane][k];
-                } else if (!sElig[wave][lane]) {
-                    // shade reads only the flag of a vertex that is not guided
-                    out[(size_t)i * GUIDE_REC_F4 + 10] = make_float4(0.f, 0.f, 0.f, 0.f);
-                } else {
-                    Mixture mx;
-                    mixtureFromRaw(sOut[wave][lane], mx);
-                    float4 *o = out + (size_t)i * GUIDE_REC_F4;
-                    for (int k = 0; k < GUIDE_LOBES; k++) o[k] = f4(mx.mu[k], mx.kappa[k]);
-                    o[8] = make_float4(mx.pi[0], mx.pi[1], mx.pi[2], mx.pi[3]);
-                    o[9] = make_float4(mx.pi[4], mx.pi[5], mx.pi[6], mx.pi[7]);
-                    o[10] = make_float4(mx.frac, 1.f, 0.f, 0.f);
-                    o[11] = make_float4(0.f, 0.f, 0.f, 0.f);
-                }
+        WAVE_SYNC();
+        if (mine < n) {
+            if (rawOut) {
+                for (int k = 0; k < GUIDE_OUT; k++) rawOut[(size_t)mine * GUIDE_OUT + k] = so[lane][k];
+            } else if (!elig) {
+                // shade reads only the flag of a vertex that is not guided
+                out[(size_t)mine * GUIDE_REC_F4 + 10] = make_float4(0.f, 0.f, 0.f, 0.f);
+            } else {
+                Mixture mx;
+                mixtureFromRaw(so[lane], mx);
+                float4 *o = out + (size_t)mine * GUIDE_REC_F4;
+                for (int k = 0; k < GUIDE_LOBES; k++) o[k] = f4(mx.mu[k], mx.kappa[k]);
+                o[8] = make_float4(mx.pi[0], mx.pi[1], mx.pi[2], mx.pi[3]);
+                o[9] = make_float4(mx.pi[4], mx.pi[5], mx.pi[6], mx.pi[7]);
+                o[10] = make_float4(mx.frac, 1.f, 0.f, 0.f);
+                o[11] = make_float4(0.f, 0.f, 0.f, 0.f);
             }
         }
         WAVE_SYNC();
@@ -557,6 +579,7 @@ __device__ __forceinline__ void storeParam(int p, float v, __bf16 *w, __bf16 *wt
         const int out = off / GUIDE_W, in = off % GUIDE_W;
         const __bf16 h = (__bf16)v;
         w[(size_t)L * GUIDE_W * GUIDE_W + out * GUIDE_W + in] = h;
+        w[GUIDE_WFRAG_OFF + wFragIndex(L, out, in)] = h;
         wt[(size_t)L * GUIDE_W * GUIDE_W + in * GUIDE_W + out] = h;
     } else {
         b[L * GUIDE_W + (off - GUIDE_W * GUIDE_W)] = v;
@@ -641,6 +664,21 @@ __global__ void k_mfma_tile(const float *A, const float *B, float *C) {
     for (int r = 0; r < 16; r++) C[cRow(lane, r) * 32 + (lane & 31)] = acc[r];
 }
 
+__global__ void k_mfma_tile16(const float *A, const float *B, float *C) {
+    // one wave: C (16 x 16) = A (16 x 32) * B (32 x 16), all row-major FP32 in and out
+    const int lane = threadIdx.x;
+    if (lane >= 64) return;
+    bf16x8 a, b;
+    for (int i = 0; i < 8; i++) {
+        const int k = (lane >> 4) * 8 + i;
+        a[i] = (__bf16)A[(lane & 15) * 32 + k];
+        b[i] = (__bf16)B[k * 16 + (lane & 15)];
+    }
+    f32x4 acc = {};
+    acc = mfma16(a, b, acc);
+    for (int r = 0; r < 4; r++) C[(4 * (lane >> 4) + r) * 16 + (lane & 15)] = acc[r];
+}
+
 }  // namespace pt
 
 // ============================================================================================
@@ -683,13 +721,13 @@ void guideInfer(const PtSceneView &sc, GuideSceneNorm norm, GuideNet net, const 
     const int maxRows = PT_QUEUE_SEGS * segLen;
     const int fgrid = std::max(1, std::min((maxRows + 255) / 256, cus * 8));
     k_guide_features<<<fgrid, 256, 0, s>>>(sc, norm, segCount, segStride, segLen, rayO, rayD, camO, hits, maxRows, feat);
-    const int grid = std::max(1, std::min((maxRows + 32 * INFER_WAVES - 1) / (32 * INFER_WAVES), cus));
+    const int grid = std::max(1, std::min((maxRows + INFER_ROWS * INFER_WAVES - 1) / (INFER_ROWS * INFER_WAVES), cus));
     k_guide_infer<<<grid, 64 * INFER_WAVES, 0, s>>>(net, segCount, segStride, segLen, feat, maxRows, out, nullptr, nullptr);
     PT_HIP_TRY(hipGetLastError());
 }
 
 void guideInferRaw(GuideNet net, const float *feats, int n, float *rawOut, hipStream_t s) {
-    const int grid = std::max(1, (n + 32 * INFER_WAVES - 1) / (32 * INFER_WAVES));
+    const int grid = std::max(1, (n + INFER_ROWS * INFER_WAVES - 1) / (INFER_ROWS * INFER_WAVES));
     k_guide_infer<<<grid, 64 * INFER_WAVES, 0, s>>>(net, nullptr, 0, 64, nullptr, n, nullptr, feats, rawOut);
     PT_HIP_TRY(hipGetLastError());
 }
@@ -713,6 +751,11 @@ void guideAdam(float *params, float *m, float *v, const float *grad, int step, f
 
 void mfmaTile(const float *A, const float *B, float *C, hipStream_t s) {
     k_mfma_tile<<<1, 64, 0, s>>>(A, B, C);
+    PT_HIP_TRY(hipGetLastError());
+}
+
+void mfmaTile16(const float *A, const float *B, float *C, hipStream_t s) {
+    k_mfma_tile16<<<1, 64, 0, s>>>(A, B, C);
     PT_HIP_TRY(hipGetLastError());
 }
 

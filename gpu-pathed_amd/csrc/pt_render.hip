@@ -113,7 +113,7 @@ struct ChunkParams {
     // per render the rays its paths spawned (continuation + shadow)
     const BatchGroup *groups = nullptr;
     int numGroups = 0;
-    const int *bucketGroup = nullptr;   // group of path b << kBatchBucketShift, per bucket b
+    const struct BatchBucket *buckets = nullptr;   // per bucket of kBatchBucketShift paths
     const BatchRender *renders = nullptr;
     unsigned long long *renderRays = nullptr;
 };
@@ -121,27 +121,71 @@ struct ChunkParams {
 // Paths per bucket of the batch wavefront's path-to-group table.
 static constexpr int kBatchBucketShift = 12;
 
-// The group of chunk path `path` (groups ordered by pathBegin, the first at 0): the group of
-// the path's bucket, then forward over the groups that start inside the bucket (none when
-// groups are longer than a bucket, as one sample row of 64 x 64 pixels already is).
-__device__ __forceinline__ int batchGroupOf(const ChunkParams &cp, int path) {
-    int g = cp.bucketGroup[path >> kBatchBucketShift];
+// The group holding a bucket's first path, the first path of the group after it (a bucket
+// rarely reaches past its group: one sample row of 64 x 64 pixels is a bucket already) and
+// the group's render's PCG32 seed: one 32-byte load gives a path's render, pixel, sample
+// and stream.
+struct BatchBucket {
+    int group, pathBegin, render, npix;
+    int sampleBegin, nextBegin;
+    uint64_t seed;
+};
+
+// n / d and n % d for 0 <= n < 2^31, d > 0, from a float estimate with the float reciprocal
+// of d, corrected to the exact integer result (the estimate is off by at most a few units for
+// quotients below 2^26): a per-lane integer division otherwise costs about 40 instructions.
+__device__ __forceinline__ int divmodEst(int n, int d, int *r) {
+    int q = (int)((float)n * __builtin_amdgcn_rcpf((float)d));
+    int rr = n - q * d;
+    while (rr < 0) {
+        q--;
+        rr += d;
+    }
+    while (rr >= d) {
+        q++;
+        rr -= d;
+    }
+    *r = rr;
+    return q;
+}
+
+// The group of a chunk path that lies past its bucket's group (in a group that starts inside
+// the bucket).
+__device__ __forceinline__ int batchLaterGroup(const ChunkParams &cp, const BatchBucket &bk, int path) {
+    int g = bk.group + 1;
     while (g + 1 < cp.numGroups && cp.groups[g + 1].pathBegin <= path) g++;
     return g;
 }
 
-// Render, pixel and sample row of a chunk path of a batch wavefront.
+// The render of a chunk path of a batch wavefront.
+__device__ __forceinline__ int batchRenderOf(const ChunkParams &cp, int path) {
+    const BatchBucket bk = cp.buckets[path >> kBatchBucketShift];
+    return path < bk.nextBegin ? bk.render : cp.groups[batchLaterGroup(cp, bk, path)].render;
+}
+
+// Render, pixel, sample row and PCG32 seed of a chunk path of a batch wavefront.
 struct PathSource {
     int render, pixel, sample;
+    uint64_t seed;
 };
 __device__ __forceinline__ PathSource batchPathSource(const ChunkParams &cp, int path) {
-    const BatchGroup g = cp.groups[batchGroupOf(cp, path)];
-    const int local = path - g.pathBegin;
-    const int s = local / g.npix;
+    const BatchBucket bk = cp.buckets[path >> kBatchBucketShift];
     PathSource ps;
-    ps.render = g.render;
-    ps.pixel = local - s * g.npix;
-    ps.sample = g.sampleBegin + s;
+    int begin = bk.pathBegin, npix = bk.npix, sample0 = bk.sampleBegin;
+    ps.render = bk.render;
+    ps.seed = bk.seed;
+    if (path >= bk.nextBegin) {
+        const BatchGroup gr = cp.groups[batchLaterGroup(cp, bk, path)];
+        begin = gr.pathBegin;
+        npix = gr.npix;
+        sample0 = gr.sampleBegin;
+        ps.render = gr.render;
+        ps.seed = cp.renders[gr.render].seed;
+    }
+    int pixel;
+    const int s = divmodEst(path - begin, npix, &pixel);
+    ps.pixel = pixel;
+    ps.sample = sample0 + s;
     return ps;
 }
 
@@ -240,10 +284,11 @@ __global__ __launch_bounds__(256) void k_queue_reset(int *counters, int field, i
 __global__ __launch_bounds__(256) void k_raygen(ChunkParams cp, PtCamera cam, float4 *rayD) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= cp.nPaths) return;
-    const int s = i / cp.npix;
-    const int pixel = cp.pixBegin + (i - s * cp.npix);
+    int rem, col;
+    const int s = divmodEst(i, cp.npix, &rem);
+    const int pixel = cp.pixBegin + rem;
     const int sample = cp.sampleBase + s;
-    const int row = pixel / cp.W, col = pixel - (pixel / cp.W) * cp.W;
+    const int row = divmodEst(pixel, cp.W, &col);
     Rng rng = seedRng(cp.seed, (uint32_t)pixel, (uint32_t)sample);
     const float sx = rng.uniform(), sy = rng.uniform();
     f3 o, d;
@@ -251,27 +296,31 @@ __global__ __launch_bounds__(256) void k_raygen(ChunkParams cp, PtCamera cam, fl
     rayD[ptCameraSlot(i, cp.segLen)] = f4(d, FLT_MAX);
 }
 
-// Batch wavefront: every path of the chunk from its render's camera; stores origin (with the
-// render in w) and direction.
-__global__ __launch_bounds__(256) void k_raygen_batch(ChunkParams cp, float4 *rayO, float4 *rayD) {
+// Batch wavefront: every path of the chunk from its render's camera; like k_raygen it stores
+// only the direction (the bounce-0 kernels find the render, its camera origin and root node
+// from the path index).
+__global__ __launch_bounds__(256) void k_raygen_batch(ChunkParams cp, float4 *rayD) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= cp.nPaths) return;
     const PathSource ps = batchPathSource(cp, i);
     const BatchRender &R = cp.renders[ps.render];
-    const int row = ps.pixel / R.W, col = ps.pixel - row * R.W;
-    Rng rng = seedRng(R.seed, (uint32_t)ps.pixel, (uint32_t)ps.sample);
+    int col;
+    const int row = divmodEst(ps.pixel, R.W, &col);
+    Rng rng = seedRng(ps.seed, (uint32_t)ps.pixel, (uint32_t)ps.sample);
     const float sx = rng.uniform(), sy = rng.uniform();
     f3 o, d;
     cameraRay(R.cam, row, col, sx, sy, &o, &d);
-    const int slot = ptCameraSlot(i, cp.segLen);
-    rayO[slot] = f4(o, asFloat(ps.render));
-    rayD[slot] = f4(d, FLT_MAX);
+    rayD[ptCameraSlot(i, cp.segLen)] = f4(d, FLT_MAX);
 }
+
+// The camera origin of a batch render.
+__device__ __forceinline__ float4 batchCameraOrigin(const BatchRender &R) { return make_float4(R.cam.m[0][3], R.cam.m[1][3], R.cam.m[2][3], 0.f); }
 
 // The camera-ray state k_raygen does not store: the RNG after the pixel's two jitter draws.
 __device__ __forceinline__ Rng cameraRng(const ChunkParams &cp, int path) {
-    const int s = path / cp.npix;
-    const int pixel = cp.pixBegin + (path - s * cp.npix);
+    int rem;
+    const int s = divmodEst(path, cp.npix, &rem);
+    const int pixel = cp.pixBegin + rem;
     Rng rng = seedRng(cp.seed, (uint32_t)pixel, (uint32_t)(cp.sampleBase + s));
     rng.next();
     rng.next();
@@ -603,14 +652,14 @@ __device__ __forceinline__ void traceQueue(const PtSceneView &sc, int *counters,
 }
 
 // Persistent closest-hit kernel over the rays of `bounce`.  camO.w != 0: the rays start at
-// camO (camera rays, tmin 0) and rayO is not read.  BATCH (a batch wavefront): rayO.w holds
-// the ray's render, whose root node starts the traversal; tmin is 0 at bounce 0, else the
-// continuation offset.
+// camO (camera rays, tmin 0) and rayO is not read.  BATCH (a batch wavefront): the traversal
+// starts at the root node of the ray's render; at bounce 0 the render comes from the path
+// index (chunk groups) and the origin is its camera's, later rayO.w holds the render.
 template <int STACK, bool BATCH = false>
 __global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 7 : 5) void k_extend(PtSceneView sc, int *counters, int bounce, int segLen,
                                                                               const float4 *rayO, const float4 *rayD, float4 camO, float4 *hits,
                                                                               int refill, int *spill, int spillCap, int *overflow,
-                                                                              const BatchRender *renders = nullptr) {
+                                                                              ChunkParams cp = ChunkParams{}) {
     __shared__ int sstack[STACK * kTraceThreads];
     const TraceStack ts = traceStack(sstack, spill, spillCap);
     traceQueue<false, STACK>(
@@ -618,10 +667,19 @@ __global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 7 : 5) void k_extend(P
         [&](int i, f3 &o, f3 &d, float &tmin, float &tmax, int &root) {
             const float4 rd = rayD[i];
             if constexpr (BATCH) {
-                const float4 ro = rayO[i];
-                o = xyz(ro);
-                tmin = bounce == 0 ? 0.f : kBounceEps;
-                root = renders[asInt(ro.w)].root;
+                if (bounce == 0) {
+                    const int seg = i / segLen;
+                    const int p = ptCameraPath(seg, i - seg * segLen);
+                    const BatchRender &R = cp.renders[batchRenderOf(cp, p)];
+                    o = xyz(batchCameraOrigin(R));
+                    tmin = 0.f;
+                    root = R.root;
+                } else {
+                    const float4 ro = rayO[i];
+                    o = xyz(ro);
+                    tmin = kBounceEps;
+                    root = cp.renders[asInt(ro.w)].root;
+                }
             } else if (camO.w != 0.f) {
                 o = xyz(camO);
                 tmin = 0.f;
@@ -828,9 +886,9 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
     // the path's radiance is written instead of added to.
     //
     // BATCH (a batch wavefront): every path belongs to one render of the batch (state.w above
-    // the depth; rayO.w at bounce 0), whose lights, environment, depth limit and PCG32 streams
-    // it uses; continuation rays carry the render in rayO.w and shadow rays in sL.w, and each
-    // render counts the rays its paths spawn.
+    // the depth; at bounce 0 found from the path index), whose camera, lights, environment,
+    // depth limit and PCG32 streams it uses; continuation rays carry the render in rayO.w and
+    // shadow rays in sL.w, and each render counts the rays its paths spawn.
     //
     // Misses (a fifth to most of a queue) take a few instructions, hits the whole BSDF and light
     // code.  A wave shades the misses of each claimed run at once and parks the hits' slots in
@@ -890,10 +948,11 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
                 bp = make_float4(1.f, 1.f, 1.f, -1.f);
                 if constexpr (BATCH) {
                     const PathSource ps = batchPathSource(cp, p);
-                    Rng g = seedRng(cp.renders[ps.render].seed, (uint32_t)ps.pixel, (uint32_t)ps.sample);
+                    const BatchRender &R = cp.renders[ps.render];
+                    Rng g = seedRng(ps.seed, (uint32_t)ps.pixel, (uint32_t)ps.sample);
                     g.next();
                     g.next();
-                    ro = rayO[i];
+                    ro = batchCameraOrigin(R);
                     st = make_uint4((uint32_t)g.state, (uint32_t)(g.state >> 32), (uint32_t)p, packDepth(0, ps.render));
                 } else {
                     const Rng g = cameraRng(cp, p);
@@ -954,10 +1013,11 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
                     Rng rng;
                     if constexpr (BATCH) {
                         const PathSource ps = batchPathSource(cp, pathId);
-                        rng.inc = streamInc(cp.renders[ps.render].seed, (uint32_t)ps.pixel, (uint32_t)ps.sample);
+                        rng.inc = streamInc(ps.seed, (uint32_t)ps.pixel, (uint32_t)ps.sample);
                     } else {
-                        const int s = pathId / cp.npix;
-                        const int pixel = cp.pixBegin + (pathId - s * cp.npix);
+                        int rem;
+                        const int s = divmodEst(pathId, cp.npix, &rem);
+                        const int pixel = cp.pixBegin + rem;
                         rng.inc = streamInc(cp.seed, (uint32_t)pixel, (uint32_t)(cp.sampleBase + s));
                     }
                     rng.state = ((uint64_t)st.y << 32) | st.x;
@@ -1532,17 +1592,17 @@ int traceGrid(const pt_renderer *r) {
 }
 
 void launchExtend(pt_renderer *r, int bounce, int segLen, const float4 *ro, const float4 *rd, float4 *hits, int kind,
-                  float4 camO = make_float4(0.f, 0.f, 0.f, 0.f)) {
+                  float4 camO = make_float4(0.f, 0.f, 0.f, 0.f), const ChunkParams *batchChunk = nullptr) {
     const int refill = refillLanes(r, kind);
     const int grid = traceGrid(r);
     int *ctr = r->pp->counters.as<int>();
     int *ov = r->pp->overflow.as<int>();
     int cap = 0;
     int *sp = ensureSpill(r, grid, &cap);
-    const BatchRender *br = r->batchRenders;
-    if (br) {
-        if (r->ldsStack == 32) k_extend<32, true><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, ro, rd, camO, hits, refill, sp, cap, ov, br);
-        else k_extend<16, true><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, ro, rd, camO, hits, refill, sp, cap, ov, br);
+    if (batchChunk) {
+        const ChunkParams &cp = *batchChunk;
+        if (r->ldsStack == 32) k_extend<32, true><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, ro, rd, camO, hits, refill, sp, cap, ov, cp);
+        else k_extend<16, true><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, ro, rd, camO, hits, refill, sp, cap, ov, cp);
     } else {
         if (r->ldsStack == 32) k_extend<32><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, ro, rd, camO, hits, refill, sp, cap, ov);
         else k_extend<16><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, ro, rd, camO, hits, refill, sp, cap, ov);
@@ -1585,7 +1645,7 @@ void ensureGuide(pt_renderer *r) {
         r->gV.alloc(np * 4);
         r->gGrad.alloc(np * 4);
         r->gLoss.alloc(4);
-        r->gW.alloc(GUIDE_LAYERS * GUIDE_W * GUIDE_W * 2);
+        r->gW.alloc(GUIDE_WBUF * 2);
         r->gWt.alloc(GUIDE_LAYERS * GUIDE_W * GUIDE_W * 2);
         r->gB.alloc(GUIDE_LAYERS * GUIDE_W * 4);
         std::vector<float> init(np);
@@ -2591,6 +2651,8 @@ struct pt_batch {
     std::vector<WaveChunk> waveChunks;
     std::vector<BatchGroup> waveGroups;
     std::vector<int> waveLead;     // renders with no rows before the first chunk (depth 0)
+    std::vector<int64_t> waveCamRays;   // per render: its camera rays in chunks of several renders
+    std::vector<char> waveOwn;          // per render: some chunk runs the render's own kernels
     DevBuf stamps;
     double clockKHz = 100000.0;
     // last run
@@ -2825,8 +2887,8 @@ void planWave(pt_batch *b, int scale) {
     if (!ok) throw HipFailure("batch: renders completing with a chunk are not consecutive");
     b->groupTable.alloc(sizeof(BatchGroup) * std::max<size_t>(b->waveGroups.size(), 1));
     PT_HIP_TRY(hipMemcpy(b->groupTable.p, b->waveGroups.data(), sizeof(BatchGroup) * b->waveGroups.size(), hipMemcpyHostToDevice));
-    // per chunk: the group of each bucket's first path
-    std::vector<int> buckets;
+    // per chunk: the group of each bucket's first path, with what a path lookup needs
+    std::vector<BatchBucket> buckets;
     std::vector<size_t> bucketAt;
     for (const WaveChunk &c : b->waveChunks) {
         bucketAt.push_back(buckets.size());
@@ -2835,36 +2897,77 @@ void planWave(pt_batch *b, int scale) {
         for (int k = 0; k < nb; k++) {
             const int path = k << kBatchBucketShift;
             while (g + 1 < c.cp.numGroups && b->waveGroups[c.firstGroup + g + 1].pathBegin <= path) g++;
-            buckets.push_back(g);
+            const BatchGroup &gr = b->waveGroups[c.firstGroup + g];
+            BatchBucket bk;
+            bk.group = g;
+            bk.pathBegin = gr.pathBegin;
+            bk.render = gr.render;
+            bk.npix = gr.npix;
+            bk.sampleBegin = gr.sampleBegin;
+            bk.nextBegin = g + 1 < c.cp.numGroups ? b->waveGroups[c.firstGroup + g + 1].pathBegin : INT32_MAX;
+            bk.seed = b->jobs[gr.render]->seed;
+            buckets.push_back(bk);
         }
     }
-    b->bucketTable.alloc(sizeof(int) * std::max<size_t>(buckets.size(), 1));
-    PT_HIP_TRY(hipMemcpy(b->bucketTable.p, buckets.data(), sizeof(int) * buckets.size(), hipMemcpyHostToDevice));
+    b->bucketTable.alloc(sizeof(BatchBucket) * std::max<size_t>(buckets.size(), 1));
+    PT_HIP_TRY(hipMemcpy(b->bucketTable.p, buckets.data(), sizeof(BatchBucket) * buckets.size(), hipMemcpyHostToDevice));
+    b->waveCamRays.assign(b->jobs.size(), 0);
+    b->waveOwn.assign(b->jobs.size(), 0);
+    for (const WaveChunk &c : b->waveChunks) {
+        if (c.cp.numGroups == 1) b->waveOwn[b->waveGroups[c.firstGroup].render] = 1;
+        else
+            for (int k = 0; k < c.cp.numGroups; k++) {
+                const BatchGroup &g = b->waveGroups[c.firstGroup + k];
+                b->waveCamRays[g.render] += (int64_t)g.npix * g.count;
+            }
+    }
     for (size_t k = 0; k < b->waveChunks.size(); k++) {
         WaveChunk &c = b->waveChunks[k];
         c.cp.groups = b->groupTable.as<BatchGroup>() + c.firstGroup;
-        c.cp.bucketGroup = b->bucketTable.as<int>() + bucketAt[k];
+        c.cp.buckets = b->bucketTable.as<BatchBucket>() + bucketAt[k];
         c.cp.renders = b->renderTable.as<BatchRender>();
         c.cp.renderRays = b->renderRays.as<unsigned long long>();
     }
 }
 
 // One chunk of the batch wavefront: raygen from every group's camera, the bounces with the
-// batch kernels, accumulation into each render's accumulator.
+// batch kernels, accumulation into each render's accumulator.  A chunk that holds sample rows
+// of one render only (the large renders) runs that render's own kernels instead, exactly its
+// chunk of the render-after-render schedule: the batch kernels' per-path render lookups buy
+// nothing there.
 void runWaveChunk(pt_batch *b, const WaveChunk &c) {
     pt_renderer *w = b->wave;
     const ChunkParams &cp = c.cp;
     PathPool &pp = *b->pp;
+    if (cp.numGroups == 1) {
+        const BatchGroup &g = b->waveGroups[c.firstGroup];
+        pt_renderer *r = b->jobs[g.render];
+        ChunkParams one;
+        one.pixBegin = 0;
+        one.npix = g.npix;
+        one.samples = g.count;
+        one.sampleBase = g.sampleBegin;
+        one.nPaths = g.npix * g.count;
+        one.segLen = ptSegLen(one.nPaths);
+        one.maxDepth = r->maxDepth;
+        one.nee = r->nee;
+        one.misSample = r->misSample;
+        one.W = r->W;
+        one.H = r->H;
+        one.seed = r->seed;
+        runChunk(r, one);
+        return;
+    }
     int *ctr = pp.counters.as<int>();
     k_queue_reset<<<PT_QUEUE_SEGS + 1, 256, 0, b->stream>>>(ctr, PT_Q_RAYS, cp.nPaths, cp.segLen, 1);
-    k_raygen_batch<<<(cp.nPaths + 255) / 256, 256, 0, b->stream>>>(cp, pp.rayO[0].as<float4>(), pp.rayD[0].as<float4>());
+    k_raygen_batch<<<(cp.nPaths + 255) / 256, 256, 0, b->stream>>>(cp, pp.rayD[0].as<float4>());
     PT_HIP_TRY(hipGetLastError());
     const int shadeGrid = (int)std::max<int64_t>(PT_QUEUE_SEGS / 4, std::min<int64_t>((cp.nPaths + 255) / 256, (int64_t)w->cus * w->shadeBlocksPerCU));
     int cur = 0;
     for (int bn = 0; bn < cp.maxDepth; bn++) {
         launchExtend(w, bn, cp.segLen, pp.rayO[cur].as<float4>(), pp.rayD[cur].as<float4>(), pp.hits.as<float4>(),
-                     bn == 0 ? TRACE_CAMERA : TRACE_INCOHERENT);
-        // w = 1 marks bounce 0 (camera vertices); batch kernels take the origin from rayO
+                     bn == 0 ? TRACE_CAMERA : TRACE_INCOHERENT, make_float4(0.f, 0.f, 0.f, 0.f), &cp);
+        // w = 1 marks bounce 0 (camera vertices); batch kernels take each render's camera
         const float4 camFlag = make_float4(0.f, 0.f, 0.f, bn == 0 ? 1.f : 0.f);
         k_shade<false, true><<<shadeGrid, 256, 0, b->stream>>>(
             w->view, cp, bn, ctr, pp.rayO[cur].as<float4>(), pp.rayD[cur].as<float4>(), pp.st[cur].as<uint4>(), pp.beta[cur].as<float4>(),
@@ -2887,6 +2990,9 @@ void enqueueWave(pt_batch *b, int scale) {
     k_zero_u64<<<1, 512, 0, b->stream>>>(b->waveStats.as<unsigned long long>(), S_COUNT);
     const int nj = (int)b->jobs.size();
     k_zero_u64<<<(nj + 255) / 256, 256, 0, b->stream>>>(b->renderRays.as<unsigned long long>(), nj);
+    // statistics of the chunks that run a render's own kernels
+    for (int j = 0; j < nj; j++)
+        if (b->waveOwn[j]) k_zero_u64<<<1, 512, 0, b->stream>>>(b->jobs[j]->stats.as<unsigned long long>(), S_COUNT);
     PT_HIP_TRY(hipGetLastError());
     // the renders completing with a chunk are consecutive in the batch order
     if (!b->waveLead.empty()) k_stamp_range<<<1, 64, 0, b->stream>>>(st + 1 + b->waveLead.front(), (int)b->waveLead.size());
@@ -2978,8 +3084,10 @@ int64_t renderRays(pt_batch *b, int j) {
     if (!b->wavefront) return jobRays(b->jobs[j]);
     unsigned long long n = 0;
     PT_HIP_TRY(hipMemcpy(&n, b->renderRays.as<unsigned long long>() + j, sizeof n, hipMemcpyDeviceToHost));
-    const pt_renderer *r = b->jobs[j];
-    return (int64_t)n + (int64_t)r->W * r->H * r->spp;
+    // spawned rays and camera rays of the chunks shared with other renders, and the rays of
+    // the chunks that ran the render's own kernels
+    if (j >= (int)b->waveOwn.size()) return (int64_t)n;   // not planned yet
+    return (int64_t)n + b->waveCamRays[j] + (b->waveOwn[j] ? jobRays(b->jobs[j]) : 0);
 }
 
 int64_t batchRays(pt_batch *b) {
@@ -2988,6 +3096,8 @@ int64_t batchRays(pt_batch *b) {
         std::vector<unsigned long long> s(S_COUNT);
         PT_HIP_TRY(hipMemcpy(s.data(), b->waveStats.p, S_COUNT * sizeof(unsigned long long), hipMemcpyDeviceToHost));
         for (int k = 0; k < kMaxBounces; k++) n += (int64_t)(s[S_RAYS + k] + s[S_SHADOW + k]);
+        for (size_t j = 0; j < b->waveOwn.size(); j++)
+            if (b->waveOwn[j]) n += jobRays(b->jobs[j]);
     } else {
         for (pt_renderer *r : b->jobs) n += jobRays(r);
     }

@@ -330,7 +330,7 @@ struct NetStage {
     GuideNet net;
     NetStage(const float *host, hipStream_t s) {
         params.alloc(GUIDE_PARAMS * sizeof(float));
-        w.alloc(GUIDE_LAYERS * GUIDE_W * GUIDE_W * sizeof(__bf16));
+        w.alloc(GUIDE_WBUF * sizeof(__bf16));
         wt.alloc(GUIDE_LAYERS * GUIDE_W * GUIDE_W * sizeof(__bf16));
         b.alloc(GUIDE_LAYERS * GUIDE_W * sizeof(float));
         PT_HIP_TRY(hipMemcpyAsync(params.p, host, GUIDE_PARAMS * sizeof(float), hipMemcpyHostToDevice, s));
@@ -407,6 +407,19 @@ int pt_unit_mfma_tile(const float *a, const float *b, float *c) {
     st.sync();
     return PT_OK;
     PT_UNIT_END("pt_unit_mfma_tile")
+}
+
+int pt_unit_mfma_tile16(const float *a, const float *b, float *c) {
+    if (!a || !b || !c) return fail(PT_ERR_INVALID, "pt_unit_mfma_tile16: bad arguments");
+    PT_UNIT_BEGIN("pt_unit_mfma_tile16")
+    Stage st;
+    const float *da = st.in(a, 16 * 32), *db = st.in(b, 32 * 16);
+    float *dc = st.out<float>(16 * 16);
+    mfmaTile16(da, db, dc, st.s);
+    st.back(c, dc, 16 * 16);
+    st.sync();
+    return PT_OK;
+    PT_UNIT_END("pt_unit_mfma_tile16")
 }
 
 }  // extern "C"

@@ -40,7 +40,7 @@ EXPORTS = [
     "pt_exr_read_rgba", "pt_exr_write_rgb",
     "pt_renderer_autotune", "pt_batch_create", "pt_batch_destroy", "pt_batch_add", "pt_batch_set_option", "pt_batch_get_option",
     "pt_batch_autotune", "pt_batch_run", "pt_batch_get_radiance", "pt_batch_get_render_stats", "pt_batch_get_stats",
-    "pt_env_tables_noalloc", "pt_renderer_get_stream",
+    "pt_env_tables_noalloc", "pt_renderer_get_stream", "pt_unit_mfma_tile16",
 ]
 GUIDE_PARAMS = 20800
 
@@ -170,6 +170,7 @@ def lib():
         L.pt_unit_guide_gradients.argtypes = [vp, vp, C.c_int64, vp, vp]
         L.pt_unit_guide_adam.argtypes = [vp, vp, vp, vp, C.c_int, C.c_float]
         L.pt_unit_mfma_tile.argtypes = [vp, vp, vp]
+        L.pt_unit_mfma_tile16.argtypes = [vp, vp, vp]
         L.pt_unit_vmf_sample.argtypes = [vp, vp, vp, C.c_int64, vp, vp]
         L.pt_unit_tonemap.argtypes = [vp, C.c_int64, vp]
         L.pt_pool_layout.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
@@ -568,6 +569,14 @@ def mfma_tile(a, b):
     b = _f32(b).reshape(16, 32)
     c = np.zeros((32, 32), np.float32)
     check(lib().pt_unit_mfma_tile(_p(a), _p(b), _p(c)))
+    return c
+
+
+def mfma_tile16(a, b):
+    a = _f32(a).reshape(16, 32)
+    b = _f32(b).reshape(32, 16)
+    c = np.zeros((16, 16), np.float32)
+    check(lib().pt_unit_mfma_tile16(_p(a), _p(b), _p(c)))
     return c
 
 

@@ -338,6 +338,9 @@ int pt_unit_guide_forward(const float *params, const float *features, int64_t n,
 int pt_unit_guide_gradients(const float *params, const float *records, int64_t n, float *grad, float *loss);
 int pt_unit_guide_adam(float *params, float *m, float *v, const float *grad, int step, float lr);
 int pt_unit_mfma_tile(const float *a, const float *b, float *c);
+/* One v_mfma_f32_16x16x32_bf16 with the lane layout of the guide inference kernel:
+ * c[16x16] = bf16(a[16x32]) * bf16(b[32x16]), row-major FP32. */
+int pt_unit_mfma_tile16(const float *a, const float *b, float *c);
 /* von Mises-Fisher lobes as the guided vertices use them: for n queries (mean mu[3n], unit;
  * concentration kappa[n]; uniforms xi[2n]) -> sampled directions dir[3n] and the density
  * of the lobe at them, pdf[n]. */

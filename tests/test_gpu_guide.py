@@ -26,6 +26,16 @@ def test_mfma_lane_layout():
     assert np.allclose(c, a @ b, rtol=1e-5, atol=1e-5)
 
 
+def test_mfma16_lane_layout():
+    """One v_mfma_f32_16x16x32_bf16 with the operand/result lane layout of the inference kernel."""
+    print("case: mfma 16x16x32 tile", flush=True)
+    rng = np.random.default_rng(3)
+    a = G.bf16(rng.normal(size=(16, 32)))
+    b = G.bf16(rng.normal(size=(32, 16)))
+    c = P.mfma_tile16(a, b)
+    assert np.allclose(c, a @ b, rtol=1e-5, atol=1e-5)
+
+
 def _params(rng):
     p = G.init_params(5)
     last = (G.LAYERS - 1) * G.LAYER_PARAMS
@@ -43,8 +53,10 @@ def test_forward_matches_oracle():
     ref, _ = G.forward(p, feats.astype(np.float64), emulate_bf16=True)
     ref32, _ = G.forward(p, feats.astype(np.float64))
     scale = np.abs(ref32).max()
+    print(f"max |gpu - bf16 oracle| / scale {np.abs(g - ref).max() / scale:.2e}, "
+          f"vs fp32 oracle {np.abs(g - ref32).max() / scale:.2e}", flush=True)
     # BF16 activations: within the oracle's own BF16 emulation up to accumulation order
-    assert np.abs(g - ref).max() <= 2e-2 * scale
+    assert np.abs(g - ref).max() <= 3e-3 * scale  # measured 5.6e-4
     assert np.abs(g - ref32).max() <= 5e-2 * scale
 
 
