@@ -37,14 +37,12 @@ constexpr int GUIDE_REC_F4 = 12;
 // continuation's contribution, weight)
 constexpr int GUIDE_TREC_F4 = 4;
 
-// The BF16 forward-weight buffer holds two copies of the weights: [5][64 out][64 in]
-// row-major, then the same values as MFMA A fragments of the inference kernel's transposed
-// layer product (GUIDE_W * GUIDE_W per layer).
-constexpr int GUIDE_WFRAG_OFF = GUIDE_LAYERS * GUIDE_W * GUIDE_W;
-constexpr int GUIDE_WBUF = 2 * GUIDE_WFRAG_OFF;   // bf16 elements of the forward-weight buffer
+// BF16 weights as MFMA A fragments of the transposed layer products (GUIDE_W * GUIDE_W bf16
+// per layer, layout in pt_guide.hip): w for the forward pass (W), wt for the backward pass
+// (W^T).
 struct GuideNet {
-    const __bf16 *w;    // [5][64 out][64 in], then the fragment copy at GUIDE_WFRAG_OFF
-    const __bf16 *wt;   // [5][64 in][64 out]
+    const __bf16 *w;
+    const __bf16 *wt;
     const float *b;     // [5][64]
 };
 
@@ -75,19 +73,18 @@ void guideUploadBf16(const float *dParams, __bf16 *dW, __bf16 *dWt, float *dB, h
 void guideInfer(const PtSceneView &sc, GuideSceneNorm norm, GuideNet net, const int *segCount, int segStride, int segLen,
                 const float4 *rayO, const float4 *rayD, float4 camO, const float4 *hits, float4 *feat, float4 *out, int cus, hipStream_t s);
 void guideInferRaw(GuideNet net, const float *feats, int n, float *rawOut, hipStream_t s);
-// Scratch of one training step over `batch` records: transposed BF16 layer inputs and output
-// deltas (GUIDE_LAYERS x GUIDE_W x guideRows(batch) each), FP32 gradient partials
-// (guideSlices(batch) x GUIDE_PARAMS), per-block loss sums of the forward/backward kernel
-// (fwdBlocks of them).
-constexpr int GUIDE_WG_ROWS = 2048;   // rows per slice of the weight-gradient product
-inline int guideRows(int batch) { return (batch + 31) / 32 * 32; }
-inline int guideSlices(int batch) { return (guideRows(batch) + GUIDE_WG_ROWS - 1) / GUIDE_WG_ROWS; }
-inline size_t guideActBytes(int batch) { return (size_t)GUIDE_LAYERS * GUIDE_W * guideRows(batch) * 2; }
+// Scratch of one training step over `batch` records: FP32 gradient partials (blocks x
+// GUIDE_PARAMS) and per-block loss sums of the training kernel, which runs
+// guideTrainBlocks(batch, cus) blocks of GUIDE_TRAIN_ROWS records per iteration.
+constexpr int GUIDE_TRAIN_ROWS = 128;
+inline int guideTrainBlocks(int batch, int cus) {
+    const int iters = (batch + GUIDE_TRAIN_ROWS - 1) / GUIDE_TRAIN_ROWS;
+    return iters < 1 ? 1 : (iters < cus ? iters : cus);
+}
 struct GuideTrainBufs {
-    __bf16 *act, *delta;
     float *partials;
     float *lossPart;
-    int fwdBlocks;
+    int blocks;
 };
 // Gradients of the mean loss over `batch` records (rows (base + i) mod *count of the record
 // sequence; weights scaled to mean one by *wsum when wsum is given) into `grad`, the mean loss

@@ -1,27 +1,22 @@
 // pt_guide.hip -- BF16 MFMA kernels of the guiding network (gfx950).
 //
-// Every layer is 64 x 64 (the 48-output last layer is zero-padded to 64), weights BF16
-// ([out][in] for the forward pass, [in][out] for the backward pass), activations BF16,
-// accumulation FP32 with v_mfma_f32_32x32x16_bf16: one wave multiplies a 32-row tile by a
-// 64-wide layer as 2 output tiles x 4 K-steps.  Fragment layout of the 32x32x16 instruction
-// used throughout (A: M x K, B: K x N, lane l = 0..63):
-//   A: row (l & 31), 8 consecutive k starting at 8 * (l >> 5) of the 16-wide K step
-//   B: col (l & 31), the same 8 k
-//   C: 16 floats per lane, element r at row 8 * (r >> 2) + 4 * (l >> 5) + (r & 3), col (l & 31)
-// (checked on the device by pt_unit_mfma_tile).  Operands are staged so that each lane's 8
-// k-values are contiguous: activations row-major for the forward A operand, transposed copies
-// for the weight-gradient products.
+// Every layer is 64 x 64 (the 48-output last layer is zero-padded to 64), weights and
+// activations BF16, accumulation FP32.  Every layer product runs transposed, H^T = W A^T, on
+// v_mfma_f32_16x16x32_bf16 with the weights as the A operand, so that a layer's output
+// fragments are the next layer's input fragments and activations stay in registers (the
+// K-permuted weight layout is described at kFeature below).
 //
 // Inference: blocks of 8 waves, each wave loops over 64-row tiles of the ray queue, encodes
-// the rows' raw features one-blob straight into MFMA operands, runs the 5 layers with the
-// activations in registers (v_mfma_f32_16x16x32_bf16 on the transposed product, weights
-// permuted to match) and writes the mixture (8 vMF lobes + selection probability) per row.
-// Training step: (1) one wave per block, persistent over 32-row tiles of the record batch:
-// forward, KL loss gradient at the output, backward through the layers; every layer's input
-// activations and output deltas are written transposed (BF16) to HBM; (2) the weight and bias
-// gradients as one MFMA product per layer with K = the batch rows, split into slices with FP32
-// partials; (3) the partials summed in slice order (deterministic) and Adam on the FP32
-// master weights, refreshing the BF16 copies.
+// the rows' raw features one-blob straight into MFMA operands, runs the 5 layers and writes
+// the mixture (8 vMF lobes + selection probability) per row.
+// Training step: (1) one kernel runs forward, loss gradient, backward and the weight and bias
+// gradients over 128-row iterations per block, each block keeping its share of the gradient in
+// registers and writing it once as an FP32 partial; (2) the partials summed in block order
+// (deterministic) and Adam on the FP32 master weights, refreshing the BF16 fragments.
+//
+// The 32x32x16 instruction is kept for its lane-layout check (pt_unit_mfma_tile): A: row
+// (l & 31), 8 consecutive k from 8 (l >> 5); B: col (l & 31), the same k; C: element r at row
+// 8 (r >> 2) + 4 (l >> 5) + (r & 3), col (l & 31).
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -39,7 +34,6 @@ typedef __bf16 bf16x8 __attribute__((__vector_size__(8 * sizeof(__bf16))));
 typedef float f32x16 __attribute__((__vector_size__(16 * sizeof(float))));
 
 constexpr int ACT_STRIDE = 72;   // bf16 per row of a row-major 32 x 64 tile (padded)
-constexpr int T_STRIDE = 40;     // bf16 per row of a transposed 64 x 32 tile (padded)
 
 #define WAVE_SYNC()                                         \
     do {                                                    \
@@ -53,11 +47,10 @@ __device__ __forceinline__ bf16x8 ld8(const __bf16 *p) { return *(const bf16x8 *
 
 __device__ __forceinline__ f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0); }
 
-// v_mfma_f32_16x16x32_bf16 (inference): A: row (l & 15), 8 consecutive k starting at
-// 8 * (l >> 4) of the 32-wide K step; B: col (l & 15), the same 8 k; C: 4 floats per lane,
-// element r at row 4 * (l >> 4) + r, col (l & 15) (checked on the device by
-// pt_unit_mfma_tile16).  A 32 x 64 layer output is 2 x 4 such tiles of two K steps each:
-// eight independent accumulation chains per wave instead of two.
+// v_mfma_f32_16x16x32_bf16, the network's instruction: A: row (l & 15), 8 consecutive k
+// starting at 8 * (l >> 4) of the 32-wide K step; B: col (l & 15), the same 8 k; C: 4 floats
+// per lane, element r at row 4 * (l >> 4) + r, col (l & 15) (checked on the device by
+// pt_unit_mfma_tile16).
 typedef float f32x4 __attribute__((__vector_size__(4 * sizeof(float))));
 __device__ __forceinline__ f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 
@@ -66,16 +59,6 @@ __device__ __forceinline__ float luminance(float4 c) { return 0.2126f * c.x + 0.
 __device__ __forceinline__ float oneBlob(float s, int j) {
     const float x = s * (float)GUIDE_BINS - ((float)j + 0.5f);
     return __expf(-0.5f * x * x);
-}
-
-// forward through layer L for one 32-row tile: A from `act` (row-major), B from W[L]
-__device__ __forceinline__ f32x16 layerTile(const __bf16 *act, const __bf16 *W, int L, int ct, int lane) {
-    f32x16 acc = {};
-    const __bf16 *arow = act + (lane & 31) * ACT_STRIDE + (lane >> 5) * 8;
-    const __bf16 *wrow = W + (size_t)L * GUIDE_W * GUIDE_W + (ct * 32 + (lane & 31)) * GUIDE_W + (lane >> 5) * 8;
-#pragma unroll
-    for (int ks = 0; ks < GUIDE_W / 16; ks++) acc = mfma(ld8(arow + ks * 16), ld8(wrow + ks * 16), acc);
-    return acc;
 }
 
 // mixture parameters from the 41 used raw outputs
@@ -204,7 +187,7 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void k_guide_infer(GuideNet net, 
     __shared__ float sOut[INFER_WAVES][INFER_ROWS][GUIDE_OUT + 1];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, lr = lane & 15;
-    for (int c = threadIdx.x; c < GUIDE_LAYERS * 8 * 64; c += blockDim.x) sWf[c] = ld8(net.w + GUIDE_WFRAG_OFF + 8 * (size_t)c);
+    for (int c = threadIdx.x; c < GUIDE_LAYERS * 8 * 64; c += blockDim.x) sWf[c] = ld8(net.w + 8 * (size_t)c);
     for (int c = threadIdx.x; c < GUIDE_LAYERS * GUIDE_W / 4; c += blockDim.x) sB[c] = ((const float4 *)net.b)[c];
     __syncthreads();
     float(*so)[GUIDE_OUT + 1] = sOut[wave];
@@ -362,42 +345,85 @@ __device__ float lossGradient(const float *o, f3 w, float pb, float wt, __bf16 *
     return loss;
 }
 
-// Training, phase 1: forward and backward of 32-row tiles of the record batch, one wave per
-// block, persistent over tiles.  Each layer's input activations and its output deltas leave
-// the kernel transposed ([layer][feature][row], BF16) for the weight-gradient product of phase
-// 2.  In LDS a wave keeps one activation tile and one delta tile, each overwritten in place
-// (both 32-column halves of a layer are computed before either is stored), and the ReLU
-// masks of the backward pass stay in registers: the lane that computed an output element in
-// the forward pass is the lane that masks its delta.  Rows past the batch get weight 0 and so
-// zero deltas.
-__global__ __launch_bounds__(64) void k_guide_fwdbwd(GuideNet net, const float4 *recs, const int *countPtr, const int *prefix, int recSeg,
-                                                     const float *wsumPtr, int base, int batch, int rows, __bf16 *actT, __bf16 *deltaT,
-                                                     float *lossOut) {
-    __shared__ __bf16 sAct[32 * ACT_STRIDE];
-    __shared__ __bf16 sD[32 * ACT_STRIDE];
-    __shared__ float sOut[32][GUIDE_OUT + 1];
-    __shared__ float sRec[32][8];
-    const int lane = threadIdx.x;
+// Training step in one kernel: blocks of four waves, persistent over iterations of
+// GUIDE_TRAIN_ROWS records (32 rows per wave).  Per iteration each wave runs the forward pass
+// of its rows with the activations in registers (as inference), the loss gradient at the
+// output, and the backward pass (the transposed product W^T delta^T with permuted W^T
+// fragments, ReLU masks kept as bits in the fragment order).  At every layer, from the last to
+// the first, the waves write the layer's input activations and output deltas of their rows to
+// LDS feature-major, and wave w adds the block's 128 rows to its share of the weight
+// gradient, output features 16 w .. 16 w + 15 of every layer, dW = delta^T a with K = rows on
+// the MFMA; the bias gradient sums the same delta fragments.  Each block keeps its gradient in
+// registers across iterations and writes it once as a partial, summed in block order by
+// k_guide_reduce: the result does not depend on scheduling.  Rows past the batch get weight 0
+// and so zero deltas.
+constexpr int TRAIN_WAVES = 4;
+constexpr int TRAIN_TILE = 32;                        // rows per wave per iteration
+constexpr int FM_PITCH = GUIDE_TRAIN_ROWS + 8;        // bf16 per feature of a feature-major tile
+static_assert(TRAIN_WAVES * TRAIN_TILE == GUIDE_TRAIN_ROWS, "an iteration is one tile per wave");
+static_assert(TRAIN_WAVES * 16 == GUIDE_W, "one 16-output block of the weight gradient per wave");
+
+struct TrainLossLds {   // one wave's loss stage
+    float out[TRAIN_TILE][GUIDE_OUT + 1];
+    __bf16 g[TRAIN_TILE][ACT_STRIDE];
+};
+struct TrainGradLds {   // the block's weight-gradient stage
+    __bf16 a[GUIDE_W][FM_PITCH];
+    __bf16 d[GUIDE_W][FM_PITCH];
+};
+union TrainLds {
+    TrainLossLds loss[TRAIN_WAVES];
+    TrainGradLds grad;
+};
+
+// Layer-0 B fragments of one 16-row block: the one-blob encoding of raw features (row-major
+// float rows of GUIDE_RAW), as in the inference kernel.
+__device__ __forceinline__ void oneBlobFrags(const float (*raw)[GUIDE_RAW], int row, int g, bf16x8 *frag) {
+    float f[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) f[i] = raw[row][2 * i + (g >> 1)];
+#pragma unroll
+    for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+        for (int p = 0; p < 8; p++) {
+            const int kf = kFeature(g, p, ks);
+            frag[ks][p] = (__bf16)oneBlob(f[(kf >> 3) >> 1], kf & 7);
+        }
+}
+
+__global__ __launch_bounds__(64 * TRAIN_WAVES) void k_guide_train(GuideNet net, const float4 *recs, const int *countPtr, const int *prefix,
+                                                                  int recSeg, const float *wsumPtr, int base, int batch, float *partials,
+                                                                  float *lossOut) {
+    __shared__ bf16x8 sWf[GUIDE_LAYERS * 8 * 64];
+    __shared__ bf16x8 sWb[(GUIDE_LAYERS - 1) * 8 * 64];   // layers 1 .. 4 (layer 0 passes no delta back)
+    __shared__ float4 sB[GUIDE_LAYERS * GUIDE_W / 4];
+    __shared__ float sRaw[TRAIN_WAVES][TRAIN_TILE][GUIDE_RAW];
+    __shared__ float sRec[TRAIN_WAVES][TRAIN_TILE][5];
+    __shared__ float sLoss[TRAIN_WAVES];
+    __shared__ __align__(16) TrainLds sx;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = lane >> 4, lr = lane & 15;
+    for (int c = threadIdx.x; c < GUIDE_LAYERS * 8 * 64; c += blockDim.x) sWf[c] = ld8(net.w + 8 * (size_t)c);
+    for (int c = threadIdx.x; c < (GUIDE_LAYERS - 1) * 8 * 64; c += blockDim.x) sWb[c] = ld8(net.wt + 8 * (size_t)(8 * 64 + c));
+    for (int c = threadIdx.x; c < GUIDE_LAYERS * GUIDE_W / 4; c += blockDim.x) sB[c] = ((const float4 *)net.b)[c];
+    __syncthreads();
     const int count = *countPtr;
     const float wsum = wsumPtr ? *wsumPtr : 0.f;
     // normalise the weights to mean one over the record buffer
     const float wscale = (wsumPtr == nullptr) ? 1.f : (wsum > 0.f ? (float)count / wsum : 0.f);
-    // C-fragment rows of this lane: groups g = 0..3 of 4 consecutive rows from 8 g + 4 (lane >> 5)
-    const int half4 = 4 * (lane >> 5);
-    auto storeT = [&](__bf16 *dst, const float *v) {
-        // 16 values of one column (C-fragment order) as four 8-byte runs of 4 consecutive rows
+    f32x4 dW[GUIDE_LAYERS][4];
+    float db[GUIDE_LAYERS];
 #pragma unroll
-        for (int g = 0; g < 4; g++) {
-            typedef __bf16 bf16x4 __attribute__((__vector_size__(4 * sizeof(__bf16))));
-            bf16x4 q = {(__bf16)v[4 * g], (__bf16)v[4 * g + 1], (__bf16)v[4 * g + 2], (__bf16)v[4 * g + 3]};
-            *(bf16x4 *)(dst + 8 * g + half4) = q;
-        }
-    };
+    for (int L = 0; L < GUIDE_LAYERS; L++) {
+        db[L] = 0.f;
+#pragma unroll
+        for (int ib = 0; ib < 4; ib++) dW[L][ib] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
     float lossAcc = 0.f;
-    const int ntiles = rows / 32;
-    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const int rowBase = tile * 32;
-        if (lane < 32) {
+    const int iters = (batch + GUIDE_TRAIN_ROWS - 1) / GUIDE_TRAIN_ROWS;
+    for (int it = blockIdx.x; it < iters; it += gridDim.x) {
+        const int rowBase = it * GUIDE_TRAIN_ROWS + wave * TRAIN_TILE;
+        if (lane < TRAIN_TILE) {
             const int r = rowBase + lane;
             float f[GUIDE_RAW], om[3] = {0.f, 0.f, 0.f}, pb = 0.f, wt = 0.f;
             for (int k = 0; k < GUIDE_RAW; k++) f[k] = 0.f;
@@ -422,152 +448,147 @@ __global__ __launch_bounds__(64) void k_guide_fwdbwd(GuideNet net, const float4 
                 pb = c.w;
                 wt = fminf(d.w * wscale, 64.f);
             }
-            for (int k = 0; k < GUIDE_RAW; k++) sOut[lane][k] = f[k];
-            sRec[lane][0] = om[0]; sRec[lane][1] = om[1]; sRec[lane][2] = om[2];
-            sRec[lane][3] = pb;
-            sRec[lane][4] = wt;
+            for (int k = 0; k < GUIDE_RAW; k++) sRaw[wave][lane][k] = f[k];
+            sRec[wave][lane][0] = om[0];
+            sRec[wave][lane][1] = om[1];
+            sRec[wave][lane][2] = om[2];
+            sRec[wave][lane][3] = pb;
+            sRec[wave][lane][4] = wt;
         }
         WAVE_SYNC();
-        {
-            // one-blob input: lane = input column (raw feature lane >> 3, bin lane & 7); all 32
-            // rows row-major into LDS and transposed to layer 0's activations
-            const int k = lane / GUIDE_BINS, j = lane % GUIDE_BINS;
-            __bf16 *dst = actT + (size_t)lane * rows + rowBase;
-            for (int q = 0; q < 4; q++) {
-                bf16x8 v;
+        // forward; act[L] = the input of layer L as B fragments [row block][K step]
+        bf16x8 act[GUIDE_LAYERS][2][2];
 #pragma unroll
-                for (int i = 0; i < 8; i++) {
-                    const int row = 8 * q + i;
-                    v[i] = (__bf16)oneBlob(sOut[row][k], j);
-                    sAct[row * ACT_STRIDE + lane] = v[i];
+        for (int rb = 0; rb < 2; rb++) oneBlobFrags(sRaw[wave], rb * 16 + lr, g, act[0][rb]);
+        uint32_t mask[GUIDE_LAYERS - 1];   // ReLU' of hidden layer L's outputs, bit (2 rb + ks) * 8 + p
+        TrainLossLds &ls = sx.loss[wave];
+#pragma unroll
+        for (int L = 0; L < GUIDE_LAYERS; L++) {
+            const bool last = L == GUIDE_LAYERS - 1;
+            f32x4 acc[2][4];
+#pragma unroll
+            for (int ob = 0; ob < 4; ob++) {
+                if (last && ob * 16 >= GUIDE_OUT) continue;
+                const float4 bias = sB[(L * GUIDE_W + ob * 16) / 4 + g];
+                const bf16x8 w0 = sWf[((L * 4 + ob) * 2) * 64 + lane], w1 = sWf[((L * 4 + ob) * 2 + 1) * 64 + lane];
+#pragma unroll
+                for (int rb = 0; rb < 2; rb++) {
+                    const f32x4 c = {bias.x, bias.y, bias.z, bias.w};
+                    acc[rb][ob] = mfma16(w1, act[L][rb][1], mfma16(w0, act[L][rb][0], c));
                 }
-                *(bf16x8 *)(dst + 8 * q) = v;
+            }
+            if (!last) {
+                uint32_t m = 0;
+#pragma unroll
+                for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+                    for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+                        for (int p = 0; p < 8; p++) {
+                            const __bf16 h = (__bf16)fmaxf(acc[rb][2 * ks + (p >> 2)][p & 3], 0.f);
+                            act[L + 1][rb][ks][p] = h;
+                            m |= ((float)h > 0.f ? 1u : 0u) << ((2 * rb + ks) * 8 + p);
+                        }
+                mask[L] = m;
+            } else {
+#pragma unroll
+                for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+                    for (int ob = 0; ob * 16 < GUIDE_OUT; ob++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++) ls.out[rb * 16 + lr][ob * 16 + 4 * g + r] = acc[rb][ob][r];
             }
         }
         WAVE_SYNC();
-        // ReLU masks of the four hidden layers' outputs, 32 bits per layer (bit 16 ct + r for
-        // this lane's C-fragment element r of column half ct): layers 0-1 in maskLo, 2-3 in
-        // maskHi, so that rolled layer loops select them without indexing registers
-        uint64_t maskLo = 0, maskHi = 0;
-#pragma unroll 1
-        for (int L = 0; L < GUIDE_LAYERS; L++) {
-            const f32x16 acc0 = layerTile(sAct, net.w, L, 0, lane);
-            const f32x16 acc1 = layerTile(sAct, net.w, L, 1, lane);
-            WAVE_SYNC();
-            uint32_t m = 0;
+        if (lane < TRAIN_TILE) {
+            const f3 w = mk(sRec[wave][lane][0], sRec[wave][lane][1], sRec[wave][lane][2]);
+            lossAcc += lossGradient(ls.out[lane], w, sRec[wave][lane][3], sRec[wave][lane][4], ls.g[lane]);
+        }
+        WAVE_SYNC();
+        // the output deltas as B fragments
+        typedef __bf16 bf16x4 __attribute__((__vector_size__(4 * sizeof(__bf16))));
+        bf16x8 dl[2][2];
 #pragma unroll
-            for (int ct = 0; ct < 2; ct++) {
-                const int col = ct * 32 + (lane & 31);
-                const float bias = net.b[L * GUIDE_W + col];
-                float h[16];
+        for (int rb = 0; rb < 2; rb++)
 #pragma unroll
-                for (int r = 0; r < 16; r++) {
-                    const int row = cRow(lane, r);
-                    const float v = (ct ? acc1[r] : acc0[r]) + bias;
-                    h[r] = fmaxf(v, 0.f);
-                    if (L < GUIDE_LAYERS - 1) {
-                        const __bf16 hb = (__bf16)h[r];
-                        sAct[row * ACT_STRIDE + col] = hb;
-                        m |= ((float)hb > 0.f ? 1u : 0u) << (16 * ct + r);
-                    } else if (col < GUIDE_OUT) {
-                        sOut[row][col] = v;
+            for (int ks = 0; ks < 2; ks++) {
+                const __bf16 *gr = ls.g[rb * 16 + lr] + 32 * ks + 4 * g;
+                const bf16x4 lo = *(const bf16x4 *)gr, hi = *(const bf16x4 *)(gr + 16);
+                dl[rb][ks] = bf16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+            }
+        __syncthreads();   // the loss buffers become the gradient stage's tiles
+        TrainGradLds &gs = sx.grad;
+#pragma unroll
+        for (int L = GUIDE_LAYERS - 1; L >= 0; L--) {
+#pragma unroll
+            for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+                for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+                    for (int p = 0; p < 8; p++) {
+                        const int kf = kFeature(g, p, ks), col = wave * TRAIN_TILE + rb * 16 + lr;
+                        gs.a[kf][col] = act[L][rb][ks][p];
+                        gs.d[kf][col] = dl[rb][ks][p];
+                    }
+            if (L > 0) {
+                // delta of layer L - 1's outputs: W_L^T delta_L, masked by ReLU'
+                f32x4 bacc[2][4];
+#pragma unroll
+                for (int ib = 0; ib < 4; ib++) {
+                    const bf16x8 w0 = sWb[(((L - 1) * 4 + ib) * 2) * 64 + lane], w1 = sWb[(((L - 1) * 4 + ib) * 2 + 1) * 64 + lane];
+#pragma unroll
+                    for (int rb = 0; rb < 2; rb++) {
+                        const f32x4 z = {0.f, 0.f, 0.f, 0.f};
+                        bacc[rb][ib] = mfma16(w1, dl[rb][1], mfma16(w0, dl[rb][0], z));
                     }
                 }
-                if (L < GUIDE_LAYERS - 1) storeT(actT + ((size_t)(L + 1) * GUIDE_W + col) * rows + rowBase, h);
+                const uint32_t m = mask[L - 1];
+#pragma unroll
+                for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+                    for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+                        for (int p = 0; p < 8; p++) {
+                            const float v = bacc[rb][2 * ks + (p >> 2)][p & 3];
+                            dl[rb][ks][p] = (__bf16)((m >> ((2 * rb + ks) * 8 + p)) & 1u ? v : 0.f);
+                        }
             }
-            if (L < GUIDE_LAYERS - 1) {
-                const uint64_t mm = (uint64_t)m << (32 * (L & 1));
-                if (L < 2) maskLo |= mm;
-                else maskHi |= mm;
-            }
-            WAVE_SYNC();
-        }
-        if (lane < 32) {
-            const f3 w = mk(sRec[lane][0], sRec[lane][1], sRec[lane][2]);
-            lossAcc += lossGradient(sOut[lane], w, sRec[lane][3], sRec[lane][4], sD + lane * ACT_STRIDE);
-        }
-        WAVE_SYNC();
-        {
-            // output deltas transposed: lane = output column
-            __bf16 *dst = deltaT + ((size_t)(GUIDE_LAYERS - 1) * GUIDE_W + lane) * rows + rowBase;
-            for (int q = 0; q < 4; q++) {
-                bf16x8 v;
+            __syncthreads();
+            // this wave's output block of dW_L += delta_L^T a_L over the block's rows
 #pragma unroll
-                for (int i = 0; i < 8; i++) v[i] = sD[(8 * q + i) * ACT_STRIDE + lane];
-                *(bf16x8 *)(dst + 8 * q) = v;
-            }
-        }
-#pragma unroll 1
-        for (int L = GUIDE_LAYERS - 1; L > 0; L--) {
-            // delta_{L-1}[row][in] = sum_out delta_L[row][out] W_L[out][in], masked by relu'
-            f32x16 acc[2];
+            for (int kk = 0; kk < GUIDE_TRAIN_ROWS / 32; kk++) {
+                const bf16x8 ad = *(const bf16x8 *)&gs.d[wave * 16 + lr][32 * kk + 8 * g];
 #pragma unroll
-            for (int nt = 0; nt < 2; nt++) {
-                f32x16 a = {};
-                const __bf16 *arow = sD + (lane & 31) * ACT_STRIDE + (lane >> 5) * 8;
-                const __bf16 *brow = net.wt + (size_t)L * GUIDE_W * GUIDE_W + (nt * 32 + (lane & 31)) * GUIDE_W + (lane >> 5) * 8;
+                for (int i = 0; i < 8; i++) db[L] += (float)ad[i];
 #pragma unroll
-                for (int ks = 0; ks < GUIDE_W / 16; ks++) a = mfma(ld8(arow + ks * 16), ld8(brow + ks * 16), a);
-                acc[nt] = a;
-            }
-            WAVE_SYNC();
-            const uint32_t m = (uint32_t)((L - 1 < 2 ? maskLo : maskHi) >> (32 * ((L - 1) & 1)));
-#pragma unroll
-            for (int nt = 0; nt < 2; nt++) {
-                const int in = nt * 32 + (lane & 31);
-                float dv[16];
-#pragma unroll
-                for (int r = 0; r < 16; r++) {
-                    const int row = cRow(lane, r);
-                    dv[r] = (m >> (16 * nt + r)) & 1u ? acc[nt][r] : 0.f;
-                    sD[row * ACT_STRIDE + in] = (__bf16)dv[r];
+                for (int ib = 0; ib < 4; ib++) {
+                    const bf16x8 ba = *(const bf16x8 *)&gs.a[ib * 16 + lr][32 * kk + 8 * g];
+                    dW[L][ib] = mfma16(ad, ba, dW[L][ib]);
                 }
-                storeT(deltaT + ((size_t)(L - 1) * GUIDE_W + in) * rows + rowBase, dv);
             }
-            WAVE_SYNC();
+            __syncthreads();
         }
     }
-    // loss of this block: lanes 0..31 hold partial sums
+    // the block's partial: out = 16 wave + 4 g + r, in = 16 ib + lr
+    float *P = partials + (size_t)blockIdx.x * GUIDE_PARAMS;
+#pragma unroll
+    for (int L = 0; L < GUIDE_LAYERS; L++) {
+        float *PL = P + (size_t)L * GUIDE_LAYER_PARAMS;
+#pragma unroll
+        for (int ib = 0; ib < 4; ib++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) PL[(wave * 16 + 4 * g + r) * GUIDE_W + ib * 16 + lr] = dW[L][ib][r];
+        // lanes lr, lr + 16, lr + 32, lr + 48 summed complementary rows of output 16 wave + lr
+        float v = db[L];
+        v += __shfl_xor(v, 16);
+        v += __shfl_xor(v, 32);
+        if (g == 0) PL[GUIDE_W * GUIDE_W + wave * 16 + lr] = v;
+    }
+    // loss of this block: lanes 0 .. 31 of each wave hold partial sums, added in wave order
     for (int off = 32; off >= 1; off >>= 1) lossAcc += __shfl_down(lossAcc, off);
-    if (lane == 0 && lossOut) lossOut[blockIdx.x] = lossAcc;
-}
-
-// Training, phase 2: weight and bias gradients of one layer over one slice of
-// GUIDE_WG_ROWS rows, dW[out][in] = sum_rows delta[row][out] a[row][in] with K = rows on the
-// MFMA (four waves, one 32 x 32 output tile each) and db[out] = sum_rows delta[row][out]; the
-// slice's partials are summed in slice order afterwards.
-__global__ __launch_bounds__(256) void k_guide_wgrad(const __bf16 *actT, const __bf16 *deltaT, int rows, float *partials) {
-    const int slice = blockIdx.x, L = blockIdx.y;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int mt = wave >> 1, nt = wave & 1;
-    const int r0 = slice * GUIDE_WG_ROWS, r1 = min(r0 + GUIDE_WG_ROWS, rows);
-    const __bf16 *dp = deltaT + ((size_t)L * GUIDE_W + mt * 32 + (lane & 31)) * rows + (lane >> 5) * 8;
-    const __bf16 *ap = actT + ((size_t)L * GUIDE_W + nt * 32 + (lane & 31)) * rows + (lane >> 5) * 8;
-    // two accumulators over alternate K steps (rows is a multiple of 32) keep two MFMA chains
-    // and their loads in flight
-    f32x16 acc = {}, acc2 = {};
-    float bsum = 0.f;
-#pragma unroll 2
-    for (int k = r0; k < r1; k += 32) {
-        const bf16x8 a = ld8(dp + k), b = ld8(ap + k);
-        const bf16x8 a2 = ld8(dp + k + 16), b2 = ld8(ap + k + 16);
-        acc = mfma(a, b, acc);
-        acc2 = mfma(a2, b2, acc2);
-        if (nt == 0) {
-#pragma unroll
-            for (int i = 0; i < 8; i++) bsum += (float)a[i] + (float)a2[i];
-        }
-    }
-    acc += acc2;
-    float *P = partials + (size_t)slice * GUIDE_PARAMS + (size_t)L * GUIDE_LAYER_PARAMS;
-    const int in = nt * 32 + (lane & 31);
-#pragma unroll
-    for (int r = 0; r < 16; r++) P[(mt * 32 + cRow(lane, r)) * GUIDE_W + in] = acc[r];
-    if (nt == 0) {
-        // lanes l and l + 32 summed complementary rows of output column mt * 32 + (l & 31)
-        bsum += __shfl_down(bsum, 32);
-        if (lane < 32) P[GUIDE_W * GUIDE_W + mt * 32 + lane] = bsum;
-    }
+    if (lane == 0) sLoss[wave] = lossAcc;
+    __syncthreads();
+    if (threadIdx.x == 0 && lossOut) lossOut[blockIdx.x] = (sLoss[0] + sLoss[1]) + (sLoss[2] + sLoss[3]);
 }
 
 // --------------------------------------------------------------------------------------------
@@ -578,9 +599,8 @@ __device__ __forceinline__ void storeParam(int p, float v, __bf16 *w, __bf16 *wt
     if (off < GUIDE_W * GUIDE_W) {
         const int out = off / GUIDE_W, in = off % GUIDE_W;
         const __bf16 h = (__bf16)v;
-        w[(size_t)L * GUIDE_W * GUIDE_W + out * GUIDE_W + in] = h;
-        w[GUIDE_WFRAG_OFF + wFragIndex(L, out, in)] = h;
-        wt[(size_t)L * GUIDE_W * GUIDE_W + in * GUIDE_W + out] = h;
+        w[wFragIndex(L, out, in)] = h;    // forward: W[out][in], K = in
+        wt[wFragIndex(L, in, out)] = h;   // backward: W^T[in][out], K = out
     } else {
         b[L * GUIDE_W + (off - GUIDE_W * GUIDE_W)] = v;
     }
@@ -734,10 +754,8 @@ void guideInferRaw(GuideNet net, const float *feats, int n, float *rawOut, hipSt
 
 void guideGradients(GuideNet net, const float4 *records, const int *count, const int *prefix, int recSeg, int base, int batch,
                     const float *wsum, const GuideTrainBufs &tb, float *grad, float *lossOut, hipStream_t s) {
-    const int rows = guideRows(batch), slices = guideSlices(batch);
-    k_guide_fwdbwd<<<tb.fwdBlocks, 64, 0, s>>>(net, records, count, prefix, recSeg, wsum, base, batch, rows, tb.act, tb.delta, tb.lossPart);
-    k_guide_wgrad<<<dim3(slices, GUIDE_LAYERS), 256, 0, s>>>(tb.act, tb.delta, rows, tb.partials);
-    k_guide_reduce<<<(GUIDE_PARAMS + 63) / 64 + 1, 256, 0, s>>>(tb.partials, slices, tb.lossPart, tb.fwdBlocks, 1.f / (float)std::max(batch, 1),
+    k_guide_train<<<tb.blocks, 64 * TRAIN_WAVES, 0, s>>>(net, records, count, prefix, recSeg, wsum, base, batch, tb.partials, tb.lossPart);
+    k_guide_reduce<<<(GUIDE_PARAMS + 63) / 64 + 1, 256, 0, s>>>(tb.partials, tb.blocks, tb.lossPart, tb.blocks, 1.f / (float)std::max(batch, 1),
                                                                grad, lossOut);
     PT_HIP_TRY(hipGetLastError());
 }

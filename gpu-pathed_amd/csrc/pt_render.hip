@@ -64,9 +64,6 @@ static constexpr int kDefaultTraceBlocks = 11;
 // LBVH measured faster than PLOC on the displaced icosphere at 1024^2, 64 spp (4467 vs 4348
 // Mrays/s; extend 2053 vs 2160 us per launch, although PLOC's tree has 12% fewer 4-wide nodes).
 static constexpr int kDefaultBvhBuilder = PT_BVH_LBVH;
-// Blocks per CU of the guide training's forward/backward kernel: one-wave blocks of 16.5 KiB
-// LDS and 197 registers per lane, two waves per SIMD.
-static constexpr int kTrainBlocksPerCU = 8;
 static constexpr float kShadowEps = 1e-4f;   // path_tracer.cu:104-110
 static constexpr float kBounceEps = 1e-3f;   // path_tracer.cu:177
 
@@ -1436,7 +1433,7 @@ struct pt_renderer {
     int misSample = 0;
     // guiding network and its training state
     bool guideReady = false;
-    DevBuf gParams, gM, gV, gGrad, gPartials, gLossPart, gLoss, gW, gWt, gB, gMix, gRecords, gRecIdx[2], gAct, gDelta, gFeat;
+    DevBuf gParams, gM, gV, gGrad, gPartials, gLossPart, gLoss, gW, gWt, gB, gMix, gRecords, gRecIdx[2], gFeat;
     int64_t gRecCap = 0;
     int gTrainSteps = 16;   // Adam steps per chunk (room workload sweep: 2 -> 16 halves the MSE)
     int gBatch = 1 << 18;
@@ -1645,7 +1642,7 @@ void ensureGuide(pt_renderer *r) {
         r->gV.alloc(np * 4);
         r->gGrad.alloc(np * 4);
         r->gLoss.alloc(4);
-        r->gW.alloc(GUIDE_WBUF * 2);
+        r->gW.alloc(GUIDE_LAYERS * GUIDE_W * GUIDE_W * 2);
         r->gWt.alloc(GUIDE_LAYERS * GUIDE_W * GUIDE_W * 2);
         r->gB.alloc(GUIDE_LAYERS * GUIDE_W * 4);
         std::vector<float> init(np);
@@ -1765,13 +1762,9 @@ void runChunk(pt_renderer *r, const ChunkParams &cp) {
         PT_HIP_TRY(hipGetLastError());
         // training scratch for the current batch size (reallocated only when it changes)
         GuideTrainBufs tb;
-        tb.fwdBlocks = r->cus * kTrainBlocksPerCU;
-        r->gAct.alloc(guideActBytes(r->gBatch));
-        r->gDelta.alloc(guideActBytes(r->gBatch));
-        r->gPartials.alloc((size_t)guideSlices(r->gBatch) * GUIDE_PARAMS * 4);
-        r->gLossPart.alloc((size_t)tb.fwdBlocks * 4);
-        tb.act = r->gAct.as<__bf16>();
-        tb.delta = r->gDelta.as<__bf16>();
+        tb.blocks = guideTrainBlocks(r->gBatch, r->cus);
+        r->gPartials.alloc((size_t)tb.blocks * GUIDE_PARAMS * 4);
+        r->gLossPart.alloc((size_t)tb.blocks * 4);
         tb.partials = r->gPartials.as<float>();
         tb.lossPart = r->gLossPart.as<float>();
         for (int s = 0; s < r->gTrainSteps; s++) {

@@ -330,7 +330,7 @@ struct NetStage {
     GuideNet net;
     NetStage(const float *host, hipStream_t s) {
         params.alloc(GUIDE_PARAMS * sizeof(float));
-        w.alloc(GUIDE_WBUF * sizeof(__bf16));
+        w.alloc(GUIDE_LAYERS * GUIDE_W * GUIDE_W * sizeof(__bf16));
         wt.alloc(GUIDE_LAYERS * GUIDE_W * GUIDE_W * sizeof(__bf16));
         b.alloc(GUIDE_LAYERS * GUIDE_W * sizeof(float));
         PT_HIP_TRY(hipMemcpyAsync(params.p, host, GUIDE_PARAMS * sizeof(float), hipMemcpyHostToDevice, s));
@@ -365,11 +365,9 @@ int pt_unit_guide_gradients(const float *params, const float *records, int64_t n
     const int cnt = (int)n;
     const int *dcount = st.in(&cnt, 1);
     GuideTrainBufs tb;
-    tb.fwdBlocks = std::min(deviceCUs(), (int)((n + 31) / 32));
-    tb.act = st.out<__bf16>(guideActBytes(cnt) / 2);
-    tb.delta = st.out<__bf16>(guideActBytes(cnt) / 2);
-    tb.partials = st.out<float>((size_t)guideSlices(cnt) * GUIDE_PARAMS);
-    tb.lossPart = st.out<float>((size_t)tb.fwdBlocks);
+    tb.blocks = guideTrainBlocks(cnt, deviceCUs());
+    tb.partials = st.out<float>((size_t)tb.blocks * GUIDE_PARAMS);
+    tb.lossPart = st.out<float>((size_t)tb.blocks);
     float *dgrad = st.out<float>(GUIDE_PARAMS);
     float *dloss = st.out<float>(1);
     guideGradients(ns.net, dr, dcount, nullptr, 0, 0, cnt, nullptr, tb, dgrad, dloss, st.s);
