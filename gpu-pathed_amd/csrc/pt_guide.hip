@@ -61,10 +61,17 @@ __device__ __forceinline__ float oneBlob(float s, int j) {
     return __expf(-0.5f * x * x);
 }
 
+// Reciprocal with the hardware's 1-ulp v_rcp_f32 (an IEEE division expands to ten
+// instructions; the network's outputs are BF16-limited anyway).
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// max(x, 0) as one integer max on the bit pattern: negative floats (and -0) are negative as
+// int32, the others keep their order.  fmaxf would first quiet a possible NaN input.
+__device__ __forceinline__ float relu(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
+
 // mixture parameters from the 41 used raw outputs
 struct Mixture {
     f3 mu[GUIDE_LOBES];
-    float kappa[GUIDE_LOBES], pi[GUIDE_LOBES], mlen[GUIDE_LOBES];
+    float kappa[GUIDE_LOBES], pi[GUIDE_LOBES], ilen[GUIDE_LOBES];   // ilen: 1 / |mean output|, 0 if degenerate
     float frac;
 };
 __device__ __forceinline__ void mixtureFromRaw(const float *o, Mixture &mx) {
@@ -75,21 +82,24 @@ __device__ __forceinline__ void mixtureFromRaw(const float *o, Mixture &mx) {
         mx.pi[k] = __expf(o[GO_LOGIT + k] - mxl);
         s += mx.pi[k];
     }
+    const float is = frcp(s);
     for (int k = 0; k < GUIDE_LOBES; k++) {
-        mx.pi[k] /= s;
+        mx.pi[k] *= is;
         const f3 m = mk(o[GO_MEAN + 3 * k], o[GO_MEAN + 3 * k + 1], o[GO_MEAN + 3 * k + 2]);
-        const float l = sqrtf(dot(m, m));
-        mx.mlen[k] = l;
-        mx.mu[k] = l > 1e-12f ? m / l : mk(0.f, 1.f, 0.f);
+        const float l2 = dot(m, m);
+        const bool ok = l2 > 1e-24f;
+        const float il = ok ? __builtin_amdgcn_rsqf(l2) : 0.f;
+        mx.ilen[k] = il;
+        mx.mu[k] = ok ? m * il : mk(0.f, 1.f, 0.f);
         mx.kappa[k] = __expf(fminf(fmaxf(o[GO_KAPPA + k], GUIDE_LOGK_MIN), GUIDE_LOGK_MAX));
     }
-    const float c = 1.f / (1.f + __expf(-o[GO_FRAC]));
+    const float c = frcp(1.f + __expf(-o[GO_FRAC]));
     mx.frac = fminf(fmaxf(c, GUIDE_FRAC_MIN), GUIDE_FRAC_MAX);
 }
 
 // von Mises-Fisher density on S^2: kappa / (2 pi (1 - e^{-2 kappa})) exp(kappa (mu.w - 1))
 __device__ __forceinline__ float vmfPdf(f3 mu, float kappa, f3 w) {
-    return kappa / (6.28318530718f * (1.f - __expf(-2.f * kappa))) * __expf(kappa * (dot(mu, w) - 1.f));
+    return kappa * frcp(6.28318530718f * (1.f - __expf(-2.f * kappa))) * __expf(kappa * (dot(mu, w) - 1.f));
 }
 
 // --------------------------------------------------------------------------------------------
@@ -192,176 +202,218 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void k_guide_infer(GuideNet net, 
     __syncthreads();
     float(*so)[GUIDE_OUT + 1] = sOut[wave];
     const int waveStride = gridDim.x * INFER_WAVES;
-    for (int tile = blockIdx.x * INFER_WAVES + wave; tile * INFER_ROWS < maxRows; tile += waveStride) {
+    // A tile's inputs: the features of this lane's four rows rowBase + 16 rb + lr (both float4
+    // of a row; rawFeats has the same layout) and its segment's record count.  The next tile's
+    // are loaded while the current one runs through the network.
+    const float4 *src = rawFeats ? (const float4 *)rawFeats : feat;
+    auto loadTile = [&](int tile, float4 *fa, float4 *fb, int &cnt) {
         const int rowBase = tile * INFER_ROWS;
-        // rows [rowBase, n) of this tile hold records: the filled part of its queue segment
-        int n = maxRows;
-        if (count) {
-            const int seg = rowBase / segLen;
-            n = seg * segLen + min(count[seg * segStride], segLen);
-            if (rowBase >= n) continue;
-        }
-        // this lane's row in the output stage
-        const int mine = rowBase + lane;
-        const bool elig = mine < n && (rawFeats != nullptr || feat[2 * (size_t)mine + 1].w >= 0.f);
-        if (__ballot(elig) == 0) {
-            // no guided vertex in the tile (misses, mirror and glass hits): only the flag that
-            // makes shade sample the BSDF alone, no network evaluation
-            if (mine < n && !rawOut) out[(size_t)mine * GUIDE_REC_F4 + 10] = make_float4(0.f, 0.f, 0.f, 0.f);
-            continue;
-        }
-        // layer 0's B fragments: one-blob encoding.  Input feature 8 k + j is bin j of raw
-        // feature k; the lane's K positions need raw features 2 i + (g >> 1), i = 0..3, at bins
-        // 4 (g & 1) + 0..3.
-        bf16x8 act[4][2];
+        cnt = count ? count[(rowBase / segLen) * segStride] : maxRows;
 #pragma unroll
         for (int rb = 0; rb < 4; rb++) {
             const int row = rowBase + rb * 16 + lr;
-            float f[4] = {0.f, 0.f, 0.f, 0.f};
-            if (row < n) {
-                if (rawFeats) {
+            fa[rb] = fb[rb] = make_float4(0.f, 0.f, 0.f, -1.f);
+            if (row < maxRows) {
+                fa[rb] = src[2 * (size_t)row];
+                fb[rb] = src[2 * (size_t)row + 1];
+            }
+        }
+    };
+    int tile = blockIdx.x * INFER_WAVES + wave;
+    float4 fa[4], fb[4];
+    int cnt = 0;
+    if (tile * INFER_ROWS < maxRows) loadTile(tile, fa, fb, cnt);
+    for (; tile * INFER_ROWS < maxRows; tile += waveStride) {
+        float4 na[4], nb[4];
+        int ncnt = 0;
+        if ((tile + waveStride) * INFER_ROWS < maxRows) loadTile(tile + waveStride, na, nb, ncnt);
+        const int rowBase = tile * INFER_ROWS;
+        // rows [rowBase, n) of this tile hold records: the filled part of its queue segment
+        const int n = count ? (rowBase / segLen) * segLen + min(cnt, segLen) : maxRows;
+        // guided rows per 16-row block (bit lr of each ballot); this lane's output row is
+        // rowBase + lane, in block lane >> 4
+        uint64_t bal[4];
 #pragma unroll
-                    for (int i = 0; i < 4; i++) f[i] = rawFeats[(size_t)row * GUIDE_RAW + 2 * i + (g >> 1)];
-                } else {
-                    const float4 a = feat[2 * (size_t)row], b = feat[2 * (size_t)row + 1];
-                    if (b.w >= 0.f) {
-                        const bool hi = (g >> 1) != 0;
-                        f[0] = hi ? a.y : a.x;
-                        f[1] = hi ? a.w : a.z;
-                        f[2] = hi ? b.y : b.x;
-                        f[3] = hi ? b.w : b.z;
+        for (int rb = 0; rb < 4; rb++) bal[rb] = __ballot(rowBase + rb * 16 + lr < n && (rawFeats != nullptr || fb[rb].w >= 0.f));
+        const int mine = rowBase + lane;
+        const uint64_t mb = g == 0 ? bal[0] : g == 1 ? bal[1] : g == 2 ? bal[2] : bal[3];
+        const bool elig = (mb >> lr) & 1u;
+        if (rowBase < n && (bal[0] | bal[1] | bal[2] | bal[3]) == 0) {
+            // no guided vertex in the tile (misses, mirror and glass hits): only the flag that
+            // makes shade sample the BSDF alone, no network evaluation
+            if (mine < n && !rawOut) out[(size_t)mine * GUIDE_REC_F4 + 10] = make_float4(0.f, 0.f, 0.f, 0.f);
+        } else if (rowBase < n) {
+            // layer 0's B fragments: one-blob encoding.  Input feature 8 k + j is bin j of raw
+            // feature k; the lane's K positions need raw features 2 i + (g >> 1), i = 0..3, at bins
+            // 4 (g & 1) + 0..3.  Rows that are not guided enter as zeros.
+            bf16x8 act[4][2];
+#pragma unroll
+            for (int rb = 0; rb < 4; rb++) {
+                const bool on = (bal[rb] >> lr) & 1u;
+                const bool hi = (g >> 1) != 0;
+                const float4 a = fa[rb], b = fb[rb];
+                float f[4];
+                f[0] = on ? (hi ? a.y : a.x) : 0.f;
+                f[1] = on ? (hi ? a.w : a.z) : 0.f;
+                f[2] = on ? (hi ? b.y : b.x) : 0.f;
+                f[3] = on ? (hi ? b.w : b.z) : 0.f;
+#pragma unroll
+                for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+                    for (int p = 0; p < 8; p++) {
+                        const int kf = kFeature(g, p, ks);
+                        act[rb][ks][p] = (__bf16)oneBlob(f[(kf >> 3) >> 1], kf & 7);
+                    }
+            }
+#pragma unroll
+            for (int L = 0; L < GUIDE_LAYERS; L++) {
+                const bool last = L == GUIDE_LAYERS - 1;
+                f32x4 acc[4][4];
+#pragma unroll
+                for (int ob = 0; ob < 4; ob++) {
+                    if (last && ob * 16 >= GUIDE_OUT) continue;
+                    const float4 bias = sB[(L * GUIDE_W + ob * 16) / 4 + g];
+                    const bf16x8 w0 = sWf[((L * 4 + ob) * 2) * 64 + lane], w1 = sWf[((L * 4 + ob) * 2 + 1) * 64 + lane];
+#pragma unroll
+                    for (int rb = 0; rb < 4; rb++) {
+                        const f32x4 c = {bias.x, bias.y, bias.z, bias.w};
+                        acc[rb][ob] = mfma16(w1, act[rb][1], mfma16(w0, act[rb][0], c));
                     }
                 }
-            }
+                if (!last) {
 #pragma unroll
-            for (int ks = 0; ks < 2; ks++)
+                    for (int rb = 0; rb < 4; rb++)
 #pragma unroll
-                for (int p = 0; p < 8; p++) {
-                    const int kf = kFeature(g, p, ks);
-                    act[rb][ks][p] = (__bf16)oneBlob(f[(kf >> 3) >> 1], kf & 7);
-                }
-        }
+                        for (int ks = 0; ks < 2; ks++)
 #pragma unroll
-        for (int L = 0; L < GUIDE_LAYERS; L++) {
-            const bool last = L == GUIDE_LAYERS - 1;
-            f32x4 acc[4][4];
+                            for (int p = 0; p < 8; p++) act[rb][ks][p] = (__bf16)relu(acc[rb][2 * ks + (p >> 2)][p & 3]);
+                } else {
 #pragma unroll
-            for (int ob = 0; ob < 4; ob++) {
-                if (last && ob * 16 >= GUIDE_OUT) continue;
-                const float4 bias = sB[(L * GUIDE_W + ob * 16) / 4 + g];
-                const bf16x8 w0 = sWf[((L * 4 + ob) * 2) * 64 + lane], w1 = sWf[((L * 4 + ob) * 2 + 1) * 64 + lane];
+                    for (int rb = 0; rb < 4; rb++)
 #pragma unroll
-                for (int rb = 0; rb < 4; rb++) {
-                    const f32x4 c = {bias.x, bias.y, bias.z, bias.w};
-                    acc[rb][ob] = mfma16(w1, act[rb][1], mfma16(w0, act[rb][0], c));
+                        for (int ob = 0; ob * 16 < GUIDE_OUT; ob++)
+#pragma unroll
+                            for (int r = 0; r < 4; r++) so[rb * 16 + lr][ob * 16 + 4 * g + r] = acc[rb][ob][r];
                 }
             }
-            if (!last) {
-#pragma unroll
-                for (int rb = 0; rb < 4; rb++)
-#pragma unroll
-                    for (int ks = 0; ks < 2; ks++)
-#pragma unroll
-                        for (int p = 0; p < 8; p++) act[rb][ks][p] = (__bf16)fmaxf(acc[rb][2 * ks + (p >> 2)][p & 3], 0.f);
-            } else {
-#pragma unroll
-                for (int rb = 0; rb < 4; rb++)
-#pragma unroll
-                    for (int ob = 0; ob * 16 < GUIDE_OUT; ob++)
-#pragma unroll
-                        for (int r = 0; r < 4; r++) so[rb * 16 + lr][ob * 16 + 4 * g + r] = acc[rb][ob][r];
+            WAVE_SYNC();
+            if (mine < n) {
+                if (rawOut) {
+                    for (int k = 0; k < GUIDE_OUT; k++) rawOut[(size_t)mine * GUIDE_OUT + k] = so[lane][k];
+                } else if (!elig) {
+                    // shade reads only the flag of a vertex that is not guided
+                    out[(size_t)mine * GUIDE_REC_F4 + 10] = make_float4(0.f, 0.f, 0.f, 0.f);
+                } else {
+                    Mixture mx;
+                    mixtureFromRaw(so[lane], mx);
+                    float4 *o = out + (size_t)mine * GUIDE_REC_F4;
+                    for (int k = 0; k < GUIDE_LOBES; k++) o[k] = f4(mx.mu[k], mx.kappa[k]);
+                    o[8] = make_float4(mx.pi[0], mx.pi[1], mx.pi[2], mx.pi[3]);
+                    o[9] = make_float4(mx.pi[4], mx.pi[5], mx.pi[6], mx.pi[7]);
+                    o[10] = make_float4(mx.frac, 1.f, 0.f, 0.f);
+                    o[11] = make_float4(0.f, 0.f, 0.f, 0.f);
+                }
             }
+            WAVE_SYNC();
         }
-        WAVE_SYNC();
-        if (mine < n) {
-            if (rawOut) {
-                for (int k = 0; k < GUIDE_OUT; k++) rawOut[(size_t)mine * GUIDE_OUT + k] = so[lane][k];
-            } else if (!elig) {
-                // shade reads only the flag of a vertex that is not guided
-                out[(size_t)mine * GUIDE_REC_F4 + 10] = make_float4(0.f, 0.f, 0.f, 0.f);
-            } else {
-                Mixture mx;
-                mixtureFromRaw(so[lane], mx);
-                float4 *o = out + (size_t)mine * GUIDE_REC_F4;
-                for (int k = 0; k < GUIDE_LOBES; k++) o[k] = f4(mx.mu[k], mx.kappa[k]);
-                o[8] = make_float4(mx.pi[0], mx.pi[1], mx.pi[2], mx.pi[3]);
-                o[9] = make_float4(mx.pi[4], mx.pi[5], mx.pi[6], mx.pi[7]);
-                o[10] = make_float4(mx.frac, 1.f, 0.f, 0.f);
-                o[11] = make_float4(0.f, 0.f, 0.f, 0.f);
-            }
+#pragma unroll
+        for (int rb = 0; rb < 4; rb++) {
+            fa[rb] = na[rb];
+            fb[rb] = nb[rb];
         }
-        WAVE_SYNC();
+        cnt = ncnt;
     }
 }
 
 // --------------------------------------------------------------------------------------------
 // training: gradients
 // --------------------------------------------------------------------------------------------
-// d loss / d raw outputs for one record; loss = -w (log q + log(c q + (1 - c) p_bsdf))
-// Stores the clamped gradient as one BF16 row (GUIDE_W entries, zero where the loss does not
-// depend on the output) and returns the loss.
-__device__ float lossGradient(const float *o, f3 w, float pb, float wt, __bf16 *g) {
+// d loss / d raw outputs for one record, loss = -w (log q + log(c q + (1 - c) p_bsdf)), as one
+// BF16 row (GUIDE_W entries, zero where the loss does not depend on the output; single
+// outliers clamped to 1e3).  Four lanes per record (lanes l, l ^ 16, l ^ 32, l ^ 48; quarter
+// q = l >> 4 handles lobes 2 q and 2 q + 1): the mixture's softmax maximum and sum and its
+// density are reduced across the four lanes, each lane writes its lobes' entries of the row
+// (and quarter 0 the selection entry).  The four lanes must be active together.  Returns the
+// loss in quarter 0, 0 in the others.
+__device__ float lossGradient4(const float *o, f3 w, float pb, float wt, __bf16 *g, int q) {
     const bf16x8 zero = {};
-#pragma unroll
-    for (int k = 0; k < GUIDE_W; k += 8) *(bf16x8 *)(g + k) = zero;
+    *(bf16x8 *)(g + 16 * q) = zero;
+    *(bf16x8 *)(g + 16 * q + 8) = zero;
     if (!(wt > 0.f)) return 0.f;
-    // keep single outliers from dominating a step
     auto put = [&](int k, float v) { g[k] = (__bf16)fminf(fmaxf(v, -1e3f), 1e3f); };
-    Mixture mx;
-    mixtureFromRaw(o, mx);
-    float fk[GUIDE_LOBES];
-    float q = 0.f;
-    for (int k = 0; k < GUIDE_LOBES; k++) {
-        fk[k] = vmfPdf(mx.mu[k], mx.kappa[k], w);
-        q += mx.pi[k] * fk[k];
+    auto sum4 = [](float v) {
+        v += __shfl_xor(v, 16);
+        return v + __shfl_xor(v, 32);
+    };
+    auto max4 = [](float v) {
+        v = fmaxf(v, __shfl_xor(v, 16));
+        return fmaxf(v, __shfl_xor(v, 32));
+    };
+    const int k0 = 2 * q;
+    const float lg0 = o[GO_LOGIT + k0], lg1 = o[GO_LOGIT + k0 + 1];
+    const float mxl = max4(fmaxf(lg0, lg1));
+    const float e0 = __expf(lg0 - mxl), e1 = __expf(lg1 - mxl);
+    const float is = frcp(sum4(e0 + e1));
+    float pi[2], kap[2], il[2], fk[2];
+    f3 mu[2];
+    float qp = 0.f;
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int k = k0 + j;
+        pi[j] = (j ? e1 : e0) * is;
+        const f3 m = mk(o[GO_MEAN + 3 * k], o[GO_MEAN + 3 * k + 1], o[GO_MEAN + 3 * k + 2]);
+        const float l2 = dot(m, m);
+        const bool ok = l2 > 1e-24f;
+        il[j] = ok ? __builtin_amdgcn_rsqf(l2) : 0.f;
+        mu[j] = ok ? m * il[j] : mk(0.f, 1.f, 0.f);
+        kap[j] = __expf(fminf(fmaxf(o[GO_KAPPA + k], GUIDE_LOGK_MIN), GUIDE_LOGK_MAX));
+        fk[j] = vmfPdf(mu[j], kap[j], w);
+        qp += pi[j] * fk[j];
     }
-    q = fmaxf(q, 1e-12f);
-    const float c = mx.frac;
-    const float mix = fmaxf(c * q + (1.f - c) * pb, 1e-12f);
-    const float loss = -wt * (__logf(q) + __logf(mix));
-    const float dq = -wt * (1.f / q + c / mix);
-    // selection probability (sigmoid, clamped)
-    const float craw = 1.f / (1.f + __expf(-o[GO_FRAC]));
-    if (craw > GUIDE_FRAC_MIN && craw < GUIDE_FRAC_MAX) put(GO_FRAC, -wt * (q - pb) / mix * craw * (1.f - craw));
-    for (int k = 0; k < GUIDE_LOBES; k++) {
-        // softmax logits
-        put(GO_LOGIT + k, dq * mx.pi[k] * (fk[k] - q));
-        const float df = dq * mx.pi[k] * fk[k];  // d loss / d log f_k
-        const float kap = mx.kappa[k];
-        const float cosw = dot(mx.mu[k], w);
-        // d log f / d kappa = 1/kappa - 2 e^{-2k} / (1 - e^{-2k}) + (mu.w - 1); kappa = exp(raw)
-        const float e2 = __expf(-2.f * kap);
-        const float dlk = 1.f / kap - 2.f * e2 / (1.f - e2) + (cosw - 1.f);
+    const float qq = fmaxf(sum4(qp), 1e-12f);
+    const float craw = frcp(1.f + __expf(-o[GO_FRAC]));
+    const float c = fminf(fmaxf(craw, GUIDE_FRAC_MIN), GUIDE_FRAC_MAX);
+    const float mix = fmaxf(c * qq + (1.f - c) * pb, 1e-12f);
+    const float imix = frcp(mix);
+    const float dq = -wt * (frcp(qq) + c * imix);
+    if (q == 0 && craw > GUIDE_FRAC_MIN && craw < GUIDE_FRAC_MAX) put(GO_FRAC, -wt * (qq - pb) * imix * craw * (1.f - craw));
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int k = k0 + j;
+        put(GO_LOGIT + k, dq * pi[j] * (fk[j] - qq));
+        const float df = dq * pi[j] * fk[j];
+        const float cosw = dot(mu[j], w);
+        const float e2 = __expf(-2.f * kap[j]);
+        const float dlk = frcp(kap[j]) - 2.f * e2 * frcp(1.f - e2) + (cosw - 1.f);
         const float lk = o[GO_KAPPA + k];
-        if (lk > GUIDE_LOGK_MIN && lk < GUIDE_LOGK_MAX) put(GO_KAPPA + k, df * dlk * kap);
-        // d log f / d m = kappa (w - (mu.w) mu) / |m|
-        if (mx.mlen[k] > 1e-12f) {
-            const float s = df * kap / mx.mlen[k];
-            put(GO_MEAN + 3 * k, s * (w.x - cosw * mx.mu[k].x));
-            put(GO_MEAN + 3 * k + 1, s * (w.y - cosw * mx.mu[k].y));
-            put(GO_MEAN + 3 * k + 2, s * (w.z - cosw * mx.mu[k].z));
+        if (lk > GUIDE_LOGK_MIN && lk < GUIDE_LOGK_MAX) put(GO_KAPPA + k, df * dlk * kap[j]);
+        if (il[j] > 0.f) {
+            const float sc = df * kap[j] * il[j];
+            put(GO_MEAN + 3 * k, sc * (w.x - cosw * mu[j].x));
+            put(GO_MEAN + 3 * k + 1, sc * (w.y - cosw * mu[j].y));
+            put(GO_MEAN + 3 * k + 2, sc * (w.z - cosw * mu[j].z));
         }
     }
-    return loss;
+    return q == 0 ? -wt * (__logf(qq) + __logf(mix)) : 0.f;
 }
 
-// Training step in one kernel: blocks of four waves, persistent over iterations of
-// GUIDE_TRAIN_ROWS records (32 rows per wave).  Per iteration each wave runs the forward pass
+// Training step in one kernel: blocks of eight waves, persistent over iterations of
+// GUIDE_TRAIN_ROWS records (16 rows per wave).  Per iteration each wave runs the forward pass
 // of its rows with the activations in registers (as inference), the loss gradient at the
-// output, and the backward pass (the transposed product W^T delta^T with permuted W^T
-// fragments, ReLU masks kept as bits in the fragment order).  At every layer, from the last to
-// the first, the waves write the layer's input activations and output deltas of their rows to
-// LDS feature-major, and wave w adds the block's 128 rows to its share of the weight
-// gradient, output features 16 w .. 16 w + 15 of every layer, dW = delta^T a with K = rows on
-// the MFMA; the bias gradient sums the same delta fragments.  Each block keeps its gradient in
-// registers across iterations and writes it once as a partial, summed in block order by
-// k_guide_reduce: the result does not depend on scheduling.  Rows past the batch get weight 0
-// and so zero deltas.
-constexpr int TRAIN_WAVES = 4;
-constexpr int TRAIN_TILE = 32;                        // rows per wave per iteration
+// output (four lanes per record), and the backward pass (the transposed product W^T delta^T
+// with permuted W^T fragments, ReLU masks kept as bits in the fragment order).  At every
+// layer, from the last to the first, the waves write the layer's input activations and output
+// deltas of their rows to LDS feature-major, and each wave adds the block's 128 rows to its
+// share of the weight gradient (output block w >> 1, input blocks 2 (w & 1) and 2 (w & 1) + 1
+// of every layer), dW = delta^T a with K = rows on the MFMA; the bias gradient sums the same
+// delta fragments.  Each block keeps its gradient in registers across iterations and writes it
+// once as a partial, summed in block order by k_guide_reduce: the result does not depend on
+// scheduling.  Rows past the batch get weight 0 and so zero deltas.
+constexpr int TRAIN_WAVES = 8;
+constexpr int TRAIN_TILE = 16;                        // rows per wave per iteration
 constexpr int FM_PITCH = GUIDE_TRAIN_ROWS + 8;        // bf16 per feature of a feature-major tile
 static_assert(TRAIN_WAVES * TRAIN_TILE == GUIDE_TRAIN_ROWS, "an iteration is one tile per wave");
-static_assert(TRAIN_WAVES * 16 == GUIDE_W, "one 16-output block of the weight gradient per wave");
+static_assert(TRAIN_WAVES * 8 == GUIDE_W, "a 16 x 32 block of the weight gradient per wave");
 
 struct TrainLossLds {   // one wave's loss stage
     float out[TRAIN_TILE][GUIDE_OUT + 1];
@@ -376,33 +428,18 @@ union TrainLds {
     TrainGradLds grad;
 };
 
-// Layer-0 B fragments of one 16-row block: the one-blob encoding of raw features (row-major
-// float rows of GUIDE_RAW), as in the inference kernel.
-__device__ __forceinline__ void oneBlobFrags(const float (*raw)[GUIDE_RAW], int row, int g, bf16x8 *frag) {
-    float f[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) f[i] = raw[row][2 * i + (g >> 1)];
-#pragma unroll
-    for (int ks = 0; ks < 2; ks++)
-#pragma unroll
-        for (int p = 0; p < 8; p++) {
-            const int kf = kFeature(g, p, ks);
-            frag[ks][p] = (__bf16)oneBlob(f[(kf >> 3) >> 1], kf & 7);
-        }
-}
-
 __global__ __launch_bounds__(64 * TRAIN_WAVES) void k_guide_train(GuideNet net, const float4 *recs, const int *countPtr, const int *prefix,
                                                                   int recSeg, const float *wsumPtr, int base, int batch, float *partials,
                                                                   float *lossOut) {
     __shared__ bf16x8 sWf[GUIDE_LAYERS * 8 * 64];
     __shared__ bf16x8 sWb[(GUIDE_LAYERS - 1) * 8 * 64];   // layers 1 .. 4 (layer 0 passes no delta back)
     __shared__ float4 sB[GUIDE_LAYERS * GUIDE_W / 4];
-    __shared__ float sRaw[TRAIN_WAVES][TRAIN_TILE][GUIDE_RAW];
-    __shared__ float sRec[TRAIN_WAVES][TRAIN_TILE][5];
+    __shared__ float4 sRec[TRAIN_WAVES][TRAIN_TILE][GUIDE_TREC_F4];
     __shared__ float sLoss[TRAIN_WAVES];
     __shared__ __align__(16) TrainLds sx;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, lr = lane & 15;
+    const int gob = wave >> 1, gib = 2 * (wave & 1);   // this wave's block of the weight gradient
     for (int c = threadIdx.x; c < GUIDE_LAYERS * 8 * 64; c += blockDim.x) sWf[c] = ld8(net.w + 8 * (size_t)c);
     for (int c = threadIdx.x; c < (GUIDE_LAYERS - 1) * 8 * 64; c += blockDim.x) sWb[c] = ld8(net.wt + 8 * (size_t)(8 * 64 + c));
     for (int c = threadIdx.x; c < GUIDE_LAYERS * GUIDE_W / 4; c += blockDim.x) sB[c] = ((const float4 *)net.b)[c];
@@ -411,184 +448,175 @@ __global__ __launch_bounds__(64 * TRAIN_WAVES) void k_guide_train(GuideNet net, 
     const float wsum = wsumPtr ? *wsumPtr : 0.f;
     // normalise the weights to mean one over the record buffer
     const float wscale = (wsumPtr == nullptr) ? 1.f : (wsum > 0.f ? (float)count / wsum : 0.f);
-    f32x4 dW[GUIDE_LAYERS][4];
+    f32x4 dW[GUIDE_LAYERS][2];
     float db[GUIDE_LAYERS];
 #pragma unroll
     for (int L = 0; L < GUIDE_LAYERS; L++) {
         db[L] = 0.f;
-#pragma unroll
-        for (int ib = 0; ib < 4; ib++) dW[L][ib] = f32x4{0.f, 0.f, 0.f, 0.f};
+        dW[L][0] = dW[L][1] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
     float lossAcc = 0.f;
     const int iters = (batch + GUIDE_TRAIN_ROWS - 1) / GUIDE_TRAIN_ROWS;
     for (int it = blockIdx.x; it < iters; it += gridDim.x) {
-        const int rowBase = it * GUIDE_TRAIN_ROWS + wave * TRAIN_TILE;
-        if (lane < TRAIN_TILE) {
-            const int r = rowBase + lane;
-            float f[GUIDE_RAW], om[3] = {0.f, 0.f, 0.f}, pb = 0.f, wt = 0.f;
-            for (int k = 0; k < GUIDE_RAW; k++) f[k] = 0.f;
+        {
+            // the tile's records: lane (g, lr) loads float4 g of row lr
+            const int r = it * GUIDE_TRAIN_ROWS + wave * TRAIN_TILE + lr;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
             if (r < batch && count > 0) {
-                const int v = (int)(((int64_t)base + r) % count);
-                size_t idx = (size_t)v;
+                const int vi = (int)(((int64_t)base + r) % count);
+                size_t idx = (size_t)vi;
                 if (prefix) {
-                    // the last segment whose prefix does not exceed v holds record v
+                    // the last segment whose prefix does not exceed vi holds record vi
                     int lo = 0, hi = PT_QUEUE_SEGS;
                     while (hi - lo > 1) {
                         const int mid = (lo + hi) >> 1;
-                        if (prefix[mid] <= v) lo = mid;
+                        if (prefix[mid] <= vi) lo = mid;
                         else hi = mid;
                     }
-                    idx = (size_t)lo * recSeg + (v - prefix[lo]);
+                    idx = (size_t)lo * recSeg + (vi - prefix[lo]);
                 }
-                const float4 a = recs[idx * GUIDE_TREC_F4], b = recs[idx * GUIDE_TREC_F4 + 1];
-                const float4 c = recs[idx * GUIDE_TREC_F4 + 2], d = recs[idx * GUIDE_TREC_F4 + 3];
-                f[0] = a.x; f[1] = a.y; f[2] = a.z; f[3] = a.w;
-                f[4] = b.x; f[5] = b.y; f[6] = b.z; f[7] = b.w;
-                om[0] = c.x; om[1] = c.y; om[2] = c.z;
-                pb = c.w;
-                wt = fminf(d.w * wscale, 64.f);
+                v = recs[idx * GUIDE_TREC_F4 + g];
+                if (g == 3) v.w = fminf(v.w * wscale, 64.f);
             }
-            for (int k = 0; k < GUIDE_RAW; k++) sRaw[wave][lane][k] = f[k];
-            sRec[wave][lane][0] = om[0];
-            sRec[wave][lane][1] = om[1];
-            sRec[wave][lane][2] = om[2];
-            sRec[wave][lane][3] = pb;
-            sRec[wave][lane][4] = wt;
+            sRec[wave][lr][g] = v;
         }
         WAVE_SYNC();
-        // forward; act[L] = the input of layer L as B fragments [row block][K step]
-        bf16x8 act[GUIDE_LAYERS][2][2];
+        // forward; act[L] = the input of layer L as B fragments [K step]
+        bf16x8 act[GUIDE_LAYERS][2];
+        {
+            // one-blob encoding as in the inference kernel
+            const float4 a = sRec[wave][lr][0], b = sRec[wave][lr][1];
+            const bool hi = (g >> 1) != 0;
+            const float f[4] = {hi ? a.y : a.x, hi ? a.w : a.z, hi ? b.y : b.x, hi ? b.w : b.z};
 #pragma unroll
-        for (int rb = 0; rb < 2; rb++) oneBlobFrags(sRaw[wave], rb * 16 + lr, g, act[0][rb]);
-        uint32_t mask[GUIDE_LAYERS - 1];   // ReLU' of hidden layer L's outputs, bit (2 rb + ks) * 8 + p
+            for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+                for (int p = 0; p < 8; p++) {
+                    const int kf = kFeature(g, p, ks);
+                    act[0][ks][p] = (__bf16)oneBlob(f[(kf >> 3) >> 1], kf & 7);
+                }
+        }
+        uint32_t mask[GUIDE_LAYERS - 1];   // ReLU' of hidden layer L's outputs, bit ks * 8 + p
         TrainLossLds &ls = sx.loss[wave];
 #pragma unroll
         for (int L = 0; L < GUIDE_LAYERS; L++) {
             const bool last = L == GUIDE_LAYERS - 1;
-            f32x4 acc[2][4];
+            f32x4 acc[4];
 #pragma unroll
             for (int ob = 0; ob < 4; ob++) {
                 if (last && ob * 16 >= GUIDE_OUT) continue;
                 const float4 bias = sB[(L * GUIDE_W + ob * 16) / 4 + g];
                 const bf16x8 w0 = sWf[((L * 4 + ob) * 2) * 64 + lane], w1 = sWf[((L * 4 + ob) * 2 + 1) * 64 + lane];
-#pragma unroll
-                for (int rb = 0; rb < 2; rb++) {
-                    const f32x4 c = {bias.x, bias.y, bias.z, bias.w};
-                    acc[rb][ob] = mfma16(w1, act[L][rb][1], mfma16(w0, act[L][rb][0], c));
-                }
+                const f32x4 c = {bias.x, bias.y, bias.z, bias.w};
+                acc[ob] = mfma16(w1, act[L][1], mfma16(w0, act[L][0], c));
             }
             if (!last) {
                 uint32_t m = 0;
 #pragma unroll
-                for (int rb = 0; rb < 2; rb++)
+                for (int ks = 0; ks < 2; ks++)
 #pragma unroll
-                    for (int ks = 0; ks < 2; ks++)
-#pragma unroll
-                        for (int p = 0; p < 8; p++) {
-                            const __bf16 h = (__bf16)fmaxf(acc[rb][2 * ks + (p >> 2)][p & 3], 0.f);
-                            act[L + 1][rb][ks][p] = h;
-                            m |= ((float)h > 0.f ? 1u : 0u) << ((2 * rb + ks) * 8 + p);
-                        }
+                    for (int p = 0; p < 8; p++) {
+                        const __bf16 h = (__bf16)relu(acc[2 * ks + (p >> 2)][p & 3]);
+                        act[L + 1][ks][p] = h;
+                        m |= ((float)h > 0.f ? 1u : 0u) << (ks * 8 + p);
+                    }
                 mask[L] = m;
             } else {
 #pragma unroll
-                for (int rb = 0; rb < 2; rb++)
+                for (int ob = 0; ob * 16 < GUIDE_OUT; ob++)
 #pragma unroll
-                    for (int ob = 0; ob * 16 < GUIDE_OUT; ob++)
-#pragma unroll
-                        for (int r = 0; r < 4; r++) ls.out[rb * 16 + lr][ob * 16 + 4 * g + r] = acc[rb][ob][r];
+                    for (int r = 0; r < 4; r++) ls.out[lr][ob * 16 + 4 * g + r] = acc[ob][r];
             }
         }
         WAVE_SYNC();
-        if (lane < TRAIN_TILE) {
-            const f3 w = mk(sRec[wave][lane][0], sRec[wave][lane][1], sRec[wave][lane][2]);
-            lossAcc += lossGradient(ls.out[lane], w, sRec[wave][lane][3], sRec[wave][lane][4], ls.g[lane]);
+        {
+            const float4 c = sRec[wave][lr][2];
+            lossAcc += lossGradient4(ls.out[lr], mk(c.x, c.y, c.z), c.w, sRec[wave][lr][3].w, ls.g[lr], g);
         }
         WAVE_SYNC();
         // the output deltas as B fragments
         typedef __bf16 bf16x4 __attribute__((__vector_size__(4 * sizeof(__bf16))));
-        bf16x8 dl[2][2];
+        bf16x8 dl[2];
 #pragma unroll
-        for (int rb = 0; rb < 2; rb++)
-#pragma unroll
-            for (int ks = 0; ks < 2; ks++) {
-                const __bf16 *gr = ls.g[rb * 16 + lr] + 32 * ks + 4 * g;
-                const bf16x4 lo = *(const bf16x4 *)gr, hi = *(const bf16x4 *)(gr + 16);
-                dl[rb][ks] = bf16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
-            }
+        for (int ks = 0; ks < 2; ks++) {
+            const __bf16 *gr = ls.g[lr] + 32 * ks + 4 * g;
+            const bf16x4 lo = *(const bf16x4 *)gr, hi = *(const bf16x4 *)(gr + 16);
+            dl[ks] = bf16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+        }
         __syncthreads();   // the loss buffers become the gradient stage's tiles
         TrainGradLds &gs = sx.grad;
 #pragma unroll
         for (int L = GUIDE_LAYERS - 1; L >= 0; L--) {
 #pragma unroll
-            for (int rb = 0; rb < 2; rb++)
+            for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+                for (int p = 0; p < 8; p++) {
+                    const int kf = kFeature(g, p, ks), col = wave * TRAIN_TILE + lr;
+                    gs.a[kf][col] = act[L][ks][p];
+                    gs.d[kf][col] = dl[ks][p];
+                }
+            if (L > 0) {
+                // delta of layer L - 1's outputs: W_L^T delta_L, masked by ReLU'
+                f32x4 bacc[4];
+#pragma unroll
+                for (int ib = 0; ib < 4; ib++) {
+                    const bf16x8 w0 = sWb[(((L - 1) * 4 + ib) * 2) * 64 + lane], w1 = sWb[(((L - 1) * 4 + ib) * 2 + 1) * 64 + lane];
+                    const f32x4 z = {0.f, 0.f, 0.f, 0.f};
+                    bacc[ib] = mfma16(w1, dl[1], mfma16(w0, dl[0], z));
+                }
+                const uint32_t m = mask[L - 1];
 #pragma unroll
                 for (int ks = 0; ks < 2; ks++)
 #pragma unroll
                     for (int p = 0; p < 8; p++) {
-                        const int kf = kFeature(g, p, ks), col = wave * TRAIN_TILE + rb * 16 + lr;
-                        gs.a[kf][col] = act[L][rb][ks][p];
-                        gs.d[kf][col] = dl[rb][ks][p];
+                        const float v = bacc[2 * ks + (p >> 2)][p & 3];
+                        dl[ks][p] = (__bf16)((m >> (ks * 8 + p)) & 1u ? v : 0.f);
                     }
-            if (L > 0) {
-                // delta of layer L - 1's outputs: W_L^T delta_L, masked by ReLU'
-                f32x4 bacc[2][4];
-#pragma unroll
-                for (int ib = 0; ib < 4; ib++) {
-                    const bf16x8 w0 = sWb[(((L - 1) * 4 + ib) * 2) * 64 + lane], w1 = sWb[(((L - 1) * 4 + ib) * 2 + 1) * 64 + lane];
-#pragma unroll
-                    for (int rb = 0; rb < 2; rb++) {
-                        const f32x4 z = {0.f, 0.f, 0.f, 0.f};
-                        bacc[rb][ib] = mfma16(w1, dl[rb][1], mfma16(w0, dl[rb][0], z));
-                    }
-                }
-                const uint32_t m = mask[L - 1];
-#pragma unroll
-                for (int rb = 0; rb < 2; rb++)
-#pragma unroll
-                    for (int ks = 0; ks < 2; ks++)
-#pragma unroll
-                        for (int p = 0; p < 8; p++) {
-                            const float v = bacc[rb][2 * ks + (p >> 2)][p & 3];
-                            dl[rb][ks][p] = (__bf16)((m >> ((2 * rb + ks) * 8 + p)) & 1u ? v : 0.f);
-                        }
             }
             __syncthreads();
-            // this wave's output block of dW_L += delta_L^T a_L over the block's rows
+            // this wave's block of dW_L += delta_L^T a_L over the block's rows
 #pragma unroll
             for (int kk = 0; kk < GUIDE_TRAIN_ROWS / 32; kk++) {
-                const bf16x8 ad = *(const bf16x8 *)&gs.d[wave * 16 + lr][32 * kk + 8 * g];
+                const bf16x8 ad = *(const bf16x8 *)&gs.d[gob * 16 + lr][32 * kk + 8 * g];
+                if (gib == 0) {
 #pragma unroll
-                for (int i = 0; i < 8; i++) db[L] += (float)ad[i];
+                    for (int i = 0; i < 8; i++) db[L] += (float)ad[i];
+                }
 #pragma unroll
-                for (int ib = 0; ib < 4; ib++) {
-                    const bf16x8 ba = *(const bf16x8 *)&gs.a[ib * 16 + lr][32 * kk + 8 * g];
-                    dW[L][ib] = mfma16(ad, ba, dW[L][ib]);
+                for (int j = 0; j < 2; j++) {
+                    const bf16x8 ba = *(const bf16x8 *)&gs.a[(gib + j) * 16 + lr][32 * kk + 8 * g];
+                    dW[L][j] = mfma16(ad, ba, dW[L][j]);
                 }
             }
             __syncthreads();
         }
     }
-    // the block's partial: out = 16 wave + 4 g + r, in = 16 ib + lr
+    // the block's partial: out = 16 gob + 4 g + r, in = 16 (gib + j) + lr
     float *P = partials + (size_t)blockIdx.x * GUIDE_PARAMS;
 #pragma unroll
     for (int L = 0; L < GUIDE_LAYERS; L++) {
         float *PL = P + (size_t)L * GUIDE_LAYER_PARAMS;
 #pragma unroll
-        for (int ib = 0; ib < 4; ib++)
+        for (int j = 0; j < 2; j++)
 #pragma unroll
-            for (int r = 0; r < 4; r++) PL[(wave * 16 + 4 * g + r) * GUIDE_W + ib * 16 + lr] = dW[L][ib][r];
-        // lanes lr, lr + 16, lr + 32, lr + 48 summed complementary rows of output 16 wave + lr
-        float v = db[L];
-        v += __shfl_xor(v, 16);
-        v += __shfl_xor(v, 32);
-        if (g == 0) PL[GUIDE_W * GUIDE_W + wave * 16 + lr] = v;
+            for (int r = 0; r < 4; r++) PL[(gob * 16 + 4 * g + r) * GUIDE_W + (gib + j) * 16 + lr] = dW[L][j][r];
+        if (gib == 0) {
+            // lanes lr, lr + 16, lr + 32, lr + 48 summed complementary rows of output 16 gob + lr
+            float v = db[L];
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            if (g == 0) PL[GUIDE_W * GUIDE_W + gob * 16 + lr] = v;
+        }
     }
-    // loss of this block: lanes 0 .. 31 of each wave hold partial sums, added in wave order
+    // loss of this block: quarter 0 of each wave holds the partial sums, added in wave order
     for (int off = 32; off >= 1; off >>= 1) lossAcc += __shfl_down(lossAcc, off);
     if (lane == 0) sLoss[wave] = lossAcc;
     __syncthreads();
-    if (threadIdx.x == 0 && lossOut) lossOut[blockIdx.x] = (sLoss[0] + sLoss[1]) + (sLoss[2] + sLoss[3]);
+    if (threadIdx.x == 0 && lossOut) {
+        float t = 0.f;
+        for (int w = 0; w < TRAIN_WAVES; w++) t += sLoss[w];
+        lossOut[blockIdx.x] = t;
+    }
 }
 
 // --------------------------------------------------------------------------------------------
@@ -619,39 +647,41 @@ __global__ void k_guide_pack(const float *params, __bf16 *w, __bf16 *wt, float *
     storeParam(p, liveParam(p) ? params[p] : 0.f, w, wt, b);
 }
 
-// Sum of the per-slice partial gradients: 64 parameters per block, four slices of the
-// partials per parameter (k = slice, slice + 4, ...), the slices added in a fixed order, so
-// the result does not depend on scheduling.  The last block sums the per-block losses the
-// same way (256 fixed strided partial sums, then a fixed tree).
-__global__ __launch_bounds__(256) void k_guide_reduce(const float *partials, int nblocks, const float *lossPart, int nloss, float invBatch,
-                                                      float *grad, float *lossOut) {
-    __shared__ float part[4][64];
-    __shared__ float lsum[256];
+// Sum of the per-block partial gradients: 64 parameters per block, REDUCE_SLICES slices of the
+// partials per parameter (k = slice, slice + REDUCE_SLICES, ...), the slices added in a fixed
+// tree, so the result does not depend on scheduling.  The last block sums the per-block losses
+// the same way (fixed strided partial sums, then a fixed tree).
+constexpr int REDUCE_SLICES = 16;
+__global__ __launch_bounds__(64 * REDUCE_SLICES) void k_guide_reduce(const float *partials, int nblocks, const float *lossPart, int nloss,
+                                                                    float invBatch, float *grad, float *lossOut) {
+    __shared__ float part[REDUCE_SLICES][64];
+    __shared__ float lsum[64 * REDUCE_SLICES];
+    const int t = threadIdx.x;
     if (blockIdx.x == gridDim.x - 1) {
         if (!lossOut) return;
         float l = 0.f;
-        for (int k = threadIdx.x; k < nloss; k += 256) l += lossPart[k];
-        lsum[threadIdx.x] = l;
+        for (int k = t; k < nloss; k += 64 * REDUCE_SLICES) l += lossPart[k];
+        lsum[t] = l;
         __syncthreads();
-        for (int off = 128; off >= 1; off >>= 1) {
-            if ((int)threadIdx.x < off) lsum[threadIdx.x] += lsum[threadIdx.x + off];
+        for (int off = 32 * REDUCE_SLICES; off >= 1; off >>= 1) {
+            if (t < off) lsum[t] += lsum[t + off];
             __syncthreads();
         }
-        if (threadIdx.x == 0) lossOut[0] = lsum[0] * invBatch;
+        if (t == 0) lossOut[0] = lsum[0] * invBatch;
         return;
     }
-    const int p = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int slice = threadIdx.x >> 6;
+    const int p = blockIdx.x * 64 + (t & 63);
+    const int slice = t >> 6;
     float g = 0.f;
     if (p < GUIDE_PARAMS)
-        for (int k = slice; k < nblocks; k += 4) g += partials[(size_t)k * GUIDE_PARAMS + p];
-    part[slice][threadIdx.x & 63] = g;
+        for (int k = slice; k < nblocks; k += REDUCE_SLICES) g += partials[(size_t)k * GUIDE_PARAMS + p];
+    part[slice][t & 63] = g;
     __syncthreads();
-    if (slice == 0 && p < GUIDE_PARAMS) {
-        const int j = threadIdx.x;
-        const float t = (part[0][j] + part[1][j]) + (part[2][j] + part[3][j]);
-        grad[p] = liveParam(p) ? t * invBatch : 0.f;
+    for (int half = REDUCE_SLICES / 2; half >= 1; half >>= 1) {
+        if (slice < half) part[slice][t & 63] += part[slice + half][t & 63];
+        __syncthreads();
     }
+    if (slice == 0 && p < GUIDE_PARAMS) grad[p] = liveParam(p) ? part[0][t] * invBatch : 0.f;
 }
 
 __global__ void k_guide_adam(float *params, float *m, float *v, const float *grad, float c1, float c2, float lr, __bf16 *w, __bf16 *wt,
@@ -755,8 +785,8 @@ void guideInferRaw(GuideNet net, const float *feats, int n, float *rawOut, hipSt
 void guideGradients(GuideNet net, const float4 *records, const int *count, const int *prefix, int recSeg, int base, int batch,
                     const float *wsum, const GuideTrainBufs &tb, float *grad, float *lossOut, hipStream_t s) {
     k_guide_train<<<tb.blocks, 64 * TRAIN_WAVES, 0, s>>>(net, records, count, prefix, recSeg, wsum, base, batch, tb.partials, tb.lossPart);
-    k_guide_reduce<<<(GUIDE_PARAMS + 63) / 64 + 1, 256, 0, s>>>(tb.partials, tb.blocks, tb.lossPart, tb.blocks, 1.f / (float)std::max(batch, 1),
-                                                               grad, lossOut);
+    k_guide_reduce<<<(GUIDE_PARAMS + 63) / 64 + 1, 64 * REDUCE_SLICES, 0, s>>>(tb.partials, tb.blocks, tb.lossPart, tb.blocks,
+                                                                              1.f / (float)std::max(batch, 1), grad, lossOut);
     PT_HIP_TRY(hipGetLastError());
 }
 
