@@ -356,6 +356,18 @@ def main():
             "units": k["units"], "bytes_per_launch": k["bytes"] / k["launches"],
             "gbs": (k["bytes"] / (k["ms"] * 1e-3) / 1e9) if k["ms"] > 0 else 0.0,
         }
+    # traversal statistics: one more pass, untimed, with the instrumented trace kernels
+    r.set_option(P.OPT_PROFILE_KERNELS, 0)
+    r.set_option(P.OPT_TRACE_STATS, 1)
+    r.reset_stats()
+    r.render(SPP)
+    ts = r.stats()
+    r.set_option(P.OPT_TRACE_STATS, 0)
+    traversal = {}
+    for q, name in ((0, "closest_hit"), (1, "shadow")):
+        n = max(1, ts.trace_rays[q])
+        traversal[name] = {"rays": ts.trace_rays[q], "node_visits_per_ray": round(ts.trace_nodes[q] / n, 2),
+                           "prim_tests_per_ray": round(ts.trace_prims[q] / n, 2)}
     dom = max(kernels, key=lambda n: kernels[n]["total_ms"])
     d = kernels[dom]
     roof = {
@@ -381,6 +393,7 @@ def main():
                    "scene_upload_s": round(build_s, 3), "parallelism": "1 GPU"},
         "roofline": roof,
         "queue_roofline": queue_roof,
+        "traversal": traversal,
         "kernels": kernels,
     }
     if args.cpu_budget > 0:

@@ -50,7 +50,15 @@ static constexpr int kWsumSlot = kSegCounters + 3;
 static constexpr int kRecPrefixSlot = kSegCounters + 8;
 static constexpr int kNumCounters = kRecPrefixSlot + PT_QUEUE_SEGS + 8;
 // device statistics (uint64): per bounce rays, shadows, shade adds, shadow adds; totals
-enum { S_RAYS = 0, S_SHADOW = kMaxBounces, S_LADD_SHADE = 2 * kMaxBounces, S_LADD_SHADOW = 3 * kMaxBounces, S_PATHS = 4 * kMaxBounces, S_COUNT };
+enum {
+    S_RAYS = 0,
+    S_SHADOW = kMaxBounces,
+    S_LADD_SHADE = 2 * kMaxBounces,
+    S_LADD_SHADOW = 3 * kMaxBounces,
+    S_PATHS = 4 * kMaxBounces,
+    S_TRACE = 5 * kMaxBounces,   // traversal counts (PT_OPT_TRACE_STATS): rays, nodes, primitives; closest hit, then shadow
+    S_COUNT = S_TRACE + 6
+};
 
 static constexpr int kTraceThreads = 128;
 // Default path-pool capacity: 2^26 paths (about 13 GB of SoA path state), a whole 1024^2 x
@@ -430,8 +438,13 @@ static constexpr int kTraceDone = INT32_MIN;   // LaneRay::node once the stack i
 // 12 -> 4420, 16 -> 4408)
 static constexpr int kDescentTail = 12;
 
-template <bool ANY, int STACK>
-__device__ __forceinline__ bool traceRound(const PtSceneView &sc, LaneRay &r, const TraceStack &ts, int *overflow) {
+// Per-lane traversal counts of the instrumented trace kernels (STATS).
+struct TraceCount {
+    int rays = 0, nodes = 0, prims = 0;
+};
+
+template <bool ANY, int STACK, bool STATS = false>
+__device__ __forceinline__ bool traceRound(const PtSceneView &sc, LaneRay &r, const TraceStack &ts, int *overflow, TraceCount *tc = nullptr) {
     const int stride = blockDim.x;
     // general push (LDS, then the spill area); the common closest-hit case writes LDS directly
     auto push = [&](int ref) {
@@ -467,6 +480,7 @@ __device__ __forceinline__ bool traceRound(const PtSceneView &sc, LaneRay &r, co
         if (searching == 0) break;
         if (__popcll(searching) <= kDescentTail && __ballot(leaf != 0) != 0) break;
         if (r.node >= 0) {
+            if constexpr (STATS) tc->nodes++;
             // 32-bit offsets from the array base (a node array stays below 4 GiB); a node is
             // 128 bytes, so the near bound's offset is an OR and the far bound of the same axis
             // (the other float4 of the pair) is that offset XOR 16
@@ -531,6 +545,7 @@ __device__ __forceinline__ bool traceRound(const PtSceneView &sc, LaneRay &r, co
             // all three records of the slot (spheres have three too) are fetched before the
             // type test, so the loads overlap
             const size_t k = 3 * (size_t)(first + j);
+            if constexpr (STATS) tc->prims++;
             const float4 p0 = sc.prims[k];
             float4 p1 = sc.prims[k + 1], p2 = sc.prims[k + 2];
             // keeps the triangle-only components' loads ahead of the type test (the compiler
@@ -608,9 +623,10 @@ __device__ __forceinline__ int segClaim(int *counters, int bounce, int countFiel
 // rays once `refill` or more of its lanes are idle, as a contiguous run of one segment so that
 // the rays in flight on a CU stay neighbours, and leaves once its segments are exhausted and
 // its lanes idle.
-template <bool ANY, int STACK, class Load, class Finish>
+template <bool ANY, int STACK, bool STATS = false, class Load, class Finish>
 __device__ __forceinline__ void traceQueue(const PtSceneView &sc, int *counters, int bounce, int countField, int fetchField, int segLen,
-                                           int refill, const TraceStack &ts, int *overflow, Load load, Finish finish) {
+                                           int refill, const TraceStack &ts, int *overflow, Load load, Finish finish,
+                                           unsigned long long *tstat = nullptr) {
     SegCursor cur = segStart();
     if (sc.numNodes == 0) {
         TraceHit th;
@@ -622,6 +638,7 @@ __device__ __forceinline__ void traceQueue(const PtSceneView &sc, int *counters,
     }
     LaneRay r;
     int ray = -1;
+    TraceCount tc;
     while (true) {
         const uint64_t idle = __ballot(ray < 0);
         if (cur.left > 0 && __popcll(idle) >= refill) {
@@ -636,14 +653,23 @@ __device__ __forceinline__ void traceQueue(const PtSceneView &sc, int *counters,
                     int root = 0;
                     load(ray, o, d, tmin, tmax, root);
                     laneRayStart(r, o, d, tmin, tmax, root);
+                    if constexpr (STATS) tc.rays++;
                 }
             }
         }
         // all lanes idle after a refill attempt means this wave's segments are exhausted
         if (__ballot(ray >= 0) == 0) break;
-        if (ray >= 0 && traceRound<ANY, STACK>(sc, r, ts, overflow)) {
+        if (ray >= 0 && traceRound<ANY, STACK, STATS>(sc, r, ts, overflow, &tc)) {
             finish(ray, r.prim >= 0, laneRayHit(r));
             ray = -1;
+        }
+    }
+    if constexpr (STATS) {
+        int v[3] = {tc.rays, tc.nodes, tc.prims};
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            for (int off = 32; off >= 1; off >>= 1) v[k] += __shfl_xor(v[k], off);
+            if (laneId() == 0) atomicAdd(&tstat[k], (unsigned long long)v[k]);
         }
     }
 }
@@ -652,14 +678,16 @@ __device__ __forceinline__ void traceQueue(const PtSceneView &sc, int *counters,
 // camO (camera rays, tmin 0) and rayO is not read.  BATCH (a batch wavefront): the traversal
 // starts at the root node of the ray's render; at bounce 0 the render comes from the path
 // index (chunk groups) and the origin is its camera's, later rayO.w holds the render.
-template <int STACK, bool BATCH = false>
+// STATS: the instrumented build that adds its rays, node visits and primitive tests to
+// tstat[0..2].
+template <int STACK, bool BATCH = false, bool STATS = false>
 __global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 7 : 5) void k_extend(PtSceneView sc, int *counters, int bounce, int segLen,
                                                                               const float4 *rayO, const float4 *rayD, float4 camO, float4 *hits,
                                                                               int refill, int *spill, int spillCap, int *overflow,
-                                                                              ChunkParams cp = ChunkParams{}) {
+                                                                              ChunkParams cp = ChunkParams{}, unsigned long long *tstat = nullptr) {
     __shared__ int sstack[STACK * kTraceThreads];
     const TraceStack ts = traceStack(sstack, spill, spillCap);
-    traceQueue<false, STACK>(
+    traceQueue<false, STACK, STATS>(
         sc, counters, bounce, PT_Q_RAYS, PT_Q_FETCH_EXT, segLen, refill, ts, overflow,
         [&](int i, f3 &o, f3 &d, float &tmin, float &tmax, int &root) {
             const float4 rd = rayD[i];
@@ -688,24 +716,24 @@ __global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 7 : 5) void k_extend(P
             d = xyz(rd);
             tmax = rd.w;
         },
-        [&](int i, bool, const TraceHit &th) { hits[i] = make_float4(th.t, th.u, th.v, asFloat(th.prim)); });
+        [&](int i, bool, const TraceHit &th) { hits[i] = make_float4(th.t, th.u, th.v, asFloat(th.prim)); }, tstat);
 }
 
 // Persistent any-hit kernel for the shadow rays of `bounce` (interval [1e-4, tmax]); an
 // unoccluded ray adds its light contribution to its path.
 // BATCH: sL.w holds the ray's render, whose root node starts the traversal.
-template <int STACK, bool BATCH = false>
+template <int STACK, bool BATCH = false, bool STATS = false>
 __global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 7 : 5) void k_shadow(PtSceneView sc, int *counters, int bounce, int segLen,
                                                                               const float4 *sO, const float4 *sD, const float4 *sL, float4 *Lpath,
                                                                               int refill, int *spill, int spillCap, int *overflow,
-                                                                              const BatchRender *renders = nullptr) {
+                                                                              const BatchRender *renders = nullptr, unsigned long long *tstat = nullptr) {
     __shared__ int sstack[STACK * kTraceThreads];
     // each lane's path index, kept from the ray's load (a re-read after the traversal would be a
     // dependent round trip ahead of the radiance update)
     __shared__ int sPath[kTraceThreads];
     const TraceStack ts = traceStack(sstack, spill, spillCap);
     int adds = 0;
-    traceQueue<true, STACK>(
+    traceQueue<true, STACK, STATS>(
         sc, counters, bounce, PT_Q_SHADOW, PT_Q_FETCH_SHADOW, segLen, refill, ts, overflow,
         [&](int i, f3 &o, f3 &d, float &tmin, float &tmax, int &root) {
             const float4 so = sO[i], sd = sD[i];
@@ -725,7 +753,8 @@ __global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 7 : 5) void k_shadow(P
                 Lpath[pid] = acc;
                 adds++;
             }
-        });
+        },
+        tstat);
     waveSum(adds, &counters[ptQueueField(homeSegment(), bounce, PT_Q_LADD_SHADOW)]);
 }
 
@@ -1395,6 +1424,7 @@ struct pt_renderer {
     int stackNeed = 1;    // deepest traversal stack the tree can need (3 per 4-wide level)
     int ldsStack = 16;    // stack entries per lane kept in LDS; the rest spill to the pool's `spill`
     int refillLanes = 0;  // idle lanes of a trace wave that trigger taking new rays (0: per launch)
+    bool traceStats = false;   // PT_OPT_TRACE_STATS: instrumented trace kernels
     bool hasScene = false;
     // host copies needed for edits
     std::vector<HostMaterial> mats;
@@ -1600,6 +1630,14 @@ void launchExtend(pt_renderer *r, int bounce, int segLen, const float4 *ro, cons
         const ChunkParams &cp = *batchChunk;
         if (r->ldsStack == 32) k_extend<32, true><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, ro, rd, camO, hits, refill, sp, cap, ov, cp);
         else k_extend<16, true><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, ro, rd, camO, hits, refill, sp, cap, ov, cp);
+    } else if (r->traceStats) {
+        unsigned long long *ts = r->stats.as<unsigned long long>() + S_TRACE;
+        if (r->ldsStack == 32)
+            k_extend<32, false, true><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, ro, rd, camO, hits, refill, sp, cap, ov,
+                                                                            ChunkParams{}, ts);
+        else
+            k_extend<16, false, true><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, ro, rd, camO, hits, refill, sp, cap, ov,
+                                                                            ChunkParams{}, ts);
     } else {
         if (r->ldsStack == 32) k_extend<32><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, ro, rd, camO, hits, refill, sp, cap, ov);
         else k_extend<16><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, ro, rd, camO, hits, refill, sp, cap, ov);
@@ -1619,6 +1657,12 @@ void launchShadow(pt_renderer *r, int bounce, int segLen) {
     if (br) {
         if (r->ldsStack == 32) k_shadow<32, true><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, o, d, l, L, refill, sp, cap, ov, br);
         else k_shadow<16, true><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, o, d, l, L, refill, sp, cap, ov, br);
+    } else if (r->traceStats) {
+        unsigned long long *ts = r->stats.as<unsigned long long>() + S_TRACE + 3;
+        if (r->ldsStack == 32)
+            k_shadow<32, false, true><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, o, d, l, L, refill, sp, cap, ov, nullptr, ts);
+        else
+            k_shadow<16, false, true><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, o, d, l, L, refill, sp, cap, ov, nullptr, ts);
     } else {
         if (r->ldsStack == 32) k_shadow<32><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, o, d, l, L, refill, sp, cap, ov);
         else k_shadow<16><<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, o, d, l, L, refill, sp, cap, ov);
@@ -2110,6 +2154,7 @@ int pt_renderer_set_option(pt_renderer *r, int opt, int64_t v) {
         break;
     case PT_OPT_SHADE_BLOCKS_PER_CU: r->shadeBlocksPerCU = v > 0 ? (int)std::min<int64_t>(v, 64) : 8; break;
     case PT_OPT_PROFILE_KERNELS: r->profile = v != 0; break;
+    case PT_OPT_TRACE_STATS: r->traceStats = v != 0; break;
     case PT_OPT_CONSISTENT_MIS: r->misSample = v != 0; r->resetPending = true; break;
     case PT_OPT_GUIDE_TRAIN_STEPS: r->gTrainSteps = v < 0 ? 16 : (int)std::min<int64_t>(v, 1024); break;
     case PT_OPT_GUIDE_BOUNCES: r->gBounces = v <= 0 ? 0 : (int)std::min<int64_t>(v, kMaxBounces); break;
@@ -2152,6 +2197,7 @@ int pt_renderer_get_option(pt_renderer *r, int opt, int64_t *v) {
     case PT_OPT_TRACE_REFILL_LANES: *v = r->refillLanes; break;
     case PT_OPT_SHADE_BLOCKS_PER_CU: *v = r->shadeBlocksPerCU; break;
     case PT_OPT_PROFILE_KERNELS: *v = r->profile ? 1 : 0; break;
+    case PT_OPT_TRACE_STATS: *v = r->traceStats ? 1 : 0; break;
     case PT_OPT_CONSISTENT_MIS: *v = r->misSample; break;
     case PT_OPT_GUIDE_TRAIN_STEPS: *v = r->gTrainSteps; break;
     case PT_OPT_GUIDE_BOUNCES: *v = r->gBounces; break;
@@ -2362,6 +2408,11 @@ int pt_renderer_get_stats(pt_renderer *r, pt_render_stats *out) {
         }
     }
     out->rays_total = out->extension_rays + out->shadow_rays;
+    for (int q = 0; q < 2; q++) {
+        out->trace_rays[q] = (int64_t)s[S_TRACE + 3 * q];
+        out->trace_nodes[q] = (int64_t)s[S_TRACE + 3 * q + 1];
+        out->trace_prims[q] = (int64_t)s[S_TRACE + 3 * q + 2];
+    }
     out->paths = (int64_t)s[S_PATHS];
     out->passes = r->passes;
     out->gpu_ms = r->gpuMs;

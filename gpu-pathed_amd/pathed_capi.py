@@ -18,7 +18,7 @@ RENDERER_CUDA, RENDERER_OPTIX, RENDERER_NORMALS, RENDERER_GUIDED = 0, 1, 2, 3
 OPT_POOL_PATHS, OPT_TRACE_BLOCKS_PER_CU, OPT_SHADE_BLOCKS_PER_CU, OPT_PROFILE_KERNELS = 0, 1, 2, 3
 OPT_CONSISTENT_MIS, OPT_GUIDE_TRAIN_STEPS, OPT_GUIDE_BATCH, OPT_GUIDE_LR_MICRO, OPT_GUIDE_RESET = 4, 5, 6, 7, 8
 OPT_TRACE_LDS_STACK, OPT_BATCH_GRAPH, OPT_GUIDE_MAX_FRACTION_MILLI, OPT_TRACE_REFILL_LANES, OPT_GUIDE_BOUNCES = 9, 10, 11, 12, 13
-OPT_BVH_BUILDER, OPT_BATCH_WAVEFRONT = 14, 15
+OPT_BVH_BUILDER, OPT_BATCH_WAVEFRONT, OPT_TRACE_STATS = 14, 15, 16
 BVH_LBVH, BVH_PLOC = 1, 2
 MISS = np.iinfo(np.int32).min
 
@@ -65,6 +65,7 @@ class RenderStats(C.Structure):
         ("passes", C.c_int64), ("gpu_ms", C.c_double), ("bvh_depth", C.c_int), ("bvh_nodes", C.c_int64),
         ("stack_overflows", C.c_int), ("guide_steps", C.c_int64), ("guide_loss", C.c_double),
         ("bounce_rays", C.c_int64 * 16), ("bounce_shadow", C.c_int64 * 16),
+        ("trace_rays", C.c_int64 * 2), ("trace_nodes", C.c_int64 * 2), ("trace_prims", C.c_int64 * 2),
     ]
 
 

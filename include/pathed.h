@@ -185,10 +185,14 @@ typedef enum {
     PT_OPT_BVH_BUILDER = 14,     /* binary tree under the 4-wide BVH, used from the next
                                     pt_renderer_set_scene: 1 = LBVH (Karras radix tree),
                                     2 = PLOC (locally-ordered clustering); 0 = the default */
-    PT_OPT_BATCH_WAVEFRONT = 15  /* batches only: 1 (default) = one wavefront over every render
+    PT_OPT_BATCH_WAVEFRONT = 15, /* batches only: 1 (default) = one wavefront over every render
                                     (paths of different renders share the queues; chunks hold
-                                    sample rows of several renders); 0 = each render's own
-                                    chunks, one render after the other */
+                                    sample rows of several renders, a chunk of one render runs
+                                    that render's own kernels); 0 = each render's own chunks,
+                                    one render after the other */
+    PT_OPT_TRACE_STATS = 16      /* 1: instrumented trace kernels count rays, BVH node visits
+                                    and primitive tests (pt_render_stats.trace_*); slower, for
+                                    measurement only.  Images are unchanged. */
 } pt_option;
 int pt_renderer_set_option(pt_renderer *r, int option, int64_t value);
 int pt_renderer_get_option(pt_renderer *r, int option, int64_t *value);
@@ -234,6 +238,11 @@ typedef struct {
     double guide_loss;         /* guided: mean loss of the last training step */
     int64_t bounce_rays[16];   /* extension rays traced at bounce 0..15 (0 = camera rays) */
     int64_t bounce_shadow[16]; /* shadow rays traced at bounce 0..15 */
+    /* With PT_OPT_TRACE_STATS on: rays traced, BVH node visits and primitive tests of the
+     * closest-hit ([0]) and shadow ([1]) kernels (0 otherwise). */
+    int64_t trace_rays[2];
+    int64_t trace_nodes[2];
+    int64_t trace_prims[2];
 } pt_render_stats;
 int pt_renderer_get_stats(pt_renderer *r, pt_render_stats *stats);
 int pt_renderer_reset_stats(pt_renderer *r);

@@ -92,3 +92,29 @@ def test_queue_statistics_add_up():
     assert all(list(st.bounce_shadow)[b] <= rays[b] for b in range(6))
     assert st.rays_total == sum(rays) + sum(list(st.bounce_shadow)[:6])
     r.close()
+
+
+def test_trace_statistics_leave_images_unchanged():
+    """The instrumented trace kernels (PT_OPT_TRACE_STATS) count each traced ray once, visit at
+    least the root node per ray and test primitives; the image is bit-identical to the one the
+    uninstrumented kernels make."""
+    say("case: traversal statistics")
+    s = P.Scene.preset(1, seed=3, param=4)  # displaced icosphere, 5120 triangles
+    imgs, sts = [], []
+    for on in (0, 1):
+        r = P.Renderer(64, 48)
+        r.set_scene(s)
+        r.set_max_depth(6)
+        r.set_option(P.OPT_TRACE_STATS, on)
+        assert r.get_option(P.OPT_TRACE_STATS) == on
+        r.render(4)
+        imgs.append(r.radiance().copy())
+        sts.append(r.stats())
+        r.close()
+    off, on = sts
+    say(f"rays {list(on.trace_rays)}, node visits {list(on.trace_nodes)}, primitive tests {list(on.trace_prims)}")
+    assert np.array_equal(imgs[0], imgs[1])
+    assert list(off.trace_rays) == [0, 0] and list(off.trace_nodes) == [0, 0]
+    assert on.trace_rays[0] == on.extension_rays and on.trace_rays[1] == on.shadow_rays
+    assert on.trace_nodes[0] >= on.trace_rays[0] and on.trace_nodes[1] >= on.trace_rays[1]
+    assert on.trace_prims[0] > 0 and on.trace_prims[1] > 0
