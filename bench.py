@@ -12,7 +12,8 @@ the wall time of the timed steps), with the roofline of the dominant kernel meas
 from HIP events on the renderer's stream, the CPU oracle timed on a bounded sample of the
 same workload on the host cores, and (mse_ratio) the other half of the headline metric: the
 equal-time mean squared error of the plain and the guided renderer on the hard-indirect room
-against the FP64 oracle's 65,536-sample reference, as their ratio.
+against the FP64 oracle's 65,536-sample reference, as their ratio; (batch) the 48-render
+batch's aggregate Mrays/s and per-render latency.
 """
 import argparse
 import json
@@ -281,6 +282,8 @@ def main():
                     help="default workload: seconds per renderer of the equal-time guided-vs-plain room measurement "
                          "reported as mse_ratio (0: skip)")
     ap.add_argument("--room-repeats", type=int, default=2, help="default workload: seeds per renderer for mse_ratio")
+    ap.add_argument("--batch-steps", type=int, default=2,
+                    help="default workload: timed runs of the 48-render batch reported as batch (0: skip)")
     ap.add_argument("--level", type=int, default=8, help="icosphere subdivision level")
     ap.add_argument("--pool", type=int, default=0, help="path pool size (0: library default)")
     ap.add_argument("--bvh", choices=["default", "lbvh", "ploc"], default="default",
@@ -372,6 +375,9 @@ def main():
     d = kernels[dom]
     roof = {
         "kernel": dom, "bound": "hbm", "achieved": round(d["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        # the traversal kernels are bound by neither: they issue VALU instructions most of their
+        # time over divergent lanes (rocprofv3 counters in profiles/)
+        "limiter": "VALU issue over divergent lanes (node and primitive tests), not HBM or MFMA",
         "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
         "bytes_per_launch": d["bytes_per_launch"], "avg_launch_us": d["avg_us"],
     }
@@ -401,6 +407,9 @@ def main():
     r.close()
     if args.room_budget > 0:
         out["mse_ratio"] = room_summary(room_equal_time(P, args.room_budget, 1, args.room_repeats))
+    if args.batch_steps > 0:
+        b = batch_renders(P, args.batch_steps, args.probe_divisor)
+        out["batch"] = {k: b[k] for k in ("metric", "value", "unit", "ms_per_step", "steps", "latency_ms", "config", "autotune")}
     print(json.dumps(out), flush=True)
 
 

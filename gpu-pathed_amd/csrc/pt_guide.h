@@ -28,9 +28,97 @@ constexpr int GO_FRAC = 40;    // guide selection probability (logit)
 constexpr float GUIDE_LOGK_MIN = -3.f, GUIDE_LOGK_MAX = 9.f;
 constexpr float GUIDE_FRAC_MIN = 0.02f, GUIDE_FRAC_MAX = 0.98f;
 
-// Mixture record written by the inference kernel, 12 float4 per path vertex:
-//   [0..7] (mu.xyz, kappa) per lobe, [8] pi 0..3, [9] pi 4..7, [10] (selection prob, eligible, 0, 0)
-constexpr int GUIDE_REC_F4 = 12;
+// The last layer's outputs in the network kernels' order.  Output position op = 16 ob + 4 g
+// + r is the C-fragment element r of output block ob in lane group g, so the four lanes that
+// hold a row's outputs hold two lobes each: slot s = 4 ob + r of group g is the logit of lobe
+// 2 g + s (s < 2), a mean coordinate of lobe 2 g + (s - 2) / 3 (s < 8), the log concentration
+// of lobe 2 g + s - 8 (s < 10), and s = 10, 11 hold the selection logit (group 3, s = 10) and
+// the seven computed outputs the loss does not use (41 .. 47).  Positions 48 .. 63 (block 3,
+// never computed) map to themselves.  guideOutOf and guideOpOf are inverse permutations of
+// 0 .. 63.
+__host__ __device__ __forceinline__ int guideOutOf(int op) {
+    if (op >= GUIDE_OUT) return op;
+    const int g = (op >> 2) & 3, s = 4 * (op >> 4) + (op & 3);
+    if (s < 2) return GO_LOGIT + 2 * g + s;
+    if (s < 8) return GO_MEAN + 3 * (2 * g + (s - 2) / 3) + (s - 2) % 3;
+    if (s < 10) return GO_KAPPA + 2 * g + (s - 8);
+    if (g == 3) return s == 10 ? GO_FRAC : GUIDE_OUT - 1;
+    return GO_FRAC + 1 + 2 * g + (s - 10);
+}
+__host__ __device__ __forceinline__ int guideOpOf(int o) {
+    if (o >= GUIDE_OUT) return o;
+    int g, s;
+    if (o < GO_MEAN) {
+        g = (o - GO_LOGIT) >> 1;
+        s = (o - GO_LOGIT) & 1;
+    } else if (o < GO_KAPPA) {
+        const int k = (o - GO_MEAN) / 3;
+        g = k >> 1;
+        s = 2 + 3 * (k & 1) + (o - GO_MEAN) % 3;
+    } else if (o < GO_FRAC) {
+        g = (o - GO_KAPPA) >> 1;
+        s = 8 + ((o - GO_KAPPA) & 1);
+    } else if (o == GO_FRAC) {
+        g = 3;
+        s = 10;
+    } else if (o == GUIDE_OUT - 1) {
+        g = 3;
+        s = 11;
+    } else {
+        g = (o - GO_FRAC - 1) >> 1;
+        s = 10 + ((o - GO_FRAC - 1) & 1);
+    }
+    return 16 * (s >> 2) + 4 * g + (s & 3);
+}
+
+// Mixture record written by the inference kernel, 64 bytes (16 words) per path vertex:
+// lobe k in words 2k (its unit mean, octahedral, two unorm16) and 2k + 1 (kappa as FP16 in
+// the low half; the lobe's weight as FP16 in the high half for k < 7, the selection
+// probability for k = 7, whose weight is 1 minus the others').  A vertex that is not guided
+// has word 15 = 0 (selection probability 0).
+constexpr int GUIDE_REC_F4 = 4;
+
+__device__ __forceinline__ uint32_t guidePackHalf2(float lo, float hi) {
+    return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)lo) | ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)hi) << 16);
+}
+__device__ __forceinline__ float guideHalfLo(uint32_t w) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu)); }
+__device__ __forceinline__ float guideHalfHi(uint32_t w) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16)); }
+// Unit vector <-> octahedral map (Cigolle et al. 2014, "A Survey of Efficient Representations
+// for Independent Unit Vectors"), 16 bits per coordinate.
+__device__ __forceinline__ uint32_t guideOctEncode(f3 n) {
+    const float l1 = fabsf(n.x) + fabsf(n.y) + fabsf(n.z);
+    float u = n.x / l1, v = n.y / l1;
+    if (n.z < 0.f) {
+        const float ou = u;
+        u = (1.f - fabsf(v)) * (ou >= 0.f ? 1.f : -1.f);
+        v = (1.f - fabsf(ou)) * (v >= 0.f ? 1.f : -1.f);
+    }
+    const uint32_t qu = (uint32_t)(fminf(fmaxf(u * 0.5f + 0.5f, 0.f), 1.f) * 65535.f + 0.5f);
+    const uint32_t qv = (uint32_t)(fminf(fmaxf(v * 0.5f + 0.5f, 0.f), 1.f) * 65535.f + 0.5f);
+    return qu | (qv << 16);
+}
+__device__ __forceinline__ f3 guideOctDecode(uint32_t w) {
+    const float u = (float)(w & 0xffffu) * (2.f / 65535.f) - 1.f, v = (float)(w >> 16) * (2.f / 65535.f) - 1.f;
+    f3 n = mk(u, v, 1.f - fabsf(u) - fabsf(v));
+    const float t = fmaxf(-n.z, 0.f);
+    n.x += n.x >= 0.f ? -t : t;
+    n.y += n.y >= 0.f ? -t : t;
+    return n * __builtin_amdgcn_rsqf(dot(n, n));
+}
+// The lobe weights of a record, renormalised so that sampling and density use the same
+// distribution whatever the FP16 rounding of the stored weights.
+__device__ __forceinline__ void guideLobeWeights(const uint32_t *w, float *pi) {
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < GUIDE_LOBES - 1; k++) {
+        pi[k] = guideHalfHi(w[2 * k + 1]);
+        s += pi[k];
+    }
+    pi[GUIDE_LOBES - 1] = fmaxf(1.f - s, 0.f);
+    const float inv = __builtin_amdgcn_rcpf(s + pi[GUIDE_LOBES - 1]);
+#pragma unroll
+    for (int k = 0; k < GUIDE_LOBES; k++) pi[k] *= inv;
+}
 
 // Training record, 4 float4: raw features (2 float4), (omega.xyz, bsdf pdf),
 // (1 / luminance of the throughput at the vertex, path id bits, radiance luminance before the

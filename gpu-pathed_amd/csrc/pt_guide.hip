@@ -33,7 +33,6 @@ namespace pt {
 typedef __bf16 bf16x8 __attribute__((__vector_size__(8 * sizeof(__bf16))));
 typedef float f32x16 __attribute__((__vector_size__(16 * sizeof(float))));
 
-constexpr int ACT_STRIDE = 72;   // bf16 per row of a row-major 32 x 64 tile (padded)
 
 #define WAVE_SYNC()                                         \
     do {                                                    \
@@ -68,33 +67,44 @@ __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x)
 // int32, the others keep their order.  fmaxf would first quiet a possible NaN input.
 __device__ __forceinline__ float relu(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
 
-// mixture parameters from the 41 used raw outputs
-struct Mixture {
-    f3 mu[GUIDE_LOBES];
-    float kappa[GUIDE_LOBES], pi[GUIDE_LOBES], ilen[GUIDE_LOBES];   // ilen: 1 / |mean output|, 0 if degenerate
-    float frac;
-};
-__device__ __forceinline__ void mixtureFromRaw(const float *o, Mixture &mx) {
-    float mxl = o[GO_LOGIT];
-    for (int k = 1; k < GUIDE_LOBES; k++) mxl = fmaxf(mxl, o[GO_LOGIT + k]);
-    float s = 0.f;
-    for (int k = 0; k < GUIDE_LOBES; k++) {
-        mx.pi[k] = __expf(o[GO_LOGIT + k] - mxl);
-        s += mx.pi[k];
-    }
-    const float is = frcp(s);
-    for (int k = 0; k < GUIDE_LOBES; k++) {
-        mx.pi[k] *= is;
-        const f3 m = mk(o[GO_MEAN + 3 * k], o[GO_MEAN + 3 * k + 1], o[GO_MEAN + 3 * k + 2]);
-        const float l2 = dot(m, m);
-        const bool ok = l2 > 1e-24f;
-        const float il = ok ? __builtin_amdgcn_rsqf(l2) : 0.f;
-        mx.ilen[k] = il;
-        mx.mu[k] = ok ? m * il : mk(0.f, 1.f, 0.f);
-        mx.kappa[k] = __expf(fminf(fmaxf(o[GO_KAPPA + k], GUIDE_LOGK_MIN), GUIDE_LOGK_MAX));
-    }
-    const float c = frcp(1.f + __expf(-o[GO_FRAC]));
-    mx.frac = fminf(fmaxf(c, GUIDE_FRAC_MIN), GUIDE_FRAC_MAX);
+// Sums and maxima over the four lanes that hold one record's outputs (lanes l, l ^ 16,
+// l ^ 32, l ^ 48); all four must be active.
+__device__ __forceinline__ float sum4(float v) {
+    v += __shfl_xor(v, 16);
+    return v + __shfl_xor(v, 32);
+}
+__device__ __forceinline__ float max4(float v) {
+    v = fmaxf(v, __shfl_xor(v, 16));
+    return fmaxf(v, __shfl_xor(v, 32));
+}
+
+// A lobe's parameters from its raw outputs: unit mean (or +y for a degenerate output, with
+// ilen = 0) and concentration exp(clamp(log kappa)).
+__device__ __forceinline__ void lobeFromRaw(float mx, float my, float mz, float lk, f3 &mu, float &il, float &kappa) {
+    const f3 m = mk(mx, my, mz);
+    const float l2 = dot(m, m);
+    const bool ok = l2 > 1e-24f;
+    il = ok ? __builtin_amdgcn_rsqf(l2) : 0.f;
+    mu = ok ? m * il : mk(0.f, 1.f, 0.f);
+    kappa = __expf(fminf(fmaxf(lk, GUIDE_LOGK_MIN), GUIDE_LOGK_MAX));
+}
+
+// One record's mixture from its last-layer outputs, four lanes per record: lane group g holds
+// slots v[s] = output guideOutOf(16 (s >> 2) + 4 g + (s & 3)), i.e. lobes 2 g and 2 g + 1 (and
+// group 3 the selection logit), and returns its 16-byte quarter of the mixture record: the
+// two lobes' octahedral means and (kappa, weight) pairs, the selection probability in place
+// of lobe 7's weight.  Softmax weights, unit means, kappa = exp(clamp(log kappa, -3, 9)),
+// selection probability clamp(sigmoid, 0.02, 0.98).
+__device__ __forceinline__ uint4 mixtureQuarter(const float *v, int g) {
+    const float mxl = max4(fmaxf(v[0], v[1]));
+    const float e0 = __expf(v[0] - mxl), e1 = __expf(v[1] - mxl);
+    const float is = frcp(sum4(e0 + e1));
+    f3 mu0, mu1;
+    float il, k0, k1;
+    lobeFromRaw(v[2], v[3], v[4], v[8], mu0, il, k0);
+    lobeFromRaw(v[5], v[6], v[7], v[9], mu1, il, k1);
+    const float c = fminf(fmaxf(frcp(1.f + __expf(-v[10])), GUIDE_FRAC_MIN), GUIDE_FRAC_MAX);
+    return make_uint4(guideOctEncode(mu0), guidePackHalf2(k0, e0 * is), guideOctEncode(mu1), guidePackHalf2(k1, g == 3 ? c : e1 * is));
 }
 
 // von Mises-Fisher density on S^2: kappa / (2 pi (1 - e^{-2 kappa})) exp(kappa (mu.w - 1))
@@ -105,9 +115,11 @@ __device__ __forceinline__ float vmfPdf(f3 mu, float kappa, f3 w) {
 // --------------------------------------------------------------------------------------------
 // inference
 // --------------------------------------------------------------------------------------------
-// Inference blocks: eight waves share the LDS-staged weight fragments (40 KiB) and biases;
-// each wave owns a 64-row tile at a time and 12.25 KiB of LDS for its raw outputs.
-constexpr int INFER_WAVES = 8;
+// Inference blocks: four waves share the LDS-staged weight fragments (40 KiB) and biases;
+// each wave owns a 64-row tile at a time, with its activations and outputs in registers.
+// Three blocks per CU: three waves per SIMD (at most 168 registers per lane).
+constexpr int INFER_WAVES = 4;
+constexpr int INFER_BLOCKS_PER_CU = 3;
 constexpr int INFER_ROWS = 64;
 
 // Raw features of the vertex at queue slot i (hit point normalised to the scene box, shading
@@ -186,21 +198,27 @@ __host__ __device__ __forceinline__ int wFragIndex(int L, int out, int in) {
     return (((L * 4 + ob) * 2 + ks) * 64 + 16 * g + (out & 15)) * 8 + p;
 }
 
+// The biases in LDS, the last layer's in guideOutOf order.
+__device__ __forceinline__ void stageBiases(const float *b, float *sb) {
+    for (int c = threadIdx.x; c < GUIDE_LAYERS * GUIDE_W; c += blockDim.x) {
+        const int L = c / GUIDE_W, o = c % GUIDE_W;
+        sb[c] = b[L * GUIDE_W + (L == GUIDE_LAYERS - 1 ? guideOutOf(o) : o)];
+    }
+}
+
 // Rows are the slots [0, maxRows) of a segmented queue (count of segment s at
 // count[s * segStride], segments segLen slots long, a multiple of 64, so that a tile of 64 rows
 // lies in one segment) whose features k_guide_features wrote to feat, or with count = nullptr
 // all maxRows rows of rawFeats.
-__global__ __launch_bounds__(64 * INFER_WAVES) void k_guide_infer(GuideNet net, const int *count, int segStride, int segLen, const float4 *feat,
+__global__ __launch_bounds__(64 * INFER_WAVES, INFER_BLOCKS_PER_CU) void k_guide_infer(GuideNet net, const int *count, int segStride, int segLen, const float4 *feat,
                                                                   int maxRows, float4 *out, const float *rawFeats, float *rawOut) {
     __shared__ bf16x8 sWf[GUIDE_LAYERS * 8 * 64];
     __shared__ float4 sB[GUIDE_LAYERS * GUIDE_W / 4];
-    __shared__ float sOut[INFER_WAVES][INFER_ROWS][GUIDE_OUT + 1];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, lr = lane & 15;
     for (int c = threadIdx.x; c < GUIDE_LAYERS * 8 * 64; c += blockDim.x) sWf[c] = ld8(net.w + 8 * (size_t)c);
-    for (int c = threadIdx.x; c < GUIDE_LAYERS * GUIDE_W / 4; c += blockDim.x) sB[c] = ((const float4 *)net.b)[c];
+    stageBiases(net.b, (float *)sB);
     __syncthreads();
-    float(*so)[GUIDE_OUT + 1] = sOut[wave];
     const int waveStride = gridDim.x * INFER_WAVES;
     // A tile's inputs: the features of this lane's four rows rowBase + 16 rb + lr (both float4
     // of a row; rawFeats has the same layout) and its segment's record count.  The next tile's
@@ -236,12 +254,10 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void k_guide_infer(GuideNet net, 
 #pragma unroll
         for (int rb = 0; rb < 4; rb++) bal[rb] = __ballot(rowBase + rb * 16 + lr < n && (rawFeats != nullptr || fb[rb].w >= 0.f));
         const int mine = rowBase + lane;
-        const uint64_t mb = g == 0 ? bal[0] : g == 1 ? bal[1] : g == 2 ? bal[2] : bal[3];
-        const bool elig = (mb >> lr) & 1u;
         if (rowBase < n && (bal[0] | bal[1] | bal[2] | bal[3]) == 0) {
             // no guided vertex in the tile (misses, mirror and glass hits): only the flag that
             // makes shade sample the BSDF alone, no network evaluation
-            if (mine < n && !rawOut) out[(size_t)mine * GUIDE_REC_F4 + 10] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (mine < n && !rawOut) out[(size_t)mine * GUIDE_REC_F4 + 3] = make_float4(0.f, 0.f, 0.f, 0.f);
         } else if (rowBase < n) {
             // layer 0's B fragments: one-blob encoding.  Input feature 8 k + j is bin j of raw
             // feature k; the lane's K positions need raw features 2 i + (g >> 1), i = 0..3, at bins
@@ -288,33 +304,30 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void k_guide_infer(GuideNet net, 
 #pragma unroll
                             for (int p = 0; p < 8; p++) act[rb][ks][p] = (__bf16)relu(acc[rb][2 * ks + (p >> 2)][p & 3]);
                 } else {
+                    // the last layer's outputs: the four lanes l, l ^ 16, l ^ 32, l ^ 48 hold a
+                    // row's outputs in guideOutOf order (two lobes each)
 #pragma unroll
-                    for (int rb = 0; rb < 4; rb++)
+                    for (int rb = 0; rb < 4; rb++) {
+                        float v[12];
 #pragma unroll
-                        for (int ob = 0; ob * 16 < GUIDE_OUT; ob++)
+                        for (int k = 0; k < 12; k++) v[k] = acc[rb][k >> 2][k & 3];
+                        const int row = rowBase + rb * 16 + lr;
+                        if (rawOut) {
+                            if (row < n) {
 #pragma unroll
-                            for (int r = 0; r < 4; r++) so[rb * 16 + lr][ob * 16 + 4 * g + r] = acc[rb][ob][r];
+                                for (int k = 0; k < 12; k++) rawOut[(size_t)row * GUIDE_OUT + guideOutOf(16 * (k >> 2) + 4 * g + (k & 3))] = v[k];
+                            }
+                        } else {
+                            const uint4 q = mixtureQuarter(v, g);
+                            if (row < n) {
+                                // a vertex that is not guided gets only the zero selection probability
+                                if ((bal[rb] >> lr) & 1u) ((uint4 *)out)[(size_t)row * GUIDE_REC_F4 + g] = q;
+                                else if (g == 3) ((uint4 *)out)[(size_t)row * GUIDE_REC_F4 + 3] = make_uint4(0u, 0u, 0u, 0u);
+                            }
+                        }
+                    }
                 }
             }
-            WAVE_SYNC();
-            if (mine < n) {
-                if (rawOut) {
-                    for (int k = 0; k < GUIDE_OUT; k++) rawOut[(size_t)mine * GUIDE_OUT + k] = so[lane][k];
-                } else if (!elig) {
-                    // shade reads only the flag of a vertex that is not guided
-                    out[(size_t)mine * GUIDE_REC_F4 + 10] = make_float4(0.f, 0.f, 0.f, 0.f);
-                } else {
-                    Mixture mx;
-                    mixtureFromRaw(so[lane], mx);
-                    float4 *o = out + (size_t)mine * GUIDE_REC_F4;
-                    for (int k = 0; k < GUIDE_LOBES; k++) o[k] = f4(mx.mu[k], mx.kappa[k]);
-                    o[8] = make_float4(mx.pi[0], mx.pi[1], mx.pi[2], mx.pi[3]);
-                    o[9] = make_float4(mx.pi[4], mx.pi[5], mx.pi[6], mx.pi[7]);
-                    o[10] = make_float4(mx.frac, 1.f, 0.f, 0.f);
-                    o[11] = make_float4(0.f, 0.f, 0.f, 0.f);
-                }
-            }
-            WAVE_SYNC();
         }
 #pragma unroll
         for (int rb = 0; rb < 4; rb++) {
@@ -328,73 +341,58 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void k_guide_infer(GuideNet net, 
 // --------------------------------------------------------------------------------------------
 // training: gradients
 // --------------------------------------------------------------------------------------------
-// d loss / d raw outputs for one record, loss = -w (log q + log(c q + (1 - c) p_bsdf)), as one
-// BF16 row (GUIDE_W entries, zero where the loss does not depend on the output; single
-// outliers clamped to 1e3).  Four lanes per record (lanes l, l ^ 16, l ^ 32, l ^ 48; quarter
-// q = l >> 4 handles lobes 2 q and 2 q + 1): the mixture's softmax maximum and sum and its
-// density are reduced across the four lanes, each lane writes its lobes' entries of the row
-// (and quarter 0 the selection entry).  The four lanes must be active together.  Returns the
-// loss in quarter 0, 0 in the others.
-__device__ float lossGradient4(const float *o, f3 w, float pb, float wt, __bf16 *g, int q) {
-    const bf16x8 zero = {};
-    *(bf16x8 *)(g + 16 * q) = zero;
-    *(bf16x8 *)(g + 16 * q + 8) = zero;
+// d loss / d last-layer outputs for one record, loss = -w (log q + log(c q + (1 - c) p_bsdf)),
+// four lanes per record in the layout of mixtureQuarter: d[s] is the gradient of output slot
+// s (0 where the loss does not depend on it; single outliers clamped to 1e3).  The mixture's
+// softmax maximum and sum and its density are reduced across the four lanes; the selection
+// logit (group 3, slot 10) is read from group 3.  Returns the loss in group 0, 0 in the others.
+__device__ float lossGradientQ(const float *v, f3 w, float pb, float wt, int g, float *d) {
+#pragma unroll
+    for (int k = 0; k < 12; k++) d[k] = 0.f;
+    const float fracRaw = __shfl(v[10], (int)(threadIdx.x & 15) + 48);
     if (!(wt > 0.f)) return 0.f;
-    auto put = [&](int k, float v) { g[k] = (__bf16)fminf(fmaxf(v, -1e3f), 1e3f); };
-    auto sum4 = [](float v) {
-        v += __shfl_xor(v, 16);
-        return v + __shfl_xor(v, 32);
-    };
-    auto max4 = [](float v) {
-        v = fmaxf(v, __shfl_xor(v, 16));
-        return fmaxf(v, __shfl_xor(v, 32));
-    };
-    const int k0 = 2 * q;
-    const float lg0 = o[GO_LOGIT + k0], lg1 = o[GO_LOGIT + k0 + 1];
-    const float mxl = max4(fmaxf(lg0, lg1));
-    const float e0 = __expf(lg0 - mxl), e1 = __expf(lg1 - mxl);
+    auto clampG = [](float x) { return fminf(fmaxf(x, -1e3f), 1e3f); };
+    const float mxl = max4(fmaxf(v[0], v[1]));
+    const float e0 = __expf(v[0] - mxl), e1 = __expf(v[1] - mxl);
     const float is = frcp(sum4(e0 + e1));
     float pi[2], kap[2], il[2], fk[2];
     f3 mu[2];
+    lobeFromRaw(v[2], v[3], v[4], v[8], mu[0], il[0], kap[0]);
+    lobeFromRaw(v[5], v[6], v[7], v[9], mu[1], il[1], kap[1]);
+    pi[0] = e0 * is;
+    pi[1] = e1 * is;
     float qp = 0.f;
 #pragma unroll
     for (int j = 0; j < 2; j++) {
-        const int k = k0 + j;
-        pi[j] = (j ? e1 : e0) * is;
-        const f3 m = mk(o[GO_MEAN + 3 * k], o[GO_MEAN + 3 * k + 1], o[GO_MEAN + 3 * k + 2]);
-        const float l2 = dot(m, m);
-        const bool ok = l2 > 1e-24f;
-        il[j] = ok ? __builtin_amdgcn_rsqf(l2) : 0.f;
-        mu[j] = ok ? m * il[j] : mk(0.f, 1.f, 0.f);
-        kap[j] = __expf(fminf(fmaxf(o[GO_KAPPA + k], GUIDE_LOGK_MIN), GUIDE_LOGK_MAX));
         fk[j] = vmfPdf(mu[j], kap[j], w);
         qp += pi[j] * fk[j];
     }
     const float qq = fmaxf(sum4(qp), 1e-12f);
-    const float craw = frcp(1.f + __expf(-o[GO_FRAC]));
+    const float craw = frcp(1.f + __expf(-fracRaw));
     const float c = fminf(fmaxf(craw, GUIDE_FRAC_MIN), GUIDE_FRAC_MAX);
     const float mix = fmaxf(c * qq + (1.f - c) * pb, 1e-12f);
     const float imix = frcp(mix);
     const float dq = -wt * (frcp(qq) + c * imix);
-    if (q == 0 && craw > GUIDE_FRAC_MIN && craw < GUIDE_FRAC_MAX) put(GO_FRAC, -wt * (qq - pb) * imix * craw * (1.f - craw));
+    if (g == 3 && craw > GUIDE_FRAC_MIN && craw < GUIDE_FRAC_MAX) d[10] = clampG(-wt * (qq - pb) * imix * craw * (1.f - craw));
 #pragma unroll
     for (int j = 0; j < 2; j++) {
-        const int k = k0 + j;
-        put(GO_LOGIT + k, dq * pi[j] * (fk[j] - qq));
-        const float df = dq * pi[j] * fk[j];
+        d[j] = clampG(dq * pi[j] * (fk[j] - qq));   // softmax logit
+        const float df = dq * pi[j] * fk[j];       // d loss / d log f_j
         const float cosw = dot(mu[j], w);
+        // d log f / d kappa = 1/kappa - 2 e^{-2k} / (1 - e^{-2k}) + (mu.w - 1); kappa = exp(raw)
         const float e2 = __expf(-2.f * kap[j]);
         const float dlk = frcp(kap[j]) - 2.f * e2 * frcp(1.f - e2) + (cosw - 1.f);
-        const float lk = o[GO_KAPPA + k];
-        if (lk > GUIDE_LOGK_MIN && lk < GUIDE_LOGK_MAX) put(GO_KAPPA + k, df * dlk * kap[j]);
+        const float lk = v[8 + j];
+        if (lk > GUIDE_LOGK_MIN && lk < GUIDE_LOGK_MAX) d[8 + j] = clampG(df * dlk * kap[j]);
+        // d log f / d m = kappa (w - (mu.w) mu) / |m|
         if (il[j] > 0.f) {
             const float sc = df * kap[j] * il[j];
-            put(GO_MEAN + 3 * k, sc * (w.x - cosw * mu[j].x));
-            put(GO_MEAN + 3 * k + 1, sc * (w.y - cosw * mu[j].y));
-            put(GO_MEAN + 3 * k + 2, sc * (w.z - cosw * mu[j].z));
+            d[2 + 3 * j] = clampG(sc * (w.x - cosw * mu[j].x));
+            d[3 + 3 * j] = clampG(sc * (w.y - cosw * mu[j].y));
+            d[4 + 3 * j] = clampG(sc * (w.z - cosw * mu[j].z));
         }
     }
-    return q == 0 ? -wt * (__logf(qq) + __logf(mix)) : 0.f;
+    return g == 0 ? -wt * (__logf(qq) + __logf(mix)) : 0.f;
 }
 
 // Training step in one kernel: blocks of eight waves, persistent over iterations of
@@ -415,17 +413,9 @@ constexpr int FM_PITCH = GUIDE_TRAIN_ROWS + 8;        // bf16 per feature of a f
 static_assert(TRAIN_WAVES * TRAIN_TILE == GUIDE_TRAIN_ROWS, "an iteration is one tile per wave");
 static_assert(TRAIN_WAVES * 8 == GUIDE_W, "a 16 x 32 block of the weight gradient per wave");
 
-struct TrainLossLds {   // one wave's loss stage
-    float out[TRAIN_TILE][GUIDE_OUT + 1];
-    __bf16 g[TRAIN_TILE][ACT_STRIDE];
-};
 struct TrainGradLds {   // the block's weight-gradient stage
     __bf16 a[GUIDE_W][FM_PITCH];
     __bf16 d[GUIDE_W][FM_PITCH];
-};
-union TrainLds {
-    TrainLossLds loss[TRAIN_WAVES];
-    TrainGradLds grad;
 };
 
 __global__ __launch_bounds__(64 * TRAIN_WAVES) void k_guide_train(GuideNet net, const float4 *recs, const int *countPtr, const int *prefix,
@@ -436,13 +426,13 @@ __global__ __launch_bounds__(64 * TRAIN_WAVES) void k_guide_train(GuideNet net, 
     __shared__ float4 sB[GUIDE_LAYERS * GUIDE_W / 4];
     __shared__ float4 sRec[TRAIN_WAVES][TRAIN_TILE][GUIDE_TREC_F4];
     __shared__ float sLoss[TRAIN_WAVES];
-    __shared__ __align__(16) TrainLds sx;
+    __shared__ __align__(16) TrainGradLds gs;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, lr = lane & 15;
     const int gob = wave >> 1, gib = 2 * (wave & 1);   // this wave's block of the weight gradient
     for (int c = threadIdx.x; c < GUIDE_LAYERS * 8 * 64; c += blockDim.x) sWf[c] = ld8(net.w + 8 * (size_t)c);
     for (int c = threadIdx.x; c < (GUIDE_LAYERS - 1) * 8 * 64; c += blockDim.x) sWb[c] = ld8(net.wt + 8 * (size_t)(8 * 64 + c));
-    for (int c = threadIdx.x; c < GUIDE_LAYERS * GUIDE_W / 4; c += blockDim.x) sB[c] = ((const float4 *)net.b)[c];
+    stageBiases(net.b, (float *)sB);
     __syncthreads();
     const int count = *countPtr;
     const float wsum = wsumPtr ? *wsumPtr : 0.f;
@@ -497,7 +487,7 @@ __global__ __launch_bounds__(64 * TRAIN_WAVES) void k_guide_train(GuideNet net, 
                 }
         }
         uint32_t mask[GUIDE_LAYERS - 1];   // ReLU' of hidden layer L's outputs, bit ks * 8 + p
-        TrainLossLds &ls = sx.loss[wave];
+        float v[12];                       // this lane's last-layer outputs (guideOutOf order)
 #pragma unroll
         for (int L = 0; L < GUIDE_LAYERS; L++) {
             const bool last = L == GUIDE_LAYERS - 1;
@@ -523,28 +513,24 @@ __global__ __launch_bounds__(64 * TRAIN_WAVES) void k_guide_train(GuideNet net, 
                 mask[L] = m;
             } else {
 #pragma unroll
-                for (int ob = 0; ob * 16 < GUIDE_OUT; ob++)
-#pragma unroll
-                    for (int r = 0; r < 4; r++) ls.out[lr][ob * 16 + 4 * g + r] = acc[ob][r];
+                for (int k = 0; k < 12; k++) v[k] = acc[k >> 2][k & 3];
             }
         }
-        WAVE_SYNC();
+        // the output deltas, in registers in the C-fragment layout of the last layer, as the
+        // backward pass's B fragments (output block 3 is not computed: zero)
+        bf16x8 dl[2];
         {
             const float4 c = sRec[wave][lr][2];
-            lossAcc += lossGradient4(ls.out[lr], mk(c.x, c.y, c.z), c.w, sRec[wave][lr][3].w, ls.g[lr], g);
-        }
-        WAVE_SYNC();
-        // the output deltas as B fragments
-        typedef __bf16 bf16x4 __attribute__((__vector_size__(4 * sizeof(__bf16))));
-        bf16x8 dl[2];
+            float d[12];
+            lossAcc += lossGradientQ(v, mk(c.x, c.y, c.z), c.w, sRec[wave][lr][3].w, g, d);
 #pragma unroll
-        for (int ks = 0; ks < 2; ks++) {
-            const __bf16 *gr = ls.g[lr] + 32 * ks + 4 * g;
-            const bf16x4 lo = *(const bf16x4 *)gr, hi = *(const bf16x4 *)(gr + 16);
-            dl[ks] = bf16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+            for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+                for (int p = 0; p < 8; p++) {
+                    const int ob = 2 * ks + (p >> 2);
+                    dl[ks][p] = (__bf16)(ob < 3 ? d[4 * ob + (p & 3)] : 0.f);
+                }
         }
-        __syncthreads();   // the loss buffers become the gradient stage's tiles
-        TrainGradLds &gs = sx.grad;
 #pragma unroll
         for (int L = GUIDE_LAYERS - 1; L >= 0; L--) {
 #pragma unroll
@@ -596,16 +582,18 @@ __global__ __launch_bounds__(64 * TRAIN_WAVES) void k_guide_train(GuideNet net, 
 #pragma unroll
     for (int L = 0; L < GUIDE_LAYERS; L++) {
         float *PL = P + (size_t)L * GUIDE_LAYER_PARAMS;
+        // the last layer's rows back from the kernels' output order
+        auto row = [&](int op) { return L == GUIDE_LAYERS - 1 ? guideOutOf(op) : op; };
 #pragma unroll
         for (int j = 0; j < 2; j++)
 #pragma unroll
-            for (int r = 0; r < 4; r++) PL[(gob * 16 + 4 * g + r) * GUIDE_W + (gib + j) * 16 + lr] = dW[L][j][r];
+            for (int r = 0; r < 4; r++) PL[row(gob * 16 + 4 * g + r) * GUIDE_W + (gib + j) * 16 + lr] = dW[L][j][r];
         if (gib == 0) {
             // lanes lr, lr + 16, lr + 32, lr + 48 summed complementary rows of output 16 gob + lr
-            float v = db[L];
-            v += __shfl_xor(v, 16);
-            v += __shfl_xor(v, 32);
-            if (g == 0) PL[GUIDE_W * GUIDE_W + gob * 16 + lr] = v;
+            float b = db[L];
+            b += __shfl_xor(b, 16);
+            b += __shfl_xor(b, 32);
+            if (g == 0) PL[GUIDE_W * GUIDE_W + row(gob * 16 + lr)] = b;
         }
     }
     // loss of this block: quarter 0 of each wave holds the partial sums, added in wave order
@@ -627,8 +615,10 @@ __device__ __forceinline__ void storeParam(int p, float v, __bf16 *w, __bf16 *wt
     if (off < GUIDE_W * GUIDE_W) {
         const int out = off / GUIDE_W, in = off % GUIDE_W;
         const __bf16 h = (__bf16)v;
-        w[wFragIndex(L, out, in)] = h;    // forward: W[out][in], K = in
-        wt[wFragIndex(L, in, out)] = h;   // backward: W^T[in][out], K = out
+        // the last layer's rows in the kernels' output order
+        const int o = L == GUIDE_LAYERS - 1 ? guideOpOf(out) : out;
+        w[wFragIndex(L, o, in)] = h;    // forward: W[out][in], K = in
+        wt[wFragIndex(L, in, o)] = h;   // backward: W^T[in][out], K = out
     } else {
         b[L * GUIDE_W + (off - GUIDE_W * GUIDE_W)] = v;
     }
@@ -771,7 +761,7 @@ void guideInfer(const PtSceneView &sc, GuideSceneNorm norm, GuideNet net, const 
     const int maxRows = PT_QUEUE_SEGS * segLen;
     const int fgrid = std::max(1, std::min((maxRows + 255) / 256, cus * 8));
     k_guide_features<<<fgrid, 256, 0, s>>>(sc, norm, segCount, segStride, segLen, rayO, rayD, camO, hits, maxRows, feat);
-    const int grid = std::max(1, std::min((maxRows + INFER_ROWS * INFER_WAVES - 1) / (INFER_ROWS * INFER_WAVES), cus));
+    const int grid = std::max(1, std::min((maxRows + INFER_ROWS * INFER_WAVES - 1) / (INFER_ROWS * INFER_WAVES), cus * INFER_BLOCKS_PER_CU));
     k_guide_infer<<<grid, 64 * INFER_WAVES, 0, s>>>(net, segCount, segStride, segLen, feat, maxRows, out, nullptr, nullptr);
     PT_HIP_TRY(hipGetLastError());
 }

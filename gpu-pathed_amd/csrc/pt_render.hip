@@ -878,15 +878,30 @@ struct GuideShadeArgs {
 __device__ __forceinline__ float lum(float4 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
 __device__ __forceinline__ float lum(f3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
 
-__device__ __forceinline__ float mixturePdf(const float4 *G, f3 w) {
-    const float4 p0 = G[8], p1 = G[9];
-    const float pi[GUIDE_LOBES] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
+// A guided vertex's mixture record (16 words, layout at GUIDE_REC_F4) in registers.
+struct GuideRec {
+    uint32_t w[16];
+};
+__device__ __forceinline__ GuideRec loadGuideRec(const float4 *G) {
+    GuideRec r;
+    const uint4 *q = (const uint4 *)G;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const uint4 v = q[i];
+        r.w[4 * i] = v.x;
+        r.w[4 * i + 1] = v.y;
+        r.w[4 * i + 2] = v.z;
+        r.w[4 * i + 3] = v.w;
+    }
+    return r;
+}
+// vMF mixture density of direction w (world space)
+__device__ __forceinline__ float mixturePdf(const GuideRec &g, f3 w) {
+    float pi[GUIDE_LOBES];
+    guideLobeWeights(g.w, pi);
     float q = 0.f;
 #pragma unroll
-    for (int k = 0; k < GUIDE_LOBES; k++) {
-        const float4 l = G[k];
-        q += pi[k] * vmfDensity(xyz(l), l.w, w);
-    }
+    for (int k = 0; k < GUIDE_LOBES; k++) q += pi[k] * vmfDensity(guideOctDecode(g.w[2 * k]), guideHalfLo(g.w[2 * k + 1]), w);
     return q;
 }
 
@@ -1047,31 +1062,31 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
                         rng.inc = streamInc(cp.seed, (uint32_t)pixel, (uint32_t)(cp.sampleBase + s));
                     }
                     rng.state = ((uint64_t)st.y << 32) | st.x;
-                    const float4 *G = nullptr;
+                    GuideRec G;
+                    bool guidedV = false;
                     float frac = 0.f;
                     if constexpr (GUIDED) {
                         if (ga.mix) {
-                            G = ga.mix + (size_t)i * GUIDE_REC_F4;
-                            const float4 g10 = G[10];
-                            if (g10.y > 0.f) frac = fminf(g10.x, ga.fracCap);
-                            else G = nullptr;
+                            G = loadGuideRec(ga.mix + (size_t)i * GUIDE_REC_F4);
+                            const float f = guideHalfHi(G.w[15]);
+                            guidedV = f > 0.f;
+                            frac = fminf(f, ga.fracCap);
                         }
                     }
                     BsdfSample bs;
                     float pdfDir;  // density of the direction sampling technique at bs.wi
                     float pbRec = 0.f;
-                    if (GUIDED && G) {
+                    if (GUIDED && guidedV) {
                         const float usel = rng.uniform();
                         if (usel < frac) {
-                            const float4 p0 = G[8], p1 = G[9];
-                            const float pi[GUIDE_LOBES] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
+                            float pi[GUIDE_LOBES];
+                            guideLobeWeights(G.w, pi);
                             const float ul = rng.uniform();
                             int k = 0;
                             float acc = pi[0];
                             while (k < GUIDE_LOBES - 1 && ul >= acc) acc += pi[++k];
                             const float u1 = rng.uniform(), u2 = rng.uniform();
-                            const float4 lobe = G[k];
-                            const f3 ww = sampleVmf(xyz(lobe), lobe.w, u1, u2);
+                            const f3 ww = sampleVmf(guideOctDecode(G.w[2 * k]), guideHalfLo(G.w[2 * k + 1]), u1, u2);
                             bs.wi = toLocal(fr, ww);
                             bs.f = bsdfF(m, wo, bs.wi);
                             bs.delta = false;
@@ -1103,7 +1118,7 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
                         if (anyLight && dot(ls.n, ls.wi) < 0.f && ls.pdf > 0.f) {
                             const f3 wiL = toLocal(fr, ls.wi);
                             float pB;
-                            if (GUIDED && G) pB = frac * mixturePdf(G, ls.wi) + (1.f - frac) * bsdfSamplePdf(m, wo, wiL);
+                            if (GUIDED && guidedV) pB = frac * mixturePdf(G, ls.wi) + (1.f - frac) * bsdfSamplePdf(m, wo, wiL);
                             else pB = cp.misSample ? bsdfSamplePdf(m, wo, wiL) : bsdfPdf(m, wo, wiL);
                             const float w = balance(ls.pdf, pB);
                             contrib = thr * (ls.emitted * w * bsdfF(m, wo, wiL) * fabsf(dot(sp.n, ls.wi)) / ls.pdf);
@@ -1136,7 +1151,7 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
                     if constexpr (GUIDED) {
                         // a training record per guided vertex whose path continues
                         const float lt = lum(thr);
-                        const bool wantRec = pushRay && G != nullptr && lt > 0.f;
+                        const bool wantRec = pushRay && guidedV && lt > 0.f;
                         // records go to the segment's own part of the record buffer
                         const int slotRec = segAppendLanes(wantRec, counters, seg, 0, PT_Q_RECORDS, 0);
                         if (pushRay) {
