@@ -161,6 +161,8 @@ void guideUploadBf16(const float *dParams, __bf16 *dW, __bf16 *dWt, float *dB, h
 void guideInfer(const PtSceneView &sc, GuideSceneNorm norm, GuideNet net, const int *segCount, int segStride, int segLen,
                 const float4 *rayO, const float4 *rayD, float4 camO, const float4 *hits, float4 *feat, float4 *out, int cus, hipStream_t s);
 void guideInferRaw(GuideNet net, const float *feats, int n, float *rawOut, hipStream_t s);
+// The mixture records (GUIDE_REC_F4 float4 each) of n rows of raw features, every row guided.
+void guideInferRecords(GuideNet net, const float *feats, int n, float4 *records, hipStream_t s);
 // Scratch of one training step over `batch` records: FP32 gradient partials (blocks x
 // GUIDE_PARAMS) and per-block loss sums of the training kernel, which runs
 // guideTrainBlocks(batch, cus) blocks of GUIDE_TRAIN_ROWS records per iteration.

@@ -766,6 +766,12 @@ void guideInfer(const PtSceneView &sc, GuideSceneNorm norm, GuideNet net, const 
     PT_HIP_TRY(hipGetLastError());
 }
 
+void guideInferRecords(GuideNet net, const float *feats, int n, float4 *records, hipStream_t s) {
+    const int grid = std::max(1, (n + INFER_ROWS * INFER_WAVES - 1) / (INFER_ROWS * INFER_WAVES));
+    k_guide_infer<<<grid, 64 * INFER_WAVES, 0, s>>>(net, nullptr, 0, 64, nullptr, n, records, feats, nullptr);
+    PT_HIP_TRY(hipGetLastError());
+}
+
 void guideInferRaw(GuideNet net, const float *feats, int n, float *rawOut, hipStream_t s) {
     const int grid = std::max(1, (n + INFER_ROWS * INFER_WAVES - 1) / (INFER_ROWS * INFER_WAVES));
     k_guide_infer<<<grid, 64 * INFER_WAVES, 0, s>>>(net, nullptr, 0, 64, nullptr, n, nullptr, feats, rawOut);

@@ -356,6 +356,20 @@ int pt_unit_guide_forward(const float *params, const float *features, int64_t n,
     PT_UNIT_END("pt_unit_guide_forward")
 }
 
+int pt_unit_guide_records(const float *params, const float *features, int64_t n, uint32_t *records) {
+    if (!params || n < 0 || (n > 0 && (!features || !records)) || n > (1 << 24)) return fail(PT_ERR_INVALID, "pt_unit_guide_records: bad arguments");
+    PT_UNIT_BEGIN("pt_unit_guide_records")
+    Stage st;
+    NetStage ns(params, st.s);
+    const float *df = st.in(features, (size_t)n * GUIDE_RAW);
+    float4 *dout = st.out<float4>((size_t)n * GUIDE_REC_F4);
+    if (n > 0) guideInferRecords(ns.net, df, (int)n, dout, st.s);
+    st.back((float4 *)records, dout, (size_t)n * GUIDE_REC_F4);
+    st.sync();
+    return PT_OK;
+    PT_UNIT_END("pt_unit_guide_records")
+}
+
 int pt_unit_guide_gradients(const float *params, const float *records, int64_t n, float *grad, float *loss) {
     if (!params || !grad || n <= 0 || !records || n > (1 << 24)) return fail(PT_ERR_INVALID, "pt_unit_guide_gradients: bad arguments");
     PT_UNIT_BEGIN("pt_unit_guide_gradients")

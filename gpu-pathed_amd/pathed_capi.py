@@ -34,7 +34,7 @@ EXPORTS = [
     "pt_renderer_hit_test", "pt_renderer_get_stats", "pt_renderer_reset_stats", "pt_renderer_get_kernel_stats",
     "pt_renderer_get_path_radiance", "pt_renderer_intersect",
     "pt_unit_triangle_test", "pt_unit_bsdf_eval", "pt_unit_bsdf_sample", "pt_unit_env_tables", "pt_unit_env_sample",
-    "pt_unit_path_uniforms", "pt_unit_guide_forward", "pt_unit_guide_gradients", "pt_unit_guide_adam", "pt_unit_mfma_tile",
+    "pt_unit_path_uniforms", "pt_unit_guide_forward", "pt_unit_guide_records", "pt_unit_guide_gradients", "pt_unit_guide_adam", "pt_unit_mfma_tile",
     "pt_unit_vmf_sample", "pt_unit_tonemap", "pt_pool_layout",
     "pt_scene_load_obj", "pt_scene_load_ply", "pt_scene_load_exr_environment", "pt_scene_reference", "pt_exr_info",
     "pt_exr_read_rgba", "pt_exr_write_rgb",
@@ -168,6 +168,7 @@ def lib():
         L.pt_unit_path_uniforms.argtypes = [C.c_uint64, vp, vp, C.c_int64, C.c_int, vp]
         L.pt_device_count.argtypes = [C.POINTER(C.c_int)]
         L.pt_unit_guide_forward.argtypes = [vp, vp, C.c_int64, vp]
+        L.pt_unit_guide_records.argtypes = [vp, vp, C.c_int64, vp]
         L.pt_unit_guide_gradients.argtypes = [vp, vp, C.c_int64, vp, vp]
         L.pt_unit_guide_adam.argtypes = [vp, vp, vp, vp, C.c_int, C.c_float]
         L.pt_unit_mfma_tile.argtypes = [vp, vp, vp]
@@ -544,6 +545,15 @@ def guide_forward(params, feats):
     f = _f32(feats).reshape(-1, 8)
     out = np.zeros((f.shape[0], 48), np.float32)
     check(lib().pt_unit_guide_forward(_p(p), _p(f), f.shape[0], _p(out)))
+    return out
+
+
+def guide_records(params, feats):
+    """The inference kernel's 64-byte mixture records of every feature row, (n, 16) uint32."""
+    p = _f32(params)
+    f = _f32(feats).reshape(-1, 8)
+    out = np.zeros((f.shape[0], 16), np.uint32)
+    check(lib().pt_unit_guide_records(_p(p), _p(f), f.shape[0], _p(out)))
     return out
 
 

@@ -344,6 +344,12 @@ int pt_unit_path_uniforms(uint64_t seed, const uint32_t *pixel, const uint32_t *
  * the lane layout the network kernels rely on. */
 #define PT_GUIDE_PARAMS 20800
 int pt_unit_guide_forward(const float *params, const float *features, int64_t n, float *out);
+/* The inference kernel's mixture records for n feature rows (16 uint32 words per row, the
+ * record the guided shade reads: per lobe k, word 2k = octahedral unit mean as two unorm16
+ * (u low, v high), word 2k+1 = kappa (FP16, low half) and the lobe weight (FP16, high half;
+ * for lobe 7 the guided-sampling probability instead, lobe 7's weight being 1 minus the
+ * others).  No counterpart in the reference (learned guiding); for the parity tests. */
+int pt_unit_guide_records(const float *params, const float *features, int64_t n, uint32_t *records);
 int pt_unit_guide_gradients(const float *params, const float *records, int64_t n, float *grad, float *loss);
 int pt_unit_guide_adam(float *params, float *m, float *v, const float *grad, int step, float lr);
 int pt_unit_mfma_tile(const float *a, const float *b, float *c);

@@ -60,6 +60,54 @@ def test_forward_matches_oracle():
     assert np.abs(g - ref32).max() <= 5e-2 * scale
 
 
+def _decode_records(rec):
+    """The 64-byte mixture record (16 words per row): octahedral unit means (two unorm16),
+    FP16 kappa and lobe weights, the selection probability in lobe 7's weight slot; weights
+    renormalised as the guided shade does."""
+    rec = np.asarray(rec, np.uint32).reshape(-1, 16)
+    oct_w, kw = rec[:, 0::2], rec[:, 1::2]
+    u = (oct_w & 0xFFFF).astype(np.float64) * (2 / 65535) - 1
+    v = (oct_w >> 16).astype(np.float64) * (2 / 65535) - 1
+    n = np.stack([u, v, 1 - np.abs(u) - np.abs(v)], -1)
+    t = np.maximum(-n[..., 2], 0)
+    n[..., 0] -= np.where(n[..., 0] >= 0, t, -t)
+    n[..., 1] -= np.where(n[..., 1] >= 0, t, -t)
+    mu = n / np.linalg.norm(n, axis=-1, keepdims=True)
+    kappa = (kw & 0xFFFF).astype(np.uint16).view(np.float16).astype(np.float64)
+    hi = (kw >> 16).astype(np.uint16).view(np.float16).astype(np.float64)
+    frac = hi[:, 7].copy()
+    pi = hi.copy()
+    pi[:, 7] = np.maximum(1 - pi[:, :7].sum(1), 0)
+    pi /= pi.sum(1, keepdims=True)
+    return mu, kappa, pi, frac
+
+
+def test_mixture_records_match_the_network_outputs():
+    """The inference kernel's mixture (its own epilogue, four lanes per row) and the record
+    encoding, against the oracle's mixture of the same kernel's raw outputs: softmax weights,
+    unit means, clamped concentrations and selection probability, to the record's FP16 and
+    16-bit octahedral precision."""
+    print("case: guide mixture records", flush=True)
+    rng = np.random.default_rng(3)
+    p = _params(rng)
+    feats = rng.random((1000, 8)).astype(np.float32)
+    raw = P.guide_forward(p, feats)
+    rec = P.guide_records(p, feats)
+    mu, kappa, pi, frac = _decode_records(rec)
+    opi, omu, oml, okappa, _, _, oc = G.mixture(raw)
+    ok = oml > 1e-6
+    dmu = np.abs(mu - omu).max(-1)[ok].max()
+    dk = (np.abs(kappa - okappa) / okappa).max()
+    dpi = np.abs(pi - opi).max()
+    dc = (np.abs(frac - oc) / oc).max()
+    print(f"max |mean| {dmu:.2e}, kappa rel {dk:.2e}, weight {dpi:.2e}, selection rel {dc:.2e}", flush=True)
+    assert dmu <= 1e-4 and dk <= 1e-3 and dpi <= 1e-3 and dc <= 1e-3
+    # the encoding keeps the lobes in their own slots: the record's dominant lobe is the
+    # network's wherever one lobe clearly dominates
+    clear = np.sort(opi, 1)[:, -1] > np.sort(opi, 1)[:, -2] + 0.01
+    assert (pi.argmax(1) == opi.argmax(1))[clear].all()
+
+
 def test_gradients_match_oracle():
     print("case: guide gradients", flush=True)
     rng = np.random.default_rng(2)
