@@ -168,8 +168,9 @@ __global__ __launch_bounds__(256) void k_guide_features(PtSceneView sc, GuideSce
                                                         const float4 *rayO, const float4 *rayD, float4 camO, const float4 *hits, int maxRows,
                                                         float4 *feat) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < maxRows; i += gridDim.x * blockDim.x) {
-        const int seg = i / segLen;
-        if (i - seg * segLen >= count[seg * segStride]) continue;
+        int off;
+        const int seg = divmodEst(i, segLen, &off);
+        if (off >= count[seg * segStride]) continue;
         float f[GUIDE_RAW];
         for (int k = 0; k < GUIDE_RAW; k++) f[k] = 0.f;
         if (!vertexFeatures(sc, nm, i, rayO, rayD, camO, hits, f)) {

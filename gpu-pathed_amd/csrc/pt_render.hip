@@ -136,24 +136,6 @@ struct BatchBucket {
     uint64_t seed;
 };
 
-// n / d and n % d for 0 <= n < 2^31, d > 0, from a float estimate with the float reciprocal
-// of d, corrected to the exact integer result (the estimate is off by at most a few units for
-// quotients below 2^26): a per-lane integer division otherwise costs about 40 instructions.
-__device__ __forceinline__ int divmodEst(int n, int d, int *r) {
-    int q = (int)((float)n * __builtin_amdgcn_rcpf((float)d));
-    int rr = n - q * d;
-    while (rr < 0) {
-        q--;
-        rr += d;
-    }
-    while (rr >= d) {
-        q++;
-        rr -= d;
-    }
-    *r = rr;
-    return q;
-}
-
 // The group of a chunk path that lies past its bucket's group (in a group that starts inside
 // the bucket).
 __device__ __forceinline__ int batchLaterGroup(const ChunkParams &cp, const BatchBucket &bk, int path) {
@@ -693,8 +675,9 @@ __global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 7 : 5) void k_extend(P
             const float4 rd = rayD[i];
             if constexpr (BATCH) {
                 if (bounce == 0) {
-                    const int seg = i / segLen;
-                    const int p = ptCameraPath(seg, i - seg * segLen);
+                    int off;
+                    const int seg = divmodEst(i, segLen, &off);
+                    const int p = ptCameraPath(seg, off);
                     const BatchRender &R = cp.renders[batchRenderOf(cp, p)];
                     o = xyz(batchCameraOrigin(R));
                     tmin = 0.f;
@@ -1207,8 +1190,9 @@ __global__ __launch_bounds__(256) void k_guide_finalize(float4 *records, int rec
     float w = 0.f;
     const int total = PT_QUEUE_SEGS * recSeg;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
-        const int seg = i / recSeg;
-        if (i - seg * recSeg >= min(counters[ptQueueField(seg, 0, PT_Q_RECORDS)], recSeg)) continue;
+        int off;
+        const int seg = divmodEst(i, recSeg, &off);
+        if (off >= min(counters[ptQueueField(seg, 0, PT_Q_RECORDS)], recSeg)) continue;
         float4 *r = records + (size_t)i * GUIDE_TREC_F4 + 3;
         const float4 v = *r;
         const int pid = asInt(v.y);
