@@ -156,6 +156,42 @@ def test_autotune_batch_and_renderer():
     r.close()
 
 
+MIXED = [  # (seed, triangles, resolution, spp): one render larger than the pool, small ones to share chunks
+    (4001, 300_000, 512, 32),
+    (4002, 20_000, 128, 4),
+    (4003, 60_000, 256, 16),
+    (4004, 10_000, 96, 8),
+    (4005, 150_000, 128, 64),
+]
+
+
+def test_batch_schedules_agree_bit_for_bit():
+    """The wavefront schedule (chunks of several renders through the batch kernels, and a large
+    render's own chunks through its renderer's kernels) and the render-after-render schedule
+    give every render the same image and ray count, bit for bit, under a pool small enough that
+    the large render spans several chunks of its own."""
+    say("case: batch schedules, bit for bit")
+    scenes = [P.Scene.preset(4, seed=s, param=t) for s, t, _, _ in MIXED]
+    out = {}
+    for wave in (1, 0):
+        b = P.Batch()
+        for sc, (_, _, res, spp) in zip(scenes, MIXED):
+            b.add(sc, res, res, spp, 8)
+        b.set_option(P.OPT_BATCH_WAVEFRONT, wave)
+        b.set_option(P.OPT_POOL_PATHS, 1 << 21)
+        b.run()
+        st = b.stats()
+        say(f"wavefront {wave}: {st}")
+        assert st["stack_overflows"] == 0
+        out[wave] = ([b.radiance(i) for i in range(len(MIXED))], [b.render_stats(i)["rays"] for i in range(len(MIXED))], st)
+        b.close()
+    (iw, rw, sw), (is_, rs, ss) = out[1], out[0]
+    assert sw["chunks"] < ss["chunks"] and sw["rays"] == ss["rays"]
+    for i in range(len(MIXED)):
+        assert rw[i] == rs[i], i
+        assert np.array_equal(iw[i], is_[i]), i
+
+
 EXTREMES = [  # (seed, triangles, resolution, spp): the batch workload's largest and smallest renders
     (3001, 2_000_000, 1024, 256),
     (3002, 10_000, 128, 4),
