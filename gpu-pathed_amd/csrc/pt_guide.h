@@ -86,8 +86,8 @@ __device__ __forceinline__ float guideHalfHi(uint32_t w) { return (float)__built
 // Unit vector <-> octahedral map (Cigolle et al. 2014, "A Survey of Efficient Representations
 // for Independent Unit Vectors"), 16 bits per coordinate.
 __device__ __forceinline__ uint32_t guideOctEncode(f3 n) {
-    const float l1 = fabsf(n.x) + fabsf(n.y) + fabsf(n.z);
-    float u = n.x / l1, v = n.y / l1;
+    const float il1 = __builtin_amdgcn_rcpf(fabsf(n.x) + fabsf(n.y) + fabsf(n.z));
+    float u = n.x * il1, v = n.y * il1;
     if (n.z < 0.f) {
         const float ou = u;
         u = (1.f - fabsf(v)) * (ou >= 0.f ? 1.f : -1.f);

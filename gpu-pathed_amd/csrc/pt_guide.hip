@@ -55,17 +55,28 @@ __device__ __forceinline__ f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c) { return __
 
 __device__ __forceinline__ float luminance(float4 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
 
+// exp(-x^2 / 2) with x = 8 s - (j + 1/2), as exp2(-t^2) with t = x sqrt(log2(e) / 2): one fma,
+// one multiply and v_exp_f32.
 __device__ __forceinline__ float oneBlob(float s, int j) {
-    const float x = s * (float)GUIDE_BINS - ((float)j + 0.5f);
-    return __expf(-0.5f * x * x);
+    constexpr float c = 0.8493218002880191f;   // sqrt(log2(e) / 2)
+    const float t = fmaf(s, (float)GUIDE_BINS * c, -((float)j + 0.5f) * c);
+    return __builtin_amdgcn_exp2f(-t * t);
+}
+
+// ReLU of two FP32 values as one packed BF16 pair: convert, then one packed 16-bit integer max
+// with 0 (a BF16 value is negative, or -0, exactly when its bit pattern is a negative int16).
+typedef short short2v __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint32_t reluPackBf16(float a, float b) {
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    const bf16x2 h = {(__bf16)a, (__bf16)b};
+    const short2v r = __builtin_elementwise_max(__builtin_bit_cast(short2v, h), (short2v){0, 0});
+    return __builtin_bit_cast(uint32_t, r);
 }
 
 // Reciprocal with the hardware's 1-ulp v_rcp_f32 (an IEEE division expands to ten
 // instructions; the network's outputs are BF16-limited anyway).
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
-// max(x, 0) as one integer max on the bit pattern: negative floats (and -0) are negative as
-// int32, the others keep their order.  fmaxf would first quiet a possible NaN input.
-__device__ __forceinline__ float relu(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
 
 // Sums and maxima over the four lanes that hold one record's outputs (lanes l, l ^ 16,
 // l ^ 32, l ^ 48); all four must be active.
@@ -161,8 +172,9 @@ __device__ __forceinline__ bool vertexFeatures(const PtSceneView &sc, const Guid
 }
 
 // Features of every filled slot of a segmented ray queue (count of segment s at
-// count[s * segStride], segments segLen slots long): two float4 per slot, the eighth feature
-// (roughness, > 0) set to -1 where the vertex is not guided.  A kernel of its own, at full
+// count[s * segStride], segments segLen slots long): two float4 per slot (raw features 0, 2,
+// 4, 6, then 1, 3, 5, 7), the eighth feature (roughness, > 0) set to -1 where the vertex is not
+// guided.  A kernel of its own, at full
 // occupancy, so that the dependent gathers (hit -> primitive -> material) overlap.
 __global__ __launch_bounds__(256) void k_guide_features(PtSceneView sc, GuideSceneNorm nm, const int *count, int segStride, int segLen,
                                                         const float4 *rayO, const float4 *rayD, float4 camO, const float4 *hits, int maxRows,
@@ -177,8 +189,10 @@ __global__ __launch_bounds__(256) void k_guide_features(PtSceneView sc, GuideSce
             for (int k = 0; k < GUIDE_RAW; k++) f[k] = 0.f;
             f[GUIDE_RAW - 1] = -1.f;
         }
-        feat[2 * (size_t)i] = make_float4(f[0], f[1], f[2], f[3]);
-        feat[2 * (size_t)i + 1] = make_float4(f[4], f[5], f[6], f[7]);
+        // even raw features, then odd ones: the inference lanes that encode raw features
+        // 2 i (2 i + 1) of a row read one float4
+        feat[2 * (size_t)i] = make_float4(f[0], f[2], f[4], f[6]);
+        feat[2 * (size_t)i + 1] = make_float4(f[1], f[3], f[5], f[7]);
     }
 }
 
@@ -221,39 +235,45 @@ __global__ __launch_bounds__(64 * INFER_WAVES, INFER_BLOCKS_PER_CU) void k_guide
     stageBiases(net.b, (float *)sB);
     __syncthreads();
     const int waveStride = gridDim.x * INFER_WAVES;
-    // A tile's inputs: the features of this lane's four rows rowBase + 16 rb + lr (both float4
-    // of a row; rawFeats has the same layout) and its segment's record count.  The next tile's
-    // are loaded while the current one runs through the network.
-    const float4 *src = rawFeats ? (const float4 *)rawFeats : feat;
-    auto loadTile = [&](int tile, float4 *fa, float4 *fb, int &cnt) {
+    // A tile's inputs: for each of this lane's four rows rowBase + 16 rb + lr the four raw
+    // features it encodes (2 i + hi, i = 0..3: one float4 of k_guide_features' interleaved
+    // layout; gathered from the row-major rawFeats), and the segment's record count.  The next
+    // tile's are loaded while the current one runs through the network.
+    const int hi = g >> 1;
+    auto loadTile = [&](int tile, float4 *fv, int &cnt) {
         const int rowBase = tile * INFER_ROWS;
         cnt = count ? count[(rowBase / segLen) * segStride] : maxRows;
 #pragma unroll
         for (int rb = 0; rb < 4; rb++) {
             const int row = rowBase + rb * 16 + lr;
-            fa[rb] = fb[rb] = make_float4(0.f, 0.f, 0.f, -1.f);
+            fv[rb] = make_float4(0.f, 0.f, 0.f, -1.f);
             if (row < maxRows) {
-                fa[rb] = src[2 * (size_t)row];
-                fb[rb] = src[2 * (size_t)row + 1];
+                if (rawFeats) {
+                    const float *r = rawFeats + (size_t)row * GUIDE_RAW + hi;
+                    fv[rb] = make_float4(r[0], r[2], r[4], r[6]);
+                } else {
+                    fv[rb] = feat[2 * (size_t)row + hi];
+                }
             }
         }
     };
     int tile = blockIdx.x * INFER_WAVES + wave;
-    float4 fa[4], fb[4];
+    float4 fv[4];
     int cnt = 0;
-    if (tile * INFER_ROWS < maxRows) loadTile(tile, fa, fb, cnt);
+    if (tile * INFER_ROWS < maxRows) loadTile(tile, fv, cnt);
     for (; tile * INFER_ROWS < maxRows; tile += waveStride) {
-        float4 na[4], nb[4];
+        float4 nv[4];
         int ncnt = 0;
-        if ((tile + waveStride) * INFER_ROWS < maxRows) loadTile(tile + waveStride, na, nb, ncnt);
+        if ((tile + waveStride) * INFER_ROWS < maxRows) loadTile(tile + waveStride, nv, ncnt);
         const int rowBase = tile * INFER_ROWS;
         // rows [rowBase, n) of this tile hold records: the filled part of its queue segment
         const int n = count ? (rowBase / segLen) * segLen + min(cnt, segLen) : maxRows;
-        // guided rows per 16-row block (bit lr of each ballot); this lane's output row is
-        // rowBase + lane, in block lane >> 4
+        // guided rows per 16-row block: bit 32 + lr of each ballot (the lanes with hi = 1 hold
+        // raw feature 7, the roughness, -1 where the vertex is not guided)
         uint64_t bal[4];
 #pragma unroll
-        for (int rb = 0; rb < 4; rb++) bal[rb] = __ballot(rowBase + rb * 16 + lr < n && (rawFeats != nullptr || fb[rb].w >= 0.f));
+        for (int rb = 0; rb < 4; rb++)
+            bal[rb] = __ballot(rowBase + rb * 16 + lr < n && (rawFeats != nullptr || (hi && fv[rb].w >= 0.f)));
         const int mine = rowBase + lane;
         if (rowBase < n && (bal[0] | bal[1] | bal[2] | bal[3]) == 0) {
             // no guided vertex in the tile (misses, mirror and glass hits): only the flag that
@@ -266,14 +286,8 @@ __global__ __launch_bounds__(64 * INFER_WAVES, INFER_BLOCKS_PER_CU) void k_guide
             bf16x8 act[4][2];
 #pragma unroll
             for (int rb = 0; rb < 4; rb++) {
-                const bool on = (bal[rb] >> lr) & 1u;
-                const bool hi = (g >> 1) != 0;
-                const float4 a = fa[rb], b = fb[rb];
-                float f[4];
-                f[0] = on ? (hi ? a.y : a.x) : 0.f;
-                f[1] = on ? (hi ? a.w : a.z) : 0.f;
-                f[2] = on ? (hi ? b.y : b.x) : 0.f;
-                f[3] = on ? (hi ? b.w : b.z) : 0.f;
+                const bool on = (bal[rb] >> (32 + lr)) & 1u;
+                const float f[4] = {on ? fv[rb].x : 0.f, on ? fv[rb].y : 0.f, on ? fv[rb].z : 0.f, on ? fv[rb].w : 0.f};
 #pragma unroll
                 for (int ks = 0; ks < 2; ks++)
 #pragma unroll
@@ -301,9 +315,13 @@ __global__ __launch_bounds__(64 * INFER_WAVES, INFER_BLOCKS_PER_CU) void k_guide
 #pragma unroll
                     for (int rb = 0; rb < 4; rb++)
 #pragma unroll
-                        for (int ks = 0; ks < 2; ks++)
+                        for (int ks = 0; ks < 2; ks++) {
+                            u32x4 q;
 #pragma unroll
-                            for (int p = 0; p < 8; p++) act[rb][ks][p] = (__bf16)relu(acc[rb][2 * ks + (p >> 2)][p & 3]);
+                            for (int h = 0; h < 4; h++)
+                                q[h] = reluPackBf16(acc[rb][2 * ks + (h >> 1)][2 * (h & 1)], acc[rb][2 * ks + (h >> 1)][2 * (h & 1) + 1]);
+                            act[rb][ks] = __builtin_bit_cast(bf16x8, q);
+                        }
                 } else {
                     // the last layer's outputs: the four lanes l, l ^ 16, l ^ 32, l ^ 48 hold a
                     // row's outputs in guideOutOf order (two lobes each)
@@ -322,7 +340,7 @@ __global__ __launch_bounds__(64 * INFER_WAVES, INFER_BLOCKS_PER_CU) void k_guide
                             const uint4 q = mixtureQuarter(v, g);
                             if (row < n) {
                                 // a vertex that is not guided gets only the zero selection probability
-                                if ((bal[rb] >> lr) & 1u) ((uint4 *)out)[(size_t)row * GUIDE_REC_F4 + g] = q;
+                                if ((bal[rb] >> (32 + lr)) & 1u) ((uint4 *)out)[(size_t)row * GUIDE_REC_F4 + g] = q;
                                 else if (g == 3) ((uint4 *)out)[(size_t)row * GUIDE_REC_F4 + 3] = make_uint4(0u, 0u, 0u, 0u);
                             }
                         }
@@ -331,10 +349,7 @@ __global__ __launch_bounds__(64 * INFER_WAVES, INFER_BLOCKS_PER_CU) void k_guide
             }
         }
 #pragma unroll
-        for (int rb = 0; rb < 4; rb++) {
-            fa[rb] = na[rb];
-            fb[rb] = nb[rb];
-        }
+        for (int rb = 0; rb < 4; rb++) fv[rb] = nv[rb];
         cnt = ncnt;
     }
 }
@@ -504,13 +519,17 @@ __global__ __launch_bounds__(64 * TRAIN_WAVES) void k_guide_train(GuideNet net, 
             if (!last) {
                 uint32_t m = 0;
 #pragma unroll
-                for (int ks = 0; ks < 2; ks++)
+                for (int ks = 0; ks < 2; ks++) {
+                    u32x4 q;
 #pragma unroll
-                    for (int p = 0; p < 8; p++) {
-                        const __bf16 h = (__bf16)relu(acc[2 * ks + (p >> 2)][p & 3]);
-                        act[L + 1][ks][p] = h;
-                        m |= ((float)h > 0.f ? 1u : 0u) << (ks * 8 + p);
+                    for (int h = 0; h < 4; h++) {
+                        q[h] = reluPackBf16(acc[2 * ks + (h >> 1)][2 * (h & 1)], acc[2 * ks + (h >> 1)][2 * (h & 1) + 1]);
+                        // a positive BF16 value is a positive int16
+                        m |= ((q[h] & 0xffffu) != 0u ? 1u : 0u) << (ks * 8 + 2 * h);
+                        m |= ((q[h] >> 16) != 0u ? 1u : 0u) << (ks * 8 + 2 * h + 1);
                     }
+                    act[L + 1][ks] = __builtin_bit_cast(bf16x8, q);
+                }
                 mask[L] = m;
             } else {
 #pragma unroll
