@@ -850,11 +850,11 @@ __device__ __forceinline__ float pdfSceneLight(const PtSceneView &sc, f3 ref, co
 // Guided-vertex inputs of the shade kernel (ignored by the plain integrator)
 struct GuideShadeArgs {
     const float4 *mix;      // GUIDE_REC_F4 float4 per queue slot (inference output)
+    const float4 *feat;     // the slot's raw features (k_guide_features: even, then odd ones)
     float4 *records;        // GUIDE_TREC_F4 float4 per training record, recSeg per queue segment
     int recSeg;
     const int *recIn;       // training record of the vertex that spawned each queued ray (-1: none)
     int *recOut;
-    GuideSceneNorm norm;
     float fracCap;          // renderer-side cap on the guided-sampling probability (defensive share)
 };
 
@@ -1141,12 +1141,11 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
                             int ridx = -1;
                             if (wantRec && slotRec < ga.recSeg) {
                                 ridx = seg * ga.recSeg + slotRec;
-                                float f[GUIDE_RAW];
-                                const float rough = type == PT_MAT_LAMBERTIAN ? 1.f : fminf(m.b.w * (1.f / 0.6f), 1.f);
-                                guideFeatures(ga.norm, sp.p, sp.n, -d, rough, f);
+                                // the features the inference of this vertex used, in raw order
+                                const float4 fe = ga.feat[2 * (size_t)i], fo = ga.feat[2 * (size_t)i + 1];
                                 float4 *R = ga.records + (size_t)ridx * GUIDE_TREC_F4;
-                                R[0] = make_float4(f[0], f[1], f[2], f[3]);
-                                R[1] = make_float4(f[4], f[5], f[6], f[7]);
+                                R[0] = make_float4(fe.x, fo.x, fe.y, fo.y);
+                                R[1] = make_float4(fe.z, fo.z, fe.w, fo.w);
                                 R[2] = f4(wworld, pbRec);
                                 R[3] = make_float4(1.f / lt, asFloat(pathId), -1.f, 0.f);
                             }
@@ -1763,11 +1762,11 @@ void runChunk(pt_renderer *r, const ChunkParams &cp) {
         }
         if (guided) {
             ga.mix = guidedBounce ? r->gMix.as<float4>() : nullptr;
+            ga.feat = guidedBounce ? r->gFeat.as<float4>() : nullptr;
             ga.records = r->gRecords.as<float4>();
             ga.recSeg = (int)(r->gRecCap / PT_QUEUE_SEGS);
             ga.recIn = r->gRecIdx[cur].as<int>();
             ga.recOut = r->gRecIdx[1 - cur].as<int>();
-            ga.norm = guideNorm(r);
             ga.fracCap = r->gFracCap;
         }
         {
