@@ -419,6 +419,11 @@ static constexpr int kTraceDone = INT32_MIN;   // LaneRay::node once the stack i
 // 32 -> 3250 Mrays/s; 0, wait for every lane: 3207; on the current tree 8 -> 4395,
 // 12 -> 4420, 16 -> 4408)
 static constexpr int kDescentTail = 12;
+// Node steps per check of the descent's exit conditions: two steps save the ballots of every
+// other step (C1 4452 -> 4498 Mrays/s with the trace kernels' register cap raised from 72 to
+// 80, six waves per SIMD, which the default 11 blocks per CU do not exceed; at 72 registers the
+// second step spills).
+static constexpr int kDescentSteps = 2;
 
 // Per-lane traversal counts of the instrumented trace kernels (STATS).
 struct TraceCount {
@@ -461,6 +466,11 @@ __device__ __forceinline__ bool traceRound(const PtSceneView &sc, LaneRay &r, co
         const uint64_t searching = __ballot(leaf == 0 && r.node != kTraceDone);
         if (searching == 0) break;
         if (__popcll(searching) <= kDescentTail && __ballot(leaf != 0) != 0) break;
+        // kDescentSteps node steps per check of the exit conditions: a lane that parks a leaf
+        // in the first step keeps descending internal nodes in the next (as it would while
+        // other lanes search) and idles at its second leaf, so only the checks' cost changes
+#pragma unroll
+        for (int step = 0; step < kDescentSteps; step++) {
         if (r.node >= 0) {
             if constexpr (STATS) tc->nodes++;
             // 32-bit offsets from the array base (a node array stays below 4 GiB); a node is
@@ -518,6 +528,7 @@ __device__ __forceinline__ bool traceRound(const PtSceneView &sc, LaneRay &r, co
             pop();
         }
         // lanes holding a parked leaf and a second one, or done, wait for the others
+        }
     }
     if (leaf != 0) {
         const int e = ~leaf;
@@ -526,10 +537,13 @@ __device__ __forceinline__ bool traceRound(const PtSceneView &sc, LaneRay &r, co
         for (int j = 0; j < cnt; j++) {
             // all three records of the slot (spheres have three too) are fetched before the
             // type test, so the loads overlap
-            const size_t k = 3 * (size_t)(first + j);
+            // 32-bit byte offsets from the array base (the primitive records stay below 4 GiB:
+            // pt_renderer_set_scene's limit), like the node loads
+            const uint32_t off = (uint32_t)(first + j) * 48u;
             if constexpr (STATS) tc->prims++;
-            const float4 p0 = sc.prims[k];
-            float4 p1 = sc.prims[k + 1], p2 = sc.prims[k + 2];
+            const char *pr = (const char *)sc.prims;
+            const float4 p0 = *(const float4 *)(pr + off);
+            float4 p1 = *(const float4 *)(pr + (off + 16u)), p2 = *(const float4 *)(pr + (off + 32u));
             // keeps the triangle-only components' loads ahead of the type test (the compiler
             // would otherwise issue them inside the triangle branch, after the first ones return)
             asm volatile("" : "+v"(p1.y), "+v"(p1.z), "+v"(p2.x), "+v"(p2.y), "+v"(p2.z));
@@ -663,7 +677,7 @@ __device__ __forceinline__ void traceQueue(const PtSceneView &sc, int *counters,
 // STATS: the instrumented build that adds its rays, node visits and primitive tests to
 // tstat[0..2].
 template <int STACK, bool BATCH = false, bool STATS = false>
-__global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 7 : 5) void k_extend(PtSceneView sc, int *counters, int bounce, int segLen,
+__global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 6 : 5) void k_extend(PtSceneView sc, int *counters, int bounce, int segLen,
                                                                               const float4 *rayO, const float4 *rayD, float4 camO, float4 *hits,
                                                                               int refill, int *spill, int spillCap, int *overflow,
                                                                               ChunkParams cp = ChunkParams{}, unsigned long long *tstat = nullptr) {
@@ -706,7 +720,7 @@ __global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 7 : 5) void k_extend(P
 // unoccluded ray adds its light contribution to its path.
 // BATCH: sL.w holds the ray's render, whose root node starts the traversal.
 template <int STACK, bool BATCH = false, bool STATS = false>
-__global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 7 : 5) void k_shadow(PtSceneView sc, int *counters, int bounce, int segLen,
+__global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 6 : 5) void k_shadow(PtSceneView sc, int *counters, int bounce, int segLen,
                                                                               const float4 *sO, const float4 *sD, const float4 *sL, float4 *Lpath,
                                                                               int refill, int *spill, int spillCap, int *overflow,
                                                                               const BatchRender *renders = nullptr, unsigned long long *tstat = nullptr) {
@@ -1957,7 +1971,9 @@ int pt_renderer_set_scene(pt_renderer *r, const pt_scene *s) {
     if (!s) return fail(PT_ERR_INVALID, "pt_renderer_set_scene: scene is NULL");
     if (r->inFlight) return fail(PT_ERR_STATE, "pt_renderer_set_scene: a pass is still running");
     const int64_t nt = (int64_t)s->triMat.size(), ns = (int64_t)s->sphMat.size();
-    if (nt + ns >= (1ll << PT_LEAF_COUNT_SHIFT)) return fail(PT_ERR_INVALID, "pt_renderer_set_scene: too many primitives");
+    // the trace kernels address primitive records (48 B) with 32-bit byte offsets
+    if (nt + ns >= (1ll << PT_LEAF_COUNT_SHIFT) || (nt + ns) * 48 >= (1ll << 32))
+        return fail(PT_ERR_INVALID, "pt_renderer_set_scene: too many primitives (at most 89,478,485)");
     if (s->mats.empty()) return fail(PT_ERR_INVALID, "pt_renderer_set_scene: scene has no materials");
     PT_API_BEGIN
     PT_HIP_TRY(hipStreamSynchronize(r->stream));
@@ -2018,6 +2034,8 @@ int pt_renderer_set_scene(pt_renderer *r, const pt_scene *s) {
         r->sceneHi[k] = std::isfinite(hi[k]) ? hi[k] : 1.f;
     }
     buildBvh(r->primsIn.as<float4>(), (int)n, lo, hi, r->nodes, r->prims.as<float4>(), r->stream, &r->bvh, r->bvhBuilder);
+    // and 4-wide nodes (128 B) with 32-bit byte offsets
+    if ((int64_t)r->bvh.numNodes * PT_NODE_F4 * 16 >= (1ll << 32)) throw HipFailure("scene upload: BVH beyond 4 GiB of nodes");
     r->primsIn.release();
     // ordered 4-wide traversal pushes at most three references per level
     r->stackNeed = 3 * r->bvh.depth + 1;
@@ -2789,7 +2807,8 @@ void buildWave(pt_batch *b) {
         nE += r->view.numEmitters;
         w->stackNeed = std::max(w->stackNeed, r->stackNeed);
     }
-    if (nP >= (1ll << PT_LEAF_COUNT_SHIFT) || nN >= INT32_MAX / 2) throw HipFailure("batch: too many primitives for one wavefront");
+    if (nP >= (1ll << PT_LEAF_COUNT_SHIFT) || nP * 48 >= (1ll << 32) || nN * (int64_t)PT_NODE_F4 * 16 >= (1ll << 32))
+        throw HipFailure("batch: too many primitives for one wavefront");
     w->nodes.alloc(sizeof(float4) * PT_NODE_F4 * (size_t)std::max<int64_t>(nN, 1));
     w->prims.alloc(sizeof(float4) * 3 * (size_t)std::max<int64_t>(nP, 1));
     w->normals.alloc(sizeof(float4) * (size_t)std::max<int64_t>(nNrm, 1));
