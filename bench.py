@@ -286,6 +286,7 @@ def main():
                     help="default workload: timed runs of the 48-render batch reported as batch (0: skip)")
     ap.add_argument("--level", type=int, default=8, help="icosphere subdivision level")
     ap.add_argument("--pool", type=int, default=0, help="path pool size (0: library default)")
+    ap.add_argument("--trace-blocks", type=int, default=0, help="trace blocks per CU (0: library default)")
     ap.add_argument("--bvh", choices=["default", "lbvh", "ploc"], default="default",
                     help="binary tree under the 4-wide BVH (default: the library's)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -324,6 +325,8 @@ def main():
     r = P.Renderer(WIDTH, HEIGHT)
     if args.pool:
         r.set_option(P.OPT_POOL_PATHS, args.pool)
+    if args.trace_blocks:
+        r.set_option(P.OPT_TRACE_BLOCKS_PER_CU, args.trace_blocks)
     if args.bvh != "default":
         r.set_option(P.OPT_BVH_BUILDER, P.BVH_LBVH if args.bvh == "lbvh" else P.BVH_PLOC)
     t0 = time.time()
