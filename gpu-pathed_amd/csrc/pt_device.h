@@ -30,8 +30,21 @@ PT_INLINE f3 operator*(float s, f3 a) { return mk(a.x * s, a.y * s, a.z * s); }
 PT_INLINE f3 operator/(f3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
 PT_INLINE float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 PT_INLINE f3 cross(f3 a, f3 b) { return mk(a.y * b.z - a.z * b.y, -(a.x * b.z - a.z * b.x), a.x * b.y - a.y * b.x); }
-PT_INLINE float length(f3 a) { return sqrtf(dot(a, a)); }
-PT_INLINE f3 normalize(f3 a) { return a / length(a); }
+// Shading arithmetic uses the hardware reciprocal, square root and reciprocal square root
+// (v_rcp_f32, v_sqrt_f32, v_rsq_f32: within 1 ulp) instead of the correctly rounded quotient
+// and root (about 10 and 16 instructions each, most of a shading vertex's instructions).  Two
+// places keep IEEE arithmetic: the intersection tests (wtSetup, wtHit, sphereHit), whose
+// decisions are compared bit for bit, and the Beckmann half vector and exponent (mfD), where
+// exp(-tan^2 / alpha^2) at small alpha turns a last-bit difference into a relative error of
+// 1e-4 and more.
+PT_INLINE float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+PT_INLINE float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+PT_INLINE float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+PT_INLINE float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+constexpr float kInvPi = 0.318309886183790671538f;
+PT_INLINE float length(f3 a) { return fsqrt(dot(a, a)); }
+PT_INLINE f3 normalize(f3 a) { return a * frsq(dot(a, a)); }
+PT_INLINE f3 normalizeIeee(f3 a) { return a / sqrtf(dot(a, a)); }
 
 // n / d and n % d for 0 <= n < 2^31, d > 0, from a float estimate with the float reciprocal
 // of d, corrected to the exact integer result (the estimate is off by at most a few units for
@@ -104,10 +117,10 @@ PT_INLINE Frame makeFrame(f3 n) {
     Frame f;
     f.n = n;
     if (fabsf(n.x) > fabsf(n.y)) {
-        const float inv = 1.0f / sqrtf(n.x * n.x + n.z * n.z);
+        const float inv = frsq(n.x * n.x + n.z * n.z);
         f.t = mk(n.z * inv, 0.f, -n.x * inv);
     } else {
-        const float inv = 1.0f / sqrtf(n.y * n.y + n.z * n.z);
+        const float inv = frsq(n.y * n.y + n.z * n.z);
         f.t = mk(0.f, n.z * inv, -n.y * inv);
     }
     f.s = cross(f.t, n);
@@ -131,32 +144,31 @@ struct BsdfSample {
     bool delta;
 };
 
-// sample.h:7-17 uniform hemisphere (z = xi1)
+// sample.h:7-17 uniform hemisphere (z = xi1); the azimuth 2 pi xi2 through sincospi, whose
+// argument reduction is exact (sincos of a radian angle reduces by a multi-word 2 pi)
 PT_INLINE f3 uniformHemisphere(float xi1, float xi2) {
     const float z = xi1;
-    const float r = sqrtf(fmaxf(0.f, 1.f - z * z));
-    const float phi = 6.28318530717958647692f * xi2;
+    const float r = fsqrt(fmaxf(0.f, 1.f - z * z));
     float sp, cp;
-    sincosf(phi, &sp, &cp);
+    sincospif(2.f * xi2, &sp, &cp);
     return mk(r * cp, r * sp, z);
 }
 // sample.h:26-37
 PT_INLINE f3 uniformSphere(float xi1, float xi2) {
     const float z = xi1 * 2.f - 1.f;
-    const float r = sqrtf(fmaxf(0.f, 1.f - z * z));
-    const float phi = 6.28318530717958647692f * xi2;
+    const float r = fsqrt(fmaxf(0.f, 1.f - z * z));
     float sp, cp;
-    sincosf(phi, &sp, &cp);
+    sincospif(2.f * xi2, &sp, &cp);
     return mk(r * cp, r * sp, z);
 }
 
 // fresnel.h:17-40
 PT_INLINE float fresnelDielectric(float cosI, float etaI, float etaT) {
-    const float sinT = (etaI / etaT) * sqrtf(fmaxf(0.f, 1.f - cosI * cosI));
+    const float sinT = fdiv(etaI, etaT) * fsqrt(fmaxf(0.f, 1.f - cosI * cosI));
     if (sinT > 1.f) return 1.f;
-    const float cosT = sqrtf(fmaxf(0.f, 1.f - sinT * sinT));
-    const float rPar = (etaT * cosI - etaI * cosT) / (etaT * cosI + etaI * cosT);
-    const float rPer = (etaI * cosI - etaT * cosT) / (etaI * cosI + etaT * cosT);
+    const float cosT = fsqrt(fmaxf(0.f, 1.f - sinT * sinT));
+    const float rPar = fdiv(etaT * cosI - etaI * cosT, etaT * cosI + etaI * cosT);
+    const float rPer = fdiv(etaI * cosI - etaT * cosT, etaI * cosI + etaT * cosT);
     return 0.5f * (rPar * rPar + rPer * rPer);
 }
 // snell.h:10-43 (including its second eta swap for vectors below the normal)
@@ -167,9 +179,9 @@ PT_INLINE f3 snellRefract(f3 wi, float etaI, float etaT) {
         const float t = etaI; etaI = etaT; etaT = t;
     }
     const f3 perpI = wi - n * dot(wi, n);
-    const f3 perpT = -perpI * (etaI / etaT);
+    const f3 perpT = -perpI * fdiv(etaI, etaT);
     const float l = length(perpT);
-    const float parLen = sqrtf(fmaxf(0.f, 1.f - l * l));
+    const float parLen = fsqrt(fmaxf(0.f, 1.f - l * l));
     return normalize(n * -parLen + perpT);
 }
 
@@ -182,14 +194,14 @@ PT_INLINE float mfD(float alpha, f3 w, f3 wh) {
     const float c2 = wh.z * wh.z;
     const float t2 = (1.f - c2) / c2;
     if (isinf(t2)) return 0.f;
-    return expf(-t2 / a2) / (3.14159265358979f * a2 * c2 * c2);
+    return expf(-t2 / a2) * frcp(3.14159265358979f * a2 * c2 * c2);
 }
 // microfacet.h:74-84 Walter 2007 rational G1
 PT_INLINE float mfG1(float alpha, f3 v) {
-    const float sinT = sqrtf(fmaxf(0.f, 1.f - v.z * v.z));
-    const float a = 1.f / (alpha * fabsf(sinT / v.z));
+    const float sinT = fsqrt(fmaxf(0.f, 1.f - v.z * v.z));
+    const float a = frcp(alpha * fabsf(fdiv(sinT, v.z)));
     if (a >= 1.6f) return 1.f;
-    return (3.535f * a + 2.181f * a * a) / (1.f + 2.276f * a + 2.577f * a * a);
+    return fdiv(3.535f * a + 2.181f * a * a, 1.f + 2.276f * a + 2.577f * a * a);
 }
 PT_INLINE float signOf(float x) { return x < 0.f ? -1.f : 1.f; }
 // microfacet.h:98-122
@@ -199,19 +211,19 @@ PT_INLINE f3 microfacetF(float alpha, f3 wo, f3 wi) {
     if (cosO == 0.f || cosI == 0.f) return mk(0.f);
     const f3 sum = wo + wi;
     if (isZero(sum)) return mk(0.f);
-    const f3 wh = normalize(sum);
+    const f3 wh = normalizeIeee(sum);
     if (signOf(dot(wo, wh)) != signOf(wo.z) || signOf(dot(wi, wh)) != signOf(wi.z)) return mk(0.f);
     const float cosInc = fminf(1.f, fmaxf(dot(wi, wh), 0.f));
     const float F = fresnelDielectric(cosInc, 1.f, 1.5f);
     const float D = mfD(alpha, wi, wh);
     const float G = mfG1(alpha, wi) * mfG1(alpha, wo);
-    return mk(D * G * F / (4.f * cosI * cosO));
+    return mk(fdiv(D * G * F, 4.f * cosI * cosO));
 }
 // microfacet.h:124-134 (D with the widened alpha at |cos wi|)
 PT_INLINE float microfacetPdf(float alpha, f3 wo, f3 wi) {
     if (wo.z < 0.f || wi.z < 0.f) return 0.f;
-    const f3 wh = normalize(wo + wi);
-    return mfD(alpha, wi, wh) * fabsf(wh.z) / (4.f * fabsf(dot(wo, wh)));
+    const f3 wh = normalizeIeee(wo + wi);
+    return fdiv(mfD(alpha, wi, wh) * fabsf(wh.z), 4.f * fabsf(dot(wo, wh)));
 }
 
 // bsdf.h:20-39
@@ -219,7 +231,7 @@ PT_INLINE f3 bsdfF(const PtMaterial &m, f3 wo, f3 wi) {
     const int t = matType(m);
     if (t == PT_MAT_LAMBERTIAN) {  // lambertian.h:43-49
         if (wo.z < 0.f || wi.z < 0.f) return mk(0.f);
-        return mk(m.a.x, m.a.y, m.a.z) / 3.14159265358979f;
+        return mk(m.a.x, m.a.y, m.a.z) * kInvPi;
     }
     if (t == PT_MAT_MICROFACET) return microfacetF(m.b.w, wo, wi);
     return mk(0.f);
@@ -241,11 +253,11 @@ PT_INLINE float bsdfSamplePdf(const PtMaterial &m, f3 wo, f3 wi) {
     if (t == PT_MAT_MICROFACET) {
         const f3 s = wo + wi;
         if (isZero(s)) return 0.f;
-        f3 wh = normalize(s);
+        f3 wh = normalizeIeee(s);
         if (wh.z < 0.f) wh = -wh;
         const float d = fabsf(dot(wi, wh));
         if (!(d > 0.f)) return 0.f;
-        return mfD(m.b.w, wo, wh) * fabsf(wh.z) / (4.f * d);
+        return fdiv(mfD(m.b.w, wo, wh) * fabsf(wh.z), 4.f * d);
     }
     return 0.f;
 }
@@ -254,13 +266,13 @@ PT_INLINE float bsdfSamplePdf(const PtMaterial &m, f3 wo, f3 wi) {
 // Mises-Fisher distribution on S^2") and its density kappa / (2 pi (1 - e^{-2 kappa}))
 // exp(kappa (mu.w - 1)).
 PT_INLINE float vmfDensity(f3 mu, float kappa, f3 w) {
-    return kappa / (6.28318530718f * (1.f - __expf(-2.f * kappa))) * __expf(kappa * (dot(mu, w) - 1.f));
+    return kappa * frcp(6.28318530718f * (1.f - __expf(-2.f * kappa))) * __expf(kappa * (dot(mu, w) - 1.f));
 }
 PT_INLINE f3 sampleVmf(f3 mu, float kappa, float u1, float u2) {
-    const float w = fminf(fmaxf(1.f + __logf(u1 + (1.f - u1) * __expf(-2.f * kappa)) / kappa, -1.f), 1.f);
-    const float s = sqrtf(fmaxf(0.f, 1.f - w * w));
+    const float w = fminf(fmaxf(1.f + __logf(u1 + (1.f - u1) * __expf(-2.f * kappa)) * frcp(kappa), -1.f), 1.f);
+    const float s = fsqrt(fmaxf(0.f, 1.f - w * w));
     float sp, cp;
-    sincosf(6.28318530717958647692f * u2, &sp, &cp);
+    sincospif(2.f * u2, &sp, &cp);
     return toWorld(makeFrame(mu), mk(s * cp, s * sp, w));
 }
 
@@ -275,12 +287,12 @@ PT_INLINE BsdfSample bsdfSample(const PtMaterial &m, f3 wo, float xi1, float xi2
     if (t == PT_MAT_LAMBERTIAN) {  // lambertian.h:67-76
         s.wi = uniformHemisphere(xi1, xi2);
         s.pdf = 0.15915494309189535f;
-        s.f = (wo.z < 0.f || s.wi.z < 0.f) ? mk(0.f) : mk(m.a.x, m.a.y, m.a.z) / 3.14159265358979f;
+        s.f = (wo.z < 0.f || s.wi.z < 0.f) ? mk(0.f) : mk(m.a.x, m.a.y, m.a.z) * kInvPi;
         s.delta = false;
     } else if (t == PT_MAT_MIRROR) {  // mirror.h:45-53
         s.wi = mk(-wo.x, -wo.y, wo.z);
         s.pdf = 1.f;
-        s.f = mk(fmaxf(0.f, 1.f / s.wi.z));
+        s.f = mk(fmaxf(0.f, frcp(s.wi.z)));
         s.delta = true;
     } else if (t == PT_MAT_GLASS) {  // glass.h:44-109
         float etaI = 1.f, etaT = m.b.w;
@@ -291,27 +303,27 @@ PT_INLINE BsdfSample bsdfSample(const PtMaterial &m, f3 wo, float xi1, float xi2
             s.wi = mk(-wo.x, -wo.y, wo.z);
             const float c = fabsf(s.wi.z);
             s.pdf = F;
-            s.f = c == 0.f ? mk(0.f) : mk(F / c);
+            s.f = c == 0.f ? mk(0.f) : mk(fdiv(F, c));
         } else {
             s.wi = snellRefract(wo, etaI, etaT);
             const float Tr = 1.f - F;
             const float c = fabsf(s.wi.z);
-            const float corr = (etaI / etaT) * (etaI / etaT);
+            const float eta = fdiv(etaI, etaT);
             s.pdf = Tr;
-            s.f = c == 0.f ? mk(0.f) : mk(Tr / c * corr);
+            s.f = c == 0.f ? mk(0.f) : mk(fdiv(Tr, c) * (eta * eta));
         }
     } else if (t == PT_MAT_MICROFACET) {  // microfacet.h:53-63, 148-158
         const float alpha = m.b.w;
-        const float phi = xi1 * 6.28318530717958647692f;
         const float a = mfAlpha(alpha, fabsf(wo.z));
         const float sq = -(a * a) * logf(1.f - xi2);
-        const float theta = atanf(sqrtf(sq));
-        float st, ct, sp, cp;
-        sincosf(theta, &st, &ct);
-        sincosf(phi, &sp, &cp);
+        // theta = atan(sqrt(sq)): cos theta = 1 / sqrt(1 + sq), sin theta = sqrt(sq) cos theta
+        const float ct = frsq(1.f + sq);
+        const float st = fsqrt(sq) * ct;
+        float sp, cp;
+        sincospif(2.f * xi1, &sp, &cp);  // phi = 2 pi xi1
         const f3 wh = mk(st * cp, st * sp, ct);
         s.wi = reflect(wo, wh);
-        s.pdf = mfD(alpha, wo, wh) * fabsf(wh.z) / (4.f * fabsf(dot(s.wi, wh)));
+        s.pdf = fdiv(mfD(alpha, wo, wh) * fabsf(wh.z), 4.f * fabsf(dot(s.wi, wh)));
         s.f = microfacetF(alpha, wo, s.wi);
         s.delta = false;
     } else {
@@ -360,8 +372,8 @@ PT_INLINE f3 envEmitPdf(const PtEnv &e, f3 w, float *pdf) {
     int x, y;
     envTexel(e, phi, theta, &x, &y);
     const float4 v = e.rgba[(size_t)y * e.width + x];
-    const float st = sqrtf(w.x * w.x + w.z * w.z);
-    *pdf = st > 0.f ? v.w / st : 0.f;
+    const float st = fsqrt(w.x * w.x + w.z * w.z);
+    *pdf = st > 0.f ? fdiv(v.w, st) : 0.f;
     return mk(v.x, v.y, v.z);
 }
 PT_INLINE float envPdf(const PtEnv &e, f3 w) {
@@ -379,23 +391,21 @@ PT_INLINE int aliasDraw(const float2 *tab, int n, float xi, float *rem) {
     // one 8-byte load: the alias is otherwise fetched after the comparison, a second round trip
     asm volatile("" : "+v"(e.y));
     if (f < e.x) {
-        *rem = f / e.x;
+        *rem = fdiv(f, e.x);
         return k;
     }
-    *rem = (f - e.x) / (1.0f - e.x);
+    *rem = fdiv(f - e.x, 1.0f - e.x);
     return asInt(e.y);
 }
 PT_INLINE f3 envSample(const PtEnv &e, float xi1, float xi2, float *pdf, f3 *emit) {
     float ry, rx;
     const int y = aliasDraw(e.marginal, e.height, xi1, &ry);
     const int x = aliasDraw(e.rows + (size_t)y * e.width, e.width, xi2, &rx);
-    const float u = fminf((x + rx) / (float)e.width, 0x1.fffffep-1f);
-    const float v = fminf((y + ry) / (float)e.height, 0x1.fffffep-1f);
-    const float phi = u * 6.28318530717958647692f;
-    const float theta = v * 3.14159265358979323846f;
+    const float u = fminf(fdiv(x + rx, (float)e.width), 0x1.fffffep-1f);
+    const float v = fminf(fdiv(y + ry, (float)e.height), 0x1.fffffep-1f);
     float st, ct, sp, cp;
-    sincosf(theta, &st, &ct);
-    sincosf(phi, &sp, &cp);
+    sincospif(v, &st, &ct);
+    sincospif(2.f * u, &sp, &cp);
     const f3 w = mk(st * cp, ct, st * sp);
     *emit = envEmitPdf(e, w, pdf);
     return w;
@@ -406,7 +416,7 @@ PT_INLINE f3 envSample(const PtEnv &e, float xi1, float xi2, float *pdf, f3 *emi
 // p2 = (v2, original index); tag >= 0: index into the per-vertex normal array, -1: flat.
 // Sphere: p0 = (centre, mat), p1 = (radius, 0, 0, -2), p2 = (0, 0, 0, ~sphere index).
 // ------------------------------------------------------------------------------------------
-PT_INLINE float triArea(f3 p0, f3 p1, f3 p2) { return fabsf(length(cross(p1 - p0, p2 - p0)) / 2.f); }
+PT_INLINE float triArea(f3 p0, f3 p1, f3 p2) { return fabsf(length(cross(p1 - p0, p2 - p0)) * 0.5f); }
 PT_INLINE float sphereArea(float r) { return 12.566370614359172f * r * r; }
 
 struct Ray {
@@ -486,7 +496,7 @@ PT_INLINE bool sphereHit(f3 c, float rad, f3 o, f3 d, float tMin, float tMax, fl
     return false;
 }
 
-PT_INLINE float balance(float p1, float p2) { return p1 / (p1 + p2); }  // mis.h:5-8
+PT_INLINE float balance(float p1, float p2) { return fdiv(p1, p1 + p2); }  // mis.h:5-8
 
 // color.h:7-28 sRGB transfer and framebuffer.cu:35-40's byte conversion (clamp to [0, 1],
 // times 255, truncated).  The power is evaluated in FP64 and rounded once to float (a

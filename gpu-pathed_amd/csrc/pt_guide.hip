@@ -74,10 +74,6 @@ __device__ __forceinline__ uint32_t reluPackBf16(float a, float b) {
     return __builtin_bit_cast(uint32_t, r);
 }
 
-// Reciprocal with the hardware's 1-ulp v_rcp_f32 (an IEEE division expands to ten
-// instructions; the network's outputs are BF16-limited anyway).
-__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
-
 // Sums and maxima over the four lanes that hold one record's outputs (lanes l, l ^ 16,
 // l ^ 32, l ^ 48); all four must be active.
 __device__ __forceinline__ float sum4(float v) {

@@ -239,8 +239,8 @@ __device__ __forceinline__ void waveSum(int v, int *counter) {
 // ray generation (camera.h:40-61, PCG32 stream per (pixel, sample))
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ void cameraRay(const PtCamera &c, int row, int col, float sx, float sy, f3 *o, f3 *d) {
-    const float xc = (col + sx) / c.W;
-    const float yc = (row + sy) / c.H;
+    const float xc = fdiv(col + sx, (float)c.W);
+    const float yc = fdiv(row + sy, (float)c.H);
     const float y = yc * c.height - c.top;
     const float x = xc * c.width - c.right;
     const f3 dl = normalize(mk(x, y, -1.f));
@@ -776,7 +776,7 @@ __device__ __forceinline__ SurfacePoint surfaceAt(const PtSceneView &sc, int slo
         const f3 c = xyz(p0);
         const float r = p1.x;
         s.p = o + d * t;
-        s.n = normalize((s.p - c) / r);
+        s.n = normalize((s.p - c) * frcp(r));
         s.area = sphereArea(r);
     } else {
         const f3 a = xyz(p0), b = xyz(p1), c = xyz(p2);
@@ -804,7 +804,7 @@ __device__ __forceinline__ LightSample sampleLights(const PtSceneView &sc, f3 p,
     const int nl = sc.numEmitters;
     const int choiceSize = nl + (sc.env.present ? 1 : 0);
     const int choice = (int)fminf((float)(choiceSize - 1), floorf(l1 * (float)choiceSize));
-    const float choicePdf = 1.f / (float)choiceSize;
+    const float choicePdf = frcp((float)choiceSize);
     if (choice < nl) {
         const float4 e0 = sc.emitters[3 * choice];
         float4 e1 = sc.emitters[3 * choice + 1], e2 = sc.emitters[3 * choice + 2];
@@ -817,21 +817,21 @@ __device__ __forceinline__ LightSample sampleLights(const PtSceneView &sc, f3 p,
             const f3 pl = uniformSphere(l2, l3);
             pt = pl * e1.x + xyz(e0);
             n = pl;
-            pdfA = 1.f / sphereArea(e1.x);
+            pdfA = frcp(sphereArea(e1.x));
         } else {
             const f3 a = xyz(e0), b = xyz(e1), c = xyz(e2);
-            const float sr = sqrtf(l2);
+            const float sr = fsqrt(l2);
             const float wa = 1.f - sr, wb = sr * (1.f - l3);
             const float wc = 1.f - wa - wb;
             pt = a * wa + b * wb + c * wc;
             n = normalize(cross(b - a, c - a));
-            pdfA = 1.f / triArea(a, b, c);
+            pdfA = frcp(triArea(a, b, c));
         }
         const f3 dir = pt - p;
         const float dist = length(dir);
         const f3 wdir = normalize(dir);
         const float proj = fabsf(dot(n, -wdir));
-        const float sa = proj == 0.f ? 0.f : pdfA * (dist * dist) / proj;
+        const float sa = proj == 0.f ? 0.f : fdiv(pdfA * (dist * dist), proj);
         const PtMaterial m = sc.materials[asInt(e0.w)];
         ls.wi = wdir;
         ls.n = n;
@@ -857,8 +857,8 @@ __device__ __forceinline__ float pdfSceneLight(const PtSceneView &sc, f3 ref, co
     const f3 wo = normalize(dir);
     const float dist = length(dir);
     const float proj = fmaxf(0.f, dot(s.n, wo));
-    const float pdfSA = (1.f / s.area) * (dist * dist) / proj;
-    return pdfSA / (float)(sc.numEmitters + (sc.env.present ? 1 : 0));
+    const float pdfSA = fdiv(frcp(s.area) * (dist * dist), proj);
+    return fdiv(pdfSA, (float)(sc.numEmitters + (sc.env.present ? 1 : 0)));
 }
 
 // Guided-vertex inputs of the shade kernel (ignored by the plain integrator)
@@ -1033,7 +1033,7 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
                     // sampler.h:127-140: light-selection share of the environment density
                     float pe;
                     const f3 e = envEmitPdf(lv.env, d, &pe);
-                    L = thr * e * balance(prevPdf, pe * (1.f / (float)(lv.numEmitters + 1)));
+                    L = thr * e * balance(prevPdf, pe * frcp((float)(lv.numEmitters + 1)));
                 }
             } else {
                 const SurfacePoint sp = surfaceAt(lv, slot, o, d, h.x, h.y, h.z);
@@ -1118,7 +1118,7 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
                             if (GUIDED && guidedV) pB = frac * mixturePdf(G, ls.wi) + (1.f - frac) * bsdfSamplePdf(m, wo, wiL);
                             else pB = cp.misSample ? bsdfSamplePdf(m, wo, wiL) : bsdfPdf(m, wo, wiL);
                             const float w = balance(ls.pdf, pB);
-                            contrib = thr * (ls.emitted * w * bsdfF(m, wo, wiL) * fabsf(dot(sp.n, ls.wi)) / ls.pdf);
+                            contrib = thr * (ls.emitted * w * bsdfF(m, wo, wiL) * (fabsf(dot(sp.n, ls.wi)) * frcp(ls.pdf)));
                             pushShadow = !isZero(contrib) && !(contrib.x != contrib.x);
                         }
                         spawned += pushShadow ? 1 : 0;
@@ -1132,7 +1132,7 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
                     bool pushRay = false;
                     f3 nb = mk(0.f);
                     if (!isZero(bs.f) && bs.pdf > 0.f) {
-                        nb = thr * (bs.f * fabsf(bs.wi.z) / bs.pdf);
+                        nb = thr * (bs.f * (fabsf(bs.wi.z) * frcp(bs.pdf)));
                         pushRay = !isZero(nb) && nb.x == nb.x;
                     }
                     spawned += pushRay ? 1 : 0;
@@ -1161,7 +1161,7 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
                                 R[0] = make_float4(fe.x, fo.x, fe.y, fo.y);
                                 R[1] = make_float4(fe.z, fo.z, fe.w, fo.w);
                                 R[2] = f4(wworld, pbRec);
-                                R[3] = make_float4(1.f / lt, asFloat(pathId), -1.f, 0.f);
+                                R[3] = make_float4(frcp(lt), asFloat(pathId), -1.f, 0.f);
                             }
                             ga.recOut[rs] = ridx;
                         }
