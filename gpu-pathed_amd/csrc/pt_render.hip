@@ -216,6 +216,36 @@ __device__ __forceinline__ int segAppendLanes(bool want, int *counters, int seg,
     }
     return k;
 }
+// segAppendLanes in two steps, so that the atomic's round trip overlaps other work: the issue
+// (the same atomics, made by the first appending lane of each segment) and, later, the slot.
+struct PendingAppend {
+    int base;     // the atomic's result, in the leader lanes
+    int leader;   // this lane's leader, or -1 for a lane without a record
+    int below;    // this lane's index among its segment's appending lanes
+};
+__device__ __forceinline__ PendingAppend segAppendIssue(bool want, int *counters, int seg, int bounce, int field) {
+    PendingAppend a{0, -1, 0};
+    uint64_t todo = __ballot(want);
+    while (todo != 0) {
+        const int lead = __ffsll((unsigned long long)todo) - 1;
+        const int s = __shfl(seg, lead);
+        const bool mine = want && seg == s;
+        const uint64_t m = __ballot(mine);
+        if (laneId() == lead) a.base = atomicAdd(counters + ptQueueField(s, bounce, field), __popcll(m));
+        if (mine) {
+            a.leader = lead;
+            a.below = (int)lanesBelow(m);
+        }
+        todo &= ~m;
+    }
+    return a;
+}
+// The slot (seg * segLen + index, or -1) of an issued append.  Every lane that was active at
+// the issue must be active here: the leaders hand their atomic's result to their lanes.
+__device__ __forceinline__ int segAppendSlot(const PendingAppend &a, int seg, int segLen) {
+    const int base = __shfl(a.base, a.leader < 0 ? laneId() : a.leader);
+    return a.leader < 0 ? -1 : seg * segLen + base + a.below;
+}
 // Adds each lane's v (> 0) to counter[key]: one atomic per distinct key among those lanes.
 __device__ __forceinline__ void waveAddKeyed(int v, int key, unsigned long long *counter) {
     uint64_t todo = __ballot(v > 0);
@@ -1103,14 +1133,23 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
                         bs = bsdfSample(m, wo, x1, x2);
                         pdfDir = bs.pdf;
                     }
+                    // the continuation's queue slot is claimed here and used after the light
+                    // sample, whose work hides the atomic's round trip
+                    bool pushRay = false;
+                    f3 nb = mk(0.f);
+                    if (!isZero(bs.f) && bs.pdf > 0.f) {
+                        nb = thr * (bs.f * (fabsf(bs.wi.z) * frcp(bs.pdf)));
+                        pushRay = !isZero(nb) && nb.x == nb.x;
+                    }
+                    const PendingAppend rayAppend = segAppendIssue(pushRay, counters, seg, bounce + 1, PT_Q_RAYS);
+                    bool pushShadow = false;
+                    f3 contrib = mk(0.f);
+                    LightSample ls;
                     if (cp.nee && !bs.delta) {
                         // directSampleLights (path_tracer.cu:83-126); a scene without lights
                         // draws the three uniforms and adds nothing
                         const float l1 = rng.uniform(), l2 = rng.uniform(), l3 = rng.uniform();
                         const bool anyLight = lv.numEmitters > 0 || lv.env.present;
-                        bool pushShadow = false;
-                        f3 contrib = mk(0.f);
-                        LightSample ls;
                         if (anyLight) ls = sampleLights(lv, sp.p, l1, l2, l3);
                         if (anyLight && dot(ls.n, ls.wi) < 0.f && ls.pdf > 0.f) {
                             const f3 wiL = toLocal(fr, ls.wi);
@@ -1121,22 +1160,10 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
                             contrib = thr * (ls.emitted * w * bsdfF(m, wo, wiL) * (fabsf(dot(sp.n, ls.wi)) * frcp(ls.pdf)));
                             pushShadow = !isZero(contrib) && !(contrib.x != contrib.x);
                         }
-                        spawned += pushShadow ? 1 : 0;
-                        const int ss = segAppendLanes(pushShadow, counters, seg, bounce, PT_Q_SHADOW, cp.segLen);
-                        if (pushShadow) {
-                            sO[ss] = f4(sp.p, ls.dist - 2e-4f);
-                            sD[ss] = f4(ls.wi, asFloat(pathId));
-                            sL[ss] = f4(contrib, BATCH ? asFloat(rid) : 0.f);
-                        }
                     }
-                    bool pushRay = false;
-                    f3 nb = mk(0.f);
-                    if (!isZero(bs.f) && bs.pdf > 0.f) {
-                        nb = thr * (bs.f * (fabsf(bs.wi.z) * frcp(bs.pdf)));
-                        pushRay = !isZero(nb) && nb.x == nb.x;
-                    }
-                    spawned += pushRay ? 1 : 0;
-                    const int rs = segAppendLanes(pushRay, counters, seg, bounce + 1, PT_Q_RAYS, cp.segLen);
+                    const PendingAppend shadowAppend = segAppendIssue(pushShadow, counters, seg, bounce, PT_Q_SHADOW);
+                    spawned += (pushShadow ? 1 : 0) + (pushRay ? 1 : 0);
+                    const int rs = segAppendSlot(rayAppend, seg, cp.segLen);
                     const f3 wworld = toWorld(fr, bs.wi);
                     if (pushRay) {
                         nRayO[rs] = f4(sp.p, BATCH ? asFloat(rid) : kBounceEps);
@@ -1144,6 +1171,12 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
                         nState[rs] = make_uint4((uint32_t)rng.state, (uint32_t)(rng.state >> 32), (uint32_t)pathId,
                                                 BATCH ? packDepth(depth + 1, rid) : (uint32_t)(depth + 1));
                         nBeta[rs] = f4(nb, bs.delta ? -1.f : pdfDir);
+                    }
+                    const int ss = segAppendSlot(shadowAppend, seg, cp.segLen);
+                    if (pushShadow) {
+                        sO[ss] = f4(sp.p, ls.dist - 2e-4f);
+                        sD[ss] = f4(ls.wi, asFloat(pathId));
+                        sL[ss] = f4(contrib, BATCH ? asFloat(rid) : 0.f);
                     }
                     if constexpr (GUIDED) {
                         // a training record per guided vertex whose path continues
