@@ -138,8 +138,8 @@ struct BatchBucket {
 
 // The group of a chunk path that lies past its bucket's group (in a group that starts inside
 // the bucket).
-__device__ __forceinline__ int batchLaterGroup(const ChunkParams &cp, const BatchBucket &bk, int path) {
-    int g = bk.group + 1;
+__device__ __forceinline__ int batchLaterGroup(const ChunkParams &cp, int bucketGroup, int path) {
+    int g = bucketGroup + 1;
     while (g + 1 < cp.numGroups && cp.groups[g + 1].pathBegin <= path) g++;
     return g;
 }
@@ -147,7 +147,7 @@ __device__ __forceinline__ int batchLaterGroup(const ChunkParams &cp, const Batc
 // The render of a chunk path of a batch wavefront.
 __device__ __forceinline__ int batchRenderOf(const ChunkParams &cp, int path) {
     const BatchBucket bk = cp.buckets[path >> kBatchBucketShift];
-    return path < bk.nextBegin ? bk.render : cp.groups[batchLaterGroup(cp, bk, path)].render;
+    return path < bk.nextBegin ? bk.render : cp.groups[batchLaterGroup(cp, bk.group, path)].render;
 }
 
 // Render, pixel, sample row and PCG32 seed of a chunk path of a batch wavefront.
@@ -162,7 +162,7 @@ __device__ __forceinline__ PathSource batchPathSource(const ChunkParams &cp, int
     ps.render = bk.render;
     ps.seed = bk.seed;
     if (path >= bk.nextBegin) {
-        const BatchGroup gr = cp.groups[batchLaterGroup(cp, bk, path)];
+        const BatchGroup gr = cp.groups[batchLaterGroup(cp, bk.group, path)];
         begin = gr.pathBegin;
         npix = gr.npix;
         sample0 = gr.sampleBegin;
@@ -615,30 +615,33 @@ static constexpr int kQueueSteal = 8;
 struct SegCursor {
     int seg;    // segment being drained (wave-uniform)
     int left;   // segments this wave may still visit
+    int n;      // the segment's record count once read (-1: not yet)
 };
 
 __device__ __forceinline__ int homeSegment() { return (int)(((blockIdx.x * blockDim.x + threadIdx.x) >> 6) % PT_QUEUE_SEGS); }
-__device__ __forceinline__ SegCursor segStart() { return SegCursor{homeSegment(), 1 + kQueueSteal}; }
+__device__ __forceinline__ SegCursor segStart() { return SegCursor{homeSegment(), 1 + kQueueSteal, -1}; }
 
 // Claims up to `want` records of the bounce's queue (count field `countField`, cursor field
 // `fetchField`): returns how many were granted, the first at slot *slot, or 0 when this wave
-// has no segment left.  Wave-uniform.
+// has no segment left.  Wave-uniform.  A queue being drained is complete (its kernel appends
+// to other queues), so a segment's count is read once per visit.
 __device__ __forceinline__ int segClaim(int *counters, int bounce, int countField, int fetchField, int segLen, int want, SegCursor &c,
                                         int *slot) {
     while (c.left > 0) {
         int *seg = counters + c.seg * PT_SEG_STRIDE;
-        const int n = seg[ptQueueField(0, bounce, countField)];
-        if (n > 0) {
+        if (c.n < 0) c.n = seg[ptQueueField(0, bounce, countField)];
+        if (c.n > 0) {
             int b = 0;
             if (laneId() == 0) b = atomicAdd(&seg[ptQueueField(0, bounce, fetchField)], want);
             b = __shfl(b, 0);
-            if (b < n) {
+            if (b < c.n) {
                 *slot = c.seg * segLen + b;
-                return min(want, n - b);
+                return min(want, c.n - b);
             }
         }
         c.seg = c.seg + 1 == PT_QUEUE_SEGS ? 0 : c.seg + 1;
         c.left--;
+        c.n = -1;
     }
     return 0;
 }
@@ -959,47 +962,60 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
     // shadow rays in sL.w, and each render counts the rays its paths spawn.
     //
     // Misses (a fifth to most of a queue) take a few instructions, hits the whole BSDF and light
-    // code.  A wave shades the misses of each claimed run at once and parks the hits' slots in
-    // LDS, and shades hits 64 at a time, so the long hit code runs on full waves.  Every vertex
-    // is shaded exactly once (the last partial batch when the queue is exhausted); results do
-    // not depend on the order.
+    // code.  A wave claims 128 records at a time (two per lane, their hit words loaded
+    // together), parks the slots of hits and of misses in two LDS lists and shades either 64
+    // at a time, so both codes run on full waves.  Every vertex is shaded exactly once (the
+    // last partial batches when the queue is exhausted); results do not depend on the order.
     const bool camera = camO.w != 0.f;
-    __shared__ int sPendSlot[4][128], sPendSeg[4][128];   // per wave: parked hits (< 128)
-    int *pSlot = sPendSlot[threadIdx.x >> 6], *pSeg = sPendSeg[threadIdx.x >> 6];
+    // per wave: parked slots (the segment is slot / segLen), each list below 192 (at most 63
+    // left over plus one claim)
+    __shared__ int sHit[4][192], sMiss[4][192];
+    int *pHit = sHit[threadIdx.x >> 6], *pMiss = sMiss[threadIdx.x >> 6];
     SegCursor cur = segStart();
     int adds = 0;
-    int npend = 0;          // parked hits (wave-uniform)
-    bool claiming = true;   // the wave's segments still have rays
+    int nHit = 0, nMiss = 0;   // parked records (wave-uniform)
+    bool claiming = true;      // the wave's segments still have rays
     while (true) {
         int i = 0, seg = 0;
         bool active;
-        if (npend >= 64 || (!claiming && npend > 0)) {
-            const int nb = min(npend, 64);
-            npend -= nb;
+        const bool hitBatch = nHit >= 64 || (!claiming && nHit > 0);
+        if (hitBatch || nMiss >= 64 || (!claiming && nMiss > 0)) {
+            const int n = hitBatch ? nHit : nMiss;
+            const int nb = min(n, 64);
+            const int at = n - nb;
+            if (hitBatch) nHit = at;
+            else nMiss = at;
             active = laneId() < nb;
             if (active) {
-                i = pSlot[npend + laneId()];
-                seg = pSeg[npend + laneId()];
+                i = (hitBatch ? pHit : pMiss)[at + laneId()];
+                int k;
+                seg = divmodEst(i, cp.segLen, &k);
             }
         } else if (claiming) {
             int base = 0;
-            const int got = segClaim(counters, bounce, PT_Q_RAYS, PT_Q_FETCH_SHADE, cp.segLen, 64, cur, &base);
+            const int got = segClaim(counters, bounce, PT_Q_RAYS, PT_Q_FETCH_SHADE, cp.segLen, 128, cur, &base);
             if (got == 0) {
                 claiming = false;
                 continue;
             }
-            seg = cur.seg;
-            i = base + laneId();
-            const bool valid = laneId() < got;
-            const bool hit = valid && asInt(hits[i].w) != INT32_MIN;
-            const uint64_t hm = __ballot(hit);
-            if (hit) {
-                const int at = npend + (int)lanesBelow(hm);
-                pSlot[at] = i;
-                pSeg[at] = seg;
-            }
-            npend += __popcll(hm);
-            active = valid && !hit;
+            // both hit words in flight at once (indices clamped to the claimed run)
+            const int last = base + got - 1;
+            const int i0 = base + laneId(), i1 = i0 + 64;
+            const int w0 = asInt(hits[min(i0, last)].w), w1 = asInt(hits[min(i1, last)].w);
+            const bool v0 = i0 <= last, v1 = i1 <= last;
+            const bool h0 = v0 && w0 != INT32_MIN, m0 = v0 && !h0, h1 = v1 && w1 != INT32_MIN, m1 = v1 && !h1;
+            const uint64_t bh0 = __ballot(h0), bm0 = __ballot(m0), bh1 = __ballot(h1), bm1 = __ballot(m1);
+            if (h0) pHit[nHit + (int)lanesBelow(bh0)] = i0;
+            if (m0) pMiss[nMiss + (int)lanesBelow(bm0)] = i0;
+            nHit += __popcll(bh0);
+            nMiss += __popcll(bm0);
+            if (h1) pHit[nHit + (int)lanesBelow(bh1)] = i1;
+            if (m1) pMiss[nMiss + (int)lanesBelow(bm1)] = i1;
+            nHit += __popcll(bh1);
+            nMiss += __popcll(bm1);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            continue;
         } else {
             break;
         }
@@ -1109,11 +1125,22 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
                             float pi[GUIDE_LOBES];
                             guideLobeWeights(G.w, pi);
                             const float ul = rng.uniform();
-                            int k = 0;
-                            float acc = pi[0];
-                            while (k < GUIDE_LOBES - 1 && ul >= acc) acc += pi[++k];
+                            // lobe k: the first whose running weight sum exceeds ul (the last
+                            // when rounding leaves none), its record words picked by selects
+                            // (a dynamically indexed record would live in scratch memory)
+                            uint32_t wMu = G.w[2 * (GUIDE_LOBES - 1)], wKappa = G.w[2 * (GUIDE_LOBES - 1) + 1];
+                            float acc = 0.f;
+                            bool found = false;
+#pragma unroll
+                            for (int j = 0; j < GUIDE_LOBES - 1; j++) {
+                                acc += pi[j];
+                                const bool take = !found && ul < acc;
+                                wMu = take ? G.w[2 * j] : wMu;
+                                wKappa = take ? G.w[2 * j + 1] : wKappa;
+                                found = found || take;
+                            }
                             const float u1 = rng.uniform(), u2 = rng.uniform();
-                            const f3 ww = sampleVmf(guideOctDecode(G.w[2 * k]), guideHalfLo(G.w[2 * k + 1]), u1, u2);
+                            const f3 ww = sampleVmf(guideOctDecode(wMu), guideHalfLo(wKappa), u1, u2);
                             bs.wi = toLocal(fr, ww);
                             bs.f = bsdfF(m, wo, bs.wi);
                             bs.delta = false;
