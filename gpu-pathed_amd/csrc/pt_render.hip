@@ -611,6 +611,8 @@ __device__ __forceinline__ TraceStack traceStack(int *sstack, int *spill, int sp
 // it leaves.  Every segment has home waves (each kernel launches at least PT_QUEUE_SEGS
 // waves), so every record is claimed; stealing only evens out the segments' ends.
 static constexpr int kQueueSteal = 8;
+// Queue records a shade wave claims at a time, per lane.
+static constexpr int kShadeClaimWords = 2;
 
 struct SegCursor {
     int seg;    // segment being drained (wave-uniform)
@@ -962,14 +964,14 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
     // shadow rays in sL.w, and each render counts the rays its paths spawn.
     //
     // Misses (a fifth to most of a queue) take a few instructions, hits the whole BSDF and light
-    // code.  A wave claims 128 records at a time (two per lane, their hit words loaded
+    // code.  A wave claims 64 x kShadeClaimWords records at a time (their hit words loaded
     // together), parks the slots of hits and of misses in two LDS lists and shades either 64
     // at a time, so both codes run on full waves.  Every vertex is shaded exactly once (the
     // last partial batches when the queue is exhausted); results do not depend on the order.
     const bool camera = camO.w != 0.f;
-    // per wave: parked slots (the segment is slot / segLen), each list below 192 (at most 63
-    // left over plus one claim)
-    __shared__ int sHit[4][192], sMiss[4][192];
+    // per wave: parked slots (the segment is slot / segLen), each list below 64 + one claim
+    // (at most 63 left over plus the claim)
+    __shared__ int sHit[4][64 * (kShadeClaimWords + 1)], sMiss[4][64 * (kShadeClaimWords + 1)];
     int *pHit = sHit[threadIdx.x >> 6], *pMiss = sMiss[threadIdx.x >> 6];
     SegCursor cur = segStart();
     int adds = 0;
@@ -993,26 +995,26 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
             }
         } else if (claiming) {
             int base = 0;
-            const int got = segClaim(counters, bounce, PT_Q_RAYS, PT_Q_FETCH_SHADE, cp.segLen, 128, cur, &base);
+            const int got = segClaim(counters, bounce, PT_Q_RAYS, PT_Q_FETCH_SHADE, cp.segLen, 64 * kShadeClaimWords, cur, &base);
             if (got == 0) {
                 claiming = false;
                 continue;
             }
-            // both hit words in flight at once (indices clamped to the claimed run)
+            // the lane's hit words all in flight at once (indices clamped to the claimed run)
             const int last = base + got - 1;
-            const int i0 = base + laneId(), i1 = i0 + 64;
-            const int w0 = asInt(hits[min(i0, last)].w), w1 = asInt(hits[min(i1, last)].w);
-            const bool v0 = i0 <= last, v1 = i1 <= last;
-            const bool h0 = v0 && w0 != INT32_MIN, m0 = v0 && !h0, h1 = v1 && w1 != INT32_MIN, m1 = v1 && !h1;
-            const uint64_t bh0 = __ballot(h0), bm0 = __ballot(m0), bh1 = __ballot(h1), bm1 = __ballot(m1);
-            if (h0) pHit[nHit + (int)lanesBelow(bh0)] = i0;
-            if (m0) pMiss[nMiss + (int)lanesBelow(bm0)] = i0;
-            nHit += __popcll(bh0);
-            nMiss += __popcll(bm0);
-            if (h1) pHit[nHit + (int)lanesBelow(bh1)] = i1;
-            if (m1) pMiss[nMiss + (int)lanesBelow(bm1)] = i1;
-            nHit += __popcll(bh1);
-            nMiss += __popcll(bm1);
+            int w[kShadeClaimWords];
+#pragma unroll
+            for (int q = 0; q < kShadeClaimWords; q++) w[q] = asInt(hits[min(base + laneId() + 64 * q, last)].w);
+#pragma unroll
+            for (int q = 0; q < kShadeClaimWords; q++) {
+                const int iq = base + laneId() + 64 * q;
+                const bool hq = iq <= last && w[q] != INT32_MIN, mq = iq <= last && !hq;
+                const uint64_t bh = __ballot(hq), bm = __ballot(mq);
+                if (hq) pHit[nHit + (int)lanesBelow(bh)] = iq;
+                if (mq) pMiss[nMiss + (int)lanesBelow(bm)] = iq;
+                nHit += __popcll(bh);
+                nMiss += __popcll(bm);
+            }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
             continue;
