@@ -114,6 +114,10 @@ struct ChunkParams {
     int W, H;
     uint64_t seed;
     int segLen;       // queue segment length, ptSegLen(nPaths)
+    // Path numbering of the chunk (the path index in state.z, the Lpath slot, the camera queue's
+    // order): 1: pixel-major, path = pixel * samples + sample row, so the camera bounce's waves
+    // trace the samples of one pixel together; 0: sample-major, path = sample row * npix + pixel
+    int pixelMajor = 0;
     // batch wavefront only (nullptr otherwise): the chunk's groups, the batch's renders, and
     // per render the rays its paths spawned (continuation + shadow)
     const BatchGroup *groups = nullptr;
@@ -174,6 +178,16 @@ __device__ __forceinline__ PathSource batchPathSource(const ChunkParams &cp, int
     ps.pixel = pixel;
     ps.sample = sample0 + s;
     return ps;
+}
+
+// Pixel within the chunk and sample row of chunk path `path`.  Pixel-major numbering puts the
+// samples of one pixel (nearly the same ray: the same BVH nodes, the same primitive) in
+// neighbouring lanes of the camera bounce's waves, and their radiance in neighbouring slots.
+__device__ __forceinline__ int chunkPixelOf(const ChunkParams &cp, int path, int *sample) {
+    if (cp.pixelMajor) return divmodEst(path, cp.samples, sample);
+    int px;
+    *sample = divmodEst(path, cp.npix, &px);
+    return px;
 }
 
 // Batch wavefront path state: depth in the low 16 bits of state.w, render above.
@@ -301,8 +315,8 @@ __global__ __launch_bounds__(256) void k_queue_reset(int *counters, int field, i
 __global__ __launch_bounds__(256) void k_raygen(ChunkParams cp, PtCamera cam, float4 *rayD) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= cp.nPaths) return;
-    int rem, col;
-    const int s = divmodEst(i, cp.npix, &rem);
+    int col, s;
+    const int rem = chunkPixelOf(cp, i, &s);
     const int pixel = cp.pixBegin + rem;
     const int sample = cp.sampleBase + s;
     const int row = divmodEst(pixel, cp.W, &col);
@@ -335,9 +349,8 @@ __device__ __forceinline__ float4 batchCameraOrigin(const BatchRender &R) { retu
 
 // The camera-ray state k_raygen does not store: the RNG after the pixel's two jitter draws.
 __device__ __forceinline__ Rng cameraRng(const ChunkParams &cp, int path) {
-    int rem;
-    const int s = divmodEst(path, cp.npix, &rem);
-    const int pixel = cp.pixBegin + rem;
+    int s;
+    const int pixel = cp.pixBegin + chunkPixelOf(cp, path, &s);
     Rng rng = seedRng(cp.seed, (uint32_t)pixel, (uint32_t)(cp.sampleBase + s));
     rng.next();
     rng.next();
@@ -1101,9 +1114,8 @@ __global__ __launch_bounds__(256, GUIDED ? 3 : 4) void k_shade(PtSceneView sc, C
                         const PathSource ps = batchPathSource(cp, pathId);
                         rng.inc = streamInc(ps.seed, (uint32_t)ps.pixel, (uint32_t)ps.sample);
                     } else {
-                        int rem;
-                        const int s = divmodEst(pathId, cp.npix, &rem);
-                        const int pixel = cp.pixBegin + rem;
+                        int s;
+                        const int pixel = cp.pixBegin + chunkPixelOf(cp, pathId, &s);
                         rng.inc = streamInc(cp.seed, (uint32_t)pixel, (uint32_t)(cp.sampleBase + s));
                     }
                     rng.state = ((uint64_t)st.y << 32) | st.x;
@@ -1307,13 +1319,46 @@ __global__ __launch_bounds__(256) void k_accumulate(ChunkParams cp, const float4
         }
         if (threadIdx.x == 0) atomicAdd(&stats[S_PATHS], (unsigned long long)cp.nPaths);
     }
-    if (p >= cp.npix) return;
     double sx = 0.0, sy = 0.0, sz = 0.0;
-    for (int s = 0; s < cp.samples; s++) {
-        const float4 l = Lpath[(size_t)s * cp.npix + p];
-        sx += (double)l.x;
-        sy += (double)l.y;
-        sz += (double)l.z;
+    if (cp.pixelMajor) {
+        // A pixel's samples are contiguous: the block's 256 pixels are read in tiles of
+        // kAccTile samples per pixel (each pixel's run of records loaded by neighbouring
+        // lanes) into LDS, and each thread sums its pixel's samples in sample order, the same
+        // FP64 sums as the sample-major layout's.
+        constexpr int kAccTile = 8;
+        __shared__ float tx[256][kAccTile + 1], ty[256][kAccTile + 1], tz[256][kAccTile + 1];
+        const int p0 = blockIdx.x * blockDim.x;
+        const int npb = min((int)blockDim.x, cp.npix - p0);
+        for (int c0 = 0; c0 < cp.samples; c0 += kAccTile) {
+            const int cn = min(kAccTile, cp.samples - c0);
+            for (int e = threadIdx.x; e < npb * kAccTile; e += blockDim.x) {
+                const int j = e / kAccTile, k = e % kAccTile;
+                if (k < cn) {
+                    const float4 l = Lpath[(size_t)(p0 + j) * cp.samples + c0 + k];
+                    tx[j][k] = l.x;
+                    ty[j][k] = l.y;
+                    tz[j][k] = l.z;
+                }
+            }
+            __syncthreads();
+            if ((int)threadIdx.x < npb) {
+                for (int k = 0; k < cn; k++) {
+                    sx += (double)tx[threadIdx.x][k];
+                    sy += (double)ty[threadIdx.x][k];
+                    sz += (double)tz[threadIdx.x][k];
+                }
+            }
+            __syncthreads();
+        }
+        if (p >= cp.npix) return;
+    } else {
+        if (p >= cp.npix) return;
+        for (int s = 0; s < cp.samples; s++) {
+            const float4 l = Lpath[(size_t)s * cp.npix + p];
+            sx += (double)l.x;
+            sy += (double)l.y;
+            sz += (double)l.z;
+        }
     }
     double *a = accum + 3 * (size_t)(cp.pixBegin + p);
     a[0] += sx;
@@ -1524,6 +1569,8 @@ struct pt_renderer {
     int passSpp = 0;
     int64_t lastPassPaths = 0;
     bool lastPassSingleChunk = false;
+    int lastPassSamples = 0;        // sample rows of the last pass's (single) chunk
+    bool lastPassPixelMajor = false;
     double gpuMs = 0.0;
     int64_t passes = 0;
     // kernel profiling
@@ -1916,6 +1963,7 @@ void planChunks(const pt_renderer *r, int spp, std::vector<ChunkParams> &out) {
         cp.W = r->W;
         cp.H = r->H;
         cp.seed = r->seed;
+        cp.pixelMajor = 1;
         out.push_back(cp);
     };
     if (npix <= pool) {
@@ -2328,6 +2376,8 @@ int pt_renderer_render_async(pt_renderer *r, int spp) {
     }
     r->lastPassPaths = (int64_t)npix * spp;
     r->lastPassSingleChunk = r->lastChunks.size() == 1;
+    r->lastPassSamples = r->lastChunks.empty() ? 0 : r->lastChunks[0].samples;
+    r->lastPassPixelMajor = !r->lastChunks.empty() && r->lastChunks[0].pixelMajor;
     tonemap(r);
     PT_HIP_TRY(hipEventRecord(r->evEnd, r->stream));
     r->inFlight = true;
@@ -2584,12 +2634,16 @@ int pt_renderer_get_path_radiance(pt_renderer *r, float *out, int64_t cap, int64
     PT_HIP_TRY(hipStreamSynchronize(r->stream));
     if (r->inFlight) finishPass(r);
     const int64_t n = std::min<int64_t>(cap, r->lastPassPaths);
-    std::vector<float4> tmp((size_t)n);
-    PT_HIP_TRY(hipMemcpy(tmp.data(), r->pp->Lpath.p, (size_t)n * sizeof(float4), hipMemcpyDeviceToHost));
+    std::vector<float4> tmp((size_t)r->lastPassPaths);
+    PT_HIP_TRY(hipMemcpy(tmp.data(), r->pp->Lpath.p, tmp.size() * sizeof(float4), hipMemcpyDeviceToHost));
+    // the pass's paths in sample-major order (path = s * npix + pixel), whatever the numbering
+    // the chunk used on the device
+    const int64_t samples = std::max(1, r->lastPassSamples), npix = r->lastPassPaths / samples;
     for (int64_t i = 0; i < n; i++) {
-        out[i * 3] = tmp[i].x;
-        out[i * 3 + 1] = tmp[i].y;
-        out[i * 3 + 2] = tmp[i].z;
+        const int64_t j = r->lastPassPixelMajor ? (i % npix) * samples + i / npix : i;
+        out[i * 3] = tmp[(size_t)j].x;
+        out[i * 3 + 1] = tmp[(size_t)j].y;
+        out[i * 3 + 2] = tmp[(size_t)j].z;
     }
     *count = n;
     return PT_OK;
