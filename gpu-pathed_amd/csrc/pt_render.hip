@@ -135,7 +135,7 @@ static constexpr int kBatchBucketShift = 12;
 // the group's render's PCG32 seed: one 32-byte load gives a path's render, pixel, sample
 // and stream.
 struct BatchBucket {
-    int group, pathBegin, render, npix;
+    int group, pathBegin, render, count;
     int sampleBegin, nextBegin;
     uint64_t seed;
 };
@@ -154,7 +154,8 @@ __device__ __forceinline__ int batchRenderOf(const ChunkParams &cp, int path) {
     return path < bk.nextBegin ? bk.render : cp.groups[batchLaterGroup(cp, bk.group, path)].render;
 }
 
-// Render, pixel, sample row and PCG32 seed of a chunk path of a batch wavefront.
+// Render, pixel, sample row and PCG32 seed of a chunk path of a batch wavefront (a group's
+// paths are numbered pixel-major, the samples of a pixel side by side).
 struct PathSource {
     int render, pixel, sample;
     uint64_t seed;
@@ -162,20 +163,20 @@ struct PathSource {
 __device__ __forceinline__ PathSource batchPathSource(const ChunkParams &cp, int path) {
     const BatchBucket bk = cp.buckets[path >> kBatchBucketShift];
     PathSource ps;
-    int begin = bk.pathBegin, npix = bk.npix, sample0 = bk.sampleBegin;
+    int begin = bk.pathBegin, count = bk.count, sample0 = bk.sampleBegin;
     ps.render = bk.render;
     ps.seed = bk.seed;
     if (path >= bk.nextBegin) {
         const BatchGroup gr = cp.groups[batchLaterGroup(cp, bk.group, path)];
         begin = gr.pathBegin;
-        npix = gr.npix;
+        count = gr.count;
         sample0 = gr.sampleBegin;
         ps.render = gr.render;
         ps.seed = cp.renders[gr.render].seed;
     }
-    int pixel;
-    const int s = divmodEst(path - begin, npix, &pixel);
-    ps.pixel = pixel;
+    // pixel-major inside a group: path - begin = pixel * count + sample row
+    int s;
+    ps.pixel = divmodEst(path - begin, count, &s);
     ps.sample = sample0 + s;
     return ps;
 }
@@ -1383,23 +1384,53 @@ __global__ __launch_bounds__(256) void k_accumulate_batch(ChunkParams cp, int ac
         }
         if (threadIdx.x == 0) atomicAdd(&stats[S_PATHS], (unsigned long long)cp.nPaths);
     }
-    if (t >= accThreads) return;
-    int lo = 0, hi = cp.numGroups - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (cp.groups[mid].accBegin <= t) lo = mid;
-        else hi = mid - 1;
+    // This thread's (group, pixel) and the run of records of its samples, [r0, r1).  The
+    // block's entries are consecutive, and so are their runs (groups are numbered pixel-major
+    // and laid out in order), so the block reads one contiguous range of Lpath in windows of
+    // kAccWindow records through LDS; each thread adds the records of its run that a window
+    // holds, in sample order (the sums of a per-thread loop over its samples).
+    constexpr int kAccWindow = 2048;
+    __shared__ float wx[kAccWindow], wy[kAccWindow], wz[kAccWindow];
+    __shared__ int range[2];
+    const bool valid = t < accThreads;
+    int r0 = 0, r1 = 0, render = 0, p = 0;
+    if (valid) {
+        int lo = 0, hi = cp.numGroups - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (cp.groups[mid].accBegin <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const BatchGroup g = cp.groups[lo];
+        p = t - g.accBegin;
+        render = g.render;
+        r0 = g.pathBegin + p * g.count;
+        r1 = r0 + g.count;
     }
-    const BatchGroup g = cp.groups[lo];
-    const int p = t - g.accBegin;
+    const int lastValid = min((int)blockDim.x, accThreads - (int)(blockIdx.x * blockDim.x)) - 1;
+    if ((int)threadIdx.x == 0) range[0] = r0;
+    if ((int)threadIdx.x == lastValid) range[1] = r1;
+    __syncthreads();
+    const int b0 = range[0], b1 = range[1];
     double sx = 0.0, sy = 0.0, sz = 0.0;
-    for (int s = 0; s < g.count; s++) {
-        const float4 l = Lpath[(size_t)g.pathBegin + (size_t)s * g.npix + p];
-        sx += (double)l.x;
-        sy += (double)l.y;
-        sz += (double)l.z;
+    for (int w0 = b0; w0 < b1; w0 += kAccWindow) {
+        const int wn = min(kAccWindow, b1 - w0);
+        for (int e = threadIdx.x; e < wn; e += blockDim.x) {
+            const float4 l = Lpath[(size_t)w0 + e];
+            wx[e] = l.x;
+            wy[e] = l.y;
+            wz[e] = l.z;
+        }
+        __syncthreads();
+        for (int k = max(r0, w0); k < min(r1, w0 + wn); k++) {
+            sx += (double)wx[k - w0];
+            sy += (double)wy[k - w0];
+            sz += (double)wz[k - w0];
+        }
+        __syncthreads();
     }
-    double *a = cp.renders[g.render].accum + 3 * (size_t)p;
+    if (!valid) return;
+    double *a = cp.renders[render].accum + 3 * (size_t)p;
     a[0] += sx;
     a[1] += sy;
     a[2] += sz;
@@ -3079,7 +3110,7 @@ void planWave(pt_batch *b, int scale) {
             bk.group = g;
             bk.pathBegin = gr.pathBegin;
             bk.render = gr.render;
-            bk.npix = gr.npix;
+            bk.count = gr.count;
             bk.sampleBegin = gr.sampleBegin;
             bk.nextBegin = g + 1 < c.cp.numGroups ? b->waveGroups[c.firstGroup + g + 1].pathBegin : INT32_MAX;
             bk.seed = b->jobs[gr.render]->seed;
@@ -3132,6 +3163,7 @@ void runWaveChunk(pt_batch *b, const WaveChunk &c) {
         one.W = r->W;
         one.H = r->H;
         one.seed = r->seed;
+        one.pixelMajor = 1;
         runChunk(r, one);
         return;
     }
