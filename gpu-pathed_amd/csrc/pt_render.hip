@@ -1753,16 +1753,18 @@ int *ensureSpill(pt_renderer *r, int grid, int *cap) {
     return r->pp->spill.as<int>();
 }
 
-// Idle lanes out of 64 at which a trace wave takes new rays.  Camera rays of neighbouring
-// pixels share most nodes, so their waves keep a batch together longer; the incoherent
-// queues refill earlier, which shortens the wait for a batch's longest ray.  Per-launch
-// optima on the 1.31M-triangle icosphere at 1024^2, 64 spp: camera rays 48, the later
-// closest-hit queues and the shadow rays 40 (whole batches of 64 take 1.9x as long on the
-// first bounce's queue; 16 idle lanes cost 6% of the pass).
+// Idle lanes out of 64 at which a trace wave takes new rays.  A camera run of 64 holds the
+// samples of one pixel (paths are numbered pixel-major), nearly the same ray, so a camera
+// wave takes a whole run at a time: refilling early would mix in a pixel 64 runs away.  The
+// incoherent queues refill earlier, which shortens the wait for a batch's longest ray.
+// Per-launch optima on the 1.31M-triangle icosphere at 1024^2, 64 spp: camera rays 64
+// (extend 1773 -> 1678 us per launch against 48; 40 and 56 as 48), the later closest-hit
+// queues and the shadow rays 40 (whole batches of 64 take 1.9x as long on the first bounce's
+// queue; 16 idle lanes cost 6% of the pass).
 enum { TRACE_CAMERA = 0, TRACE_INCOHERENT = 1, TRACE_SHADOW = 2 };
 int refillLanes(const pt_renderer *r, int kind) {
     if (r->refillLanes > 0) return r->refillLanes;
-    return kind == TRACE_CAMERA ? 48 : 40;
+    return kind == TRACE_CAMERA ? 64 : 40;
 }
 
 // Trace grids hold at least PT_QUEUE_SEGS waves: every queue segment needs home waves.
