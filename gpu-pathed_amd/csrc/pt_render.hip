@@ -765,6 +765,119 @@ __global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 6 : 5) void k_extend(P
         [&](int i, bool, const TraceHit &th) { hits[i] = make_float4(th.t, th.u, th.v, asFloat(th.prim)); }, tstat);
 }
 
+// Wave stack entries of the packet traversal.
+static constexpr int kPacketStack = 64;
+
+// Camera bounce of a pixel-major chunk whose runs of 64 camera rays are one pixel's samples,
+// nearly the same ray: a wave traverses the BVH once for the whole run (packet traversal on a
+// 64-wide wave, Wald et al. 2001, Aila and Laine 2009): one node sequence and one stack for
+// the wave, each lane testing a node's child boxes and a leaf's primitives against its own ray
+// and closest hit so far.  A child is visited when some lane's ray enters it before that
+// lane's closest hit; the visited children are taken nearest-first for the run's first ray,
+// those it misses last.  Every lane sees every leaf whose box any lane enters, so each ray
+// gets the closest hit among all primitives of the leaves its own box tests reach (a tie of
+// equal distances may resolve to another primitive than in the per-lane traversal).  Runs
+// are claimed 64 at a time; the wave stack holds kPacketStack entries (launched only when the
+// tree's 3 x depth + 1 fits, with an overflow counter besides).
+__global__ __launch_bounds__(kTraceThreads, 6) void k_extend_packet(PtSceneView sc, int *counters, int segLen, const float4 *rayD, float4 camO,
+                                                                   float4 *hits, int *overflow) {
+    __shared__ int wstack[kTraceThreads / 64][kPacketStack];
+    int *ws = wstack[threadIdx.x >> 6];
+    const char *nodes = (const char *)sc.nodes;
+    SegCursor cur = segStart();
+    while (true) {
+        int slot = 0;
+        const int got = segClaim(counters, 0, PT_Q_RAYS, PT_Q_FETCH_EXT, segLen, 64, cur, &slot);
+        if (got == 0) break;
+        const bool valid = laneId() < got;
+        const int i = slot + (valid ? laneId() : 0);
+        const float4 rd = rayD[i];
+        LaneRay r;
+        // a lane without a ray gets an empty interval: every box and primitive test fails
+        laneRayStart(r, xyz(camO), xyz(rd), 0.f, valid ? rd.w : -1.f, 0);
+        int node = 0, sp = 0;   // wave-uniform
+        while (true) {
+            if (node >= 0) {
+                const uint32_t base = (uint32_t)node * (uint32_t)(PT_NODE_F4 * 16);
+                const uint32_t ax = base | r.nX, ay = base | r.nY, az = base | r.nZ;
+                const float4 nx = *(const float4 *)(nodes + ax), fx = *(const float4 *)(nodes + (ax ^ 16u));
+                const float4 ny = *(const float4 *)(nodes + ay), fy = *(const float4 *)(nodes + (ay ^ 16u));
+                const float4 nz = *(const float4 *)(nodes + az), fz = *(const float4 *)(nodes + (az ^ 16u));
+                const int4 rf = *(const int4 *)(nodes + (base + 96u));
+                const float bnx[4] = {nx.x, nx.y, nx.z, nx.w}, bfx[4] = {fx.x, fx.y, fx.z, fx.w};
+                const float bny[4] = {ny.x, ny.y, ny.z, ny.w}, bfy[4] = {fy.x, fy.y, fy.z, fy.w};
+                const float bnz[4] = {nz.x, nz.y, nz.z, nz.w}, bfz[4] = {fz.x, fz.y, fz.z, fz.w};
+                int ref[4] = {__builtin_amdgcn_readfirstlane(rf.x), __builtin_amdgcn_readfirstlane(rf.y),
+                              __builtin_amdgcn_readfirstlane(rf.z), __builtin_amdgcn_readfirstlane(rf.w)};
+                float key[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const float lo = fmaxf(fmaxf(bnx[k] * r.ix - r.ox, bny[k] * r.iy - r.oy), fmaxf(bnz[k] * r.iz - r.oz, r.tmin));
+                    const float hi = fminf(fminf(bfx[k] * r.ix - r.ox, bfy[k] * r.iy - r.oy), fminf(bfz[k] * r.iz - r.oz, r.tmax));
+                    const bool in = lo <= hi;
+                    // the run's first lane orders the children; one that only other lanes enter
+                    // comes after its hits, one that no lane enters is not visited
+                    const float first = __builtin_amdgcn_readfirstlane(__float_as_int(in ? lo : INFINITY)) == __float_as_int(INFINITY)
+                                            ? FLT_MAX
+                                            : __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(in ? lo : INFINITY)));
+                    key[k] = __ballot(in) != 0 ? first : INFINITY;
+                }
+                cswap(key[0], ref[0], key[1], ref[1]);
+                cswap(key[2], ref[2], key[3], ref[3]);
+                cswap(key[0], ref[0], key[2], ref[2]);
+                cswap(key[1], ref[1], key[3], ref[3]);
+                cswap(key[1], ref[1], key[2], ref[2]);
+                if (key[0] != INFINITY) {
+                    const int more = (key[1] != INFINITY) + (key[2] != INFINITY) + (key[3] != INFINITY);
+                    if (sp + more <= kPacketStack) {
+                        // the farthest lowest, the second nearest on top
+                        if (more >= 3) ws[sp] = ref[3];
+                        if (more >= 2) ws[sp + more - 2] = ref[2];
+                        if (more >= 1) ws[sp + more - 1] = ref[1];
+                        sp += more;
+                    } else if (laneId() == 0) {
+                        atomicAdd(overflow, 1);
+                    }
+                    node = ref[0];
+                    continue;
+                }
+            } else {
+                // a leaf: every lane tests its primitives (as traceRound's leaf phase)
+                const int e = ~node;
+                const int first = e & ((1 << PT_LEAF_COUNT_SHIFT) - 1);
+                const int cnt = (e >> PT_LEAF_COUNT_SHIFT) + 1;
+                for (int j = 0; j < cnt; j++) {
+                    const uint32_t off = (uint32_t)(first + j) * 48u;
+                    const char *pr = (const char *)sc.prims;
+                    const float4 p0 = *(const float4 *)(pr + off);
+                    float4 p1 = *(const float4 *)(pr + (off + 16u)), p2 = *(const float4 *)(pr + (off + 32u));
+                    asm volatile("" : "+v"(p1.y), "+v"(p1.z), "+v"(p2.x), "+v"(p2.y), "+v"(p2.z));
+                    float t, u = 0.f, v = 0.f;
+                    bool h;
+                    if (asInt(p1.w) == PT_TAG_SPHERE) {
+                        h = sphereHit(xyz(p0), p1.x, r.wr.o, r.d, r.tmin, r.tmax, &t);
+                    } else {
+                        h = wtHit(r.wr, xyz(p0), xyz(p1), xyz(p2), r.tmin, r.tmax, &t, &u, &v);
+                    }
+                    if (h) {
+                        r.tmax = t;
+                        r.u = u;
+                        r.v = v;
+                        r.prim = first + j;
+                    }
+                }
+            }
+            if (sp == 0) break;
+            sp--;
+            node = __builtin_amdgcn_readfirstlane(ws[sp]);
+        }
+        if (valid) {
+            const TraceHit th = laneRayHit(r);
+            hits[i] = make_float4(th.t, th.u, th.v, asFloat(th.prim));
+        }
+    }
+}
+
 // Persistent any-hit kernel for the shadow rays of `bounce` (interval [1e-4, tmax]); an
 // unoccluded ray adds its light contribution to its path.
 // BATCH: sL.w holds the ray's render, whose root node starts the traversal.
@@ -1574,6 +1687,7 @@ struct pt_renderer {
     int stackNeed = 1;    // deepest traversal stack the tree can need (3 per 4-wide level)
     int ldsStack = 16;    // stack entries per lane kept in LDS; the rest spill to the pool's `spill`
     int refillLanes = 0;  // idle lanes of a trace wave that trigger taking new rays (0: per launch)
+    bool packetCamera = true;  // camera runs of one pixel's samples traced as one packet
     bool traceStats = false;   // PT_OPT_TRACE_STATS: instrumented trace kernels
     bool hasScene = false;
     // host copies needed for edits
@@ -1773,11 +1887,16 @@ int traceGrid(const pt_renderer *r) {
 }
 
 void launchExtend(pt_renderer *r, int bounce, int segLen, const float4 *ro, const float4 *rd, float4 *hits, int kind,
-                  float4 camO = make_float4(0.f, 0.f, 0.f, 0.f), const ChunkParams *batchChunk = nullptr) {
+                  float4 camO = make_float4(0.f, 0.f, 0.f, 0.f), const ChunkParams *batchChunk = nullptr, bool packet = false) {
     const int refill = refillLanes(r, kind);
     const int grid = traceGrid(r);
     int *ctr = r->pp->counters.as<int>();
     int *ov = r->pp->overflow.as<int>();
+    if (packet) {
+        k_extend_packet<<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, segLen, rd, camO, hits, ov);
+        PT_HIP_TRY(hipGetLastError());
+        return;
+    }
     int cap = 0;
     int *sp = ensureSpill(r, grid, &cap);
     if (batchChunk) {
@@ -1905,8 +2024,12 @@ void runChunk(pt_renderer *r, const ChunkParams &cp) {
         const float4 bo = b == 0 ? camO : noCam;
         {
             KTimer t(r, K_EXTEND);
+            // camera runs of one pixel's samples (pixel-major chunks of 64 sample rows or more):
+            // one traversal per run
+            const bool packet = b == 0 && cp.pixelMajor && cp.samples >= 64 && !r->traceStats && r->view.numNodes > 0 &&
+                                r->stackNeed <= kPacketStack && r->packetCamera;
             launchExtend(r, b, cp.segLen, r->pp->rayO[cur].as<float4>(), r->pp->rayD[cur].as<float4>(), r->pp->hits.as<float4>(),
-                         b == 0 ? TRACE_CAMERA : TRACE_INCOHERENT, bo);
+                         b == 0 ? TRACE_CAMERA : TRACE_INCOHERENT, bo, nullptr, packet);
         }
         GuideShadeArgs ga{};
         // vertices of bounces >= gBounces (when set) sample the BSDF only

@@ -95,6 +95,32 @@ def test_icosphere_c1_scene_per_path():
     assert np.isfinite(g).all() and g.mean() > 0
 
 
+def test_icosphere_camera_packets_per_path():
+    """A chunk of 64 sample rows traces each camera run (the 64 samples of one pixel) as one
+    packet on one wave stack: per-path parity with the oracle on the 1.31M-triangle icosphere
+    (16x16 pixels, 64 spp, depth 8, every other row compared), and the same path radiances as
+    the per-lane traversal (the instrumented trace kernels, PT_OPT_TRACE_STATS, trace per
+    lane), up to equal-distance ties."""
+    say("case: icosphere camera packets per-path (16x16x64, depth 8)")
+    s = P.Scene.preset(1, seed=0, param=8)
+    frac, g, cl, st = _per_path(s, 16, 16, 64, 8, rows=range(0, 16, 2))
+    say(f"per-path agreement {frac:.5f} over {g.shape[0]} paths; mean radiance {g.mean():.4f}")
+    assert frac >= 0.9995 and st.stack_overflows == 0
+    lanes = []
+    for stats in (0, 1):
+        r = P.Renderer(16, 16)
+        r.set_scene(s)
+        r.set_max_depth(8)
+        r.set_option(P.OPT_TRACE_STATS, stats)
+        r.render(64)
+        lanes.append(r.path_radiance(16 * 16 * 64))
+        assert r.stats().stack_overflows == 0
+        r.close()
+    same = np.all(lanes[0] == lanes[1], axis=1).mean()
+    say(f"packet vs per-lane traversal: {same:.5f} of paths identical")
+    assert same >= 0.9995
+
+
 def test_room_c4_scene_per_path():
     """The 4M-triangle room (slot light 1e3, glass sphere, environment), 64x32 pixels, 2 spp,
     depth 16, consistent MIS."""
