@@ -780,22 +780,24 @@ static constexpr int kPacketStack = 64;
 // equal distances may resolve to another primitive than in the per-lane traversal).  Runs
 // are claimed 64 at a time; the wave stack holds kPacketStack entries (launched only when the
 // tree's 3 x depth + 1 fits, with an overflow counter besides).
-__global__ __launch_bounds__(kTraceThreads, 6) void k_extend_packet(PtSceneView sc, int *counters, int segLen, const float4 *rayD, float4 camO,
-                                                                   float4 *hits, int *overflow) {
+__global__ __launch_bounds__(kTraceThreads, 6) void k_extend_packet(PtSceneView sc, int *counters, int bounce, int segLen, const float4 *rayO,
+                                                                   const float4 *rayD, float4 camO, float4 *hits, int *overflow) {
     __shared__ int wstack[kTraceThreads / 64][kPacketStack];
     int *ws = wstack[threadIdx.x >> 6];
     const char *nodes = (const char *)sc.nodes;
     SegCursor cur = segStart();
     while (true) {
         int slot = 0;
-        const int got = segClaim(counters, 0, PT_Q_RAYS, PT_Q_FETCH_EXT, segLen, 64, cur, &slot);
+        const int got = segClaim(counters, bounce, PT_Q_RAYS, PT_Q_FETCH_EXT, segLen, 64, cur, &slot);
         if (got == 0) break;
         const bool valid = laneId() < got;
         const int i = slot + (valid ? laneId() : 0);
         const float4 rd = rayD[i];
+        // camera rays (camO.w != 0) start at the camera; later ones carry origin and tmin
+        const float4 ro = camO.w != 0.f ? make_float4(camO.x, camO.y, camO.z, 0.f) : rayO[i];
         LaneRay r;
         // a lane without a ray gets an empty interval: every box and primitive test fails
-        laneRayStart(r, xyz(camO), xyz(rd), 0.f, valid ? rd.w : -1.f, 0);
+        laneRayStart(r, xyz(ro), xyz(rd), ro.w, valid ? rd.w : -1.f, 0);
         int node = 0, sp = 0;   // wave-uniform
         while (true) {
             if (node >= 0) {
@@ -1894,7 +1896,7 @@ void launchExtend(pt_renderer *r, int bounce, int segLen, const float4 *ro, cons
     int *ctr = r->pp->counters.as<int>();
     int *ov = r->pp->overflow.as<int>();
     if (packet) {
-        k_extend_packet<<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, segLen, rd, camO, hits, ov);
+        k_extend_packet<<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, ro, rd, camO, hits, ov);
         PT_HIP_TRY(hipGetLastError());
         return;
     }
