@@ -82,7 +82,8 @@ __host__ __device__ __forceinline__ int ptSegLen(int64_t n) {
 }
 // Camera queue: path p belongs to run p / 64, dealt to segment (p / 64) % PT_QUEUE_SEGS, so
 // every segment holds runs from the whole chunk (similar work per segment) and a run of 64
-// neighbouring pixels stays together.
+// consecutive paths stays together (with the renderer's pixel-major path numbering, the
+// samples of one pixel, or of a few neighbouring pixels).
 __host__ __device__ __forceinline__ int ptCameraSlot(int p, int segLen) {
     const int run = p >> 6;
     return (run % PT_QUEUE_SEGS) * segLen + (run / PT_QUEUE_SEGS) * 64 + (p & 63);
