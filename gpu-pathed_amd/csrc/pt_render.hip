@@ -765,8 +765,9 @@ __global__ __launch_bounds__(kTraceThreads, STACK <= 16 ? 6 : 5) void k_extend(P
         [&](int i, bool, const TraceHit &th) { hits[i] = make_float4(th.t, th.u, th.v, asFloat(th.prim)); }, tstat);
 }
 
-// Wave stack entries of the packet traversal.
+// Wave stack entries of the packet traversal, and its blocks per CU.
 static constexpr int kPacketStack = 64;
+static constexpr int kPacketBlocksPerCU = 16;
 
 // Camera bounce of a pixel-major chunk whose runs of 64 camera rays are one pixel's samples
 // (or a few neighbouring pixels' samples),
@@ -1896,7 +1897,10 @@ void launchExtend(pt_renderer *r, int bounce, int segLen, const float4 *ro, cons
     int *ctr = r->pp->counters.as<int>();
     int *ov = r->pp->overflow.as<int>();
     if (packet) {
-        k_extend_packet<<<grid, kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, ro, rd, camO, hits, ov);
+        // the packet kernel holds 64 registers and a 512-byte LDS stack per block: 16 two-wave
+        // blocks per CU fit (8 waves per SIMD); C1 6037-6050 against 5953-5972 Mrays/s with the
+        // per-lane kernels' 11 blocks per CU
+        k_extend_packet<<<std::max(grid, r->cus * kPacketBlocksPerCU), kTraceThreads, 0, r->stream>>>(r->view, ctr, bounce, segLen, ro, rd, camO, hits, ov);
         PT_HIP_TRY(hipGetLastError());
         return;
     }
